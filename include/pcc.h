@@ -1,0 +1,196 @@
+/*
+ * pcc.h — C ABI of the MI355X collision-pipeline library (libpcc.so).
+ *
+ * One call of pcc_step() is one timestep of the proliferating-cell-colony
+ * collision pipeline on one GPU: spatial-grid contact detection, constraint
+ * assembly, then either the hard-model ReLCP/BBPGD complementarity solve or the
+ * soft-model Hertz force assembly, then FP64 state integration and growth.
+ *
+ * It replaces, on one GPU and without MPI/PETSc:
+ *   ParticleManager::step(int, std::function<void()>)
+ *       code/cpp/src/simulation/ParticleManager.h:50, ParticleManager.cpp:150-166
+ *   solveHardModel(...)      code/cpp/src/dynamics/models/Hard.h:9, Hard.cpp:91-193
+ *   solveSoftPotential(...)  code/cpp/src/dynamics/models/Soft.h:8, Soft.cpp:21-114
+ * and returns the same SolverSolution summary
+ *   (ParticleManager.h:32-38; built at Hard.cpp:192 and Soft.cpp:113).
+ *
+ * Conventions
+ *  - Plain C: no exceptions cross this boundary; every entry point returns a
+ *    pcc_status and pcc_last_error() gives the message.  The three solver
+ *    failures of the reference map 1:1 to PCC_E_BBPGD_NOT_CONVERGED
+ *    ("BBPGD did not converge", BBPGD.cpp:118-121), PCC_E_RELCP_MAX_ITERS
+ *    ("Maximum number of constraint iterations reached", Hard.cpp:187-190) and
+ *    PCC_E_INVALID_MODE ("Invalid mode", ParticleManager.cpp:165).
+ *  - Body index == global id (gid).  The reference numbers bodies 0..N-1 by
+ *    ascending x on one rank (Domain::assignGlobalIDs, Domain.cpp:492-508);
+ *    the caller passes bodies in gid order.
+ *  - Host buffers are owned by the caller; device buffers by the context.  A
+ *    context owns one HIP stream on one device; every call is stream-ordered
+ *    and synchronous at return.  One context per host thread.
+ *  - State stays resident on the device between steps.
+ */
+#ifndef PCC_H_
+#define PCC_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum pcc_status {
+  PCC_OK = 0,
+  PCC_E_BBPGD_NOT_CONVERGED = 1, /* BBPGD.cpp:118-121 */
+  PCC_E_RELCP_MAX_ITERS = 2,     /* Hard.cpp:187-190 */
+  PCC_E_INVALID_MODE = 3,        /* ParticleManager.cpp:165 */
+  PCC_E_INVALID_ARGUMENT = 4,
+  PCC_E_DEVICE = 5,              /* a HIP runtime call failed */
+  PCC_E_OUT_OF_MEMORY = 6,
+  PCC_E_UNSUPPORTED = 7,         /* e.g. temperature > 0 (Brownian input) */
+  PCC_E_NO_STATE = 8             /* pcc_step before pcc_set_state */
+} pcc_status;
+
+enum { PCC_MODE_HARD = 0, PCC_MODE_SOFT = 1 };
+
+/* FP64: every kernel computes in double; the whole step is bit-identical to the
+ *       FP64 restatement of the reference algorithm.
+ * MIXED: constraint Jacobian/stress blocks, impulses and gradients are stored
+ *       and applied in FP32 (D, D^T, L, L^T, Hertz sums); the BB step-size dot
+ *       products, the projected-gradient residual, contact geometry and the
+ *       position/orientation/length integration stay FP64. */
+enum { PCC_PREC_FP64 = 0, PCC_PREC_MIXED = 1 };
+
+/* Mirrors SimulationParameters (code/cpp/src/util/Config.h:9-56) for the fields
+ * the step reads, with the reference's single `tolerance` (Q8: it drives the
+ * BBPGD residual, the ReLCP overlap test and the phase>=1 contact cutoff,
+ * Hard.cpp:121,132,153) split into three.  pcc_default_params() ties them as
+ * the reference does. */
+typedef struct pcc_params {
+  int32_t mode;      /* PCC_MODE_HARD | PCC_MODE_SOFT  (-mode) */
+  int32_t precision; /* PCC_PREC_FP64 | PCC_PREC_MIXED */
+  double dt;         /* sim_config.dt_s (-dt) */
+  double xi;         /* drag coefficient (-xi) */
+  double tau;        /* growth time constant (-tau) */
+  double lambda;     /* stress sensitivity (-lambda) */
+  double temperature;/* Brownian temperature; only 0 is supported */
+  double kcc;        /* soft-model collision constant (-kcc) */
+  double bbpgd_tol;      /* BBPGD projected-gradient residual bound */
+  double relcp_tol;      /* ReLCP stops once max overlap <= relcp_tol (k > 0) */
+  double cutoff_phase0;  /* contact cutoff of ReLCP phase 0 (reference: 0.25) */
+  double cutoff_phaseK;  /* contact cutoff of phases k >= 1 (reference: tolerance) */
+  int64_t max_bbpgd_iterations;     /* -max_bbpgd_iterations */
+  int64_t max_recursive_iterations; /* -max_recursive_iterations */
+  double linked_cell_size;          /* -linked_cell_size (default 2.2*l0) */
+  double grid_origin;               /* -1 on every axis (CollisionDetector.cpp:24) */
+  double warm_start_factor;         /* gamma0 = -factor*phi for phi < 0 (500, CollisionDetector.cpp:308) */
+} pcc_params;
+
+/* Host-side body state, gid order.  Vectors are interleaved per body
+ * (pos[3*i+k], quat[4*i+k] with quat = (w, x, y, z)).  Fields marked (out)
+ * are written by pcc_get_state and ignored by pcc_set_state (may be NULL). */
+typedef struct pcc_bodies {
+  int64_t n;
+  double* pos;        /* 3n */
+  double* quat;       /* 4n, unit (w, x, y, z) */
+  double* length;     /* n */
+  double* l0;         /* n */
+  double* diameter;   /* n */
+  double* ldot;       /* n */
+  double* force;      /* 3n (out): last ReLCP phase's force increment (Q9) */
+  double* torque;     /* 3n (out) */
+  double* velocity;   /* 3n (out) */
+  double* omega;      /* 3n (out) */
+  double* impedance;  /* n (out) */
+  double* stress;     /* n (out) */
+  int32_t* num_constraints; /* n (out) */
+} pcc_bodies;
+
+/* SoA image of std::vector<Constraint> (code/cpp/src/dynamics/Constraint.h:6-49).
+ * `capacity` is the length of every array on input (vectors hold 3*capacity). */
+typedef struct pcc_constraints {
+  int64_t capacity;
+  int32_t* gidI;            /* lower body index of the pair */
+  int32_t* gidJ;
+  double* signed_distance;  /* phi */
+  double* normal;           /* 3c, normI = normalize(closest_I - closest_J) */
+  double* rPosI;            /* 3c */
+  double* rPosJ;            /* 3c */
+  double* contact_point;    /* 3c */
+  double* stressI;
+  double* stressJ;
+  double* gamma;
+  int32_t* gid;             /* index within its detection batch */
+  int32_t* iteration;       /* ReLCP phase that created it */
+} pcc_constraints;
+
+/* SolverSolution without the constraint vector (fetch it with
+ * pcc_get_constraints). */
+typedef struct pcc_step_result {
+  int64_t n_constraints;
+  int32_t constraint_iterations;
+  int32_t reserved;
+  int64_t bbpgd_iterations;
+  double residual;
+  double max_overlap;
+} pcc_step_result;
+
+typedef struct pcc_ctx pcc_ctx;
+
+/* Reference defaults (code/cpp/src/util/CLIParser.h:167-194,232) with the
+ * tolerance coupling of Q8, for mode hard, FP64, l0 = 1; temperature 0. */
+void pcc_default_params(pcc_params* p);
+
+pcc_status pcc_create(const pcc_params* params, int device, pcc_ctx** out);
+void pcc_destroy(pcc_ctx* ctx);
+pcc_status pcc_set_params(pcc_ctx* ctx, const pcc_params* params);
+const char* pcc_last_error(const pcc_ctx* ctx);
+const char* pcc_status_string(pcc_status s);
+
+/* Upload/download the body state (host buffers, gid order). */
+pcc_status pcc_set_state(pcc_ctx* ctx, const pcc_bodies* bodies);
+pcc_status pcc_get_state(pcc_ctx* ctx, pcc_bodies* bodies);
+
+/* Device-side copy of the current body state into a second resident buffer,
+ * and back: lets a caller replay the same step from identical inputs without
+ * host traffic. */
+pcc_status pcc_snapshot(pcc_ctx* ctx);
+pcc_status pcc_restore(pcc_ctx* ctx);
+
+/* One timestep (ParticleManager::step).  `iter` is the step index, used by the
+ * reference only to gate tracing (Hard.cpp:109).  On PCC_E_RELCP_MAX_ITERS the
+ * state has been advanced and grown, as the reference does before throwing. */
+pcc_status pcc_step(pcc_ctx* ctx, int32_t iter, pcc_step_result* out);
+
+/* Constraint set of the last step (all ReLCP phases, in solver order).
+ * *count receives the number of constraints; arrays are filled up to
+ * out->capacity entries. */
+pcc_status pcc_get_constraints(pcc_ctx* ctx, pcc_constraints* out, int64_t* count);
+
+/* Contact detection only, at the current state and a given cutoff: fills the
+ * constraints of one detection batch (CollisionDetector::detectCollisions,
+ * CollisionDetector.cpp:84-111) and, if non-NULL, the grid ordering
+ * (bucket of every body and the stable body permutation by bucket). */
+pcc_status pcc_detect(pcc_ctx* ctx, double cutoff, pcc_constraints* out, int64_t* count,
+                      uint32_t* bucket_out, int32_t* perm_out);
+
+/* Per-kernel GPU time accounting with HIP events on the context stream.
+ * Kernel ids: see pcc_kernel_name(). */
+enum {
+  PCC_K_BODY_APPLY = 0,   /* D^T/L^T segmented reduction + mobility + growth law */
+  PCC_K_CONSTR_APPLY = 1, /* D/L gather + projection + BB dots + residual */
+  PCC_K_DETECT = 2,
+  PCC_K_ASSEMBLE = 3,
+  PCC_K_INTEGRATE = 4,
+  PCC_K_HERTZ = 5,
+  PCC_K_COUNT = 6
+};
+pcc_status pcc_set_timing(pcc_ctx* ctx, int enable);
+pcc_status pcc_get_timing(pcc_ctx* ctx, int kernel, int64_t* launches, double* total_ms,
+                          double* bytes_per_launch);
+const char* pcc_kernel_name(int kernel);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PCC_H_ */

@@ -1,0 +1,154 @@
+"""ctypes wrapper over the FP64 oracle (liboracle.so / liboracle_fast.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker or the CPU baseline, never
+as the measured or shipped path.
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_PKG = os.path.join(os.path.dirname(_HERE), "proliferatingcellcollectives-idp_amd")
+if _PKG not in sys.path:
+    sys.path.insert(0, _PKG)
+
+from pcc.abi_types import (Bodies, Constraints, Params, StepResult,  # noqa: E402
+                           reference_params)
+
+_libs = {}
+
+
+def lib(fast: bool = False):
+    name = "liboracle_fast.so" if fast else "liboracle.so"
+    if name not in _libs:
+        path = os.path.join(_HERE, name)
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} not built (run `make -C oracle`)")
+        L = C.CDLL(path)
+        dp = C.POINTER(C.c_double)
+        L.oracle_dcp.argtypes = [dp, dp, dp, dp, dp]
+        L.oracle_direction.argtypes = [dp, dp]
+        L.oracle_normalize.argtypes = [dp, dp]
+        L.oracle_exp.argtypes = [C.c_double]
+        L.oracle_exp.restype = C.c_double
+        L.oracle_block_sum.argtypes = [dp, C.c_int64]
+        L.oracle_block_sum.restype = C.c_double
+        L.oracle_table_size.argtypes = [C.c_int64]
+        L.oracle_table_size.restype = C.c_int64
+        L.oracle_grid.argtypes = [C.c_int64, dp, C.c_double, C.c_double,
+                                  C.POINTER(C.c_int32), C.POINTER(C.c_uint32),
+                                  C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int64]
+        L.oracle_grid.restype = C.c_int64
+        L.oracle_detect.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double,
+                                    C.c_int32, C.c_int, C.c_void_p, C.POINTER(C.c_int32)]
+        L.oracle_detect.restype = C.c_int64
+        L.oracle_step.argtypes = [C.POINTER(Params), C.c_void_p, C.c_int32,
+                                  C.POINTER(StepResult), C.c_void_p, C.POINTER(C.c_int64),
+                                  C.c_int64, C.c_char_p, C.c_int64]
+        L.oracle_step.restype = C.c_int
+        L.oracle_set_iteration_cap.argtypes = [C.c_int64]
+        L.oracle_threads.restype = C.c_int
+        _libs[name] = L
+    return _libs[name]
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def dcp(P0, P1, Q0, Q1):
+    out = np.zeros(10)
+    args = [np.ascontiguousarray(v, dtype=np.float64) for v in (P0, P1, Q0, Q1)]
+    lib().oracle_dcp(*[_d(a) for a in args], _d(out))
+    return out
+
+
+def direction(q):
+    t = np.zeros(3)
+    lib().oracle_direction(_d(np.ascontiguousarray(q, dtype=np.float64)), _d(t))
+    return t
+
+
+def normalize(a):
+    o = np.zeros(3)
+    lib().oracle_normalize(_d(np.ascontiguousarray(a, dtype=np.float64)), _d(o))
+    return o
+
+
+def exp(x):
+    return lib().oracle_exp(float(x))
+
+
+def block_sum(v):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    return lib().oracle_block_sum(_d(v), len(v))
+
+
+def grid(bodies: Bodies, h=2.2, origin=-1.0):
+    n = bodies.n
+    T = lib().oracle_table_size(n)
+    cell = np.zeros((max(n, 1), 3), np.int32)
+    bucket = np.zeros(max(n, 1), np.uint32)
+    perm = np.zeros(max(n, 1), np.int32)
+    start = np.zeros(T + 1, np.int32)
+    lib().oracle_grid(n, _d(bodies.pos), h, origin,
+                      cell.ctypes.data_as(C.POINTER(C.c_int32)),
+                      bucket.ctypes.data_as(C.POINTER(C.c_uint32)),
+                      perm.ctypes.data_as(C.POINTER(C.c_int32)),
+                      start.ctypes.data_as(C.POINTER(C.c_int32)), T + 1)
+    return {"table_size": T, "cell": cell[:n], "bucket": bucket[:n], "perm": perm[:n],
+            "bucket_start": start}
+
+
+def detect(bodies: Bodies, cutoff, h=2.2, origin=-1.0, warm=500.0, iteration=0, brute=False,
+           capacity=None):
+    """One detection batch sorted by (contact x, gidI, gidJ)."""
+    bs = bodies.c_struct()
+    if capacity is None:
+        capacity = 16 * max(bodies.n, 1) + 64
+    cons = Constraints(capacity)
+    cs = cons.c_struct()
+    nc = np.zeros(max(bodies.n, 1), np.int32)
+    cnt = lib().oracle_detect(C.byref(bs), cutoff, h, origin, warm, iteration, int(brute),
+                              C.byref(cs), nc.ctypes.data_as(C.POINTER(C.c_int32)))
+    if cnt > capacity:
+        return detect(bodies, cutoff, h, origin, warm, iteration, brute, capacity=cnt)
+    cons.count = cnt
+    return cons.trimmed()
+
+
+def step(params: Params, bodies: Bodies, iter_=0, capacity=None, fast=False, phase_iters=64,
+         iteration_cap=0):
+    """One timestep on a copy of `bodies`; returns (status, result dict,
+    new bodies, constraints dict, per-phase BBPGD iterations, error text)."""
+    b = bodies.copy()
+    bs = b.c_struct()
+    if capacity is None:
+        capacity = 64 * max(b.n, 1) + 64
+    cons = Constraints(capacity)
+    cs = cons.c_struct()
+    res = StepResult()
+    ph = np.zeros(phase_iters, np.int64)
+    err = C.create_string_buffer(256)
+    L = lib(fast)
+    L.oracle_set_iteration_cap(int(iteration_cap))
+    st = L.oracle_step(C.byref(params), C.byref(bs), int(iter_), C.byref(res), C.byref(cs),
+                       ph.ctypes.data_as(C.POINTER(C.c_int64)), phase_iters, err, 256)
+    L.oracle_set_iteration_cap(0)
+    if res.n_constraints > capacity:
+        return step(params, bodies, iter_, capacity=int(res.n_constraints), fast=fast,
+                    phase_iters=phase_iters, iteration_cap=iteration_cap)
+    cons.count = int(res.n_constraints)
+    return st, res.as_dict(), b, cons.trimmed(), ph[: max(res.constraint_iterations, 0) + 1], \
+        err.value.decode()
+
+
+def threads(fast=False):
+    return lib(fast).oracle_threads()
+
+
+__all__ = ["Bodies", "Params", "reference_params", "dcp", "direction", "normalize", "exp",
+           "block_sum", "grid", "detect", "step", "lib", "threads"]

@@ -1,0 +1,1110 @@
+/*
+ * pcc_oracle.c — FP64 CPU restatement of the reference's per-timestep collision
+ * pipeline.  TEST INFRASTRUCTURE ONLY (checker for libpcc and the CPU baseline
+ * of bench.py); the product never links or calls it.
+ *
+ * Built twice by oracle/Makefile:
+ *   liboracle.so        strict IEEE (-O2 -ffp-contract=off): the parity oracle;
+ *   liboracle_fast.so   the reference's Release flags (-O3 -ffast-math
+ *                       -funroll-loops, code/cpp/CMakeLists.txt:17) + OpenMP:
+ *                       the timed CPU baseline.
+ *
+ * Operation order follows the reference line by line (cited per function) so
+ * that the FP64 GPU path, which evaluates the same expressions, reproduces it
+ * bit for bit.  The deliberate deviations are listed in pcc_oracle.h.
+ */
+#include "pcc_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+int oracle_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+static int64_t g_iteration_cap = 0;
+void oracle_set_iteration_cap(int64_t cap) { g_iteration_cap = cap; }
+
+/* ------------------------------------------------------------------------ */
+/* Small vector math: ArrayMath.h                                            */
+/* ------------------------------------------------------------------------ */
+
+/* dot: `result = 0; result += a[d]*b[d]` (ArrayMath.h:152-159). */
+static inline double dot3(const double a[3], const double b[3]) {
+  double r = 0.0;
+  r += a[0] * b[0];
+  r += a[1] * b[1];
+  r += a[2] * b[2];
+  return r;
+}
+
+/* cross_product (ArrayMath.h:139-142). */
+static inline void cross3(const double a[3], const double b[3], double o[3]) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* normalize (ArrayMath.h:218-231): a * (1/|a|), or (1,0,0) if |a| <= eps. */
+void oracle_normalize(const double a[3], double o[3]) {
+  double mag = sqrt(dot3(a, a));
+  if (mag > DBL_EPSILON) {
+    double inv = 1.0 / mag;
+    o[0] = a[0] * inv;
+    o[1] = a[1] * inv;
+    o[2] = a[2] * inv;
+  } else {
+    o[0] = 1.0;
+    o[1] = 0.0;
+    o[2] = 0.0;
+  }
+}
+
+/* Quaternion::rotateVectorOfPositions with v = e_x (Quaternion.cpp:7-38). */
+void oracle_direction(const double q[4], double t[3]) {
+  const double w = q[0], x = q[1], y = q[2], z = q[3];
+  const double vx = 1.0, vy = 0.0, vz = 0.0;
+  double c1x = y * vz - z * vy;
+  double c1y = z * vx - x * vz;
+  double c1z = x * vy - y * vx;
+  double c2x = y * c1z - z * c1y;
+  double c2y = z * c1x - x * c1z;
+  double c2z = x * c1y - y * c1x;
+  t[0] = vx + 2.0 * w * c1x + 2.0 * c2x;
+  t[1] = vy + 2.0 * w * c1y + 2.0 * c2y;
+  t[2] = vz + 2.0 * w * c1z + 2.0 * c2z;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Deterministic exp: Cody & Waite (1980) argument reduction by ln2 split into */
+/* a 33-bit head and a tail, then the degree-13 Taylor polynomial of e^r on   */
+/* |r| <= ln2/2 in Horner form, then exact scaling by 2^k.  Coefficients are */
+/* 1/n! correctly rounded.  Same expression order on CPU and GPU.            */
+/* ------------------------------------------------------------------------ */
+double oracle_exp(double x) {
+  if (x != x) return x;
+  if (x > 709.782712893384) return INFINITY;
+  if (x < -745.1332191019412) return 0.0;
+  const double inv_ln2 = 0x1.71547652b82fep+0;
+  const double ln2_hi = 0x1.62e42fef00000p-1;
+  const double ln2_lo = 0x1.473de6af278edp-34;
+  double k = rint(x * inv_ln2);
+  double r = (x - k * ln2_hi) - k * ln2_lo;
+  double p = 0x1.6124613a86d09p-33;      /* 1/13! */
+  p = p * r + 0x1.1eed8eff8d898p-29;     /* 1/12! */
+  p = p * r + 0x1.ae64567f544e4p-26;     /* 1/11! */
+  p = p * r + 0x1.27e4fb7789f5cp-22;     /* 1/10! */
+  p = p * r + 0x1.71de3a556c734p-19;     /* 1/9!  */
+  p = p * r + 0x1.a01a01a01a01ap-16;     /* 1/8!  */
+  p = p * r + 0x1.a01a01a01a01ap-13;     /* 1/7!  */
+  p = p * r + 0x1.6c16c16c16c17p-10;     /* 1/6!  */
+  p = p * r + 0x1.1111111111111p-7;      /* 1/5!  */
+  p = p * r + 0x1.5555555555555p-5;      /* 1/4!  */
+  p = p * r + 0x1.5555555555555p-3;      /* 1/3!  */
+  p = p * r + 0.5;
+  p = p * r + 1.0;
+  p = p * r + 1.0;
+  return ldexp(p, (int)k);
+}
+
+/* ------------------------------------------------------------------------ */
+/* DCP: DCPQuery<double,3>::operator() (SpherocylinderCell.h:64-261)         */
+/* ------------------------------------------------------------------------ */
+void oracle_dcp(const double P0[3], const double P1[3], const double Q0[3], const double Q1[3],
+                double out[10]) {
+  double P1mP0[3], Q1mQ0[3], P0mQ0[3];
+  for (int k = 0; k < 3; ++k) {
+    P1mP0[k] = P1[k] - P0[k];
+    Q1mQ0[k] = Q1[k] - Q0[k];
+    P0mQ0[k] = P0[k] - Q0[k];
+  }
+  double a = dot3(P1mP0, P1mP0);
+  double b = dot3(P1mP0, Q1mQ0);
+  double c = dot3(Q1mQ0, Q1mQ0);
+  double d = dot3(P1mP0, P0mQ0);
+  double e = dot3(Q1mQ0, P0mQ0);
+  double det = a * c - b * b;
+  double s, t, nd, bmd, bte, ctd, bpe, ate, btd;
+  const double zero = 0.0, one = 1.0;
+  if (det > zero) {
+    bte = b * e;
+    ctd = c * d;
+    if (bte <= ctd) {
+      s = zero;
+      if (e <= zero) {
+        t = zero;
+        nd = -d;
+        if (nd >= a) {
+          s = one;
+        } else if (nd > zero) {
+          s = nd / a;
+        }
+      } else if (e < c) {
+        t = e / c;
+      } else {
+        t = one;
+        bmd = b - d;
+        if (bmd >= a) {
+          s = one;
+        } else if (bmd > zero) {
+          s = bmd / a;
+        }
+      }
+    } else {
+      s = bte - ctd;
+      if (s >= det) {
+        s = one;
+        bpe = b + e;
+        if (bpe <= zero) {
+          t = zero;
+          nd = -d;
+          if (nd <= zero) {
+            s = zero;
+          } else if (nd < a) {
+            s = nd / a;
+          }
+        } else if (bpe < c) {
+          t = bpe / c;
+        } else {
+          t = one;
+          bmd = b - d;
+          if (bmd <= zero) {
+            s = zero;
+          } else if (bmd < a) {
+            s = bmd / a;
+          }
+        }
+      } else {
+        ate = a * e;
+        btd = b * d;
+        if (ate <= btd) {
+          t = zero;
+          nd = -d;
+          if (nd <= zero) {
+            s = zero;
+          } else if (nd >= a) {
+            s = one;
+          } else {
+            s = nd / a;
+          }
+        } else {
+          t = ate - btd;
+          if (t >= det) {
+            t = one;
+            bmd = b - d;
+            if (bmd <= zero) {
+              s = zero;
+            } else if (bmd >= a) {
+              s = one;
+            } else {
+              s = bmd / a;
+            }
+          } else {
+            s /= det;
+            t /= det;
+          }
+        }
+      }
+    }
+  } else {
+    if (e <= zero) {
+      t = zero;
+      nd = -d;
+      if (nd <= zero) {
+        s = zero;
+      } else if (nd >= a) {
+        s = one;
+      } else {
+        s = nd / a;
+      }
+    } else if (e >= c) {
+      t = one;
+      bmd = b - d;
+      if (bmd <= zero) {
+        s = zero;
+      } else if (bmd >= a) {
+        s = one;
+      } else {
+        s = bmd / a;
+      }
+    } else {
+      s = zero;
+      t = e / c;
+    }
+  }
+  double c0[3], c1[3], diff[3];
+  for (int k = 0; k < 3; ++k) {
+    c0[k] = P0[k] + P1mP0[k] * s;
+    c1[k] = Q0[k] + Q1mQ0[k] * t;
+  }
+  for (int k = 0; k < 3; ++k) diff[k] = c0[k] - c1[k];
+  double sqr = dot3(diff, diff);
+  out[0] = s;
+  out[1] = t;
+  out[2] = c0[0];
+  out[3] = c0[1];
+  out[4] = c0[2];
+  out[5] = c1[0];
+  out[6] = c1[1];
+  out[7] = c1[2];
+  out[8] = sqr;
+  out[9] = sqrt(sqr);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Broad phase: hashed cell list with a stable counting sort                 */
+/* ------------------------------------------------------------------------ */
+
+/* Cell coordinate: static_cast<int>(floor((x - min)/h)) (SpatialGrid.cpp:29-36). */
+static inline int32_t cell_coord(double x, double origin, double h) {
+  return (int32_t)floor((x - origin) / h);
+}
+
+/* 32-bit mix of the three cell coordinates: odd-multiplier combination then
+ * the MurmurHash3 fmix32 finaliser (Appleby). */
+static inline uint32_t cell_hash(int32_t cx, int32_t cy, int32_t cz) {
+  uint32_t h = (uint32_t)cx * 0x9E3779B1u + (uint32_t)cy * 0x85EBCA77u + (uint32_t)cz * 0xC2B2AE3Du;
+  h ^= h >> 16;
+  h *= 0x85EBCA6Bu;
+  h ^= h >> 13;
+  h *= 0xC2B2AE35u;
+  h ^= h >> 16;
+  return h;
+}
+
+int64_t oracle_table_size(int64_t n) {
+  int64_t t = 1024;
+  while (t < 2 * n) t <<= 1;
+  return t;
+}
+
+int64_t oracle_grid(int64_t n, const double* pos, double h, double origin, int32_t* cell,
+                    uint32_t* bucket, int32_t* perm, int32_t* bucket_start,
+                    int64_t bucket_start_cap) {
+  int64_t T = oracle_table_size(n);
+  uint32_t mask = (uint32_t)(T - 1);
+  int32_t* cl = cell ? cell : (int32_t*)malloc(sizeof(int32_t) * 3 * (size_t)(n > 0 ? n : 1));
+  uint32_t* bk = bucket ? bucket : (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+  for (int64_t i = 0; i < n; ++i) {
+    for (int k = 0; k < 3; ++k) cl[3 * i + k] = cell_coord(pos[3 * i + k], origin, h);
+    bk[i] = cell_hash(cl[3 * i], cl[3 * i + 1], cl[3 * i + 2]) & mask;
+  }
+  if (perm && bucket_start && bucket_start_cap >= T + 1) {
+    memset(bucket_start, 0, sizeof(int32_t) * (size_t)(T + 1));
+    for (int64_t i = 0; i < n; ++i) bucket_start[bk[i] + 1]++;
+    for (int64_t b = 0; b < T; ++b) bucket_start[b + 1] += bucket_start[b];
+    int32_t* cur = (int32_t*)malloc(sizeof(int32_t) * (size_t)T);
+    memcpy(cur, bucket_start, sizeof(int32_t) * (size_t)T);
+    for (int64_t i = 0; i < n; ++i) perm[cur[bk[i]]++] = (int32_t)i;
+    free(cur);
+  }
+  if (!cell) free(cl);
+  if (!bucket) free(bk);
+  return T;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Narrow phase: tryCreateConstraint (CollisionDetector.cpp:201-311)         */
+/* ------------------------------------------------------------------------ */
+typedef struct rec_t {
+  int32_t I, J;
+  double phi;
+  double n[3], rI[3], rJ[3], p[3];
+  double sI, sJ, gamma;
+} rec_t;
+
+typedef struct body_view {
+  const double* pos;
+  const double* quat;
+  const double* len;
+  const double* diam;
+} body_view;
+
+/* getParticleEndpoints (CollisionDetector.cpp:61-75). */
+static inline void endpoints(const body_view* bv, int64_t i, double P0[3], double P1[3],
+                             double t[3]) {
+  oracle_direction(bv->quat + 4 * i, t);
+  double half = bv->len[i] / 2 - bv->diam[i] / 2;
+  for (int k = 0; k < 3; ++k) {
+    double hv = t[k] * half;
+    P0[k] = bv->pos[3 * i + k] - hv;
+    P1[k] = bv->pos[3 * i + k] + hv;
+  }
+}
+
+/* Returns 1 and fills *r if (I, J) yields a constraint at this cutoff. */
+static int try_pair(const body_view* bv, int32_t I, int32_t J, double cutoff, double warm,
+                    rec_t* r) {
+  const double* xi = bv->pos + 3 * (int64_t)I;
+  const double* xj = bv->pos + 3 * (int64_t)J;
+  double dx[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+  double rough = sqrt(dot3(dx, dx));
+  if (rough > (bv->len[I] + bv->len[J]) / 2.0 + cutoff) return 0;
+  double P0[3], P1[3], Q0[3], Q1[3], tI[3], tJ[3], dcp[10];
+  endpoints(bv, I, P0, P1, tI);
+  endpoints(bv, J, Q0, Q1, tJ);
+  oracle_dcp(P0, P1, Q0, Q1, dcp);
+  const double* c0 = dcp + 2;
+  const double* c1 = dcp + 5;
+  double sum_radii = (bv->diam[I] + bv->diam[J]) / 2.0;
+  double phi = dcp[9] - sum_radii;
+  if (phi > cutoff) return 0;
+  r->I = I;
+  r->J = J;
+  r->phi = phi;
+  double d01[3];
+  for (int k = 0; k < 3; ++k) {
+    r->p[k] = (c0[k] + c1[k]) * 0.5;
+    d01[k] = c0[k] - c1[k];
+  }
+  for (int k = 0; k < 3; ++k) {
+    r->rI[k] = r->p[k] - xi[k];
+    r->rJ[k] = r->p[k] - xj[k];
+  }
+  oracle_normalize(d01, r->n);
+  double mn[3] = {-r->n[0], -r->n[1], -r->n[2]};
+  r->sI = 0.5 * fabs(dot3(r->n, tI));
+  r->sJ = 0.5 * fabs(dot3(mn, tJ));
+  r->gamma = phi >= 0.0 ? 0.0 : -phi * warm;
+  return 1;
+}
+
+static int rec_cmp(const void* pa, const void* pb) {
+  const rec_t* a = (const rec_t*)pa;
+  const rec_t* b = (const rec_t*)pb;
+  if (a->p[0] < b->p[0]) return -1;
+  if (b->p[0] < a->p[0]) return 1;
+  if (a->I != b->I) return a->I < b->I ? -1 : 1;
+  if (a->J != b->J) return a->J < b->J ? -1 : 1;
+  return 0;
+}
+
+/* Detection into a malloc'd record array sorted by (p.x, I, J). */
+static int64_t detect_records(const body_view* bv, int64_t n, double cutoff, double h,
+                              double origin, double warm, int brute, rec_t** out_recs) {
+  int64_t T = oracle_table_size(n);
+  int32_t* cell = (int32_t*)malloc(sizeof(int32_t) * 3 * (size_t)(n > 0 ? n : 1));
+  uint32_t* bkt = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+  int32_t* perm = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+  int32_t* start = (int32_t*)malloc(sizeof(int32_t) * (size_t)(T + 1));
+  oracle_grid(n, bv->pos, h, origin, cell, bkt, perm, start, T + 1);
+  uint32_t mask = (uint32_t)(T - 1);
+
+  int64_t* cnt = (int64_t*)calloc((size_t)n + 1, sizeof(int64_t));
+  /* pass 1 counts, pass 2 writes (both visit the same candidates) */
+  rec_t* recs = NULL;
+  for (int pass = 0; pass < 2; ++pass) {
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t I = 0; I < n; ++I) {
+      int64_t w = pass ? cnt[I] : 0;
+      int64_t c = 0;
+      rec_t r;
+      if (brute) {
+        for (int64_t J = I + 1; J < n; ++J) {
+          if (try_pair(bv, (int32_t)I, (int32_t)J, cutoff, warm, &r)) {
+            if (pass) recs[w + c] = r;
+            ++c;
+          }
+        }
+      } else {
+        const int32_t cx = cell[3 * I], cy = cell[3 * I + 1], cz = cell[3 * I + 2];
+        for (int dz = -1; dz <= 1; ++dz)
+          for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+              int32_t nx = cx + dx, ny = cy + dy, nz = cz + dz;
+              uint32_t b = cell_hash(nx, ny, nz) & mask;
+              for (int32_t e = start[b]; e < start[b + 1]; ++e) {
+                int32_t J = perm[e];
+                if (J <= I) continue;
+                if (cell[3 * (int64_t)J] != nx || cell[3 * (int64_t)J + 1] != ny ||
+                    cell[3 * (int64_t)J + 2] != nz)
+                  continue;
+                if (try_pair(bv, (int32_t)I, J, cutoff, warm, &r)) {
+                  if (pass) recs[w + c] = r;
+                  ++c;
+                }
+              }
+            }
+      }
+      if (!pass) cnt[I] = c;
+    }
+    if (!pass) {
+      int64_t acc = 0;
+      for (int64_t I = 0; I < n; ++I) {
+        int64_t c = cnt[I];
+        cnt[I] = acc;
+        acc += c;
+      }
+      cnt[n] = acc;
+      recs = (rec_t*)malloc(sizeof(rec_t) * (size_t)(acc > 0 ? acc : 1));
+    }
+  }
+  int64_t total = cnt[n];
+  qsort(recs, (size_t)total, sizeof(rec_t), rec_cmp);
+  free(cnt);
+  free(cell);
+  free(bkt);
+  free(perm);
+  free(start);
+  *out_recs = recs;
+  return total;
+}
+
+static void write_record(pcc_constraints* out, int64_t i, const rec_t* r, int32_t gid,
+                         int32_t iteration) {
+  if (!out || i >= out->capacity) return;
+  if (out->gidI) out->gidI[i] = r->I;
+  if (out->gidJ) out->gidJ[i] = r->J;
+  if (out->signed_distance) out->signed_distance[i] = r->phi;
+  for (int k = 0; k < 3; ++k) {
+    if (out->normal) out->normal[3 * i + k] = r->n[k];
+    if (out->rPosI) out->rPosI[3 * i + k] = r->rI[k];
+    if (out->rPosJ) out->rPosJ[3 * i + k] = r->rJ[k];
+    if (out->contact_point) out->contact_point[3 * i + k] = r->p[k];
+  }
+  if (out->stressI) out->stressI[i] = r->sI;
+  if (out->stressJ) out->stressJ[i] = r->sJ;
+  if (out->gamma) out->gamma[i] = r->gamma;
+  if (out->gid) out->gid[i] = gid;
+  if (out->iteration) out->iteration[i] = iteration;
+}
+
+int64_t oracle_detect(const pcc_bodies* b, double cutoff, double h, double origin, double warm,
+                      int32_t iteration, int brute, pcc_constraints* out,
+                      int32_t* num_constraints) {
+  body_view bv = {b->pos, b->quat, b->length, b->diameter};
+  rec_t* recs = NULL;
+  int64_t C = detect_records(&bv, b->n, cutoff, h, origin, warm, brute, &recs);
+  for (int64_t i = 0; i < C; ++i) {
+    write_record(out, i, &recs[i], (int32_t)i, iteration);
+    if (num_constraints) {
+      num_constraints[recs[i].I]++;
+      num_constraints[recs[i].J]++;
+    }
+  }
+  free(recs);
+  return C;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Deterministic blocked sum (mirrors the GPU's block/wave reduction tree)   */
+/* ------------------------------------------------------------------------ */
+static double butterfly64(const double* x) {
+  double v[64], w[64];
+  memcpy(v, x, sizeof(v));
+  for (int off = 32; off >= 1; off >>= 1) {
+    for (int l = 0; l < 64; ++l) w[l] = v[l] + v[l ^ off];
+    memcpy(v, w, sizeof(v));
+  }
+  return v[0];
+}
+
+static double block256(const double* x) {
+  double w0 = butterfly64(x), w1 = butterfly64(x + 64), w2 = butterfly64(x + 128),
+         w3 = butterfly64(x + 192);
+  return ((w0 + w1) + w2) + w3;
+}
+
+/* Sum of per-tile partials: thread t accumulates partials t, t+256, ...
+ * sequentially from 0.0, then the same block tree. */
+static double reduce_partials(const double* part, int64_t nb) {
+  double acc[256];
+  for (int t = 0; t < 256; ++t) {
+    double a = 0.0;
+    for (int64_t i = t; i < nb; i += 256) a += part[i];
+    acc[t] = a;
+  }
+  return block256(acc);
+}
+
+double oracle_block_sum(const double* v, int64_t n) {
+  int64_t nb = (n + 255) / 256;
+  double* part = (double*)malloc(sizeof(double) * (size_t)(nb > 0 ? nb : 1));
+  for (int64_t b = 0; b < nb; ++b) {
+    double x[256];
+    for (int t = 0; t < 256; ++t) {
+      int64_t i = b * 256 + t;
+      x[t] = i < n ? v[i] : 0.0;
+    }
+    part[b] = block256(x);
+  }
+  double r = reduce_partials(part, nb);
+  free(part);
+  return r;
+}
+
+/* Three BB sums at once: sum(dg*dg), sum(dg*dp), sum(dp*dp). */
+static void bb_sums(const double* gi, const double* gp, const double* ri, const double* rp,
+                    int64_t C, double* Sgg, double* Sgp, double* Spp) {
+  int64_t nb = (C + 255) / 256;
+  double* part = (double*)malloc(sizeof(double) * 3 * (size_t)(nb > 0 ? nb : 1));
+#pragma omp parallel for schedule(static)
+  for (int64_t b = 0; b < nb; ++b) {
+    double x0[256], x1[256], x2[256];
+    for (int t = 0; t < 256; ++t) {
+      int64_t i = b * 256 + t;
+      if (i < C) {
+        double dgam = gi[i] - gp[i];
+        double dphi = ri[i] - rp[i];
+        x0[t] = dgam * dgam;
+        x1[t] = dgam * dphi;
+        x2[t] = dphi * dphi;
+      } else {
+        x0[t] = 0.0;
+        x1[t] = 0.0;
+        x2[t] = 0.0;
+      }
+    }
+    part[b] = block256(x0);
+    part[nb + b] = block256(x1);
+    part[2 * nb + b] = block256(x2);
+  }
+  *Sgg = reduce_partials(part, nb);
+  *Sgp = reduce_partials(part + nb, nb);
+  *Spp = reduce_partials(part + 2 * nb, nb);
+  free(part);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Hard model                                                                */
+/* ------------------------------------------------------------------------ */
+typedef struct cset_t { /* all_constraints_set (Hard.cpp:101) as SoA */
+  int64_t C, cap;
+  rec_t* r;
+  int32_t* gid;
+  int32_t* iter;
+} cset_t;
+
+static void cset_append(cset_t* S, const rec_t* recs, int64_t m, int32_t iteration) {
+  if (S->C + m > S->cap) {
+    int64_t nc = S->cap ? S->cap : 1024;
+    while (nc < S->C + m) nc *= 2;
+    S->r = (rec_t*)realloc(S->r, sizeof(rec_t) * (size_t)nc);
+    S->gid = (int32_t*)realloc(S->gid, sizeof(int32_t) * (size_t)nc);
+    S->iter = (int32_t*)realloc(S->iter, sizeof(int32_t) * (size_t)nc);
+    S->cap = nc;
+  }
+  for (int64_t i = 0; i < m; ++i) {
+    S->r[S->C + i] = recs[i];
+    S->gid[S->C + i] = (int32_t)i;
+    S->iter[S->C + i] = iteration;
+  }
+  S->C += m;
+}
+
+/* Operators of one ReLCP phase (createMatrices, Hard.cpp:10-47; Physics.cpp). */
+typedef struct phase_t {
+  int64_t N, C;
+  double dt, lambda, tau;
+  double* D;        /* C x 12: body I [n, rI x n], body J [-n, rJ x -n] (Physics.cpp:118-154) */
+  double* sI;       /* L row: (I, sI), (J, sJ) (Physics.cpp:174-190) */
+  double* sJ;
+  int32_t* I;
+  int32_t* J;
+  double* phi;      /* PHI */
+  double* gold;     /* gamma_old (Hard.cpp:146-147) */
+  double* mt;       /* mobility 1/(xi l) (Physics.cpp:203) */
+  double* mr;       /* 12/(xi l^3) (Physics.cpp:204) */
+  double* len;
+  double* ldot_prev;
+  int64_t* rowptr;  /* body -> incidences (ascending constraint index) */
+  int64_t* inc;     /* 2*c + side */
+  /* workspace (Workspace.h:5-20) */
+  double* dgam;
+  double* F;        /* 6N */
+  double* U;        /* 6N */
+  double* sig;      /* N */
+  double* imp;      /* N */
+  double* ldot;     /* N */
+  double* dldot;    /* N */
+} phase_t;
+
+static void phase_free(phase_t* ph) {
+  free(ph->D);
+  free(ph->sI);
+  free(ph->sJ);
+  free(ph->I);
+  free(ph->J);
+  free(ph->phi);
+  free(ph->gold);
+  free(ph->mt);
+  free(ph->mr);
+  free(ph->len);
+  free(ph->ldot_prev);
+  free(ph->rowptr);
+  free(ph->inc);
+  free(ph->dgam);
+  free(ph->F);
+  free(ph->U);
+  free(ph->sig);
+  free(ph->imp);
+  free(ph->ldot);
+  free(ph->dldot);
+}
+
+#define ALLOC(T, n) ((T*)malloc(sizeof(T) * (size_t)((n) > 0 ? (n) : 1)))
+
+static void phase_build(phase_t* ph, const cset_t* S, const pcc_bodies* b, const pcc_params* p) {
+  const int64_t N = b->n, C = S->C;
+  ph->N = N;
+  ph->C = C;
+  ph->dt = p->dt;
+  ph->lambda = p->lambda;
+  ph->tau = p->tau;
+  ph->D = ALLOC(double, 12 * C);
+  ph->sI = ALLOC(double, C);
+  ph->sJ = ALLOC(double, C);
+  ph->I = ALLOC(int32_t, C);
+  ph->J = ALLOC(int32_t, C);
+  ph->phi = ALLOC(double, C);
+  ph->gold = ALLOC(double, C);
+  ph->dgam = ALLOC(double, C);
+  for (int64_t c = 0; c < C; ++c) {
+    const rec_t* r = &S->r[c];
+    double nj[3] = {-r->n[0], -r->n[1], -r->n[2]};
+    double ti[3], tj[3];
+    cross3(r->rI, r->n, ti);
+    cross3(r->rJ, nj, tj);
+    double* Dc = ph->D + 12 * c;
+    Dc[0] = r->n[0];
+    Dc[1] = r->n[1];
+    Dc[2] = r->n[2];
+    Dc[3] = ti[0];
+    Dc[4] = ti[1];
+    Dc[5] = ti[2];
+    Dc[6] = nj[0];
+    Dc[7] = nj[1];
+    Dc[8] = nj[2];
+    Dc[9] = tj[0];
+    Dc[10] = tj[1];
+    Dc[11] = tj[2];
+    ph->sI[c] = r->sI;
+    ph->sJ[c] = r->sJ;
+    ph->I[c] = r->I;
+    ph->J[c] = r->J;
+    ph->phi[c] = r->phi;
+    ph->gold[c] = r->gamma;
+  }
+  ph->mt = ALLOC(double, N);
+  ph->mr = ALLOC(double, N);
+  ph->len = ALLOC(double, N);
+  ph->ldot_prev = ALLOC(double, N);
+  for (int64_t i = 0; i < N; ++i) {
+    double l = b->length[i];
+    ph->mt[i] = 1.0 / (l * p->xi);
+    ph->mr[i] = 12.0 / (l * l * l * p->xi);
+    ph->len[i] = l;
+    ph->ldot_prev[i] = b->ldot[i];
+  }
+  /* incidence CSR, ascending constraint index per body (the row order in which
+   * PETSc's MatMultTranspose accumulates). */
+  ph->rowptr = (int64_t*)calloc((size_t)N + 1, sizeof(int64_t));
+  ph->inc = ALLOC(int64_t, 2 * C);
+  for (int64_t c = 0; c < C; ++c) {
+    ph->rowptr[ph->I[c] + 1]++;
+    ph->rowptr[ph->J[c] + 1]++;
+  }
+  for (int64_t i = 0; i < N; ++i) ph->rowptr[i + 1] += ph->rowptr[i];
+  int64_t* cur = ALLOC(int64_t, N);
+  memcpy(cur, ph->rowptr, sizeof(int64_t) * (size_t)N);
+  for (int64_t c = 0; c < C; ++c) {
+    ph->inc[cur[ph->I[c]]++] = 2 * c;
+    ph->inc[cur[ph->J[c]]++] = 2 * c + 1;
+  }
+  free(cur);
+  ph->F = ALLOC(double, 6 * N);
+  ph->U = ALLOC(double, 6 * N);
+  ph->sig = ALLOC(double, N);
+  ph->imp = ALLOC(double, N);
+  ph->ldot = ALLOC(double, N);
+  ph->dldot = ALLOC(double, N);
+}
+
+/* HardModelGradient::gradient (HardModelGradient.cpp:89-121). */
+static void gradient(phase_t* ph, const double* gam, double* g) {
+  const int64_t N = ph->N, C = ph->C;
+  const double dt = ph->dt;
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < C; ++c) ph->dgam[c] = gam[c] - ph->gold[c]; /* :92 */
+  const double mlam = -ph->lambda;
+  const double itau = 1 / ph->tau;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < N; ++i) {
+    /* F = D^T dgam (:15), sigma = L^T gamma (:56) */
+    double F[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double sig = 0.0;
+    for (int64_t e = ph->rowptr[i]; e < ph->rowptr[i + 1]; ++e) {
+      int64_t c = ph->inc[e] >> 1;
+      int side = (int)(ph->inc[e] & 1);
+      const double* Dv = ph->D + 12 * c + 6 * side;
+      double a = ph->dgam[c];
+      for (int k = 0; k < 6; ++k) F[k] += a * Dv[k];
+      sig += gam[c] * (side ? ph->sJ[c] : ph->sI[c]);
+    }
+    /* U_c = M F (:18), U_total = U_c + U_ext with U_ext = 0 (:21) */
+    for (int k = 0; k < 3; ++k) {
+      ph->F[6 * i + k] = F[k];
+      ph->U[6 * i + k] = (0.0 + ph->mt[i] * F[k]) + 0.0;
+    }
+    for (int k = 3; k < 6; ++k) {
+      ph->F[6 * i + k] = F[k];
+      ph->U[6 * i + k] = (0.0 + ph->mr[i] * F[k]) + 0.0;
+    }
+    /* impedance = exp(-lambda sigma), ldot = (l .* impedance) * (1/tau) (:34-52) */
+    ph->sig[i] = sig;
+    double imp = oracle_exp(sig * mlam);
+    ph->imp[i] = imp;
+    double ld = (ph->len[i] * imp) * itau;
+    ph->ldot[i] = ld;
+    ph->dldot[i] = ld - ph->ldot_prev[i]; /* :108 */
+  }
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < C; ++c) {
+    /* phi_dot = D U_total (:24): 12 products in column order (I < J). */
+    const double* Dc = ph->D + 12 * c;
+    const double* UI = ph->U + 6 * (int64_t)ph->I[c];
+    const double* UJ = ph->U + 6 * (int64_t)ph->J[c];
+    double pd = 0.0;
+    for (int k = 0; k < 6; ++k) pd += Dc[k] * UI[k];
+    for (int k = 0; k < 6; ++k) pd += Dc[6 + k] * UJ[k];
+    /* phi_dot_growth = L ldot_diff (:111) */
+    double pg = 0.0;
+    pg += ph->sI[c] * ph->dldot[ph->I[c]];
+    pg += ph->sJ[c] * ph->dldot[ph->J[c]];
+    /* phi_next = PHI + dt*phi_dot + (-dt)*phi_dot_growth (VecMAXPY, :114-118) */
+    g[c] = ph->phi[c] + (dt * pd + (-dt) * pg);
+  }
+}
+
+/* HardModelGradient::residual (HardModelGradient.cpp:123-158). */
+static double residual(const double* g, const double* gam, int64_t C) {
+  double res = 0.0;
+  for (int64_t c = 0; c < C; ++c) {
+    double v;
+    if (gam[c] > 0) {
+      v = g[c];
+    } else {
+      v = g[c] < 0.0 ? g[c] : 0.0;
+    }
+    double a = fabs(v);
+    res = res < a ? a : res;
+  }
+  return res;
+}
+
+/* BBPGD (BBPGD.cpp:32-127).  On return Gamma holds the lagging iterate (Q3),
+ * g_last the gradient of the last evaluated iterate (the workspace). */
+static pcc_status bbpgd(phase_t* ph, double* Gamma, double tol, int64_t maxit, double* g_last,
+                        int64_t* iters, double* res_out) {
+  const int64_t C = ph->C;
+  double* gp = ALLOC(double, C);
+  double* gi = ALLOC(double, C);
+  double* yp = ALLOC(double, C);
+  double* yi = ALLOC(double, C);
+  memcpy(yp, Gamma, sizeof(double) * (size_t)C);
+  gradient(ph, yp, gp);             /* :49 */
+  double res = residual(gp, yp, C); /* :51 */
+  double alpha = 1.0 / res;         /* :52 */
+  int64_t it = 1;
+  pcc_status st = PCC_OK;
+  int64_t cap = g_iteration_cap;
+  while (it < maxit) {
+    const double malpha = -alpha;
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < C; ++c) { /* :66-67 */
+      double v = yp[c] + malpha * gp[c];
+      yi[c] = v < 0.0 ? 0.0 : v;
+    }
+    gradient(ph, yi, gi); /* :70 */
+    if (it % 10 == 0) {   /* :73-88 */
+      res = residual(gi, yi, C);
+      if (res <= tol) break;
+    }
+    if (cap > 0 && it >= cap) break;
+    double Sgg, Sgp, Spp; /* :91-104 */
+    bb_sums(yi, yp, gi, gp, C, &Sgg, &Sgp, &Spp);
+    double num, den;
+    if (it % 2 == 0) {
+      num = Sgg;
+      den = Sgp;
+    } else {
+      den = Spp;
+      num = Sgp;
+    }
+    if (fabs(den) < 10 * DBL_EPSILON) den += 10 * DBL_EPSILON; /* :106-108 */
+    alpha = num / den;
+    double* t = yp;
+    yp = yi;
+    yi = t;
+    t = gp;
+    gp = gi;
+    gi = t;
+    ++it;
+  }
+  if (it == maxit) st = PCC_E_BBPGD_NOT_CONVERGED; /* :118-121 */
+  memcpy(Gamma, yp, sizeof(double) * (size_t)C); /* :124 (self-move, Q3) */
+  memcpy(g_last, gi, sizeof(double) * (size_t)C);
+  *iters = it;
+  *res_out = res;
+  free(gp);
+  free(gi);
+  free(yp);
+  free(yi);
+  return st;
+}
+
+/* G U (Physics.cpp:220-265 via MatMult, Hard.cpp:161) then
+ * Particle::eulerStepPosition (Particle.cpp:37-57,91-94,110-120). */
+static void integrate_body(pcc_bodies* b, int64_t i, const double* U, double dt) {
+  double* q = b->quat + 4 * i;
+  const double s = q[0], wx = q[1], wy = q[2], wz = q[3];
+  const double xi[4][3] = {{-0.5 * wx, -0.5 * wy, -0.5 * wz},
+                           {0.5 * s, -0.5 * wz, 0.5 * wy},
+                           {0.5 * wz, 0.5 * s, -0.5 * wx},
+                           {-0.5 * wy, 0.5 * wx, 0.5 * s}};
+  double dC[7];
+  for (int k = 0; k < 3; ++k) dC[k] = 0.0 + 1.0 * U[k];
+  for (int r = 0; r < 4; ++r) {
+    double sum = 0.0;
+    sum += xi[r][0] * U[3];
+    sum += xi[r][1] * U[4];
+    sum += xi[r][2] * U[5];
+    dC[3 + r] = sum;
+  }
+  for (int k = 0; k < 3; ++k) b->pos[3 * i + k] += dt * dC[k];
+  for (int k = 0; k < 4; ++k) q[k] += dt * dC[3 + k];
+  double norm = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+  for (int k = 0; k < 4; ++k) q[k] /= norm;
+}
+
+static void set_err(char* err, int64_t len, const char* msg) {
+  if (err && len > 0) {
+    strncpy(err, msg, (size_t)len - 1);
+    err[len - 1] = 0;
+  }
+}
+
+static void write_set(pcc_constraints* out, const cset_t* S) {
+  if (!out) return;
+  for (int64_t c = 0; c < S->C; ++c) write_record(out, c, &S->r[c], S->gid[c], S->iter[c]);
+}
+
+/* solveHardModel (Hard.cpp:91-193). */
+static pcc_status hard_step(const pcc_params* p, pcc_bodies* b, pcc_step_result* result,
+                            pcc_constraints* cons, int64_t* phase_iters, int64_t max_phase,
+                            char* err, int64_t err_len) {
+  const int64_t N = b->n;
+  const double dt = p->dt;
+  body_view bv = {b->pos, b->quat, b->length, b->diameter};
+  cset_t S = {0, 0, NULL, NULL, NULL};
+  int32_t k = 0;
+  int64_t total_it = 0;
+  double res = 0.0, maxov = 0.0;
+  pcc_status st = PCC_OK;
+  for (int64_t i = 0; i < N; ++i) { /* Particle::reset (Particle.h:146-149) */
+    b->ldot[i] = 0.0;
+    if (b->num_constraints) b->num_constraints[i] = 0;
+  }
+  while (k < p->max_recursive_iterations) { /* :113 */
+    double cutoff = k == 0 ? p->cutoff_phase0 : p->cutoff_phaseK; /* :121 */
+    rec_t* recs = NULL;
+    int64_t m = detect_records(&bv, N, cutoff, p->linked_cell_size, p->grid_origin,
+                               p->warm_start_factor, 0, &recs);
+    if (b->num_constraints)
+      for (int64_t i = 0; i < m; ++i) {
+        b->num_constraints[recs[i].I]++;
+        b->num_constraints[recs[i].J]++;
+      }
+    cset_append(&S, recs, m, k); /* :123 */
+    free(recs);
+    double acc = 0.0; /* :125-130 */
+    for (int64_t c = 0; c < S.C; ++c) {
+      double ph = S.r[c].phi;
+      double v = ph < 0 ? -ph : 0;
+      acc = acc < v ? v : acc;
+    }
+    maxov = acc;
+    if (maxov <= p->relcp_tol && k > 0) break; /* :132-134 */
+
+    phase_t ph;
+    memset(&ph, 0, sizeof(ph));
+    phase_build(&ph, &S, b, p); /* :136 */
+    double* Gamma = ALLOC(double, S.C);
+    double* g_last = ALLOC(double, S.C);
+    for (int64_t c = 0; c < S.C; ++c) Gamma[c] = S.r[c].gamma;
+    int64_t its = 0;
+    st = bbpgd(&ph, Gamma, p->bbpgd_tol, p->max_bbpgd_iterations, g_last, &its, &res); /* :153 */
+    if (phase_iters && k < max_phase) phase_iters[k] = its;
+    if (st != PCC_OK) {
+      set_err(err, err_len, "BBPGD did not converge");
+      free(Gamma);
+      free(g_last);
+      phase_free(&ph);
+      total_it += its;
+      break;
+    }
+    total_it += its;
+    /* move (Hard.cpp:160-164, ParticleManager.cpp:29-87) */
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+      integrate_body(b, i, ph.U + 6 * i, dt);
+      for (int kk = 0; kk < 3; ++kk) {
+        if (b->force) b->force[3 * i + kk] = ph.F[6 * i + kk];
+        if (b->torque) b->torque[3 * i + kk] = ph.F[6 * i + 3 + kk];
+        if (b->velocity) b->velocity[3 * i + kk] = ph.U[6 * i + kk];
+        if (b->omega) b->omega[3 * i + kk] = ph.U[6 * i + 3 + kk];
+      }
+      /* setGrowParamsFromSolution (ParticleManager.cpp:89-130) */
+      b->ldot[i] = ph.ldot[i];
+      if (b->impedance) b->impedance[i] = ph.imp[i];
+      if (b->stress) b->stress[i] = ph.sig[i];
+    }
+    /* updateConstraintsFromSolution (Hard.cpp:49-89) */
+    for (int64_t c = 0; c < S.C; ++c) {
+      S.r[c].gamma = Gamma[c];
+      S.r[c].phi = g_last[c];
+    }
+    free(Gamma);
+    free(g_last);
+    phase_free(&ph);
+    ++k;
+  }
+  if (st == PCC_OK) {
+    for (int64_t i = 0; i < N; ++i) { /* grow (ParticleManager.cpp:132-137, Particle.cpp:59-65) */
+      b->length[i] += dt * b->ldot[i];
+      if (b->length[i] > 2 * b->l0[i]) b->length[i] = 2 * b->l0[i];
+    }
+    if (k == p->max_recursive_iterations) { /* :187-190 */
+      st = PCC_E_RELCP_MAX_ITERS;
+      set_err(err, err_len, "Maximum number of constraint iterations reached");
+    }
+  }
+  result->n_constraints = S.C;
+  result->constraint_iterations = k;
+  result->reserved = 0;
+  result->bbpgd_iterations = total_it;
+  result->residual = res;
+  result->max_overlap = maxov;
+  write_set(cons, &S);
+  free(S.r);
+  free(S.gid);
+  free(S.iter);
+  return st;
+}
+
+/* solveSoftPotential (Soft.cpp:21-114). */
+static pcc_status soft_step(const pcc_params* p, pcc_bodies* b, pcc_step_result* result,
+                            pcc_constraints* cons) {
+  const int64_t N = b->n;
+  const double dt = p->dt;
+  body_view bv = {b->pos, b->quat, b->length, b->diameter};
+  for (int64_t i = 0; i < N; ++i) {
+    b->ldot[i] = 0.0;
+    if (b->num_constraints) b->num_constraints[i] = 0;
+  }
+  rec_t* recs = NULL;
+  int64_t C = detect_records(&bv, N, 0.0, p->linked_cell_size, p->grid_origin,
+                             p->warm_start_factor, 0, &recs); /* :35 */
+  if (b->num_constraints)
+    for (int64_t i = 0; i < C; ++i) {
+      b->num_constraints[recs[i].I]++;
+      b->num_constraints[recs[i].J]++;
+    }
+  double maxov = 0.0; /* :37-41 */
+  for (int64_t c = 0; c < C; ++c) {
+    double ph = recs[c].phi;
+    double v = ph < 0 ? -ph : 0;
+    maxov = maxov < v ? v : maxov;
+  }
+  double* F = (double*)calloc((size_t)(6 * N > 0 ? 6 * N : 1), sizeof(double));
+  double* sig = (double*)calloc((size_t)(N > 0 ? N : 1), sizeof(double));
+  for (int64_t c = 0; c < C; ++c) { /* :46-83 */
+    const rec_t* r = &recs[c];
+    double overlap = -r->phi;
+    if (overlap <= 0.00) continue;
+    double R1 = b->diameter[r->I] / 2.0;
+    double R2 = b->diameter[r->J] / 2.0;
+    double Reff = sqrt(R1 * R2 / (R1 + R2));
+    double Fel = Reff * p->kcc * pow(overlap, 1.5);
+    double fi[3], fj[3], ti[3], tj[3];
+    for (int k = 0; k < 3; ++k) fi[k] = r->n[k] * Fel;
+    for (int k = 0; k < 3; ++k) fj[k] = -fi[k];
+    cross3(r->rI, fi, ti);
+    cross3(r->rJ, fj, tj);
+    double* FI = F + 6 * (int64_t)r->I;
+    double* FJ = F + 6 * (int64_t)r->J;
+    for (int k = 0; k < 3; ++k) {
+      FI[k] += fi[k];
+      FI[3 + k] += ti[k];
+    }
+    for (int k = 0; k < 3; ++k) {
+      FJ[k] += fj[k];
+      FJ[3 + k] += tj[k];
+    }
+    sig[r->I] += r->sI * Fel;
+    sig[r->J] += r->sJ * Fel;
+  }
+  const double mlam = -p->lambda;
+  const double itau = 1 / p->tau;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < N; ++i) {
+    double l = b->length[i];
+    double mt = 1.0 / (l * p->xi);
+    double mr = 12.0 / (l * l * l * p->xi);
+    double U[6];
+    for (int k = 0; k < 3; ++k) U[k] = (0.0 + mt * F[6 * i + k]) + 0.0; /* calculate_forces :8-19 */
+    for (int k = 3; k < 6; ++k) U[k] = (0.0 + mr * F[6 * i + k]) + 0.0;
+    integrate_body(b, i, U, dt); /* :99 */
+    for (int k = 0; k < 3; ++k) {
+      if (b->force) b->force[3 * i + k] = F[6 * i + k];
+      if (b->torque) b->torque[3 * i + k] = F[6 * i + 3 + k];
+      if (b->velocity) b->velocity[3 * i + k] = U[k];
+      if (b->omega) b->omega[3 * i + k] = U[3 + k];
+    }
+    double imp = oracle_exp(sig[i] * mlam); /* :102-107 */
+    double ld = (l * imp) * itau;
+    if (b->impedance) b->impedance[i] = imp;
+    if (b->stress) b->stress[i] = sig[i];
+    b->ldot[i] = ld;
+    b->length[i] += dt * ld; /* grow :111 */
+    if (b->length[i] > 2 * b->l0[i]) b->length[i] = 2 * b->l0[i];
+  }
+  for (int64_t c = 0; c < C; ++c) write_record(cons, c, &recs[c], (int32_t)c, 0);
+  result->n_constraints = C;
+  result->constraint_iterations = 0;
+  result->reserved = 0;
+  result->bbpgd_iterations = 0;
+  result->residual = maxov;
+  result->max_overlap = maxov;
+  free(F);
+  free(sig);
+  free(recs);
+  return PCC_OK;
+}
+
+pcc_status oracle_step(const pcc_params* p, pcc_bodies* b, int32_t iter, pcc_step_result* result,
+                       pcc_constraints* cons, int64_t* phase_iters, int64_t max_phase,
+                       char* err, int64_t err_len) {
+  (void)iter;
+  memset(result, 0, sizeof(*result));
+  if (p->temperature != 0.0) {
+    set_err(err, err_len, "temperature > 0 (Brownian input) is not supported");
+    return PCC_E_UNSUPPORTED;
+  }
+  if (p->mode == PCC_MODE_HARD)
+    return hard_step(p, b, result, cons, phase_iters, max_phase, err, err_len);
+  if (p->mode == PCC_MODE_SOFT) return soft_step(p, b, result, cons);
+  set_err(err, err_len, "Invalid mode");
+  return PCC_E_INVALID_MODE;
+}

@@ -1,0 +1,88 @@
+/*
+ * pcc_oracle.h — FP64 CPU restatement of the reference's per-timestep collision
+ * pipeline.  TEST INFRASTRUCTURE ONLY: it is the checker for libpcc (the HIP
+ * library) and the CPU baseline of bench.py.  Nothing in the product links,
+ * loads or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * the reference checkout, code/cpp/src/...).  Deviations from the reference,
+ * each deliberate and shared with the GPU path:
+ *   - pair orientation is canonical (I = lower gid); the reference's depends on
+ *     std::unordered_map iteration order (SpatialGrid.cpp:141);
+ *   - ties of the contact-x sort (CollisionDetector.cpp:94-96, an unstable
+ *     std::sort) are broken by (gidI, gidJ);
+ *   - the broad-phase grid is a hashed cell list with a stable counting sort
+ *     (same cell geometry as SpatialGrid.cpp:29-56, same candidate set);
+ *   - BB dot products (BBPGD.cpp:3-29) are summed in a fixed blocked tree
+ *     (256-wide blocks, 64-lane butterflies) instead of one sequential loop;
+ *   - exp() (HardModelGradient.cpp:40, VecExp) is a fixed Cody-Waite +
+ *     degree-13 Taylor evaluation so CPU and GPU round identically;
+ *   - Brownian input (temperature > 0) is not supported.
+ */
+#ifndef PCC_ORACLE_H_
+#define PCC_ORACLE_H_
+
+#include <stdint.h>
+
+#include "pcc.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Segment-segment closest points, SpherocylinderCell.h:64-261.
+ * out = {s, t, c0[3], c1[3], sqrDistance, distance}. */
+void oracle_dcp(const double P0[3], const double P1[3], const double Q0[3], const double Q1[3],
+                double out[10]);
+/* Quaternion.cpp:7-38 (rotate e_x by q = (w,x,y,z)). */
+void oracle_direction(const double q[4], double t[3]);
+/* ArrayMath.h:219-231. */
+void oracle_normalize(const double a[3], double out[3]);
+double oracle_exp(double x);
+
+/* Broad phase.  cell = floor((x - origin)/h) per axis (SpatialGrid.cpp:29-36);
+ * bucket = hash(cell) & (table_size-1); stable counting sort of bodies by
+ * bucket.  Returns table_size; fills cell (3n), bucket (n), perm (n) and
+ * bucket_start (table_size+1) when non-NULL and large enough
+ * (bucket_start_cap >= table_size+1). */
+int64_t oracle_grid(int64_t n, const double* pos, double h, double origin, int32_t* cell,
+                    uint32_t* bucket, int32_t* perm, int32_t* bucket_start,
+                    int64_t bucket_start_cap);
+int64_t oracle_table_size(int64_t n);
+
+/* One detection batch (CollisionDetector::detectCollisions,
+ * CollisionDetector.cpp:84-111): candidates from the grid (brute == 0) or all
+ * pairs (brute != 0), filtered and built by tryCreateConstraint
+ * (CollisionDetector.cpp:201-311), sorted by (contact x, gidI, gidJ).
+ * Returns the count; fills `out` up to its capacity.  iteration is stored in
+ * every record; num_constraints (if non-NULL) is incremented per contact. */
+int64_t oracle_detect(const pcc_bodies* b, double cutoff, double h, double origin,
+                      double warm_start_factor, int32_t iteration, int brute,
+                      pcc_constraints* out, int32_t* num_constraints);
+
+/* One timestep: ParticleManager::step (ParticleManager.cpp:150-166).
+ * Bodies are updated in place (all (out) fields written).  The constraint set
+ * of the step is written to `cons` (may be NULL) up to its capacity;
+ * result->n_constraints is the full count.  Optional diagnostics: if non-NULL,
+ * phase_iters[k] receives the BBPGD iterations of phase k (up to
+ * max_phase_entries). */
+pcc_status oracle_step(const pcc_params* p, pcc_bodies* b, int32_t iter, pcc_step_result* result,
+                       pcc_constraints* cons, int64_t* phase_iters, int64_t max_phase_entries,
+                       char* err, int64_t err_len);
+
+/* BBPGD iteration budget for timing: run the hard step but stop every BBPGD
+ * solve after `iteration_cap` iterations (no error), so a bounded CPU sample of
+ * a large step can be timed.  0 = no cap. */
+void oracle_set_iteration_cap(int64_t iteration_cap);
+
+/* Deterministic blocked sum used for the BB dot products (exposed for tests). */
+double oracle_block_sum(const double* v, int64_t n);
+
+/* Number of OpenMP threads the oracle uses (1 when built without OpenMP). */
+int oracle_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PCC_ORACLE_H_ */

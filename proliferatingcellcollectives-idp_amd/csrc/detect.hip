@@ -1,0 +1,429 @@
+// detect.hip — contact detection on the GPU.
+//
+// Replaces SpatialGrid::findPotentialCollisions (SpatialGrid.cpp:58-121),
+// CollisionDetector::checkParticlePairs / tryCreateConstraint
+// (CollisionDetector.cpp:144-183, 201-311) and the contact-x sort of
+// detectCollisions (CollisionDetector.cpp:84-111).
+//
+//  1. k_cells      cell = floor((x+1)/h) per axis, bucket = hash(cell) & (T-1),
+//                  histogram of buckets (integer atomics)
+//  2. scan         bucket offsets
+//  3. k_scatter    bodies into buckets, then k_bucket_sort orders each bucket
+//                  by body index: a stable counting sort, identical to the
+//                  CPU restatement's
+//  4. k_detect     one thread per body I visits the 27 neighbour cells and
+//                  tests every J > I of the same or an adjacent cell with the
+//                  reference's filters and exact FP64 segment-segment DCP;
+//                  pass 0 counts, pass 1 writes at the scanned offset
+//  5. k_segsort + radix sort: order = (contact x, I, J)
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include "common.hpp"
+#include "internal.hpp"
+
+namespace pcc {
+
+namespace {
+
+__global__ void k_cells(int64_t N, BodyArrays b, double origin, double h, uint32_t mask,
+                        int32_t* __restrict__ cell, uint32_t* __restrict__ bucket,
+                        int32_t* __restrict__ count) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int32_t cx = cell_coord(b.px[i], origin, h);
+  int32_t cy = cell_coord(b.py[i], origin, h);
+  int32_t cz = cell_coord(b.pz[i], origin, h);
+  cell[3 * i] = cx;
+  cell[3 * i + 1] = cy;
+  cell[3 * i + 2] = cz;
+  uint32_t bk = cell_hash(cx, cy, cz) & mask;
+  bucket[i] = bk;
+  atomicAdd(&count[bk], 1);
+}
+
+__global__ void k_scatter(int64_t N, const uint32_t* __restrict__ bucket,
+                          const int32_t* __restrict__ start, int32_t* __restrict__ cursor,
+                          int32_t* __restrict__ perm) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  uint32_t bk = bucket[i];
+  int32_t pos = start[bk] + atomicAdd(&cursor[bk], 1);
+  perm[pos] = (int32_t)i;
+}
+
+// Orders the members of each segment ascending (insertion sort; segments are
+// short: bodies per hash bucket, or incidences per body).
+__global__ void k_seg_sort(int64_t nseg, const int32_t* __restrict__ start,
+                           int32_t* __restrict__ v) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nseg) return;
+  int32_t a = start[s], e = start[s + 1];
+  for (int32_t i = a + 1; i < e; ++i) {
+    int32_t x = v[i];
+    int32_t j = i - 1;
+    while (j >= a && v[j] > x) {
+      v[j + 1] = v[j];
+      --j;
+    }
+    v[j + 1] = x;
+  }
+}
+
+// Raw contact record layout (16 doubles): phi, n[3], rI[3], rJ[3], p[3], sI, sJ, gamma0
+enum { R_PHI = 0, R_N = 1, R_RI = 4, R_RJ = 7, R_P = 10, R_SI = 13, R_SJ = 14, R_G = 15, R_LEN = 16 };
+
+__global__ void __launch_bounds__(256)
+k_detect(int pass, int64_t N, BodyArrays b, const int32_t* __restrict__ cell,
+         const int32_t* __restrict__ bstart, const int32_t* __restrict__ perm, uint32_t mask,
+         double cutoff, double warm, int32_t* __restrict__ cnt_off, int32_t* __restrict__ outI,
+         int32_t* __restrict__ outJ, double* __restrict__ rec) {
+  int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= N) return;
+  const double xI[3] = {b.px[I], b.py[I], b.pz[I]};
+  const double lenI = b.len[I], dI = b.diam[I];
+  double tI[3];
+  direction(b.qw[I], b.qx[I], b.qy[I], b.qz[I], tI);
+  const double halfI = lenI / 2 - dI / 2;
+  double P0[3], P1[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double hv = tI[k] * halfI;
+    P0[k] = xI[k] - hv;
+    P1[k] = xI[k] + hv;
+  }
+  const int32_t cx = cell[3 * I], cy = cell[3 * I + 1], cz = cell[3 * I + 2];
+  int32_t w = pass ? cnt_off[I] : 0;
+  int32_t c = 0;
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int32_t nx = cx + dx, ny = cy + dy, nz = cz + dz;
+        const uint32_t bk = cell_hash(nx, ny, nz) & mask;
+        const int32_t e1 = bstart[bk + 1];
+        for (int32_t e = bstart[bk]; e < e1; ++e) {
+          const int32_t J = perm[e];
+          if (J <= I) continue;
+          if (cell[3 * (int64_t)J] != nx || cell[3 * (int64_t)J + 1] != ny ||
+              cell[3 * (int64_t)J + 2] != nz)
+            continue;
+          const double xJ[3] = {b.px[J], b.py[J], b.pz[J]};
+          const double lenJ = b.len[J], dJ = b.diam[J];
+          double dd[3] = {xI[0] - xJ[0], xI[1] - xJ[1], xI[2] - xJ[2]};
+          double rough = sqrt(dot3(dd, dd));
+          if (rough > (lenI + lenJ) / 2.0 + cutoff) continue;
+          double tJ[3];
+          direction(b.qw[J], b.qx[J], b.qy[J], b.qz[J], tJ);
+          const double halfJ = lenJ / 2 - dJ / 2;
+          double Q0[3], Q1[3];
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            double hv = tJ[k] * halfJ;
+            Q0[k] = xJ[k] - hv;
+            Q1[k] = xJ[k] + hv;
+          }
+          DcpOut o;
+          dcp(P0, P1, Q0, Q1, o);
+          const double sum_radii = (dI + dJ) / 2.0;
+          const double phi = o.dist - sum_radii;
+          if (phi > cutoff) continue;
+          if (pass) {
+            const int64_t slot = (int64_t)w + c;
+            outI[slot] = (int32_t)I;
+            outJ[slot] = J;
+            double* r = rec + R_LEN * slot;
+            double p[3], d01[3], nrm[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              p[k] = (o.c0[k] + o.c1[k]) * 0.5;
+              d01[k] = o.c0[k] - o.c1[k];
+            }
+            normalize3(d01, nrm);
+            const double mn[3] = {-nrm[0], -nrm[1], -nrm[2]};
+            r[R_PHI] = phi;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              r[R_N + k] = nrm[k];
+              r[R_RI + k] = p[k] - xI[k];
+              r[R_RJ + k] = p[k] - xJ[k];
+              r[R_P + k] = p[k];
+            }
+            r[R_SI] = 0.5 * fabs(dot3(nrm, tI));
+            r[R_SJ] = 0.5 * fabs(dot3(mn, tJ));
+            r[R_G] = phi >= 0.0 ? 0.0 : -phi * warm;
+          }
+          ++c;
+        }
+      }
+  if (!pass) cnt_off[I] = c;
+}
+
+// Per body segment of the batch: order its records by J (insertion sort of
+// record indices).
+__global__ void k_segsort_by_J(int64_t N, const int32_t* __restrict__ off,
+                               const int32_t* __restrict__ J, int32_t* __restrict__ idx) {
+  int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= N) return;
+  int32_t a = off[I], e = off[I + 1];
+  for (int32_t i = a; i < e; ++i) idx[i] = i;
+  for (int32_t i = a + 1; i < e; ++i) {
+    int32_t x = idx[i];
+    int32_t jx = J[x];
+    int32_t j = i - 1;
+    while (j >= a && J[idx[j]] > jx) {
+      idx[j + 1] = idx[j];
+      --j;
+    }
+    idx[j + 1] = x;
+  }
+}
+
+__global__ void k_make_keys(int64_t m, const int32_t* __restrict__ idx,
+                           const double* __restrict__ rec, uint64_t* __restrict__ key) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  key[i] = order_key(rec[R_LEN * (int64_t)idx[i] + R_P]);
+}
+
+// Appends the sorted batch to the constraint set at offset C0 and builds the
+// Jacobian blocks of the new rows (Physics.cpp:118-154).
+__global__ void k_append(int64_t m, int64_t C0, int32_t iteration, const int32_t* __restrict__ idx,
+                         const int32_t* __restrict__ rI_, const int32_t* __restrict__ rJ_,
+                         const double* __restrict__ rec, int32_t* __restrict__ SI,
+                         int32_t* __restrict__ SJ, int32_t* __restrict__ Sgid,
+                         int32_t* __restrict__ Siter, double* __restrict__ Sphi,
+                         double* __restrict__ Sgam, double* __restrict__ Sn,
+                         double* __restrict__ SrI, double* __restrict__ SrJ,
+                         double* __restrict__ Sp, double* __restrict__ SsI,
+                         double* __restrict__ SsJ, double* __restrict__ StI,
+                         double* __restrict__ StJ, float* __restrict__ fn,
+                         float* __restrict__ ftI, float* __restrict__ ftJ,
+                         float* __restrict__ fsI, float* __restrict__ fsJ,
+                         int32_t* __restrict__ nc) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int32_t src = idx[i];
+  const double* r = rec + R_LEN * (int64_t)src;
+  const int64_t c = C0 + i;
+  const int32_t I = rI_[src], J = rJ_[src];
+  SI[c] = I;
+  SJ[c] = J;
+  Sgid[c] = (int32_t)i;
+  Siter[c] = iteration;
+  Sphi[c] = r[R_PHI];
+  Sgam[c] = r[R_G];
+  double n[3], ri[3], rj[3], nj[3], ti[3], tj[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    n[k] = r[R_N + k];
+    ri[k] = r[R_RI + k];
+    rj[k] = r[R_RJ + k];
+    nj[k] = -n[k];
+    Sn[3 * c + k] = n[k];
+    SrI[3 * c + k] = ri[k];
+    SrJ[3 * c + k] = rj[k];
+    Sp[3 * c + k] = r[R_P + k];
+  }
+  cross3(ri, n, ti);
+  cross3(rj, nj, tj);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    StI[3 * c + k] = ti[k];
+    StJ[3 * c + k] = tj[k];
+    fn[3 * c + k] = (float)n[k];
+    ftI[3 * c + k] = (float)ti[k];
+    ftJ[3 * c + k] = (float)tj[k];
+  }
+  SsI[c] = r[R_SI];
+  SsJ[c] = r[R_SJ];
+  fsI[c] = (float)r[R_SI];
+  fsJ[c] = (float)r[R_SJ];
+  atomicAdd(&nc[I], 1);
+  atomicAdd(&nc[J], 1);
+}
+
+inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+template <class T>
+static void excl_scan(pcc_ctx* ctx, const T* in, T* out, int64_t n) {
+  size_t bytes = 0;
+  PCC_HIP(rocprim::exclusive_scan(nullptr, bytes, in, out, T(0), (size_t)n, rocprim::plus<T>(),
+                                  ctx->stream));
+  ctx->sort_tmp.ensure((int64_t)bytes + 16);
+  PCC_HIP(rocprim::exclusive_scan(ctx->sort_tmp.p, bytes, in, out, T(0), (size_t)n,
+                                  rocprim::plus<T>(), ctx->stream));
+}
+
+void exclusive_scan_i32(pcc_ctx* ctx, const int32_t* in, int32_t* out, int64_t n) {
+  excl_scan<int32_t>(ctx, in, out, n);
+}
+
+void seg_sort_i32(pcc_ctx* ctx, int64_t nseg, const int32_t* start, int32_t* v) {
+  if (nseg > 0) k_seg_sort<<<nblk(nseg), 256, 0, ctx->stream>>>(nseg, start, v);
+}
+
+int64_t table_size_for(int64_t n) {
+  int64_t t = 1024;
+  while (t < 2 * n) t <<= 1;
+  return t;
+}
+
+// Builds the hashed cell list of the current body state (stable counting sort).
+void build_grid(pcc_ctx* ctx) {
+  const int64_t N = ctx->N;
+  const int64_t T = table_size_for(N);
+  ctx->table_size = T;
+  ctx->cell.ensure(3 * N + 3);
+  ctx->bucket.ensure(N + 1);
+  ctx->perm.ensure(N + 1);
+  ctx->bstart.ensure(T + 1);
+  ctx->bcursor.ensure(T + 1);
+  hipStream_t s = ctx->stream;
+  BodyArrays b = body_arrays(ctx->body.p, N);
+  PCC_HIP(hipMemsetAsync(ctx->bcursor.p, 0, sizeof(int32_t) * (size_t)(T + 1), s));
+  if (N > 0)
+    k_cells<<<nblk(N), 256, 0, s>>>(N, b, ctx->params.grid_origin, ctx->params.linked_cell_size,
+                                    (uint32_t)(T - 1), ctx->cell.p, ctx->bucket.p, ctx->bcursor.p);
+  excl_scan<int32_t>(ctx, ctx->bcursor.p, ctx->bstart.p, T + 1);
+  PCC_HIP(hipMemsetAsync(ctx->bcursor.p, 0, sizeof(int32_t) * (size_t)(T + 1), s));
+  if (N > 0) {
+    k_scatter<<<nblk(N), 256, 0, s>>>(N, ctx->bucket.p, ctx->bstart.p, ctx->bcursor.p, ctx->perm.p);
+    k_seg_sort<<<nblk(T), 256, 0, s>>>(T, ctx->bstart.p, ctx->perm.p);
+  }
+}
+
+// One detection batch; on return ctx->ridx[0..batch_count) lists the raw
+// records in (contact x, I, J) order.
+void detect_batch(pcc_ctx* ctx, double cutoff, int32_t iteration) {
+  (void)iteration;
+  const int64_t N = ctx->N;
+  hipStream_t s = ctx->stream;
+  KTimer tm(ctx, PCC_K_DETECT, 0.0);
+  build_grid(ctx);
+  BodyArrays b = body_arrays(ctx->body.p, N);
+  const uint32_t mask = (uint32_t)(ctx->table_size - 1);
+  ctx->dcount.ensure(N + 1);
+  PCC_HIP(hipMemsetAsync(ctx->dcount.p, 0, sizeof(int32_t) * (size_t)(N + 1), s));
+  if (N > 0)
+    k_detect<<<nblk(N), 256, 0, s>>>(0, N, b, ctx->cell.p, ctx->bstart.p, ctx->perm.p, mask,
+                                     cutoff, ctx->params.warm_start_factor, ctx->dcount.p,
+                                     nullptr, nullptr, nullptr);
+  // offsets (in place into a second array)
+  ctx->ridx2.ensure(N + 1);
+  excl_scan<int32_t>(ctx, ctx->dcount.p, ctx->ridx2.p, N + 1);
+  int32_t total = 0;
+  PCC_HIP(hipMemcpyAsync(&total, ctx->ridx2.p + N, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  PCC_HIP(hipStreamSynchronize(s));
+  const int64_t m = total;
+  ctx->batch_count = m;
+  // keep offsets in dcount
+  PCC_HIP(hipMemcpyAsync(ctx->dcount.p, ctx->ridx2.p, sizeof(int32_t) * (size_t)(N + 1),
+                         hipMemcpyDeviceToDevice, s));
+  ctx->rI_.ensure(m + 1);
+  ctx->rJ_.ensure(m + 1);
+  ctx->rrec.ensure(R_LEN * (m + 1));
+  ctx->ridx.ensure(m + 1);
+  ctx->ridx2.ensure(m + 1);
+  ctx->rkey.ensure(m + 1);
+  ctx->rkey2.ensure(m + 1);
+  if (m == 0) return;
+  k_detect<<<nblk(N), 256, 0, s>>>(1, N, b, ctx->cell.p, ctx->bstart.p, ctx->perm.p, mask, cutoff,
+                                   ctx->params.warm_start_factor, ctx->dcount.p, ctx->rI_.p,
+                                   ctx->rJ_.p, ctx->rrec.p);
+  k_segsort_by_J<<<nblk(N), 256, 0, s>>>(N, ctx->dcount.p, ctx->rJ_.p, ctx->ridx.p);
+  k_make_keys<<<nblk(m), 256, 0, s>>>(m, ctx->ridx.p, ctx->rrec.p, ctx->rkey.p);
+  size_t bytes = 0;
+  PCC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, ctx->rkey.p, ctx->rkey2.p, ctx->ridx.p,
+                                    ctx->ridx2.p, (size_t)m, 0, 64, s));
+  ctx->sort_tmp.ensure((int64_t)bytes + 16);
+  PCC_HIP(rocprim::radix_sort_pairs(ctx->sort_tmp.p, bytes, ctx->rkey.p, ctx->rkey2.p, ctx->ridx.p,
+                                    ctx->ridx2.p, (size_t)m, 0, 64, s));
+  // sorted order now in ridx2; swap so ridx holds it
+  int32_t* t = ctx->ridx.p;
+  ctx->ridx.p = ctx->ridx2.p;
+  ctx->ridx2.p = t;
+  int64_t tc = ctx->ridx.cap;
+  ctx->ridx.cap = ctx->ridx2.cap;
+  ctx->ridx2.cap = tc;
+}
+
+static void ensure_set_capacity(pcc_ctx* ctx, int64_t need) {
+  ConstraintSet& S = ctx->S;
+  if (need <= S.cap) return;
+  int64_t nc = S.cap > 0 ? S.cap : 4096;
+  while (nc < need) nc *= 2;
+  hipStream_t s = ctx->stream;
+  const int64_t keep = S.C;
+  S.I.grow_keep(nc, keep, s);
+  S.J.grow_keep(nc, keep, s);
+  S.gid.grow_keep(nc, keep, s);
+  S.iter.grow_keep(nc, keep, s);
+  S.phi.grow_keep(nc, keep, s);
+  S.gamma.grow_keep(nc, keep, s);
+  S.sI.grow_keep(nc, keep, s);
+  S.sJ.grow_keep(nc, keep, s);
+  S.n.grow_keep(3 * nc, 3 * keep, s);
+  S.rI.grow_keep(3 * nc, 3 * keep, s);
+  S.rJ.grow_keep(3 * nc, 3 * keep, s);
+  S.p.grow_keep(3 * nc, 3 * keep, s);
+  S.tI.grow_keep(3 * nc, 3 * keep, s);
+  S.tJ.grow_keep(3 * nc, 3 * keep, s);
+  S.f_n.grow_keep(3 * nc, 3 * keep, s);
+  S.f_tI.grow_keep(3 * nc, 3 * keep, s);
+  S.f_tJ.grow_keep(3 * nc, 3 * keep, s);
+  S.f_sI.grow_keep(nc, keep, s);
+  S.f_sJ.grow_keep(nc, keep, s);
+  S.cap = nc;
+}
+
+void append_batch(pcc_ctx* ctx, int32_t iteration) {
+  const int64_t m = ctx->batch_count;
+  ConstraintSet& S = ctx->S;
+  ensure_set_capacity(ctx, S.C + m + 1);
+  if (m > 0) {
+    KTimer tm(ctx, PCC_K_ASSEMBLE, 0.0);
+    k_append<<<nblk(m), 256, 0, ctx->stream>>>(
+        m, S.C, iteration, ctx->ridx.p, ctx->rI_.p, ctx->rJ_.p, ctx->rrec.p, S.I.p, S.J.p,
+        S.gid.p, S.iter.p, S.phi.p, S.gamma.p, S.n.p, S.rI.p, S.rJ.p, S.p.p, S.sI.p, S.sJ.p,
+        S.tI.p, S.tJ.p, S.f_n.p, S.f_tI.p, S.f_tJ.p, S.f_sI.p, S.f_sJ.p, ctx->nc.p);
+  }
+  S.C += m;
+}
+
+// Copies the current (sorted) detection batch to host arrays.
+void batch_to_host(pcc_ctx* ctx, pcc_constraints* out, int32_t iteration) {
+  const int64_t m = ctx->batch_count;
+  if (!out || m == 0) return;
+  std::vector<int32_t> idx(m), I(m), J(m);
+  std::vector<double> rec(R_LEN * m);
+  hipStream_t s = ctx->stream;
+  PCC_HIP(hipMemcpyAsync(idx.data(), ctx->ridx.p, sizeof(int32_t) * m, hipMemcpyDeviceToHost, s));
+  PCC_HIP(hipMemcpyAsync(I.data(), ctx->rI_.p, sizeof(int32_t) * m, hipMemcpyDeviceToHost, s));
+  PCC_HIP(hipMemcpyAsync(J.data(), ctx->rJ_.p, sizeof(int32_t) * m, hipMemcpyDeviceToHost, s));
+  PCC_HIP(hipMemcpyAsync(rec.data(), ctx->rrec.p, sizeof(double) * R_LEN * m,
+                         hipMemcpyDeviceToHost, s));
+  PCC_HIP(hipStreamSynchronize(s));
+  for (int64_t i = 0; i < m && i < out->capacity; ++i) {
+    const int32_t src = idx[i];
+    const double* r = rec.data() + R_LEN * (int64_t)src;
+    if (out->gidI) out->gidI[i] = I[src];
+    if (out->gidJ) out->gidJ[i] = J[src];
+    if (out->signed_distance) out->signed_distance[i] = r[R_PHI];
+    for (int k = 0; k < 3; ++k) {
+      if (out->normal) out->normal[3 * i + k] = r[R_N + k];
+      if (out->rPosI) out->rPosI[3 * i + k] = r[R_RI + k];
+      if (out->rPosJ) out->rPosJ[3 * i + k] = r[R_RJ + k];
+      if (out->contact_point) out->contact_point[3 * i + k] = r[R_P + k];
+    }
+    if (out->stressI) out->stressI[i] = r[R_SI];
+    if (out->stressJ) out->stressJ[i] = r[R_SJ];
+    if (out->gamma) out->gamma[i] = r[R_G];
+    if (out->gid) out->gid[i] = (int32_t)i;
+    if (out->iteration) out->iteration[i] = iteration;
+  }
+}
+
+}  // namespace pcc

@@ -1,0 +1,197 @@
+// internal.hpp — libpcc context, device arrays and kernel launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "pcc.h"
+
+namespace pcc {
+
+struct HipError {
+  hipError_t err;
+  const char* what;
+};
+
+#define PCC_HIP(call)                                                   \
+  do {                                                                  \
+    hipError_t e_ = (call);                                             \
+    if (e_ != hipSuccess) throw ::pcc::HipError{e_, #call};             \
+  } while (0)
+
+struct StatusError {
+  pcc_status status;
+  std::string msg;
+};
+
+// Growable device array (never shrinks; contents are not preserved on growth
+// unless grow_keep is used).
+template <class T>
+struct DArr {
+  T* p = nullptr;
+  int64_t cap = 0;
+  DArr() = default;
+  DArr(const DArr&) = delete;
+  DArr& operator=(const DArr&) = delete;
+  ~DArr() { release(); }
+  void ensure(int64_t n) {
+    if (n <= cap) return;
+    int64_t nc = cap > 0 ? cap : 1024;
+    while (nc < n) nc *= 2;
+    T* q = nullptr;
+    hipError_t e = hipMalloc(&q, sizeof(T) * (size_t)nc);
+    if (e != hipSuccess) throw StatusError{PCC_E_OUT_OF_MEMORY, "hipMalloc failed"};
+    if (p) (void)hipFree(p);
+    p = q;
+    cap = nc;
+  }
+  void grow_keep(int64_t n, int64_t keep, hipStream_t s) {
+    if (n <= cap) return;
+    int64_t nc = cap > 0 ? cap : 1024;
+    while (nc < n) nc *= 2;
+    T* q = nullptr;
+    hipError_t e = hipMalloc(&q, sizeof(T) * (size_t)nc);
+    if (e != hipSuccess) throw StatusError{PCC_E_OUT_OF_MEMORY, "hipMalloc failed"};
+    if (p && keep > 0) PCC_HIP(hipMemcpyAsync(q, p, sizeof(T) * (size_t)keep, hipMemcpyDeviceToDevice, s));
+    if (p) {
+      PCC_HIP(hipStreamSynchronize(s));
+      (void)hipFree(p);
+    }
+    p = q;
+    cap = nc;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+// Body state, FP64 SoA, gid order.
+struct BodyArrays {
+  double *px, *py, *pz, *qw, *qx, *qy, *qz, *len, *l0, *diam, *ldot;
+};
+static constexpr int kBodyFields = 11;
+
+// Device-side solver scalars of one BBPGD solve.
+struct SolverState {
+  double alpha;
+  double res;
+  double maxov;
+  double pad0;
+  int64_t iter;       // BBPGD iteration about to run (0 = initial gradient)
+  int32_t converged;  // residual <= tol reached at iteration `iter`
+  int32_t failed;     // iteration == max
+};
+
+// Constraint set of the current step (all ReLCP phases), FP64 SoA.
+struct ConstraintSet {
+  DArr<int32_t> I, J, gid, iter;
+  DArr<double> phi, gamma, n, rI, rJ, p, sI, sJ;  // n, rI, rJ, p hold 3*C
+  DArr<double> tI, tJ;                           // r x n blocks, 3*C
+  DArr<float> f_n, f_tI, f_tJ, f_sI, f_sJ;       // FP32 copies (mixed)
+  int64_t C = 0;
+  int64_t cap = 0;
+};
+
+// Per-kernel HIP-event accounting on the context stream (enabled on demand).
+struct Timing {
+  struct Pending {
+    hipEvent_t a, b;
+    int kernel;
+    double bytes;
+  };
+  bool enabled = false;
+  int64_t launches[PCC_K_COUNT] = {0};
+  double total_ms[PCC_K_COUNT] = {0};
+  double bytes[PCC_K_COUNT] = {0};
+  std::vector<hipEvent_t> free_events;
+  std::vector<Pending> pending;
+};
+
+}  // namespace pcc
+
+struct pcc_ctx {
+  pcc_params params;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+  int64_t N = 0;
+  bool has_state = false;
+  bool planar = false;
+
+  // body state + snapshot + outputs
+  pcc::DArr<double> body;      // kBodyFields * N
+  pcc::DArr<double> snap;      // kBodyFields * N
+  bool has_snapshot = false;
+  pcc::DArr<double> F, U, sig, imp, ldot_new;  // 6N, 6N, N, N, N
+  pcc::DArr<int32_t> nc;                        // N
+
+  // grid
+  pcc::DArr<int32_t> cell;     // 3N
+  pcc::DArr<uint32_t> bucket;  // N
+  pcc::DArr<int32_t> perm;     // N
+  pcc::DArr<int32_t> bstart;   // T+1
+  pcc::DArr<int32_t> bcursor;  // T
+  int64_t table_size = 0;
+
+  // detection batch
+  pcc::DArr<int32_t> dcount;   // N+1 (per-body counts, then offsets)
+  pcc::DArr<int32_t> rI_, rJ_; // raw pair ids
+  pcc::DArr<double> rrec;      // raw geometry: 16 doubles per contact
+  pcc::DArr<int32_t> ridx, ridx2;
+  pcc::DArr<uint64_t> rkey, rkey2;
+  pcc::DArr<uint8_t> sort_tmp;
+  int64_t batch_count = 0;
+
+  // constraint set and phase operators
+  pcc::ConstraintSet S;
+  pcc::DArr<int32_t> rowptr, inccur, inc;  // N+1, N, 2C
+  pcc::DArr<double> increc;                // 8 doubles per incidence
+  pcc::DArr<float> increc32;               // 8 floats per incidence
+  pcc::DArr<double> gam, grd, gold, phi;   // 2C, 2C, C, C
+  pcc::DArr<float> gam32, grd32, gold32, phi32;
+  pcc::DArr<double> ubuf;                  // 8 per body: U[6], dldot, pad
+  pcc::DArr<float> ubuf32;
+  pcc::DArr<double> mob;                   // 2 per body: 1/(xi l), 12/(xi l^3)
+  pcc::DArr<double> partials;              // 4 * nblocks
+  pcc::DArr<uint32_t> ticket;
+  pcc::DArr<pcc::SolverState> state;
+  pcc::SolverState* state_host = nullptr;  // pinned, 4 slots
+  pcc::DArr<double> scratch_d;             // small reductions
+
+  // soft
+  pcc::DArr<double> hertz;                 // 16 per constraint
+  pcc::DArr<float> hertz32;                // 16 per constraint
+
+  pcc::Timing timing;
+};
+
+namespace pcc {
+
+// launchers (kernels in detect.hip / solve.hip / soft.hip)
+void detect_batch(pcc_ctx* ctx, double cutoff, int32_t iteration);  // fills ctx->ridx sorted
+void append_batch(pcc_ctx* ctx, int32_t iteration);               // appends sorted batch to S
+double max_overlap(pcc_ctx* ctx);
+void build_incidence(pcc_ctx* ctx, bool records);
+pcc_status bbpgd_solve(pcc_ctx* ctx, int64_t* iters, double* res);
+void integrate_hard(pcc_ctx* ctx, int64_t kfinal);
+void grow(pcc_ctx* ctx);
+void soft_forces_and_integrate(pcc_ctx* ctx);
+void reset_step(pcc_ctx* ctx);
+
+BodyArrays body_arrays(double* base, int64_t N);
+
+// timing helpers
+struct KTimer {
+  pcc_ctx* ctx;
+  int k;
+  hipEvent_t a = nullptr, b = nullptr;
+  KTimer(pcc_ctx* c, int kernel, double bytes);
+  ~KTimer();
+};
+
+}  // namespace pcc

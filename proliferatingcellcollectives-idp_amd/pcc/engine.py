@@ -1,0 +1,187 @@
+"""Python mirror of the reference's step interface over libpcc's C ABI.
+
+`ParticleManager.step()` has the meaning of
+`ParticleManager::SolverSolution ParticleManager::step(int, std::function<void()>)`
+(code/cpp/src/simulation/ParticleManager.h:50, ParticleManager.cpp:150-166): it
+advances the colony by one timestep (hard ReLCP/BBPGD or soft Hertz model) and
+returns the SolverSolution fields.  The three solver failures raise
+`PccError` with the reference's messages ("BBPGD did not converge",
+"Maximum number of constraint iterations reached", "Invalid mode").
+
+There is no CPU fallback: if libpcc.so is missing or no GPU is visible, the
+constructor raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi_types as T
+
+_PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG_DIR, "libpcc.so")
+_lib = None
+
+
+def lib():
+    """Loads libpcc.so (the HIP library); raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.pcc_default_params.argtypes = [C.POINTER(T.Params)]
+        L.pcc_create.argtypes = [C.POINTER(T.Params), C.c_int, C.POINTER(vp)]
+        L.pcc_destroy.argtypes = [vp]
+        L.pcc_set_params.argtypes = [vp, C.POINTER(T.Params)]
+        L.pcc_last_error.argtypes = [vp]
+        L.pcc_last_error.restype = C.c_char_p
+        L.pcc_status_string.argtypes = [C.c_int]
+        L.pcc_status_string.restype = C.c_char_p
+        L.pcc_set_state.argtypes = [vp, C.POINTER(T.BodiesC)]
+        L.pcc_get_state.argtypes = [vp, C.POINTER(T.BodiesC)]
+        L.pcc_snapshot.argtypes = [vp]
+        L.pcc_restore.argtypes = [vp]
+        L.pcc_step.argtypes = [vp, C.c_int32, C.POINTER(T.StepResult)]
+        L.pcc_get_constraints.argtypes = [vp, C.POINTER(T.ConstraintsC), C.POINTER(C.c_int64)]
+        L.pcc_detect.argtypes = [vp, C.c_double, C.POINTER(T.ConstraintsC),
+                                 C.POINTER(C.c_int64), C.POINTER(C.c_uint32),
+                                 C.POINTER(C.c_int32)]
+        L.pcc_set_timing.argtypes = [vp, C.c_int]
+        L.pcc_get_timing.argtypes = [vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]
+        L.pcc_kernel_name.argtypes = [C.c_int]
+        L.pcc_kernel_name.restype = C.c_char_p
+        for f in ("pcc_create", "pcc_set_params", "pcc_set_state", "pcc_get_state",
+                  "pcc_snapshot", "pcc_restore", "pcc_step", "pcc_get_constraints",
+                  "pcc_detect", "pcc_set_timing", "pcc_get_timing"):
+            getattr(L, f).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+EXPORTED = ("pcc_default_params", "pcc_create", "pcc_destroy", "pcc_set_params",
+            "pcc_last_error", "pcc_status_string", "pcc_set_state", "pcc_get_state",
+            "pcc_snapshot", "pcc_restore", "pcc_step", "pcc_get_constraints", "pcc_detect",
+            "pcc_set_timing", "pcc_get_timing", "pcc_kernel_name")
+
+
+class PccError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(message)
+        self.status = status
+
+
+def default_params() -> T.Params:
+    p = T.Params()
+    lib().pcc_default_params(C.byref(p))
+    return p
+
+
+class ParticleManager:
+    """Owns one libpcc context (one HIP stream on one device) and the colony's
+    device-resident state."""
+
+    def __init__(self, params: T.Params, device: int = 0):
+        self._ctx = C.c_void_p()
+        self.params = params.copy()
+        st = lib().pcc_create(C.byref(self.params), int(device), C.byref(self._ctx))
+        if st != T.PCC_OK:
+            raise PccError(st, "pcc_create failed: " +
+                           lib().pcc_status_string(st).decode())
+        self.n = 0
+
+    def close(self):
+        if self._ctx:
+            lib().pcc_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != T.PCC_OK:
+            msg = lib().pcc_last_error(self._ctx).decode() or lib().pcc_status_string(st).decode()
+            raise PccError(st, msg)
+
+    def set_params(self, params: T.Params):
+        self.params = params.copy()
+        self._check(lib().pcc_set_params(self._ctx, C.byref(self.params)))
+
+    def set_state(self, bodies: T.Bodies):
+        s = bodies.c_struct()
+        self._check(lib().pcc_set_state(self._ctx, C.byref(s)))
+        self.n = bodies.n
+
+    def get_state(self) -> T.Bodies:
+        n = self.n
+        b = T.Bodies(np.zeros((n, 3)), np.zeros((n, 4)), np.zeros(n), np.zeros(n), np.zeros(n),
+                     np.zeros(n))
+        s = b.c_struct()
+        self._check(lib().pcc_get_state(self._ctx, C.byref(s)))
+        return b
+
+    def snapshot(self):
+        self._check(lib().pcc_snapshot(self._ctx))
+
+    def restore(self):
+        self._check(lib().pcc_restore(self._ctx))
+
+    def step(self, i: int = 0, raise_on_error: bool = True):
+        """One timestep; returns the SolverSolution summary as a dict."""
+        r = T.StepResult()
+        st = lib().pcc_step(self._ctx, int(i), C.byref(r))
+        out = r.as_dict()
+        out["status"] = st
+        if st != T.PCC_OK and raise_on_error:
+            self._check(st)
+        return out
+
+    def constraints(self, count=None) -> dict:
+        cnt = C.c_int64(0)
+        probe = T.Constraints(1)
+        ps = probe.c_struct()
+        ps.capacity = 0
+        self._check(lib().pcc_get_constraints(self._ctx, C.byref(ps), C.byref(cnt)))
+        n = int(cnt.value)
+        cons = T.Constraints(max(n, 1))
+        cs = cons.c_struct()
+        self._check(lib().pcc_get_constraints(self._ctx, C.byref(cs), C.byref(cnt)))
+        cons.count = n
+        return cons.trimmed()
+
+    def detect(self, cutoff: float, capacity=None):
+        """One detection batch at the current state: (constraints dict,
+        bucket per body, stable body permutation by bucket)."""
+        cnt = C.c_int64(0)
+        cap = capacity if capacity is not None else 16 * max(self.n, 1) + 64
+        cons = T.Constraints(cap)
+        cs = cons.c_struct()
+        bucket = np.zeros(max(self.n, 1), np.uint32)
+        perm = np.zeros(max(self.n, 1), np.int32)
+        self._check(lib().pcc_detect(self._ctx, float(cutoff), C.byref(cs), C.byref(cnt),
+                                     bucket.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                     perm.ctypes.data_as(C.POINTER(C.c_int32))))
+        if cnt.value > cap:
+            return self.detect(cutoff, capacity=int(cnt.value))
+        cons.count = int(cnt.value)
+        return cons.trimmed(), bucket[: self.n], perm[: self.n]
+
+    def set_timing(self, enable: bool):
+        self._check(lib().pcc_set_timing(self._ctx, int(bool(enable))))
+
+    def timing(self) -> dict:
+        out = {}
+        for k in range(T.K_COUNT):
+            n = C.c_int64(0)
+            ms = C.c_double(0)
+            by = C.c_double(0)
+            self._check(lib().pcc_get_timing(self._ctx, k, C.byref(n), C.byref(ms), C.byref(by)))
+            out[lib().pcc_kernel_name(k).decode()] = {
+                "launches": int(n.value), "total_ms": float(ms.value),
+                "bytes_per_launch": float(by.value)}
+        return out
