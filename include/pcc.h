@@ -173,6 +173,13 @@ pcc_status pcc_get_constraints(pcc_ctx* ctx, pcc_constraints* out, int64_t* coun
 pcc_status pcc_detect(pcc_ctx* ctx, double cutoff, pcc_constraints* out, int64_t* count,
                       uint32_t* bucket_out, int32_t* perm_out);
 
+/* Per-ReLCP-phase diagnostics of the last step (the data the reference's
+ * SimulationStep/BBPGD loggers print, SimulationLogger.h:7-30): BBPGD
+ * iterations and constraint-set size of each solved phase.  *count receives
+ * the number of solved phases; arrays are filled up to `max` entries. */
+pcc_status pcc_get_phase_stats(pcc_ctx* ctx, int64_t* bbpgd_iterations, int64_t* n_constraints,
+                               int64_t max, int64_t* count);
+
 /* Per-kernel GPU time accounting with HIP events on the context stream.
  * Kernel ids: see pcc_kernel_name(). */
 enum {

@@ -51,6 +51,8 @@ def lib(fast: bool = False):
         L.oracle_step.restype = C.c_int
         L.oracle_set_iteration_cap.argtypes = [C.c_int64]
         L.oracle_threads.restype = C.c_int
+        L.oracle_phase_times.argtypes = [C.POINTER(C.c_double), C.c_int64]
+        L.oracle_phase_times.restype = C.c_int64
         _libs[name] = L
     return _libs[name]
 
@@ -144,6 +146,13 @@ def step(params: Params, bodies: Bodies, iter_=0, capacity=None, fast=False, pha
     cons.count = int(res.n_constraints)
     return st, res.as_dict(), b, cons.trimmed(), ph[: max(res.constraint_iterations, 0) + 1], \
         err.value.decode()
+
+
+def phase_times(fast=False):
+    """Wall time of each ReLCP phase of the last hard step."""
+    out = np.zeros(64)
+    n = lib(fast).oracle_phase_times(out.ctypes.data_as(C.POINTER(C.c_double)), 64)
+    return out[: min(n, 64)].copy()
 
 
 def threads(fast=False):

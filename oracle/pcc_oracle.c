@@ -20,6 +20,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #ifdef _OPENMP
 #include <omp.h>
@@ -35,6 +36,20 @@ int oracle_threads(void) {
 
 static int64_t g_iteration_cap = 0;
 void oracle_set_iteration_cap(int64_t cap) { g_iteration_cap = cap; }
+
+/* wall time of each ReLCP phase of the last hard step (timing diagnostics) */
+#define ORACLE_MAX_PHASES 64
+static double g_phase_time[ORACLE_MAX_PHASES];
+static int64_t g_phase_count = 0;
+static double now_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+int64_t oracle_phase_times(double* out, int64_t max) {
+  for (int64_t i = 0; i < g_phase_count && i < max; ++i) out[i] = g_phase_time[i];
+  return g_phase_count;
+}
 
 /* ------------------------------------------------------------------------ */
 /* Small vector math: ArrayMath.h                                            */
@@ -916,7 +931,9 @@ static pcc_status hard_step(const pcc_params* p, pcc_bodies* b, pcc_step_result*
     b->ldot[i] = 0.0;
     if (b->num_constraints) b->num_constraints[i] = 0;
   }
+  g_phase_count = 0;
   while (k < p->max_recursive_iterations) { /* :113 */
+    const double t_phase = now_s();
     double cutoff = k == 0 ? p->cutoff_phase0 : p->cutoff_phaseK; /* :121 */
     rec_t* recs = NULL;
     int64_t m = detect_records(&bv, N, cutoff, p->linked_cell_size, p->grid_origin,
@@ -978,6 +995,10 @@ static pcc_status hard_step(const pcc_params* p, pcc_bodies* b, pcc_step_result*
     free(Gamma);
     free(g_last);
     phase_free(&ph);
+    if (k < ORACLE_MAX_PHASES) {
+      g_phase_time[k] = now_s() - t_phase;
+      g_phase_count = k + 1;
+    }
     ++k;
   }
   if (st == PCC_OK) {

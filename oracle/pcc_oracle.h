@@ -75,6 +75,10 @@ pcc_status oracle_step(const pcc_params* p, pcc_bodies* b, int32_t iter, pcc_ste
  * a large step can be timed.  0 = no cap. */
 void oracle_set_iteration_cap(int64_t iteration_cap);
 
+/* Wall time (s) of each completed ReLCP phase of the last hard step;
+ * returns the number of phases. */
+int64_t oracle_phase_times(double* out, int64_t max);
+
 /* Deterministic blocked sum used for the BB dot products (exposed for tests). */
 double oracle_block_sum(const double* v, int64_t n);
 

@@ -61,7 +61,7 @@ static void resolve_timing(pcc_ctx* ctx) {
     PCC_HIP(hipEventElapsedTime(&ms, p.a, p.b));
     T.launches[p.kernel] += 1;
     T.total_ms[p.kernel] += ms;
-    T.bytes[p.kernel] = p.bytes;
+    T.bytes[p.kernel] += p.bytes;  // summed; reported per launch
     T.free_events.push_back(p.a);
     T.free_events.push_back(p.b);
   }
@@ -156,6 +156,8 @@ pcc_status hard_step(pcc_ctx* ctx, pcc_step_result* out) {
   int64_t total_it = 0;
   double res = 0.0, maxov = 0.0;
   reset_step(ctx);
+  ctx->phase_iters.clear();
+  ctx->phase_C.clear();
   pcc_status st = PCC_OK;
   while (k < p.max_recursive_iterations) {  // Hard.cpp:113
     const double cutoff = k == 0 ? p.cutoff_phase0 : p.cutoff_phaseK;  // Hard.cpp:121
@@ -167,6 +169,8 @@ pcc_status hard_step(pcc_ctx* ctx, pcc_step_result* out) {
     int64_t its = 0;
     st = bbpgd_solve(ctx, &its, &res);  // Hard.cpp:153
     total_it += its;
+    ctx->phase_iters.push_back(its);
+    ctx->phase_C.push_back(ctx->S.C);
     if (st != PCC_OK) {
       ctx->last_error = "BBPGD did not converge";
       break;
@@ -461,6 +465,18 @@ pcc_status pcc_detect(pcc_ctx* ctx, double cutoff, pcc_constraints* out, int64_t
   });
 }
 
+pcc_status pcc_get_phase_stats(pcc_ctx* ctx, int64_t* bbpgd_iterations, int64_t* n_constraints,
+                               int64_t max, int64_t* count) {
+  if (!ctx) return PCC_E_INVALID_ARGUMENT;
+  const int64_t n = (int64_t)ctx->phase_iters.size();
+  if (count) *count = n;
+  for (int64_t i = 0; i < n && i < max; ++i) {
+    if (bbpgd_iterations) bbpgd_iterations[i] = ctx->phase_iters[i];
+    if (n_constraints) n_constraints[i] = ctx->phase_C[i];
+  }
+  return PCC_OK;
+}
+
 pcc_status pcc_set_timing(pcc_ctx* ctx, int enable) {
   if (!ctx) return PCC_E_INVALID_ARGUMENT;
   return guarded(ctx, [&]() {
@@ -484,7 +500,8 @@ pcc_status pcc_get_timing(pcc_ctx* ctx, int kernel, int64_t* launches, double* t
     const Timing& T = ctx->timing;
     if (launches) *launches = T.launches[kernel];
     if (total_ms) *total_ms = T.total_ms[kernel];
-    if (bytes_per_launch) *bytes_per_launch = T.bytes[kernel];
+    if (bytes_per_launch)
+      *bytes_per_launch = T.launches[kernel] ? T.bytes[kernel] / (double)T.launches[kernel] : 0.0;
     return PCC_OK;
   });
 }

@@ -196,11 +196,8 @@ __global__ void k_append(int64_t m, int64_t C0, int32_t iteration, const int32_t
                          double* __restrict__ Sgam, double* __restrict__ Sn,
                          double* __restrict__ SrI, double* __restrict__ SrJ,
                          double* __restrict__ Sp, double* __restrict__ SsI,
-                         double* __restrict__ SsJ, double* __restrict__ StI,
-                         double* __restrict__ StJ, float* __restrict__ fn,
-                         float* __restrict__ ftI, float* __restrict__ ftJ,
-                         float* __restrict__ fsI, float* __restrict__ fsJ,
-                         int32_t* __restrict__ nc) {
+                         double* __restrict__ SsJ, double* __restrict__ coef64,
+                         float* __restrict__ coef32, int32_t* __restrict__ nc) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
   const int32_t src = idx[i];
@@ -227,18 +224,16 @@ __global__ void k_append(int64_t m, int64_t C0, int32_t iteration, const int32_t
   }
   cross3(ri, n, ti);
   cross3(rj, nj, tj);
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    StI[3 * c + k] = ti[k];
-    StJ[3 * c + k] = tj[k];
-    fn[3 * c + k] = (float)n[k];
-    ftI[3 * c + k] = (float)ti[k];
-    ftJ[3 * c + k] = (float)tj[k];
-  }
   SsI[c] = r[R_SI];
   SsJ[c] = r[R_SJ];
-  fsI[c] = (float)r[R_SI];
-  fsJ[c] = (float)r[R_SJ];
+  // packed operator row: D at I = [n, rI x n], D at J = [-n, rJ x -n], L = (sI, sJ)
+  const double row[12] = {n[0],  n[1],  n[2],     ti[0],    ti[1], ti[2],
+                          tj[0], tj[1], tj[2], r[R_SI], r[R_SJ], 0.0};
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    coef64[12 * c + k] = row[k];
+    coef32[12 * c + k] = (float)row[k];
+  }
   atomicAdd(&nc[I], 1);
   atomicAdd(&nc[J], 1);
 }
@@ -369,13 +364,8 @@ static void ensure_set_capacity(pcc_ctx* ctx, int64_t need) {
   S.rI.grow_keep(3 * nc, 3 * keep, s);
   S.rJ.grow_keep(3 * nc, 3 * keep, s);
   S.p.grow_keep(3 * nc, 3 * keep, s);
-  S.tI.grow_keep(3 * nc, 3 * keep, s);
-  S.tJ.grow_keep(3 * nc, 3 * keep, s);
-  S.f_n.grow_keep(3 * nc, 3 * keep, s);
-  S.f_tI.grow_keep(3 * nc, 3 * keep, s);
-  S.f_tJ.grow_keep(3 * nc, 3 * keep, s);
-  S.f_sI.grow_keep(nc, keep, s);
-  S.f_sJ.grow_keep(nc, keep, s);
+  S.coef64.grow_keep(12 * nc, 12 * keep, s);
+  S.coef32.grow_keep(12 * nc, 12 * keep, s);
   S.cap = nc;
 }
 
@@ -388,7 +378,7 @@ void append_batch(pcc_ctx* ctx, int32_t iteration) {
     k_append<<<nblk(m), 256, 0, ctx->stream>>>(
         m, S.C, iteration, ctx->ridx.p, ctx->rI_.p, ctx->rJ_.p, ctx->rrec.p, S.I.p, S.J.p,
         S.gid.p, S.iter.p, S.phi.p, S.gamma.p, S.n.p, S.rI.p, S.rJ.p, S.p.p, S.sI.p, S.sJ.p,
-        S.tI.p, S.tJ.p, S.f_n.p, S.f_tI.p, S.f_tJ.p, S.f_sI.p, S.f_sJ.p, ctx->nc.p);
+        S.coef64.p, S.coef32.p, ctx->nc.p);
   }
   S.C += m;
 }

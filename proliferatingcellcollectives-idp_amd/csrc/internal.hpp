@@ -44,7 +44,11 @@ struct DArr {
     T* q = nullptr;
     hipError_t e = hipMalloc(&q, sizeof(T) * (size_t)nc);
     if (e != hipSuccess) throw StatusError{PCC_E_OUT_OF_MEMORY, "hipMalloc failed"};
-    if (p) (void)hipFree(p);
+    if (p) {
+      // queued work may still read the old buffer
+      PCC_HIP(hipDeviceSynchronize());
+      (void)hipFree(p);
+    }
     p = q;
     cap = nc;
   }
@@ -91,8 +95,9 @@ struct SolverState {
 struct ConstraintSet {
   DArr<int32_t> I, J, gid, iter;
   DArr<double> phi, gamma, n, rI, rJ, p, sI, sJ;  // n, rI, rJ, p hold 3*C
-  DArr<double> tI, tJ;                           // r x n blocks, 3*C
-  DArr<float> f_n, f_tI, f_tJ, f_sI, f_sJ;       // FP32 copies (mixed)
+  // packed operator row per constraint: n[3], rI x n, rJ x -n, sI, sJ, pad
+  DArr<double> coef64;                           // 12*C
+  DArr<float> coef32;                            // 12*C (mixed precision)
   int64_t C = 0;
   int64_t cap = 0;
 };
@@ -153,9 +158,7 @@ struct pcc_ctx {
   pcc::DArr<double> increc;                // 8 doubles per incidence
   pcc::DArr<float> increc32;               // 8 floats per incidence
   pcc::DArr<double> gam, grd, gold, phi;   // 2C, 2C, C, C
-  pcc::DArr<float> gam32, grd32, gold32, phi32;
   pcc::DArr<double> ubuf;                  // 8 per body: U[6], dldot, pad
-  pcc::DArr<float> ubuf32;
   pcc::DArr<double> mob;                   // 2 per body: 1/(xi l), 12/(xi l^3)
   pcc::DArr<double> partials;              // 4 * nblocks
   pcc::DArr<uint32_t> ticket;
@@ -168,6 +171,7 @@ struct pcc_ctx {
   pcc::DArr<float> hertz32;                // 16 per constraint
 
   pcc::Timing timing;
+  std::vector<int64_t> phase_iters, phase_C;  // per solved ReLCP phase, last step
 };
 
 namespace pcc {

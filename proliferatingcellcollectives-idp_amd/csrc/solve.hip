@@ -41,11 +41,6 @@ __device__ __forceinline__ double proj_step(double yp, double malpha, double gp)
   return v < 0.0 ? 0.0 : v;     // VecPointwiseMax(gamma_i, gamma_i, 0)
 }
 
-__device__ __forceinline__ float proj_step_f(float yp, double malpha, float gp) {
-  double v = (double)yp + malpha * (double)gp;
-  return v < 0.0 ? 0.0f : (float)v;
-}
-
 __global__ void k_inc_count(int64_t C, const int32_t* __restrict__ I, const int32_t* __restrict__ J,
                             int32_t* __restrict__ deg) {
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -66,32 +61,28 @@ __global__ void k_inc_scatter(int64_t C, const int32_t* __restrict__ I,
 
 // Incidence records, body-major: coefficients of the D row on this body's
 // six DOFs ([n, rI x n] for side I, [-n, rJ x -n] for side J), the stress
-// coefficient, and the constraint index.
+// coefficient, and the constraint index, in FP64 and FP32 copies.
 __global__ void k_inc_records(int64_t E, const int32_t* __restrict__ inc,
-                              const double* __restrict__ n, const double* __restrict__ tI,
-                              const double* __restrict__ tJ, const double* __restrict__ sI,
-                              const double* __restrict__ sJ, double* __restrict__ rec,
+                              const double* __restrict__ coef, double* __restrict__ rec,
                               float* __restrict__ rec32) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E) return;
   const int32_t v = inc[e];
   const int64_t c = v >> 1;
   const int side = v & 1;
+  const double* row = coef + 12 * c;
   double co[8];
   if (side == 0) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      co[k] = n[3 * c + k];
-      co[3 + k] = tI[3 * c + k];
-    }
-    co[6] = sI[c];
+    for (int k = 0; k < 6; ++k) co[k] = row[k];
+    co[6] = row[9];
   } else {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      co[k] = -n[3 * c + k];
-      co[3 + k] = tJ[3 * c + k];
+      co[k] = -row[k];
+      co[3 + k] = row[6 + k];
     }
-    co[6] = sJ[c];
+    co[6] = row[10];
   }
   co[7] = __longlong_as_double((long long)c);
   double* r = rec + 8 * e;
@@ -105,18 +96,13 @@ __global__ void k_inc_records(int64_t E, const int32_t* __restrict__ inc,
 
 __global__ void k_phase_init(int64_t C, const double* __restrict__ Sgam,
                              const double* __restrict__ Sphi, double* __restrict__ gam,
-                             double* __restrict__ gold, double* __restrict__ phi,
-                             float* __restrict__ gam32, float* __restrict__ gold32,
-                             float* __restrict__ phi32) {
+                             double* __restrict__ gold, double* __restrict__ phi) {
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= C) return;
   double g = Sgam[c], p = Sphi[c];
   gam[c] = g;
   gold[c] = g;
   phi[c] = p;
-  gam32[c] = (float)g;
-  gold32[c] = (float)g;
-  phi32[c] = (float)p;
 }
 
 __global__ void k_mobility(int64_t N, const double* __restrict__ len, double xi,
@@ -129,29 +115,65 @@ __global__ void k_mobility(int64_t N, const double* __restrict__ len, double xi,
 }
 
 // ---------------------------------------------------------------------------
-// FP64 BBPGD kernels
+// BBPGD kernels, templated on the storage type CT of the operator
+// coefficients (D blocks, L entries, incidence records):
+//   CT = double  plain FP64: bit-identical to the CPU restatement;
+//   CT = float   mixed: the operator, the bulk of the bytes, is stored in
+//                FP32; impulses, gaps, body velocities, every accumulation,
+//                the BB dot products and the residual stay FP64.  (FP32
+//                accumulation cannot reach a 1e-6 residual once impulses
+//                reach ~1e5, as they do on random dense packings.)
 // ---------------------------------------------------------------------------
 struct BodyArgs {
   int64_t N, C;
   const int32_t* rowptr;
-  const double* rec;
-  const float* rec32;
+  const void* rec;     // 8 CT per incidence: coef[6], s, constraint index bits
   const double* gam;   // 2C
   const double* grd;   // 2C
   const double* gold;  // C
-  const float* gam32;
-  const float* grd32;
-  const float* gold32;
   const double* mob;
   const double* len;
   const double* ldot_prev;
   double mlam, itau;
   const SolverState* st;
-  double* ubuf;  // 8 per body
-  float* ubuf32;
+  double* ubuf;  // 8 per body: U[6], ldot - ldot_prev, pad
 };
 
-__global__ void __launch_bounds__(256) k_body64(BodyArgs a) {
+template <class CT>
+__device__ __forceinline__ void load_rec(const void* base, int64_t e, double co[7], int64_t& c);
+
+template <>
+__device__ __forceinline__ void load_rec<double>(const void* base, int64_t e, double co[7],
+                                                 int64_t& c) {
+  const double2* r = reinterpret_cast<const double2*>(base) + 4 * e;
+  const double2 a = r[0], b = r[1], d = r[2], f = r[3];
+  co[0] = a.x;
+  co[1] = a.y;
+  co[2] = b.x;
+  co[3] = b.y;
+  co[4] = d.x;
+  co[5] = d.y;
+  co[6] = f.x;
+  c = (int64_t)__double_as_longlong(f.y);
+}
+
+template <>
+__device__ __forceinline__ void load_rec<float>(const void* base, int64_t e, double co[7],
+                                                int64_t& c) {
+  const float4* r = reinterpret_cast<const float4*>(base) + 2 * e;
+  const float4 a = r[0], b = r[1];
+  co[0] = a.x;
+  co[1] = a.y;
+  co[2] = a.z;
+  co[3] = a.w;
+  co[4] = b.x;
+  co[5] = b.y;
+  co[6] = b.z;
+  c = (int64_t)__float_as_int(b.w);
+}
+
+template <class CT>
+__global__ void __launch_bounds__(256) k_body(BodyArgs a) {
   const SolverState* st = a.st;
   if (st->converged | st->failed) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -164,16 +186,17 @@ __global__ void __launch_bounds__(256) k_body64(BodyArgs a) {
   double sig = 0.0;
   const int32_t e1 = a.rowptr[i + 1];
   for (int32_t e = a.rowptr[i]; e < e1; ++e) {
-    const double* r = a.rec + 8 * (int64_t)e;
-    const int64_t c = (int64_t)__double_as_longlong(r[7]);
+    double co[7];
+    int64_t c;
+    load_rec<CT>(a.rec, e, co, c);
     const double gc = (k == 0) ? a.gam[c] : proj_step(yp[c], malpha, gp[c]);
     const double dg = gc - a.gold[c];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) F[q] += dg * r[q];
-    sig += gc * r[6];
+    for (int q = 0; q < 6; ++q) F[q] += dg * co[q];
+    sig += gc * co[6];
   }
   const double mt = a.mob[2 * i], mr = a.mob[2 * i + 1];
-  double* u = a.ubuf + 8 * i;
+  double u[8];
 #pragma unroll
   for (int q = 0; q < 3; ++q) u[q] = (0.0 + mt * F[q]) + 0.0;
 #pragma unroll
@@ -181,36 +204,59 @@ __global__ void __launch_bounds__(256) k_body64(BodyArgs a) {
   const double imp = det_exp(sig * a.mlam);
   const double ld = (a.len[i] * imp) * a.itau;
   u[6] = ld - a.ldot_prev[i];
+  u[7] = 0.0;
+  double2* o = reinterpret_cast<double2*>(a.ubuf + 8 * i);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) o[q] = make_double2(u[2 * q], u[2 * q + 1]);
 }
 
 struct ConstrArgs {
   int64_t C, nb;
   const int32_t* I;
   const int32_t* J;
-  const double* n;
-  const double* tI;
-  const double* tJ;
-  const double* sI;
-  const double* sJ;
+  const void* coef;  // 12 CT per constraint: n[3], tI[3], tJ[3], sI, sJ, pad
   const double* phi;
-  const float* fn;
-  const float* ftI;
-  const float* ftJ;
-  const float* fsI;
-  const float* fsJ;
-  const float* phi32;
   double* gam;
   double* grd;
-  float* gam32;
-  float* grd32;
   const double* ubuf;
-  const float* ubuf32;
   double dt, tol;
   int64_t maxit;
   SolverState* st;
   double* partials;
   uint32_t* ticket;
 };
+
+template <class CT>
+__device__ __forceinline__ void load_coef(const void* base, int64_t c, double co[11]);
+
+template <>
+__device__ __forceinline__ void load_coef<double>(const void* base, int64_t c, double co[11]) {
+  const double2* r = reinterpret_cast<const double2*>(base) + 6 * c;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    const double2 v = r[q];
+    co[2 * q] = v.x;
+    co[2 * q + 1] = v.y;
+  }
+  co[10] = r[5].x;
+}
+
+template <>
+__device__ __forceinline__ void load_coef<float>(const void* base, int64_t c, double co[11]) {
+  const float4* r = reinterpret_cast<const float4*>(base) + 3 * c;
+  const float4 a = r[0], b = r[1], d = r[2];
+  co[0] = a.x;
+  co[1] = a.y;
+  co[2] = a.z;
+  co[3] = a.w;
+  co[4] = b.x;
+  co[5] = b.y;
+  co[6] = b.z;
+  co[7] = b.w;
+  co[8] = d.x;
+  co[9] = d.y;
+  co[10] = d.z;
+}
 
 // Final reduction of the per-block partials by the last block, and the BB
 // step-size / convergence update (BBPGD.cpp:49-116).
@@ -274,7 +320,8 @@ __device__ void finish_iteration(const ConstrArgs& a, int64_t k, double s0, doub
   if (k + 1 == a.maxit) st->failed = 1;
 }
 
-__global__ void __launch_bounds__(256) k_constr64(ConstrArgs a) {
+template <class CT>
+__global__ void __launch_bounds__(256) k_constr(ConstrArgs a) {
   __shared__ double lds[4];
   const SolverState* st = a.st;
   if (st->converged | st->failed) return;
@@ -286,25 +333,28 @@ __global__ void __launch_bounds__(256) k_constr64(ConstrArgs a) {
     const double* yp = a.gam + ((k - 1) & 1) * a.C;
     const double* gp = a.grd + ((k - 1) & 1) * a.C;
     const double gc = (k == 0) ? a.gam[c] : proj_step(yp[c], malpha, gp[c]);
-    const double* UI = a.ubuf + 8 * (int64_t)a.I[c];
-    const double* UJ = a.ubuf + 8 * (int64_t)a.J[c];
-    const double n0 = a.n[3 * c], n1 = a.n[3 * c + 1], n2 = a.n[3 * c + 2];
+    double co[11];
+    load_coef<CT>(a.coef, c, co);
+    const double2* UI = reinterpret_cast<const double2*>(a.ubuf + 8 * (int64_t)a.I[c]);
+    const double2* UJ = reinterpret_cast<const double2*>(a.ubuf + 8 * (int64_t)a.J[c]);
+    const double2 i0 = UI[0], i1 = UI[1], i2 = UI[2], i3 = UI[3];
+    const double2 j0 = UJ[0], j1 = UJ[1], j2 = UJ[2], j3 = UJ[3];
     double pd = 0.0;  // D U_total, columns of body I (lower gid) first
-    pd += n0 * UI[0];
-    pd += n1 * UI[1];
-    pd += n2 * UI[2];
-    pd += a.tI[3 * c] * UI[3];
-    pd += a.tI[3 * c + 1] * UI[4];
-    pd += a.tI[3 * c + 2] * UI[5];
-    pd += (-n0) * UJ[0];
-    pd += (-n1) * UJ[1];
-    pd += (-n2) * UJ[2];
-    pd += a.tJ[3 * c] * UJ[3];
-    pd += a.tJ[3 * c + 1] * UJ[4];
-    pd += a.tJ[3 * c + 2] * UJ[5];
+    pd += co[0] * i0.x;
+    pd += co[1] * i0.y;
+    pd += co[2] * i1.x;
+    pd += co[3] * i1.y;
+    pd += co[4] * i2.x;
+    pd += co[5] * i2.y;
+    pd += (-co[0]) * j0.x;
+    pd += (-co[1]) * j0.y;
+    pd += (-co[2]) * j1.x;
+    pd += co[6] * j1.y;
+    pd += co[7] * j2.x;
+    pd += co[8] * j2.y;
     double pg = 0.0;  // L ldot_diff
-    pg += a.sI[c] * UI[6];
-    pg += a.sJ[c] * UJ[6];
+    pg += co[9] * i3.x;
+    pg += co[10] * j3.x;
     const double g = a.phi[c] + (a.dt * pd + (-a.dt) * pg);
     if (k > 0) a.gam[(k & 1) * a.C + c] = gc;
     a.grd[(k & 1) * a.C + c] = g;
@@ -326,107 +376,29 @@ __global__ void __launch_bounds__(256) k_constr64(ConstrArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Mixed precision: FP32 storage and applies of D, D^T, L, L^T, gamma, g and
-// the body velocities; FP64 for the projection step, the BB dot products and
-// the residual.  The exp of the growth law runs in FP32 (HIP expf).
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_body32(BodyArgs a) {
-  const SolverState* st = a.st;
-  if (st->converged | st->failed) return;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.N) return;
-  const int64_t k = st->iter;
-  const double malpha = -st->alpha;
-  const float* yp = a.gam32 + ((k - 1) & 1) * a.C;
-  const float* gp = a.grd32 + ((k - 1) & 1) * a.C;
-  float F[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  float sig = 0.f;
-  const int32_t e1 = a.rowptr[i + 1];
-  for (int32_t e = a.rowptr[i]; e < e1; ++e) {
-    const float4* r = reinterpret_cast<const float4*>(a.rec32 + 8 * (int64_t)e);
-    const float4 r0 = r[0], r1 = r[1];
-    const int64_t c = (int64_t)__float_as_int(r1.w);
-    const float gc = (k == 0) ? a.gam32[c] : proj_step_f(yp[c], malpha, gp[c]);
-    const float dg = gc - a.gold32[c];
-    F[0] += dg * r0.x;
-    F[1] += dg * r0.y;
-    F[2] += dg * r0.z;
-    F[3] += dg * r0.w;
-    F[4] += dg * r1.x;
-    F[5] += dg * r1.y;
-    sig += gc * r1.z;
-  }
-  const float mt = (float)a.mob[2 * i], mr = (float)a.mob[2 * i + 1];
-  const float imp = expf(sig * (float)a.mlam);
-  const float ld = ((float)a.len[i] * imp) * (float)a.itau;
-  float4* u = reinterpret_cast<float4*>(a.ubuf32 + 8 * i);
-  u[0] = make_float4(mt * F[0], mt * F[1], mt * F[2], mr * F[3]);
-  u[1] = make_float4(mr * F[4], mr * F[5], ld - (float)a.ldot_prev[i], 0.f);
-}
-
-__global__ void __launch_bounds__(256) k_constr32(ConstrArgs a) {
-  __shared__ double lds[4];
-  const SolverState* st = a.st;
-  if (st->converged | st->failed) return;
-  const int64_t k = st->iter;
-  const double malpha = -st->alpha;
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double v0 = 0.0, v1 = 0.0, v2 = 0.0, vr = 0.0;
-  if (c < a.C) {
-    const float* yp = a.gam32 + ((k - 1) & 1) * a.C;
-    const float* gp = a.grd32 + ((k - 1) & 1) * a.C;
-    const float gc = (k == 0) ? a.gam32[c] : proj_step_f(yp[c], malpha, gp[c]);
-    const float4* UI = reinterpret_cast<const float4*>(a.ubuf32 + 8 * (int64_t)a.I[c]);
-    const float4* UJ = reinterpret_cast<const float4*>(a.ubuf32 + 8 * (int64_t)a.J[c]);
-    const float4 ui0 = UI[0], ui1 = UI[1], uj0 = UJ[0], uj1 = UJ[1];
-    const float n0 = a.fn[3 * c], n1 = a.fn[3 * c + 1], n2 = a.fn[3 * c + 2];
-    float pd = n0 * (ui0.x - uj0.x) + n1 * (ui0.y - uj0.y) + n2 * (ui0.z - uj0.z);
-    pd += a.ftI[3 * c] * ui0.w + a.ftI[3 * c + 1] * ui1.x + a.ftI[3 * c + 2] * ui1.y;
-    pd += a.ftJ[3 * c] * uj0.w + a.ftJ[3 * c + 1] * uj1.x + a.ftJ[3 * c + 2] * uj1.y;
-    const float pg = a.fsI[c] * ui1.z + a.fsJ[c] * uj1.z;
-    const float g = a.phi32[c] + ((float)a.dt * pd - (float)a.dt * pg);
-    if (k > 0) a.gam32[(k & 1) * a.C + c] = gc;
-    a.grd32[(k & 1) * a.C + c] = g;
-    if (k > 0) {
-      const double dgam = (double)gc - (double)yp[c];
-      const double dphi = (double)g - (double)gp[c];
-      v0 = dgam * dgam;
-      v1 = dgam * dphi;
-      v2 = dphi * dphi;
-    }
-    const float pj = gc > 0 ? g : (g < 0.0f ? g : 0.0f);
-    vr = fabs((double)pj);
-  }
-  const double s0 = block_sum(v0, lds);
-  const double s1 = block_sum(v1, lds);
-  const double s2 = block_sum(v2, lds);
-  const double r = block_max(vr, lds);
-  finish_iteration(a, k, s0, s1, s2, r, lds);
-}
-
-// ---------------------------------------------------------------------------
 // Finalisation: the workspace of the last gradient evaluation, gamma_i =
-// gam[kf & 1] (HardModelGradient.cpp:89-121), in FP64.  In mixed precision
-// the FP32 iterate is widened and the body sums are redone in FP64.
+// gam[kf & 1] (HardModelGradient.cpp:89-121), with the same operator
+// coefficients the solve used.
 // ---------------------------------------------------------------------------
-__global__ void k_body_final(BodyArgs a, int64_t kf, int mixed, double* __restrict__ F_out,
+template <class CT>
+__global__ void k_body_final(BodyArgs a, int64_t kf, double* __restrict__ F_out,
                              double* __restrict__ U_out, double* __restrict__ sig_out,
                              double* __restrict__ imp_out, double* __restrict__ ld_out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.N) return;
   const double* gi = a.gam + (kf & 1) * a.C;
-  const float* gi32 = a.gam32 + (kf & 1) * a.C;
   double F[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double sig = 0.0;
   const int32_t e1 = a.rowptr[i + 1];
   for (int32_t e = a.rowptr[i]; e < e1; ++e) {
-    const double* r = a.rec + 8 * (int64_t)e;
-    const int64_t c = (int64_t)__double_as_longlong(r[7]);
-    const double gc = mixed ? (double)gi32[c] : gi[c];
-    const double dg = gc - (mixed ? (double)a.gold32[c] : a.gold[c]);
+    double co[7];
+    int64_t c;
+    load_rec<CT>(a.rec, e, co, c);
+    const double gc = gi[c];
+    const double dg = gc - a.gold[c];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) F[q] += dg * r[q];
-    sig += gc * r[6];
+    for (int q = 0; q < 6; ++q) F[q] += dg * co[q];
+    sig += gc * co[6];
   }
   const double mt = a.mob[2 * i], mr = a.mob[2 * i + 1];
 #pragma unroll
@@ -485,20 +457,13 @@ __global__ void k_integrate(int64_t N, BodyArrays b, double dt, const double* __
 
 // updateConstraintsFromSolution (Hard.cpp:49-89): gamma <- lagging iterate,
 // phi <- linearised gap of the last evaluated iterate.
-__global__ void k_update_constraints(int64_t C, int mixed, const double* __restrict__ gam_prev,
+__global__ void k_update_constraints(int64_t C, const double* __restrict__ gam_prev,
                                      const double* __restrict__ g_last,
-                                     const float* __restrict__ gam_prev32,
-                                     const float* __restrict__ g_last32,
                                      double* __restrict__ Sgam, double* __restrict__ Sphi) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= C) return;
-  if (mixed) {
-    Sgam[c] = (double)gam_prev32[c];
-    Sphi[c] = (double)g_last32[c];
-  } else {
-    Sgam[c] = gam_prev[c];
-    Sphi[c] = g_last[c];
-  }
+  Sgam[c] = gam_prev[c];
+  Sphi[c] = g_last[c];
 }
 
 // max over the set of max(0, -phi) (Hard.cpp:125-130); max is order-free.
@@ -562,8 +527,7 @@ void build_incidence(pcc_ctx* ctx, bool records) {
                                           ctx->inc.p);
     seg_sort_i32(ctx, N, ctx->rowptr.p, ctx->inc.p);
     if (records)
-      k_inc_records<<<nblk(2 * C), 256, 0, s>>>(2 * C, ctx->inc.p, ctx->S.n.p, ctx->S.tI.p,
-                                              ctx->S.tJ.p, ctx->S.sI.p, ctx->S.sJ.p,
+      k_inc_records<<<nblk(2 * C), 256, 0, s>>>(2 * C, ctx->inc.p, ctx->S.coef64.p,
                                               ctx->increc.p, ctx->increc32.p);
   }
 }
@@ -571,18 +535,15 @@ void build_incidence(pcc_ctx* ctx, bool records) {
 static BodyArgs body_args(pcc_ctx* ctx) {
   BodyArgs a;
   const int64_t N = ctx->N;
+  const bool mixed = ctx->params.precision == PCC_PREC_MIXED;
   BodyArrays b = body_arrays(ctx->body.p, N);
   a.N = N;
   a.C = ctx->S.C;
   a.rowptr = ctx->rowptr.p;
-  a.rec = ctx->increc.p;
-  a.rec32 = ctx->increc32.p;
+  a.rec = mixed ? (const void*)ctx->increc32.p : (const void*)ctx->increc.p;
   a.gam = ctx->gam.p;
   a.grd = ctx->grd.p;
   a.gold = ctx->gold.p;
-  a.gam32 = ctx->gam32.p;
-  a.grd32 = ctx->grd32.p;
-  a.gold32 = ctx->gold32.p;
   a.mob = ctx->mob.p;
   a.len = b.len;
   a.ldot_prev = b.ldot;
@@ -590,35 +551,22 @@ static BodyArgs body_args(pcc_ctx* ctx) {
   a.itau = 1 / ctx->params.tau;
   a.st = ctx->state.p;
   a.ubuf = ctx->ubuf.p;
-  a.ubuf32 = ctx->ubuf32.p;
   return a;
 }
 
 static ConstrArgs constr_args(pcc_ctx* ctx, int64_t nb) {
   ConstrArgs a;
   ConstraintSet& S = ctx->S;
+  const bool mixed = ctx->params.precision == PCC_PREC_MIXED;
   a.C = S.C;
   a.nb = nb;
   a.I = S.I.p;
   a.J = S.J.p;
-  a.n = S.n.p;
-  a.tI = S.tI.p;
-  a.tJ = S.tJ.p;
-  a.sI = S.sI.p;
-  a.sJ = S.sJ.p;
+  a.coef = mixed ? (const void*)S.coef32.p : (const void*)S.coef64.p;
   a.phi = ctx->phi.p;
-  a.fn = S.f_n.p;
-  a.ftI = S.f_tI.p;
-  a.ftJ = S.f_tJ.p;
-  a.fsI = S.f_sI.p;
-  a.fsJ = S.f_sJ.p;
-  a.phi32 = ctx->phi32.p;
   a.gam = ctx->gam.p;
   a.grd = ctx->grd.p;
-  a.gam32 = ctx->gam32.p;
-  a.grd32 = ctx->grd32.p;
   a.ubuf = ctx->ubuf.p;
-  a.ubuf32 = ctx->ubuf32.p;
   a.dt = ctx->params.dt;
   a.tol = ctx->params.bbpgd_tol;
   a.maxit = ctx->params.max_bbpgd_iterations;
@@ -628,19 +576,20 @@ static ConstrArgs constr_args(pcc_ctx* ctx, int64_t nb) {
   return a;
 }
 
-// Bytes one launch of each BBPGD kernel must move at minimum (each array
-// touched once per launch), used for the achieved-bandwidth figure.
-static double body_bytes(int64_t N, int64_t C, bool mixed) {
-  const double sv = mixed ? 4 : 8;
-  // per incidence: record (8 values), gamma_p, g_p, gamma_old gathers
-  // per body: rowptr, mobility (2), length, ldot_prev, U+dldot out (8)
-  return 2.0 * C * (8 * sv + 3 * sv) + N * (4 + 16 + 16 + 8 * sv);
+// Algorithmic bytes of one launch of each BBPGD kernel: every array the
+// kernel must touch, counted once (gathers of per-constraint iterates count
+// once per incidence, body velocity records once per body).
+double body_bytes(int64_t N, int64_t C, bool mixed) {
+  const double sc = mixed ? 4 : 8;
+  // per incidence: record (8 coefficient slots), gamma_p, g_p, gamma_old
+  // per body: rowptr, mobility (2), length, ldot_prev, U + dldot out (8)
+  return 2.0 * C * (8 * sc + 3 * 8) + N * (4 + 16 + 16 + 64);
 }
-static double constr_bytes(int64_t N, int64_t C, bool mixed) {
-  const double sv = mixed ? 4 : 8;
-  // per constraint: I,J; n,tI,tJ (9); sI,sJ; phi; gamma_p, g_p in; gamma, g out
-  // per body: the U+dldot record read once (8 values)
-  return C * (8 + 9 * sv + 2 * sv + sv + 4 * sv) + N * 8 * sv;
+double constr_bytes(int64_t N, int64_t C, bool mixed) {
+  const double sc = mixed ? 4 : 8;
+  // per constraint: I, J; packed row (12 slots); phi; gamma_p, g_p in;
+  // gamma, g out.  Per body: the U + dldot record read once.
+  return C * (8 + 12 * sc + 8 + 4 * 8) + N * 64.0;
 }
 
 // Runs BBPGD on the current phase operators.  Returns its status; *iters and
@@ -654,18 +603,13 @@ pcc_status bbpgd_solve(pcc_ctx* ctx, int64_t* iters, double* res) {
   ctx->grd.ensure(2 * C + 2);
   ctx->gold.ensure(C + 1);
   ctx->phi.ensure(C + 1);
-  ctx->gam32.ensure(2 * C + 2);
-  ctx->grd32.ensure(2 * C + 2);
-  ctx->gold32.ensure(C + 1);
-  ctx->phi32.ensure(C + 1);
   ctx->ubuf.ensure(8 * N + 8);
-  ctx->ubuf32.ensure(8 * N + 8);
   ctx->mob.ensure(2 * N + 2);
   ctx->partials.ensure(4 * nb + 4);
   BodyArrays b = body_arrays(ctx->body.p, N);
   if (C > 0)
     k_phase_init<<<nblk(C), 256, 0, s>>>(C, ctx->S.gamma.p, ctx->S.phi.p, ctx->gam.p, ctx->gold.p,
-                                         ctx->phi.p, ctx->gam32.p, ctx->gold32.p, ctx->phi32.p);
+                                         ctx->phi.p);
   if (N > 0) k_mobility<<<nblk(N), 256, 0, s>>>(N, b.len, ctx->params.xi, ctx->mob.p);
   SolverState init = {};
   PCC_HIP(hipMemcpyAsync(ctx->state.p, &init, sizeof(SolverState), hipMemcpyHostToDevice, s));
@@ -677,16 +621,16 @@ pcc_status bbpgd_solve(pcc_ctx* ctx, int64_t* iters, double* res) {
     {
       KTimer t(ctx, PCC_K_BODY_APPLY, bb);
       if (mixed)
-        k_body32<<<gb, 256, 0, s>>>(ba);
+        k_body<float><<<gb, 256, 0, s>>>(ba);
       else
-        k_body64<<<gb, 256, 0, s>>>(ba);
+        k_body<double><<<gb, 256, 0, s>>>(ba);
     }
     {
       KTimer t(ctx, PCC_K_CONSTR_APPLY, cb);
       if (mixed)
-        k_constr32<<<gc, 256, 0, s>>>(ca);
+        k_constr<float><<<gc, 256, 0, s>>>(ca);
       else
-        k_constr64<<<gc, 256, 0, s>>>(ca);
+        k_constr<double><<<gc, 256, 0, s>>>(ca);
     }
   };
   one_iteration();  // k = 0
@@ -727,15 +671,18 @@ void integrate_hard(pcc_ctx* ctx, int64_t kf) {
   KTimer t(ctx, PCC_K_INTEGRATE, 0.0);
   if (N > 0) {
     BodyArgs ba = body_args(ctx);
-    k_body_final<<<nblk(N), 256, 0, s>>>(ba, kf, mixed ? 1 : 0, ctx->F.p, ctx->U.p, ctx->sig.p,
-                                         ctx->imp.p, ctx->ldot_new.p);
+    if (mixed)
+      k_body_final<float><<<nblk(N), 256, 0, s>>>(ba, kf, ctx->F.p, ctx->U.p, ctx->sig.p,
+                                                  ctx->imp.p, ctx->ldot_new.p);
+    else
+      k_body_final<double><<<nblk(N), 256, 0, s>>>(ba, kf, ctx->F.p, ctx->U.p, ctx->sig.p,
+                                                   ctx->imp.p, ctx->ldot_new.p);
     k_integrate<<<nblk(N), 256, 0, s>>>(N, b, ctx->params.dt, ctx->U.p, ctx->ldot_new.p, b.ldot);
   }
   if (C > 0)
-    k_update_constraints<<<nblk(C), 256, 0, s>>>(
-        C, mixed ? 1 : 0, ctx->gam.p + ((kf - 1) & 1) * C, ctx->grd.p + (kf & 1) * C,
-        ctx->gam32.p + ((kf - 1) & 1) * C, ctx->grd32.p + (kf & 1) * C, ctx->S.gamma.p,
-        ctx->S.phi.p);
+    k_update_constraints<<<nblk(C), 256, 0, s>>>(C, ctx->gam.p + ((kf - 1) & 1) * C,
+                                                 ctx->grd.p + (kf & 1) * C, ctx->S.gamma.p,
+                                                 ctx->S.phi.p);
 }
 
 void grow(pcc_ctx* ctx) {

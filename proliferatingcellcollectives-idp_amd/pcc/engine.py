@@ -53,9 +53,11 @@ def lib():
                                      C.POINTER(C.c_double)]
         L.pcc_kernel_name.argtypes = [C.c_int]
         L.pcc_kernel_name.restype = C.c_char_p
+        L.pcc_get_phase_stats.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                          C.c_int64, C.POINTER(C.c_int64)]
         for f in ("pcc_create", "pcc_set_params", "pcc_set_state", "pcc_get_state",
                   "pcc_snapshot", "pcc_restore", "pcc_step", "pcc_get_constraints",
-                  "pcc_detect", "pcc_set_timing", "pcc_get_timing"):
+                  "pcc_detect", "pcc_set_timing", "pcc_get_timing", "pcc_get_phase_stats"):
             getattr(L, f).restype = C.c_int
         _lib = L
     return _lib
@@ -64,7 +66,7 @@ def lib():
 EXPORTED = ("pcc_default_params", "pcc_create", "pcc_destroy", "pcc_set_params",
             "pcc_last_error", "pcc_status_string", "pcc_set_state", "pcc_get_state",
             "pcc_snapshot", "pcc_restore", "pcc_step", "pcc_get_constraints", "pcc_detect",
-            "pcc_set_timing", "pcc_get_timing", "pcc_kernel_name")
+            "pcc_set_timing", "pcc_get_timing", "pcc_kernel_name", "pcc_get_phase_stats")
 
 
 class PccError(RuntimeError):
@@ -170,6 +172,18 @@ class ParticleManager:
             return self.detect(cutoff, capacity=int(cnt.value))
         cons.count = int(cnt.value)
         return cons.trimmed(), bucket[: self.n], perm[: self.n]
+
+    def phase_stats(self):
+        """[(bbpgd_iterations, n_constraints)] per solved ReLCP phase of the
+        last step."""
+        it = np.zeros(256, np.int64)
+        nc = np.zeros(256, np.int64)
+        cnt = C.c_int64(0)
+        self._check(lib().pcc_get_phase_stats(self._ctx, it.ctypes.data_as(C.POINTER(C.c_int64)),
+                                              nc.ctypes.data_as(C.POINTER(C.c_int64)), 256,
+                                              C.byref(cnt)))
+        n = min(int(cnt.value), 256)
+        return list(zip(it[:n].tolist(), nc[:n].tolist()))
 
     def set_timing(self, enable: bool):
         self._check(lib().pcc_set_timing(self._ctx, int(bool(enable))))
