@@ -559,35 +559,43 @@ double oracle_block_sum(const double* v, int64_t n) {
   return r;
 }
 
-/* Three BB sums at once: sum(dg*dg), sum(dg*dp), sum(dp*dp). */
+/* Three BB sums at once: sum(dg*dg), sum(dg*dp), sum(dp*dp), in the GPU's
+ * order: the constraints form Q = ceil(C/256) tiles of 256; G = min(Q, 2048)
+ * logical blocks each own K = ceil(Q/G) consecutive tiles; thread t of a
+ * block accumulates element t of its tiles in tile order starting from 0.0;
+ * then the block tree per block and the partial reduction over blocks. */
+#define ORACLE_BB_GMAX 2048
 static void bb_sums(const double* gi, const double* gp, const double* ri, const double* rp,
                     int64_t C, double* Sgg, double* Sgp, double* Spp) {
-  int64_t nb = (C + 255) / 256;
-  double* part = (double*)malloc(sizeof(double) * 3 * (size_t)(nb > 0 ? nb : 1));
+  const int64_t Q = (C + 255) / 256;
+  const int64_t G = Q < ORACLE_BB_GMAX ? (Q > 0 ? Q : 1) : ORACLE_BB_GMAX;
+  const int64_t K = (Q + G - 1) / G;
+  double* part = (double*)malloc(sizeof(double) * 3 * (size_t)G);
 #pragma omp parallel for schedule(static)
-  for (int64_t b = 0; b < nb; ++b) {
+  for (int64_t b = 0; b < G; ++b) {
     double x0[256], x1[256], x2[256];
     for (int t = 0; t < 256; ++t) {
-      int64_t i = b * 256 + t;
-      if (i < C) {
+      double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+      for (int64_t j = 0; j < K; ++j) {
+        int64_t i = (b * K + j) * 256 + t;
+        if (i >= C) break;
         double dgam = gi[i] - gp[i];
         double dphi = ri[i] - rp[i];
-        x0[t] = dgam * dgam;
-        x1[t] = dgam * dphi;
-        x2[t] = dphi * dphi;
-      } else {
-        x0[t] = 0.0;
-        x1[t] = 0.0;
-        x2[t] = 0.0;
+        a0 += dgam * dgam;
+        a1 += dgam * dphi;
+        a2 += dphi * dphi;
       }
+      x0[t] = a0;
+      x1[t] = a1;
+      x2[t] = a2;
     }
     part[b] = block256(x0);
-    part[nb + b] = block256(x1);
-    part[2 * nb + b] = block256(x2);
+    part[G + b] = block256(x1);
+    part[2 * G + b] = block256(x2);
   }
-  *Sgg = reduce_partials(part, nb);
-  *Sgp = reduce_partials(part + nb, nb);
-  *Spp = reduce_partials(part + 2 * nb, nb);
+  *Sgg = reduce_partials(part, G);
+  *Sgp = reduce_partials(part + G, G);
+  *Spp = reduce_partials(part + 2 * G, G);
   free(part);
 }
 

@@ -157,7 +157,7 @@ struct pcc_ctx {
   pcc::DArr<int32_t> rowptr, inccur, inc;  // N+1, N, 2C
   pcc::DArr<double> increc;                // 8 doubles per incidence
   pcc::DArr<float> increc32;               // 8 floats per incidence
-  pcc::DArr<double> gam, grd, gold, phi;   // 2C, 2C, C, C
+  pcc::DArr<double> iter;                  // 8 per constraint: iterate record
   pcc::DArr<double> ubuf;                  // 8 per body: U[6], dldot, pad
   pcc::DArr<double> mob;                   // 2 per body: 1/(xi l), 12/(xi l^3)
   pcc::DArr<double> partials;              // 4 * nblocks

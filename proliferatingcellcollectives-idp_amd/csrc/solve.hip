@@ -5,21 +5,26 @@
 // BBPGD (BBPGD.cpp:32-127) and the write-back of Hard.cpp:160-170,
 // ParticleManager.cpp:29-137, Particle.cpp:37-65,110-120.
 //
-// One BBPGD iteration is two kernels:
-//   k_body    one thread per body: projection step gamma = max(gp - a*g, 0)
-//             evaluated on the fly for each incident constraint, then the
-//             segmented D^T (gamma - gamma_old) and L^T gamma sums in ascending
-//             constraint order (PETSc's MatMultTranspose row order), mobility,
-//             impedance = exp(-lambda sigma), growth rate; writes U and
-//             ldot - ldot_prev per body.  No float atomics.
-//   k_constr  one thread per constraint: the same projection, D U and L dldot
-//             gathers, g = Phi + dt*phidot - dt*phidot_growth, BB partial dot
-//             products and the projected-gradient residual with a fixed
-//             reduction tree; the last block (integer ticket) reduces the
-//             partials in a fixed order and updates alpha / iteration /
-//             convergence on the device.
-// Both kernels exit immediately once the device flags say the solve is done,
-// so the host can enqueue iterations speculatively.
+// One BBPGD iteration is three launches:
+//   k_body    one thread per body: the projection step
+//             gamma = max(gamma_p - alpha g_p, 0) evaluated on the fly for each
+//             incident constraint, then the segmented D^T (gamma - gamma_old)
+//             and L^T gamma sums in ascending constraint order (PETSc's
+//             MatMultTranspose row order), mobility, impedance exp(-lambda
+//             sigma), growth rate; writes U and ldot - ldot_prev per body.
+//             No float atomics.
+//   k_constr  the same projection per constraint, D U and L dldot gathers,
+//             g = Phi + dt*phidot - dt*phidot_growth, BB dot-product and
+//             residual partials per block (fixed tile->block assignment and
+//             reduction tree, no atomics);
+//   k_finish  one block: reduces the partials in a fixed order and updates
+//             alpha / iteration / convergence on the device.
+// All three exit immediately once the device flags say the solve is done, so
+// the host can enqueue iterations speculatively.
+//
+// Blocks are mapped XCD-aware: the hardware dispatches block h to XCD h % 8,
+// so logical block (h % 8) * (G/8) + h / 8 makes each XCD sweep one
+// contiguous range of bodies / constraints and keeps its gathers in its own L2.
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
@@ -32,6 +37,13 @@ namespace pcc {
 void exclusive_scan_i32(pcc_ctx* ctx, const int32_t* in, int32_t* out, int64_t n);
 void seg_sort_i32(pcc_ctx* ctx, int64_t nseg, const int32_t* start, int32_t* v);
 
+// Per-constraint iterate record (8 doubles, one 64-byte line):
+//   [0,1] gamma, g of parity 0   [2,3] gamma, g of parity 1
+//   [4] gamma_old   [5] Phi   [6,7] unused
+enum { IT_G0 = 0, IT_GOLD = 4, IT_PHI = 5, IT_LEN = 8 };
+static constexpr int kBbGridMax = 2048;
+static constexpr int kXcd = 8;
+
 namespace {
 
 inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
@@ -40,6 +52,13 @@ __device__ __forceinline__ double proj_step(double yp, double malpha, double gp)
   double v = yp + malpha * gp;  // VecWAXPY(gamma_i, -alpha, g_prev, gamma_prev)
   return v < 0.0 ? 0.0 : v;     // VecPointwiseMax(gamma_i, gamma_i, 0)
 }
+
+// Logical block of hardware block h in a grid of gp blocks (gp % 8 == 0).
+__device__ __forceinline__ int64_t xcd_block(int64_t h, int64_t gp) {
+  return (h % kXcd) * (gp / kXcd) + h / kXcd;
+}
+
+inline int64_t pad8(int64_t g) { return (g + kXcd - 1) / kXcd * kXcd; }
 
 __global__ void k_inc_count(int64_t C, const int32_t* __restrict__ I, const int32_t* __restrict__ J,
                             int32_t* __restrict__ deg) {
@@ -95,14 +114,15 @@ __global__ void k_inc_records(int64_t E, const int32_t* __restrict__ inc,
 }
 
 __global__ void k_phase_init(int64_t C, const double* __restrict__ Sgam,
-                             const double* __restrict__ Sphi, double* __restrict__ gam,
-                             double* __restrict__ gold, double* __restrict__ phi) {
+                             const double* __restrict__ Sphi, double* __restrict__ it) {
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= C) return;
-  double g = Sgam[c], p = Sphi[c];
-  gam[c] = g;
-  gold[c] = g;
-  phi[c] = p;
+  const double g = Sgam[c], p = Sphi[c];
+  double2* r = reinterpret_cast<double2*>(it + IT_LEN * c);
+  r[0] = make_double2(g, 0.0);
+  r[1] = make_double2(0.0, 0.0);
+  r[2] = make_double2(g, p);
+  r[3] = make_double2(0.0, 0.0);
 }
 
 __global__ void k_mobility(int64_t N, const double* __restrict__ len, double xi,
@@ -125,12 +145,10 @@ __global__ void k_mobility(int64_t N, const double* __restrict__ len, double xi,
 //                reach ~1e5, as they do on random dense packings.)
 // ---------------------------------------------------------------------------
 struct BodyArgs {
-  int64_t N, C;
+  int64_t N, C, gp;    // gp: padded grid (multiple of 8)
   const int32_t* rowptr;
   const void* rec;     // 8 CT per incidence: coef[6], s, constraint index bits
-  const double* gam;   // 2C
-  const double* grd;   // 2C
-  const double* gold;  // C
+  const double* it;    // iterate records, IT_LEN per constraint
   const double* mob;
   const double* len;
   const double* ldot_prev;
@@ -176,12 +194,11 @@ template <class CT>
 __global__ void __launch_bounds__(256) k_body(BodyArgs a) {
   const SolverState* st = a.st;
   if (st->converged | st->failed) return;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = xcd_block(blockIdx.x, a.gp) * blockDim.x + threadIdx.x;
   if (i >= a.N) return;
   const int64_t k = st->iter;
   const double malpha = -st->alpha;
-  const double* yp = a.gam + ((k - 1) & 1) * a.C;
-  const double* gp = a.grd + ((k - 1) & 1) * a.C;
+  const int po = k == 0 ? 0 : (int)((k - 1) & 1);  // parity of gamma_p, g_p
   double F[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double sig = 0.0;
   const int32_t e1 = a.rowptr[i + 1];
@@ -189,8 +206,11 @@ __global__ void __launch_bounds__(256) k_body(BodyArgs a) {
     double co[7];
     int64_t c;
     load_rec<CT>(a.rec, e, co, c);
-    const double gc = (k == 0) ? a.gam[c] : proj_step(yp[c], malpha, gp[c]);
-    const double dg = gc - a.gold[c];
+    const double* r = a.it + IT_LEN * c;
+    const double2 yg = reinterpret_cast<const double2*>(r)[po];
+    const double gold = r[IT_GOLD];
+    const double gc = (k == 0) ? yg.x : proj_step(yg.x, malpha, yg.y);
+    const double dg = gc - gold;
 #pragma unroll
     for (int q = 0; q < 6; ++q) F[q] += dg * co[q];
     sig += gc * co[6];
@@ -211,19 +231,16 @@ __global__ void __launch_bounds__(256) k_body(BodyArgs a) {
 }
 
 struct ConstrArgs {
-  int64_t C, nb;
+  int64_t C, G, K, gp;  // G logical blocks of K tiles of 256; gp padded grid
   const int32_t* I;
   const int32_t* J;
-  const void* coef;  // 12 CT per constraint: n[3], tI[3], tJ[3], sI, sJ, pad
-  const double* phi;
-  double* gam;
-  double* grd;
+  const void* coef;     // 12 CT per constraint: n[3], tI[3], tJ[3], sI, sJ, pad
+  double* it;
   const double* ubuf;
   double dt, tol;
   int64_t maxit;
   SolverState* st;
-  double* partials;
-  uint32_t* ticket;
+  double* partials;     // 4 * G
 };
 
 template <class CT>
@@ -258,30 +275,88 @@ __device__ __forceinline__ void load_coef<float>(const void* base, int64_t c, do
   co[10] = d.z;
 }
 
-// Final reduction of the per-block partials by the last block, and the BB
-// step-size / convergence update (BBPGD.cpp:49-116).
-__device__ void finish_iteration(const ConstrArgs& a, int64_t k, double s0, double s1, double s2,
-                                 double r, double* lds) {
-  __shared__ int is_last;
-  const int64_t nb = a.nb;
-  if (threadIdx.x == 0) {
-    a.partials[blockIdx.x] = s0;
-    a.partials[nb + blockIdx.x] = s1;
-    a.partials[2 * nb + blockIdx.x] = s2;
-    a.partials[3 * nb + blockIdx.x] = r;
-    __threadfence();
-    uint32_t t = atomicAdd(a.ticket, 1u);
-    is_last = (t == (uint32_t)(nb - 1));
+template <class CT>
+__global__ void __launch_bounds__(256) k_constr(ConstrArgs a) {
+  __shared__ double lds[4];
+  const SolverState* st = a.st;
+  if (st->converged | st->failed) return;
+  const int64_t b = xcd_block(blockIdx.x, a.gp);
+  if (b >= a.G) return;
+  const int64_t k = st->iter;
+  const double malpha = -st->alpha;
+  const int po = k == 0 ? 0 : (int)((k - 1) & 1);
+  const int pn = (int)(k & 1);
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0, vr = 0.0;
+  for (int64_t j = 0; j < a.K; ++j) {
+    const int64_t c = (b * a.K + j) * PCC_BLOCK + threadIdx.x;
+    if (c >= a.C) break;
+    double2* r = reinterpret_cast<double2*>(a.it + IT_LEN * c);
+    const double2 yg = r[po];
+    const double2 gp = r[2];  // (gamma_old, Phi)
+    const double gc = (k == 0) ? yg.x : proj_step(yg.x, malpha, yg.y);
+    double co[11];
+    load_coef<CT>(a.coef, c, co);
+    const double2* UI = reinterpret_cast<const double2*>(a.ubuf + 8 * (int64_t)a.I[c]);
+    const double2* UJ = reinterpret_cast<const double2*>(a.ubuf + 8 * (int64_t)a.J[c]);
+    const double2 i0 = UI[0], i1 = UI[1], i2 = UI[2], i3 = UI[3];
+    const double2 j0 = UJ[0], j1 = UJ[1], j2 = UJ[2], j3 = UJ[3];
+    double pd = 0.0;  // D U_total, columns of body I (lower gid) first
+    pd += co[0] * i0.x;
+    pd += co[1] * i0.y;
+    pd += co[2] * i1.x;
+    pd += co[3] * i1.y;
+    pd += co[4] * i2.x;
+    pd += co[5] * i2.y;
+    pd += (-co[0]) * j0.x;
+    pd += (-co[1]) * j0.y;
+    pd += (-co[2]) * j1.x;
+    pd += co[6] * j1.y;
+    pd += co[7] * j2.x;
+    pd += co[8] * j2.y;
+    double pg = 0.0;  // L ldot_diff
+    pg += co[9] * i3.x;
+    pg += co[10] * j3.x;
+    const double g = gp.y + (a.dt * pd + (-a.dt) * pg);
+    if (k > 0) {
+      r[pn] = make_double2(gc, g);
+      const double dgam = gc - yg.x;
+      const double dphi = g - yg.y;
+      v0 += dgam * dgam;
+      v1 += dgam * dphi;
+      v2 += dphi * dphi;
+    } else {
+      r[0] = make_double2(gc, g);
+    }
+    const double pj = gc > 0 ? g : (g < 0.0 ? g : 0.0);
+    const double ar = fabs(pj);
+    vr = vr < ar ? ar : vr;
   }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
+  const double s0 = block_sum(v0, lds);
+  const double s1 = block_sum(v1, lds);
+  const double s2 = block_sum(v2, lds);
+  const double rm = block_max(vr, lds);
+  if (threadIdx.x == 0) {
+    a.partials[b] = s0;
+    a.partials[a.G + b] = s1;
+    a.partials[2 * a.G + b] = s2;
+    a.partials[3 * a.G + b] = rm;
+  }
+}
+
+// Reduction of the per-block partials in a fixed order and the BB step-size /
+// convergence update (BBPGD.cpp:49-116).  One block of PCC_BLOCK threads.
+__global__ void __launch_bounds__(256) k_finish(ConstrArgs a) {
+  __shared__ double lds[4];
+  SolverState* st = a.st;
+  if (st->converged | st->failed) return;
+  const int64_t k = st->iter;
+  const int64_t G = a.G;
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, am = 0.0;
-  for (int64_t i = threadIdx.x; i < nb; i += PCC_BLOCK) {
-    a0 += __hip_atomic_load(a.partials + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    a1 += __hip_atomic_load(a.partials + nb + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    a2 += __hip_atomic_load(a.partials + 2 * nb + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    double m = __hip_atomic_load(a.partials + 3 * nb + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int64_t i = threadIdx.x; i < G; i += PCC_BLOCK) {
+    a0 += a.partials[i];
+    a1 += a.partials[G + i];
+    a2 += a.partials[2 * G + i];
+    const double m = a.partials[3 * G + i];
     am = am < m ? m : am;
   }
   const double S0 = block_sum(a0, lds);
@@ -289,8 +364,6 @@ __device__ void finish_iteration(const ConstrArgs& a, int64_t k, double s0, doub
   const double S2 = block_sum(a2, lds);
   const double R = block_max(am, lds);
   if (threadIdx.x != 0) return;
-  SolverState* st = a.st;
-  *a.ticket = 0u;
   if (k == 0) {  // initial gradient: res = residual(g, Gamma), alpha = 1/res
     st->res = R;
     st->alpha = 1.0 / R;
@@ -320,64 +393,9 @@ __device__ void finish_iteration(const ConstrArgs& a, int64_t k, double s0, doub
   if (k + 1 == a.maxit) st->failed = 1;
 }
 
-template <class CT>
-__global__ void __launch_bounds__(256) k_constr(ConstrArgs a) {
-  __shared__ double lds[4];
-  const SolverState* st = a.st;
-  if (st->converged | st->failed) return;
-  const int64_t k = st->iter;
-  const double malpha = -st->alpha;
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double v0 = 0.0, v1 = 0.0, v2 = 0.0, vr = 0.0;
-  if (c < a.C) {
-    const double* yp = a.gam + ((k - 1) & 1) * a.C;
-    const double* gp = a.grd + ((k - 1) & 1) * a.C;
-    const double gc = (k == 0) ? a.gam[c] : proj_step(yp[c], malpha, gp[c]);
-    double co[11];
-    load_coef<CT>(a.coef, c, co);
-    const double2* UI = reinterpret_cast<const double2*>(a.ubuf + 8 * (int64_t)a.I[c]);
-    const double2* UJ = reinterpret_cast<const double2*>(a.ubuf + 8 * (int64_t)a.J[c]);
-    const double2 i0 = UI[0], i1 = UI[1], i2 = UI[2], i3 = UI[3];
-    const double2 j0 = UJ[0], j1 = UJ[1], j2 = UJ[2], j3 = UJ[3];
-    double pd = 0.0;  // D U_total, columns of body I (lower gid) first
-    pd += co[0] * i0.x;
-    pd += co[1] * i0.y;
-    pd += co[2] * i1.x;
-    pd += co[3] * i1.y;
-    pd += co[4] * i2.x;
-    pd += co[5] * i2.y;
-    pd += (-co[0]) * j0.x;
-    pd += (-co[1]) * j0.y;
-    pd += (-co[2]) * j1.x;
-    pd += co[6] * j1.y;
-    pd += co[7] * j2.x;
-    pd += co[8] * j2.y;
-    double pg = 0.0;  // L ldot_diff
-    pg += co[9] * i3.x;
-    pg += co[10] * j3.x;
-    const double g = a.phi[c] + (a.dt * pd + (-a.dt) * pg);
-    if (k > 0) a.gam[(k & 1) * a.C + c] = gc;
-    a.grd[(k & 1) * a.C + c] = g;
-    if (k > 0) {
-      const double dgam = gc - yp[c];
-      const double dphi = g - gp[c];
-      v0 = dgam * dgam;
-      v1 = dgam * dphi;
-      v2 = dphi * dphi;
-    }
-    const double pj = gc > 0 ? g : (g < 0.0 ? g : 0.0);
-    vr = fabs(pj);
-  }
-  const double s0 = block_sum(v0, lds);
-  const double s1 = block_sum(v1, lds);
-  const double s2 = block_sum(v2, lds);
-  const double r = block_max(vr, lds);
-  finish_iteration(a, k, s0, s1, s2, r, lds);
-}
-
 // ---------------------------------------------------------------------------
-// Finalisation: the workspace of the last gradient evaluation, gamma_i =
-// gam[kf & 1] (HardModelGradient.cpp:89-121), with the same operator
+// Finalisation: the workspace of the last gradient evaluation, gamma_i of
+// parity kf & 1 (HardModelGradient.cpp:89-121), with the same operator
 // coefficients the solve used.
 // ---------------------------------------------------------------------------
 template <class CT>
@@ -386,7 +404,7 @@ __global__ void k_body_final(BodyArgs a, int64_t kf, double* __restrict__ F_out,
                              double* __restrict__ imp_out, double* __restrict__ ld_out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.N) return;
-  const double* gi = a.gam + (kf & 1) * a.C;
+  const int pf = (int)(kf & 1);
   double F[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double sig = 0.0;
   const int32_t e1 = a.rowptr[i + 1];
@@ -394,8 +412,9 @@ __global__ void k_body_final(BodyArgs a, int64_t kf, double* __restrict__ F_out,
     double co[7];
     int64_t c;
     load_rec<CT>(a.rec, e, co, c);
-    const double gc = gi[c];
-    const double dg = gc - a.gold[c];
+    const double* r = a.it + IT_LEN * c;
+    const double gc = r[2 * pf];
+    const double dg = gc - r[IT_GOLD];
 #pragma unroll
     for (int q = 0; q < 6; ++q) F[q] += dg * co[q];
     sig += gc * co[6];
@@ -455,15 +474,15 @@ __global__ void k_integrate(int64_t N, BodyArrays b, double dt, const double* __
   if (ld) ldot_field[i] = ld[i];
 }
 
-// updateConstraintsFromSolution (Hard.cpp:49-89): gamma <- lagging iterate,
-// phi <- linearised gap of the last evaluated iterate.
-__global__ void k_update_constraints(int64_t C, const double* __restrict__ gam_prev,
-                                     const double* __restrict__ g_last,
+// updateConstraintsFromSolution (Hard.cpp:49-89): gamma <- lagging iterate
+// (parity kf-1), phi <- linearised gap of the last evaluated iterate (kf).
+__global__ void k_update_constraints(int64_t C, int64_t kf, const double* __restrict__ it,
                                      double* __restrict__ Sgam, double* __restrict__ Sphi) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= C) return;
-  Sgam[c] = gam_prev[c];
-  Sphi[c] = g_last[c];
+  const double* r = it + IT_LEN * c;
+  Sgam[c] = r[2 * (int)((kf - 1) & 1)];
+  Sphi[c] = r[2 * (int)(kf & 1) + 1];
 }
 
 // max over the set of max(0, -phi) (Hard.cpp:125-130); max is order-free.
@@ -539,11 +558,10 @@ static BodyArgs body_args(pcc_ctx* ctx) {
   BodyArrays b = body_arrays(ctx->body.p, N);
   a.N = N;
   a.C = ctx->S.C;
+  a.gp = pad8(nblk(N > 0 ? N : 1));
   a.rowptr = ctx->rowptr.p;
   a.rec = mixed ? (const void*)ctx->increc32.p : (const void*)ctx->increc.p;
-  a.gam = ctx->gam.p;
-  a.grd = ctx->grd.p;
-  a.gold = ctx->gold.p;
+  a.it = ctx->iter.p;
   a.mob = ctx->mob.p;
   a.len = b.len;
   a.ldot_prev = b.ldot;
@@ -554,42 +572,42 @@ static BodyArgs body_args(pcc_ctx* ctx) {
   return a;
 }
 
-static ConstrArgs constr_args(pcc_ctx* ctx, int64_t nb) {
+static ConstrArgs constr_args(pcc_ctx* ctx) {
   ConstrArgs a;
   ConstraintSet& S = ctx->S;
   const bool mixed = ctx->params.precision == PCC_PREC_MIXED;
+  const int64_t Q = (S.C + PCC_BLOCK - 1) / PCC_BLOCK;
   a.C = S.C;
-  a.nb = nb;
+  a.G = Q < kBbGridMax ? (Q > 0 ? Q : 1) : kBbGridMax;
+  a.K = (Q + a.G - 1) / a.G;
+  a.gp = pad8(a.G);
   a.I = S.I.p;
   a.J = S.J.p;
   a.coef = mixed ? (const void*)S.coef32.p : (const void*)S.coef64.p;
-  a.phi = ctx->phi.p;
-  a.gam = ctx->gam.p;
-  a.grd = ctx->grd.p;
+  a.it = ctx->iter.p;
   a.ubuf = ctx->ubuf.p;
   a.dt = ctx->params.dt;
   a.tol = ctx->params.bbpgd_tol;
   a.maxit = ctx->params.max_bbpgd_iterations;
   a.st = ctx->state.p;
   a.partials = ctx->partials.p;
-  a.ticket = ctx->ticket.p;
   return a;
 }
 
 // Algorithmic bytes of one launch of each BBPGD kernel: every array the
-// kernel must touch, counted once (gathers of per-constraint iterates count
-// once per incidence, body velocity records once per body).
+// kernel must touch, counted once (the iterate record of a constraint counts
+// once per incidence in the body kernel, a body's velocity record once).
 double body_bytes(int64_t N, int64_t C, bool mixed) {
   const double sc = mixed ? 4 : 8;
-  // per incidence: record (8 coefficient slots), gamma_p, g_p, gamma_old
+  // per incidence: record (8 coefficient slots) + gamma_p, g_p, gamma_old
   // per body: rowptr, mobility (2), length, ldot_prev, U + dldot out (8)
-  return 2.0 * C * (8 * sc + 3 * 8) + N * (4 + 16 + 16 + 64);
+  return 2.0 * C * (8 * sc + 24) + N * (4 + 16 + 16 + 64);
 }
 double constr_bytes(int64_t N, int64_t C, bool mixed) {
   const double sc = mixed ? 4 : 8;
-  // per constraint: I, J; packed row (12 slots); phi; gamma_p, g_p in;
+  // per constraint: I, J; packed row (12 slots); gamma_p, g_p, Phi in;
   // gamma, g out.  Per body: the U + dldot record read once.
-  return C * (8 + 12 * sc + 8 + 4 * 8) + N * 64.0;
+  return C * (8 + 12 * sc + 24 + 16) + N * 64.0;
 }
 
 // Runs BBPGD on the current phase operators.  Returns its status; *iters and
@@ -598,24 +616,19 @@ pcc_status bbpgd_solve(pcc_ctx* ctx, int64_t* iters, double* res) {
   const int64_t N = ctx->N, C = ctx->S.C;
   const bool mixed = ctx->params.precision == PCC_PREC_MIXED;
   hipStream_t s = ctx->stream;
-  const int64_t nb = C > 0 ? (C + 255) / 256 : 1;
-  ctx->gam.ensure(2 * C + 2);
-  ctx->grd.ensure(2 * C + 2);
-  ctx->gold.ensure(C + 1);
-  ctx->phi.ensure(C + 1);
+  ctx->iter.ensure(IT_LEN * (C + 1));
   ctx->ubuf.ensure(8 * N + 8);
   ctx->mob.ensure(2 * N + 2);
-  ctx->partials.ensure(4 * nb + 4);
+  ctx->partials.ensure(4 * kBbGridMax + 4);
   BodyArrays b = body_arrays(ctx->body.p, N);
   if (C > 0)
-    k_phase_init<<<nblk(C), 256, 0, s>>>(C, ctx->S.gamma.p, ctx->S.phi.p, ctx->gam.p, ctx->gold.p,
-                                         ctx->phi.p);
+    k_phase_init<<<nblk(C), 256, 0, s>>>(C, ctx->S.gamma.p, ctx->S.phi.p, ctx->iter.p);
   if (N > 0) k_mobility<<<nblk(N), 256, 0, s>>>(N, b.len, ctx->params.xi, ctx->mob.p);
   SolverState init = {};
   PCC_HIP(hipMemcpyAsync(ctx->state.p, &init, sizeof(SolverState), hipMemcpyHostToDevice, s));
   BodyArgs ba = body_args(ctx);
-  ConstrArgs ca = constr_args(ctx, nb);
-  const unsigned gb = nblk(N > 0 ? N : 1), gc = (unsigned)nb;
+  ConstrArgs ca = constr_args(ctx);
+  const unsigned gb = (unsigned)ba.gp, gc = (unsigned)ca.gp;
   const double bb = body_bytes(N, C, mixed), cb = constr_bytes(N, C, mixed);
   auto one_iteration = [&]() {
     {
@@ -632,6 +645,7 @@ pcc_status bbpgd_solve(pcc_ctx* ctx, int64_t* iters, double* res) {
       else
         k_constr<double><<<gc, 256, 0, s>>>(ca);
     }
+    k_finish<<<1, 256, 0, s>>>(ca);
   };
   one_iteration();  // k = 0
   const int chunk = 10;
@@ -680,8 +694,7 @@ void integrate_hard(pcc_ctx* ctx, int64_t kf) {
     k_integrate<<<nblk(N), 256, 0, s>>>(N, b, ctx->params.dt, ctx->U.p, ctx->ldot_new.p, b.ldot);
   }
   if (C > 0)
-    k_update_constraints<<<nblk(C), 256, 0, s>>>(C, ctx->gam.p + ((kf - 1) & 1) * C,
-                                                 ctx->grd.p + (kf & 1) * C, ctx->S.gamma.p,
+    k_update_constraints<<<nblk(C), 256, 0, s>>>(C, kf, ctx->iter.p, ctx->S.gamma.p,
                                                  ctx->S.phi.p);
 }
 
