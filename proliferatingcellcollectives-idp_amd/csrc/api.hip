@@ -5,6 +5,7 @@
 // the soft model (solveSoftPotential, Soft.cpp:21-114).  All device work runs
 // on the context's stream; the host only reads the handful of scalars that
 // steer the ReLCP loop (constraint count, max overlap, BBPGD flags).
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -156,6 +157,7 @@ pcc_status hard_step(pcc_ctx* ctx, pcc_step_result* out) {
   int64_t total_it = 0;
   double res = 0.0, maxov = 0.0;
   reset_step(ctx);
+  ctx->planar = ctx->allow_planar && bodies_planar(ctx);
   ctx->phase_iters.clear();
   ctx->phase_C.clear();
   pcc_status st = PCC_OK;
@@ -261,6 +263,8 @@ pcc_status pcc_create(const pcc_params* params, int device, pcc_ctx** out) {
   pcc_ctx* ctx = new pcc_ctx();
   ctx->params = *params;
   ctx->device = device;
+  const char* planar_env = getenv("PCC_PLANAR");
+  ctx->allow_planar = !(planar_env && planar_env[0] == '0');
   pcc_status s = guarded(ctx, [&]() {
     PCC_HIP(hipSetDevice(device));
     PCC_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));

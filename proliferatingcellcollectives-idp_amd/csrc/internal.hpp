@@ -126,7 +126,8 @@ struct pcc_ctx {
   std::string last_error;
   int64_t N = 0;
   bool has_state = false;
-  bool planar = false;
+  bool planar = false;        // current step uses the 2D monolayer kernels
+  bool allow_planar = true;   // PCC_PLANAR=0 in the environment forces the 3D kernels
 
   // body state + snapshot + outputs
   pcc::DArr<double> body;      // kBodyFields * N
@@ -155,9 +156,14 @@ struct pcc_ctx {
   // constraint set and phase operators
   pcc::ConstraintSet S;
   pcc::DArr<int32_t> rowptr, inccur, inc;  // N+1, N, 2C
-  pcc::DArr<double> increc;                // 8 doubles per incidence
-  pcc::DArr<float> increc32;               // 8 floats per incidence
-  pcc::DArr<double> iter;                  // 8 per constraint: iterate record
+  pcc::DArr<double> increc;                // per incidence: 8 (3D) or 4 (planar) doubles
+  pcc::DArr<float> increc32;               // per incidence: 8 or 4 floats
+  pcc::DArr<int32_t> incidx;               // planar: constraint index per incidence
+  pcc::DArr<double> prow;                  // planar operator rows, 6 per constraint
+  pcc::DArr<float> prow32;
+  pcc::DArr<int32_t> flag;                 // small device flags
+  pcc::DArr<double> iter;                  // 4 per constraint: iterate record
+  pcc::DArr<double> gprev;                 // lagging iterate at convergence checks
   pcc::DArr<double> ubuf;                  // 8 per body: U[6], dldot, pad
   pcc::DArr<double> mob;                   // 2 per body: 1/(xi l), 12/(xi l^3)
   pcc::DArr<double> partials;              // 4 * nblocks
@@ -181,6 +187,7 @@ void detect_batch(pcc_ctx* ctx, double cutoff, int32_t iteration);  // fills ctx
 void append_batch(pcc_ctx* ctx, int32_t iteration);               // appends sorted batch to S
 double max_overlap(pcc_ctx* ctx);
 void build_incidence(pcc_ctx* ctx, bool records);
+bool bodies_planar(pcc_ctx* ctx);
 pcc_status bbpgd_solve(pcc_ctx* ctx, int64_t* iters, double* res);
 void integrate_hard(pcc_ctx* ctx, int64_t kfinal);
 void grow(pcc_ctx* ctx);

@@ -37,10 +37,12 @@ namespace pcc {
 void exclusive_scan_i32(pcc_ctx* ctx, const int32_t* in, int32_t* out, int64_t n);
 void seg_sort_i32(pcc_ctx* ctx, int64_t nseg, const int32_t* start, int32_t* v);
 
-// Per-constraint iterate record (8 doubles, one 64-byte line):
-//   [0,1] gamma, g of parity 0   [2,3] gamma, g of parity 1
-//   [4] gamma_old   [5] Phi   [6,7] unused
-enum { IT_G0 = 0, IT_GOLD = 4, IT_PHI = 5, IT_LEN = 8 };
+// Per-constraint iterate record (4 doubles, 32 bytes): gamma, g of the last
+// evaluated iterate, gamma_old, Phi.  The constraint kernel overwrites
+// (gamma, g) in place; the lagging iterate the reference returns (BBPGD.cpp:124)
+// is only needed where convergence can be declared (iteration % 10 == 0), so
+// it is saved to gprev on those iterations.
+enum { IT_GAM = 0, IT_G = 1, IT_GOLD = 2, IT_PHI = 3, IT_LEN = 4 };
 static constexpr int kBbGridMax = 2048;
 static constexpr int kXcd = 8;
 
@@ -113,6 +115,49 @@ __global__ void k_inc_records(int64_t E, const int32_t* __restrict__ inc,
   r32[7] = __int_as_float((int)c);
 }
 
+// Planar (2D monolayer) operator: every body has z = 0 and q = (w, 0, 0, z),
+// so n_z, r_z, the x/y torque components and U_z, omega_x, omega_y are exact
+// zeros.  Incidence record (c_x, c_y, tau_z, s) plus the constraint index;
+// constraint row (n_x, n_y, tauI_z, tauJ_z, sI, sJ).
+__global__ void k_planar_records(int64_t E, int64_t C, const int32_t* __restrict__ inc,
+                                 const double* __restrict__ coef, double* __restrict__ rec,
+                                 float* __restrict__ rec32, int32_t* __restrict__ rc,
+                                 double* __restrict__ prow, float* __restrict__ prow32) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < E) {
+    const int32_t v = inc[e];
+    const int64_t c = v >> 1;
+    const int side = v & 1;
+    const double* row = coef + 12 * c;
+    double co[4];
+    if (side == 0) {
+      co[0] = row[0];
+      co[1] = row[1];
+      co[2] = row[5];
+      co[3] = row[9];
+    } else {
+      co[0] = -row[0];
+      co[1] = -row[1];
+      co[2] = row[8];
+      co[3] = row[10];
+    }
+    reinterpret_cast<double2*>(rec)[2 * e] = make_double2(co[0], co[1]);
+    reinterpret_cast<double2*>(rec)[2 * e + 1] = make_double2(co[2], co[3]);
+    reinterpret_cast<float4*>(rec32)[e] =
+        make_float4((float)co[0], (float)co[1], (float)co[2], (float)co[3]);
+    rc[e] = (int32_t)c;
+  }
+  if (e < C) {
+    const double* row = coef + 12 * e;
+    const double p[6] = {row[0], row[1], row[5], row[8], row[9], row[10]};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      prow[6 * e + k] = p[k];
+      prow32[6 * e + k] = (float)p[k];
+    }
+  }
+}
+
 __global__ void k_phase_init(int64_t C, const double* __restrict__ Sgam,
                              const double* __restrict__ Sphi, double* __restrict__ it) {
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -120,9 +165,7 @@ __global__ void k_phase_init(int64_t C, const double* __restrict__ Sgam,
   const double g = Sgam[c], p = Sphi[c];
   double2* r = reinterpret_cast<double2*>(it + IT_LEN * c);
   r[0] = make_double2(g, 0.0);
-  r[1] = make_double2(0.0, 0.0);
-  r[2] = make_double2(g, p);
-  r[3] = make_double2(0.0, 0.0);
+  r[1] = make_double2(g, p);
 }
 
 __global__ void k_mobility(int64_t N, const double* __restrict__ len, double xi,
@@ -134,27 +177,37 @@ __global__ void k_mobility(int64_t N, const double* __restrict__ len, double xi,
   mob[2 * i + 1] = 12.0 / (l * l * l * xi);  // Physics.cpp:204
 }
 
+// 1 if every body is in the z = 0 plane with an in-plane orientation.
+__global__ void k_planar_check(int64_t N, BodyArrays b, int32_t* __restrict__ nonplanar) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  if (b.pz[i] != 0.0 || b.qx[i] != 0.0 || b.qy[i] != 0.0) atomicOr(nonplanar, 1);
+}
+
 // ---------------------------------------------------------------------------
 // BBPGD kernels, templated on the storage type CT of the operator
-// coefficients (D blocks, L entries, incidence records):
+// coefficients (D rows, L entries, incidence records):
 //   CT = double  plain FP64: bit-identical to the CPU restatement;
 //   CT = float   mixed: the operator, the bulk of the bytes, is stored in
 //                FP32; impulses, gaps, body velocities, every accumulation,
 //                the BB dot products and the residual stay FP64.  (FP32
 //                accumulation cannot reach a 1e-6 residual once impulses
 //                reach ~1e5, as they do on random dense packings.)
+// and on PLANAR (2D monolayer records, the exact-zero terms skipped; the
+// results are bit-identical to the 3D kernels).
 // ---------------------------------------------------------------------------
 struct BodyArgs {
   int64_t N, C, gp;    // gp: padded grid (multiple of 8)
   const int32_t* rowptr;
-  const void* rec;     // 8 CT per incidence: coef[6], s, constraint index bits
-  const double* it;    // iterate records, IT_LEN per constraint
+  const void* rec;     // 3D: 8 CT per incidence (coef[6], s, index bits); 2D: 4 CT
+  const int32_t* rc;   // 2D: constraint index per incidence
+  double* it;          // iterate records, IT_LEN per constraint
   const double* mob;
   const double* len;
   const double* ldot_prev;
   double mlam, itau;
   const SolverState* st;
-  double* ubuf;  // 8 per body: U[6], ldot - ldot_prev, pad
+  double* ubuf;  // 3D: 8 per body (U[6], dldot, pad); 2D: 4 (ux, uy, wz, dldot)
 };
 
 template <class CT>
@@ -191,51 +244,112 @@ __device__ __forceinline__ void load_rec<float>(const void* base, int64_t e, dou
 }
 
 template <class CT>
-__global__ void __launch_bounds__(256) k_body(BodyArgs a) {
+__device__ __forceinline__ void load_prec(const void* base, int64_t e, double co[4]);
+
+template <>
+__device__ __forceinline__ void load_prec<double>(const void* base, int64_t e, double co[4]) {
+  const double2* r = reinterpret_cast<const double2*>(base) + 2 * e;
+  const double2 a = r[0], b = r[1];
+  co[0] = a.x;
+  co[1] = a.y;
+  co[2] = b.x;
+  co[3] = b.y;
+}
+
+template <>
+__device__ __forceinline__ void load_prec<float>(const void* base, int64_t e, double co[4]) {
+  const float4 a = reinterpret_cast<const float4*>(base)[e];
+  co[0] = a.x;
+  co[1] = a.y;
+  co[2] = a.z;
+  co[3] = a.w;
+}
+
+// Body apply.  FINAL = false: one BBPGD iteration (projection on the fly,
+// writes the velocity record).  FINAL = true: the workspace of the converged
+// iterate (HardModelGradient.cpp:89-121), writes F, U, sigma, impedance, ldot.
+struct FinalOut {
+  double *F, *U, *sig, *imp, *ld;
+};
+
+template <class CT, bool PLANAR, bool FINAL>
+__global__ void __launch_bounds__(256) k_body(BodyArgs a, FinalOut fo) {
   const SolverState* st = a.st;
-  if (st->converged | st->failed) return;
-  const int64_t i = xcd_block(blockIdx.x, a.gp) * blockDim.x + threadIdx.x;
+  if (!FINAL && (st->converged | st->failed)) return;
+  const int64_t i = FINAL ? (int64_t)blockIdx.x * blockDim.x + threadIdx.x
+                          : xcd_block(blockIdx.x, a.gp) * blockDim.x + threadIdx.x;
   if (i >= a.N) return;
-  const int64_t k = st->iter;
-  const double malpha = -st->alpha;
-  const int po = k == 0 ? 0 : (int)((k - 1) & 1);  // parity of gamma_p, g_p
+  const int64_t k = FINAL ? 1 : st->iter;
+  const double malpha = FINAL ? 0.0 : -st->alpha;
   double F[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double sig = 0.0;
   const int32_t e1 = a.rowptr[i + 1];
   for (int32_t e = a.rowptr[i]; e < e1; ++e) {
     double co[7];
     int64_t c;
-    load_rec<CT>(a.rec, e, co, c);
-    const double* r = a.it + IT_LEN * c;
-    const double2 yg = reinterpret_cast<const double2*>(r)[po];
-    const double gold = r[IT_GOLD];
-    const double gc = (k == 0) ? yg.x : proj_step(yg.x, malpha, yg.y);
+    if (PLANAR) {
+      double p[4];
+      load_prec<CT>(a.rec, e, p);
+      co[0] = p[0];
+      co[1] = p[1];
+      co[5] = p[2];
+      co[6] = p[3];
+      c = a.rc[e];
+    } else {
+      load_rec<CT>(a.rec, e, co, c);
+    }
+    const double2* r = reinterpret_cast<const double2*>(a.it + IT_LEN * c);
+    const double2 yg = r[0];
+    const double gold = r[1].x;
+    const double gc = (FINAL || k == 0) ? yg.x : proj_step(yg.x, malpha, yg.y);
     const double dg = gc - gold;
+    if (PLANAR) {
+      F[0] += dg * co[0];
+      F[1] += dg * co[1];
+      F[5] += dg * co[5];
+    } else {
 #pragma unroll
-    for (int q = 0; q < 6; ++q) F[q] += dg * co[q];
+      for (int q = 0; q < 6; ++q) F[q] += dg * co[q];
+    }
     sig += gc * co[6];
   }
   const double mt = a.mob[2 * i], mr = a.mob[2 * i + 1];
-  double u[8];
+  double u[6];
 #pragma unroll
   for (int q = 0; q < 3; ++q) u[q] = (0.0 + mt * F[q]) + 0.0;
 #pragma unroll
   for (int q = 3; q < 6; ++q) u[q] = (0.0 + mr * F[q]) + 0.0;
   const double imp = det_exp(sig * a.mlam);
   const double ld = (a.len[i] * imp) * a.itau;
-  u[6] = ld - a.ldot_prev[i];
-  u[7] = 0.0;
-  double2* o = reinterpret_cast<double2*>(a.ubuf + 8 * i);
+  if (FINAL) {
 #pragma unroll
-  for (int q = 0; q < 4; ++q) o[q] = make_double2(u[2 * q], u[2 * q + 1]);
+    for (int q = 0; q < 6; ++q) {
+      fo.F[6 * i + q] = F[q];
+      fo.U[6 * i + q] = u[q];
+    }
+    fo.sig[i] = sig;
+    fo.imp[i] = imp;
+    fo.ld[i] = ld;
+  } else if (PLANAR) {
+    double2* o = reinterpret_cast<double2*>(a.ubuf + 4 * i);
+    o[0] = make_double2(u[0], u[1]);
+    o[1] = make_double2(u[5], ld - a.ldot_prev[i]);
+  } else {
+    double2* o = reinterpret_cast<double2*>(a.ubuf + 8 * i);
+    o[0] = make_double2(u[0], u[1]);
+    o[1] = make_double2(u[2], u[3]);
+    o[2] = make_double2(u[4], u[5]);
+    o[3] = make_double2(ld - a.ldot_prev[i], 0.0);
+  }
 }
 
 struct ConstrArgs {
   int64_t C, G, K, gp;  // G logical blocks of K tiles of 256; gp padded grid
   const int32_t* I;
   const int32_t* J;
-  const void* coef;     // 12 CT per constraint: n[3], tI[3], tJ[3], sI, sJ, pad
+  const void* coef;     // 3D: 12 CT per constraint (n, rIxn, rJx-n, sI, sJ, pad); 2D: 6 CT
   double* it;
+  double* gprev;        // lagging iterate saved on iterations % 10 == 0
   const double* ubuf;
   double dt, tol;
   int64_t maxit;
@@ -276,6 +390,33 @@ __device__ __forceinline__ void load_coef<float>(const void* base, int64_t c, do
 }
 
 template <class CT>
+__device__ __forceinline__ void load_pcoef(const void* base, int64_t c, double co[6]);
+
+template <>
+__device__ __forceinline__ void load_pcoef<double>(const void* base, int64_t c, double co[6]) {
+  const double2* r = reinterpret_cast<const double2*>(base) + 3 * c;
+  const double2 a = r[0], b = r[1], d = r[2];
+  co[0] = a.x;
+  co[1] = a.y;
+  co[2] = b.x;
+  co[3] = b.y;
+  co[4] = d.x;
+  co[5] = d.y;
+}
+
+template <>
+__device__ __forceinline__ void load_pcoef<float>(const void* base, int64_t c, double co[6]) {
+  const float2* r = reinterpret_cast<const float2*>(base) + 3 * c;
+  const float2 a = r[0], b = r[1], d = r[2];
+  co[0] = a.x;
+  co[1] = a.y;
+  co[2] = b.x;
+  co[3] = b.y;
+  co[4] = d.x;
+  co[5] = d.y;
+}
+
+template <class CT, bool PLANAR>
 __global__ void __launch_bounds__(256) k_constr(ConstrArgs a) {
   __shared__ double lds[4];
   const SolverState* st = a.st;
@@ -284,48 +425,62 @@ __global__ void __launch_bounds__(256) k_constr(ConstrArgs a) {
   if (b >= a.G) return;
   const int64_t k = st->iter;
   const double malpha = -st->alpha;
-  const int po = k == 0 ? 0 : (int)((k - 1) & 1);
-  const int pn = (int)(k & 1);
+  const bool save_prev = k > 0 && k % 10 == 0;
   double v0 = 0.0, v1 = 0.0, v2 = 0.0, vr = 0.0;
   for (int64_t j = 0; j < a.K; ++j) {
     const int64_t c = (b * a.K + j) * PCC_BLOCK + threadIdx.x;
     if (c >= a.C) break;
     double2* r = reinterpret_cast<double2*>(a.it + IT_LEN * c);
-    const double2 yg = r[po];
-    const double2 gp = r[2];  // (gamma_old, Phi)
+    const double2 yg = r[0];  // (gamma_p, g_p)
+    const double phi = r[1].y;
     const double gc = (k == 0) ? yg.x : proj_step(yg.x, malpha, yg.y);
-    double co[11];
-    load_coef<CT>(a.coef, c, co);
-    const double2* UI = reinterpret_cast<const double2*>(a.ubuf + 8 * (int64_t)a.I[c]);
-    const double2* UJ = reinterpret_cast<const double2*>(a.ubuf + 8 * (int64_t)a.J[c]);
-    const double2 i0 = UI[0], i1 = UI[1], i2 = UI[2], i3 = UI[3];
-    const double2 j0 = UJ[0], j1 = UJ[1], j2 = UJ[2], j3 = UJ[3];
     double pd = 0.0;  // D U_total, columns of body I (lower gid) first
-    pd += co[0] * i0.x;
-    pd += co[1] * i0.y;
-    pd += co[2] * i1.x;
-    pd += co[3] * i1.y;
-    pd += co[4] * i2.x;
-    pd += co[5] * i2.y;
-    pd += (-co[0]) * j0.x;
-    pd += (-co[1]) * j0.y;
-    pd += (-co[2]) * j1.x;
-    pd += co[6] * j1.y;
-    pd += co[7] * j2.x;
-    pd += co[8] * j2.y;
     double pg = 0.0;  // L ldot_diff
-    pg += co[9] * i3.x;
-    pg += co[10] * j3.x;
-    const double g = gp.y + (a.dt * pd + (-a.dt) * pg);
+    if (PLANAR) {
+      double co[6];
+      load_pcoef<CT>(a.coef, c, co);
+      const double2* UI = reinterpret_cast<const double2*>(a.ubuf + 4 * (int64_t)a.I[c]);
+      const double2* UJ = reinterpret_cast<const double2*>(a.ubuf + 4 * (int64_t)a.J[c]);
+      const double2 i0 = UI[0], i1 = UI[1], j0 = UJ[0], j1 = UJ[1];
+      pd += co[0] * i0.x;
+      pd += co[1] * i0.y;
+      pd += co[2] * i1.x;
+      pd += (-co[0]) * j0.x;
+      pd += (-co[1]) * j0.y;
+      pd += co[3] * j1.x;
+      pg += co[4] * i1.y;
+      pg += co[5] * j1.y;
+    } else {
+      double co[11];
+      load_coef<CT>(a.coef, c, co);
+      const double2* UI = reinterpret_cast<const double2*>(a.ubuf + 8 * (int64_t)a.I[c]);
+      const double2* UJ = reinterpret_cast<const double2*>(a.ubuf + 8 * (int64_t)a.J[c]);
+      const double2 i0 = UI[0], i1 = UI[1], i2 = UI[2], i3 = UI[3];
+      const double2 j0 = UJ[0], j1 = UJ[1], j2 = UJ[2], j3 = UJ[3];
+      pd += co[0] * i0.x;
+      pd += co[1] * i0.y;
+      pd += co[2] * i1.x;
+      pd += co[3] * i1.y;
+      pd += co[4] * i2.x;
+      pd += co[5] * i2.y;
+      pd += (-co[0]) * j0.x;
+      pd += (-co[1]) * j0.y;
+      pd += (-co[2]) * j1.x;
+      pd += co[6] * j1.y;
+      pd += co[7] * j2.x;
+      pd += co[8] * j2.y;
+      pg += co[9] * i3.x;
+      pg += co[10] * j3.x;
+    }
+    const double g = phi + (a.dt * pd + (-a.dt) * pg);
+    if (save_prev) a.gprev[c] = yg.x;
+    r[0] = make_double2(gc, g);
     if (k > 0) {
-      r[pn] = make_double2(gc, g);
       const double dgam = gc - yg.x;
       const double dphi = g - yg.y;
       v0 += dgam * dgam;
       v1 += dgam * dphi;
       v2 += dphi * dphi;
-    } else {
-      r[0] = make_double2(gc, g);
     }
     const double pj = gc > 0 ? g : (g < 0.0 ? g : 0.0);
     const double ar = fabs(pj);
@@ -393,49 +548,6 @@ __global__ void __launch_bounds__(256) k_finish(ConstrArgs a) {
   if (k + 1 == a.maxit) st->failed = 1;
 }
 
-// ---------------------------------------------------------------------------
-// Finalisation: the workspace of the last gradient evaluation, gamma_i of
-// parity kf & 1 (HardModelGradient.cpp:89-121), with the same operator
-// coefficients the solve used.
-// ---------------------------------------------------------------------------
-template <class CT>
-__global__ void k_body_final(BodyArgs a, int64_t kf, double* __restrict__ F_out,
-                             double* __restrict__ U_out, double* __restrict__ sig_out,
-                             double* __restrict__ imp_out, double* __restrict__ ld_out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.N) return;
-  const int pf = (int)(kf & 1);
-  double F[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  double sig = 0.0;
-  const int32_t e1 = a.rowptr[i + 1];
-  for (int32_t e = a.rowptr[i]; e < e1; ++e) {
-    double co[7];
-    int64_t c;
-    load_rec<CT>(a.rec, e, co, c);
-    const double* r = a.it + IT_LEN * c;
-    const double gc = r[2 * pf];
-    const double dg = gc - r[IT_GOLD];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) F[q] += dg * co[q];
-    sig += gc * co[6];
-  }
-  const double mt = a.mob[2 * i], mr = a.mob[2 * i + 1];
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    F_out[6 * i + q] = F[q];
-    U_out[6 * i + q] = (0.0 + mt * F[q]) + 0.0;
-  }
-#pragma unroll
-  for (int q = 3; q < 6; ++q) {
-    F_out[6 * i + q] = F[q];
-    U_out[6 * i + q] = (0.0 + mr * F[q]) + 0.0;
-  }
-  const double imp = det_exp(sig * a.mlam);
-  sig_out[i] = sig;
-  imp_out[i] = imp;
-  ld_out[i] = (a.len[i] * imp) * a.itau;
-}
-
 // dC = G(q) U (Physics.cpp:220-265), Euler step and renormalisation
 // (Particle.cpp:37-57,110-120), write-back of ldot/impedance/stress
 // (ParticleManager.cpp:89-130).
@@ -475,14 +587,15 @@ __global__ void k_integrate(int64_t N, BodyArrays b, double dt, const double* __
 }
 
 // updateConstraintsFromSolution (Hard.cpp:49-89): gamma <- lagging iterate
-// (parity kf-1), phi <- linearised gap of the last evaluated iterate (kf).
-__global__ void k_update_constraints(int64_t C, int64_t kf, const double* __restrict__ it,
+// (saved at the converging iteration), phi <- linearised gap of the last
+// evaluated iterate.
+__global__ void k_update_constraints(int64_t C, const double* __restrict__ it,
+                                     const double* __restrict__ gprev,
                                      double* __restrict__ Sgam, double* __restrict__ Sphi) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= C) return;
-  const double* r = it + IT_LEN * c;
-  Sgam[c] = r[2 * (int)((kf - 1) & 1)];
-  Sphi[c] = r[2 * (int)(kf & 1) + 1];
+  Sgam[c] = gprev[c];
+  Sphi[c] = it[IT_LEN * c + IT_G];
 }
 
 // max over the set of max(0, -phi) (Hard.cpp:125-130); max is order-free.
@@ -526,8 +639,22 @@ double max_overlap(pcc_ctx* ctx) {
   return m;
 }
 
+// 1 if every body lies in the z = 0 plane with an in-plane orientation.
+bool bodies_planar(pcc_ctx* ctx) {
+  const int64_t N = ctx->N;
+  if (N == 0) return true;
+  ctx->flag.ensure(1);
+  PCC_HIP(hipMemsetAsync(ctx->flag.p, 0, sizeof(int32_t), ctx->stream));
+  k_planar_check<<<nblk(N), 256, 0, ctx->stream>>>(N, body_arrays(ctx->body.p, N), ctx->flag.p);
+  int32_t h = 0;
+  PCC_HIP(hipMemcpyAsync(&h, ctx->flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  PCC_HIP(hipStreamSynchronize(ctx->stream));
+  return h == 0;
+}
+
 // Body-incidence CSR over the whole constraint set, each body's incidences in
-// ascending constraint order, plus the per-incidence coefficient records.
+// ascending constraint order, plus the per-incidence coefficient records (3D
+// or planar, FP64 and FP32) and, planar, the compact constraint rows.
 void build_incidence(pcc_ctx* ctx, bool records) {
   const int64_t N = ctx->N, C = ctx->S.C;
   hipStream_t s = ctx->stream;
@@ -535,19 +662,29 @@ void build_incidence(pcc_ctx* ctx, bool records) {
   ctx->rowptr.ensure(N + 1);
   ctx->inccur.ensure(N + 1);
   ctx->inc.ensure(2 * C + 1);
-  ctx->increc.ensure(8 * (2 * C + 1));
-  ctx->increc32.ensure(8 * (2 * C + 1));
   PCC_HIP(hipMemsetAsync(ctx->inccur.p, 0, sizeof(int32_t) * (size_t)(N + 1), s));
   if (C > 0) k_inc_count<<<nblk(C), 256, 0, s>>>(C, ctx->S.I.p, ctx->S.J.p, ctx->inccur.p);
   exclusive_scan_i32(ctx, ctx->inccur.p, ctx->rowptr.p, N + 1);
   PCC_HIP(hipMemsetAsync(ctx->inccur.p, 0, sizeof(int32_t) * (size_t)(N + 1), s));
-  if (C > 0) {
-    k_inc_scatter<<<nblk(C), 256, 0, s>>>(C, ctx->S.I.p, ctx->S.J.p, ctx->rowptr.p, ctx->inccur.p,
-                                          ctx->inc.p);
-    seg_sort_i32(ctx, N, ctx->rowptr.p, ctx->inc.p);
-    if (records)
-      k_inc_records<<<nblk(2 * C), 256, 0, s>>>(2 * C, ctx->inc.p, ctx->S.coef64.p,
-                                              ctx->increc.p, ctx->increc32.p);
+  if (C == 0) return;
+  k_inc_scatter<<<nblk(C), 256, 0, s>>>(C, ctx->S.I.p, ctx->S.J.p, ctx->rowptr.p, ctx->inccur.p,
+                                        ctx->inc.p);
+  seg_sort_i32(ctx, N, ctx->rowptr.p, ctx->inc.p);
+  if (!records) return;
+  if (ctx->planar) {
+    ctx->increc.ensure(4 * (2 * C + 1));
+    ctx->increc32.ensure(4 * (2 * C + 1));
+    ctx->incidx.ensure(2 * C + 1);
+    ctx->prow.ensure(6 * (C + 1));
+    ctx->prow32.ensure(6 * (C + 1));
+    k_planar_records<<<nblk(2 * C), 256, 0, s>>>(2 * C, C, ctx->inc.p, ctx->S.coef64.p,
+                                                 ctx->increc.p, ctx->increc32.p, ctx->incidx.p,
+                                                 ctx->prow.p, ctx->prow32.p);
+  } else {
+    ctx->increc.ensure(8 * (2 * C + 1));
+    ctx->increc32.ensure(8 * (2 * C + 1));
+    k_inc_records<<<nblk(2 * C), 256, 0, s>>>(2 * C, ctx->inc.p, ctx->S.coef64.p, ctx->increc.p,
+                                              ctx->increc32.p);
   }
 }
 
@@ -561,6 +698,7 @@ static BodyArgs body_args(pcc_ctx* ctx) {
   a.gp = pad8(nblk(N > 0 ? N : 1));
   a.rowptr = ctx->rowptr.p;
   a.rec = mixed ? (const void*)ctx->increc32.p : (const void*)ctx->increc.p;
+  a.rc = ctx->incidx.p;
   a.it = ctx->iter.p;
   a.mob = ctx->mob.p;
   a.len = b.len;
@@ -583,8 +721,12 @@ static ConstrArgs constr_args(pcc_ctx* ctx) {
   a.gp = pad8(a.G);
   a.I = S.I.p;
   a.J = S.J.p;
-  a.coef = mixed ? (const void*)S.coef32.p : (const void*)S.coef64.p;
+  if (ctx->planar)
+    a.coef = mixed ? (const void*)ctx->prow32.p : (const void*)ctx->prow.p;
+  else
+    a.coef = mixed ? (const void*)S.coef32.p : (const void*)S.coef64.p;
   a.it = ctx->iter.p;
+  a.gprev = ctx->gprev.p;
   a.ubuf = ctx->ubuf.p;
   a.dt = ctx->params.dt;
   a.tol = ctx->params.bbpgd_tol;
@@ -595,19 +737,24 @@ static ConstrArgs constr_args(pcc_ctx* ctx) {
 }
 
 // Algorithmic bytes of one launch of each BBPGD kernel: every array the
-// kernel must touch, counted once (the iterate record of a constraint counts
-// once per incidence in the body kernel, a body's velocity record once).
-double body_bytes(int64_t N, int64_t C, bool mixed) {
+// kernel must touch, counted once (a constraint's iterate record counts once
+// per incidence in the body kernel, a body's velocity record once).
+double body_bytes(int64_t N, int64_t C, bool mixed, bool planar) {
   const double sc = mixed ? 4 : 8;
-  // per incidence: record (8 coefficient slots) + gamma_p, g_p, gamma_old
-  // per body: rowptr, mobility (2), length, ldot_prev, U + dldot out (8)
-  return 2.0 * C * (8 * sc + 24) + N * (4 + 16 + 16 + 64);
+  // per incidence: record (8 or 4 coefficient slots [+ 4-byte index]) +
+  // the iterate record's (gamma, g, gamma_old)
+  // per body: rowptr, mobility (2), length, ldot_prev, velocity record out
+  const double rec = planar ? 4 * sc + 4 : 8 * sc;
+  const double ub = planar ? 32 : 64;
+  return 2.0 * C * (rec + 24) + N * (4 + 16 + 16 + ub);
 }
-double constr_bytes(int64_t N, int64_t C, bool mixed) {
+double constr_bytes(int64_t N, int64_t C, bool mixed, bool planar) {
   const double sc = mixed ? 4 : 8;
-  // per constraint: I, J; packed row (12 slots); gamma_p, g_p, Phi in;
-  // gamma, g out.  Per body: the U + dldot record read once.
-  return C * (8 + 12 * sc + 24 + 16) + N * 64.0;
+  // per constraint: I, J; operator row (12 or 6 slots); iterate record in
+  // (gamma, g, Phi) and out (gamma, g).  Per body: velocity record read once.
+  const double row = planar ? 6 * sc : 12 * sc;
+  const double ub = planar ? 32 : 64;
+  return C * (8 + row + 24 + 16) + N * ub;
 }
 
 // Runs BBPGD on the current phase operators.  Returns its status; *iters and
@@ -615,8 +762,10 @@ double constr_bytes(int64_t N, int64_t C, bool mixed) {
 pcc_status bbpgd_solve(pcc_ctx* ctx, int64_t* iters, double* res) {
   const int64_t N = ctx->N, C = ctx->S.C;
   const bool mixed = ctx->params.precision == PCC_PREC_MIXED;
+  const bool planar = ctx->planar;
   hipStream_t s = ctx->stream;
   ctx->iter.ensure(IT_LEN * (C + 1));
+  ctx->gprev.ensure(C + 1);
   ctx->ubuf.ensure(8 * N + 8);
   ctx->mob.ensure(2 * N + 2);
   ctx->partials.ensure(4 * kBbGridMax + 4);
@@ -629,21 +778,36 @@ pcc_status bbpgd_solve(pcc_ctx* ctx, int64_t* iters, double* res) {
   BodyArgs ba = body_args(ctx);
   ConstrArgs ca = constr_args(ctx);
   const unsigned gb = (unsigned)ba.gp, gc = (unsigned)ca.gp;
-  const double bb = body_bytes(N, C, mixed), cb = constr_bytes(N, C, mixed);
+  const double bb = body_bytes(N, C, mixed, planar), cb = constr_bytes(N, C, mixed, planar);
+  const FinalOut none = {};
   auto one_iteration = [&]() {
     {
       KTimer t(ctx, PCC_K_BODY_APPLY, bb);
-      if (mixed)
-        k_body<float><<<gb, 256, 0, s>>>(ba);
-      else
-        k_body<double><<<gb, 256, 0, s>>>(ba);
+      if (planar) {
+        if (mixed)
+          k_body<float, true, false><<<gb, 256, 0, s>>>(ba, none);
+        else
+          k_body<double, true, false><<<gb, 256, 0, s>>>(ba, none);
+      } else {
+        if (mixed)
+          k_body<float, false, false><<<gb, 256, 0, s>>>(ba, none);
+        else
+          k_body<double, false, false><<<gb, 256, 0, s>>>(ba, none);
+      }
     }
     {
       KTimer t(ctx, PCC_K_CONSTR_APPLY, cb);
-      if (mixed)
-        k_constr<float><<<gc, 256, 0, s>>>(ca);
-      else
-        k_constr<double><<<gc, 256, 0, s>>>(ca);
+      if (planar) {
+        if (mixed)
+          k_constr<float, true><<<gc, 256, 0, s>>>(ca);
+        else
+          k_constr<double, true><<<gc, 256, 0, s>>>(ca);
+      } else {
+        if (mixed)
+          k_constr<float, false><<<gc, 256, 0, s>>>(ca);
+        else
+          k_constr<double, false><<<gc, 256, 0, s>>>(ca);
+      }
     }
     k_finish<<<1, 256, 0, s>>>(ca);
   };
@@ -678,6 +842,7 @@ pcc_status bbpgd_solve(pcc_ctx* ctx, int64_t* iters, double* res) {
 
 // Workspace of the converged iterate, move, write-back, constraint update.
 void integrate_hard(pcc_ctx* ctx, int64_t kf) {
+  (void)kf;
   const int64_t N = ctx->N, C = ctx->S.C;
   const bool mixed = ctx->params.precision == PCC_PREC_MIXED;
   hipStream_t s = ctx->stream;
@@ -685,16 +850,23 @@ void integrate_hard(pcc_ctx* ctx, int64_t kf) {
   KTimer t(ctx, PCC_K_INTEGRATE, 0.0);
   if (N > 0) {
     BodyArgs ba = body_args(ctx);
-    if (mixed)
-      k_body_final<float><<<nblk(N), 256, 0, s>>>(ba, kf, ctx->F.p, ctx->U.p, ctx->sig.p,
-                                                  ctx->imp.p, ctx->ldot_new.p);
-    else
-      k_body_final<double><<<nblk(N), 256, 0, s>>>(ba, kf, ctx->F.p, ctx->U.p, ctx->sig.p,
-                                                   ctx->imp.p, ctx->ldot_new.p);
-    k_integrate<<<nblk(N), 256, 0, s>>>(N, b, ctx->params.dt, ctx->U.p, ctx->ldot_new.p, b.ldot);
+    const FinalOut fo = {ctx->F.p, ctx->U.p, ctx->sig.p, ctx->imp.p, ctx->ldot_new.p};
+    const unsigned g = nblk(N);
+    if (ctx->planar) {
+      if (mixed)
+        k_body<float, true, true><<<g, 256, 0, s>>>(ba, fo);
+      else
+        k_body<double, true, true><<<g, 256, 0, s>>>(ba, fo);
+    } else {
+      if (mixed)
+        k_body<float, false, true><<<g, 256, 0, s>>>(ba, fo);
+      else
+        k_body<double, false, true><<<g, 256, 0, s>>>(ba, fo);
+    }
+    k_integrate<<<g, 256, 0, s>>>(N, b, ctx->params.dt, ctx->U.p, ctx->ldot_new.p, b.ldot);
   }
   if (C > 0)
-    k_update_constraints<<<nblk(C), 256, 0, s>>>(C, kf, ctx->iter.p, ctx->S.gamma.p,
+    k_update_constraints<<<nblk(C), 256, 0, s>>>(C, ctx->iter.p, ctx->gprev.p, ctx->S.gamma.p,
                                                  ctx->S.phi.p);
 }
 

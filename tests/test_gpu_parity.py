@@ -198,3 +198,25 @@ def test_repeat_runs_bitwise_identical():
     assert_same_state(pm.get_state(), s1)
     assert_same_constraints(pm.constraints(), c1)
     pm.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_planar_kernels_equal_3d_kernels(precision, monkeypatch):
+    """The 2D monolayer kernels skip terms that are exact zeros: the results
+    equal the 3D kernels' (forced with PCC_PLANAR=0) value for value."""
+    from pcc.engine import ParticleManager
+    p = reference_params(dt=1e-3, tolerance=1e-6, tau=1 / LN2, cutoff_phase0=0.05,
+                         cutoff_phaseK=0.05, precision=precision)
+    b = G.dense_colony(4000, seed=15)
+    pm2 = ParticleManager(p)
+    pm2.set_state(b)
+    r2 = pm2.step()
+    monkeypatch.setenv("PCC_PLANAR", "0")
+    pm3 = ParticleManager(p)
+    pm3.set_state(b)
+    r3 = pm3.step()
+    assert r2 == r3
+    assert_same_state(pm2.get_state(), pm3.get_state())
+    assert_same_constraints(pm2.constraints(), pm3.constraints())
+    pm2.close()
+    pm3.close()
