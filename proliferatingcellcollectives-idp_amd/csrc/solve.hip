@@ -6,13 +6,16 @@
 // ParticleManager.cpp:29-137, Particle.cpp:37-65,110-120.
 //
 // One BBPGD iteration is three launches:
-//   k_body    one thread per body: the projection step
-//             gamma = max(gamma_p - alpha g_p, 0) evaluated on the fly for each
-//             incident constraint, then the segmented D^T (gamma - gamma_old)
-//             and L^T gamma sums in ascending constraint order (PETSc's
-//             MatMultTranspose row order), mobility, impedance exp(-lambda
-//             sigma), growth rate; writes U and ldot - ldot_prev per body.
-//             No float atomics.
+//   k_body_coop  one block per 256 consecutive bodies: the block walks their
+//             incidence range in LDS chunks; all lanes evaluate the projection
+//             step gamma = max(gamma_p - alpha g_p, 0) of one incidence each
+//             (independent gathers) and store the products with the D / L
+//             coefficients; then each thread adds its own body's products in
+//             ascending constraint order (PETSc's MatMultTranspose row order),
+//             applies mobility, impedance exp(-lambda sigma) and the growth
+//             rate, and writes U and ldot - ldot_prev.  No float atomics.
+//             (k_body, one thread per body, is the same sum and is used for
+//             the final write-back of F, U, sigma.)
 //   k_constr  the same projection per constraint, D U and L dldot gathers,
 //             g = Phi + dt*phidot - dt*phidot_growth, BB dot-product and
 //             residual partials per block (fixed tile->block assignment and
@@ -331,6 +334,103 @@ __global__ void __launch_bounds__(256) k_body(BodyArgs a, FinalOut fo) {
     fo.imp[i] = imp;
     fo.ld[i] = ld;
   } else if (PLANAR) {
+    double2* o = reinterpret_cast<double2*>(a.ubuf + 4 * i);
+    o[0] = make_double2(u[0], u[1]);
+    o[1] = make_double2(u[5], ld - a.ldot_prev[i]);
+  } else {
+    double2* o = reinterpret_cast<double2*>(a.ubuf + 8 * i);
+    o[0] = make_double2(u[0], u[1]);
+    o[1] = make_double2(u[2], u[3]);
+    o[2] = make_double2(u[4], u[5]);
+    o[3] = make_double2(ld - a.ldot_prev[i], 0.0);
+  }
+}
+
+// Body apply of one BBPGD iteration, block-cooperative: a block owns 256
+// consecutive bodies and walks their incidence range in chunks.  Phase 1: every
+// lane takes incidences of the chunk in turn, gathers the constraint's
+// iterate record (independent loads, full memory-level parallelism) and
+// writes the products dg*coef and gamma*s to LDS.  Phase 2: each thread adds
+// the products of its own body in ascending incidence order, so every sum has
+// exactly the order of the thread-per-body loop above.
+template <class CT, bool PLANAR>
+__global__ void __launch_bounds__(256) k_body_coop(BodyArgs a) {
+  constexpr int NC = PLANAR ? 4 : 7;           // products per incidence
+  constexpr int CHUNK = PLANAR ? 1024 : 512;   // incidences staged per pass
+  __shared__ double prod[NC][CHUNK];
+  const SolverState* st = a.st;
+  if (st->converged | st->failed) return;
+  const int64_t b0 = xcd_block(blockIdx.x, a.gp) * PCC_BLOCK;
+  if (b0 >= a.N) return;
+  const int64_t b1 = b0 + PCC_BLOCK < a.N ? b0 + PCC_BLOCK : a.N;
+  const int64_t i = b0 + threadIdx.x;
+  const bool active = i < a.N;
+  const int64_t k = st->iter;
+  const double malpha = -st->alpha;
+  const int32_t E0 = a.rowptr[b0], E1 = a.rowptr[b1];
+  int32_t e_my = active ? a.rowptr[i] : 0;
+  const int32_t e_end = active ? a.rowptr[i + 1] : 0;
+  double F[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double sig = 0.0;
+  for (int32_t base = E0; base < E1; base += CHUNK) {
+    const int32_t lim = base + CHUNK < E1 ? base + CHUNK : E1;
+    for (int32_t e = base + (int32_t)threadIdx.x; e < lim; e += PCC_BLOCK) {
+      double co[7];
+      int64_t c;
+      if (PLANAR) {
+        double p[4];
+        load_prec<CT>(a.rec, e, p);
+        co[0] = p[0];
+        co[1] = p[1];
+        co[5] = p[2];
+        co[6] = p[3];
+        c = a.rc[e];
+      } else {
+        load_rec<CT>(a.rec, e, co, c);
+      }
+      const double2* r = reinterpret_cast<const double2*>(a.it + IT_LEN * c);
+      const double2 yg = r[0];
+      const double gold = r[1].x;
+      const double gc = (k == 0) ? yg.x : proj_step(yg.x, malpha, yg.y);
+      const double dg = gc - gold;
+      const int q = e - base;
+      if (PLANAR) {
+        prod[0][q] = dg * co[0];
+        prod[1][q] = dg * co[1];
+        prod[2][q] = dg * co[5];
+        prod[3][q] = gc * co[6];
+      } else {
+#pragma unroll
+        for (int m = 0; m < 6; ++m) prod[m][q] = dg * co[m];
+        prod[6][q] = gc * co[6];
+      }
+    }
+    __syncthreads();
+    for (; e_my < e_end && e_my < lim; ++e_my) {
+      const int q = e_my - base;
+      if (PLANAR) {
+        F[0] += prod[0][q];
+        F[1] += prod[1][q];
+        F[5] += prod[2][q];
+        sig += prod[3][q];
+      } else {
+#pragma unroll
+        for (int m = 0; m < 6; ++m) F[m] += prod[m][q];
+        sig += prod[6][q];
+      }
+    }
+    __syncthreads();
+  }
+  if (!active) return;
+  const double mt = a.mob[2 * i], mr = a.mob[2 * i + 1];
+  double u[6];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) u[q] = (0.0 + mt * F[q]) + 0.0;
+#pragma unroll
+  for (int q = 3; q < 6; ++q) u[q] = (0.0 + mr * F[q]) + 0.0;
+  const double imp = det_exp(sig * a.mlam);
+  const double ld = (a.len[i] * imp) * a.itau;
+  if (PLANAR) {
     double2* o = reinterpret_cast<double2*>(a.ubuf + 4 * i);
     o[0] = make_double2(u[0], u[1]);
     o[1] = make_double2(u[5], ld - a.ldot_prev[i]);
@@ -779,20 +879,19 @@ pcc_status bbpgd_solve(pcc_ctx* ctx, int64_t* iters, double* res) {
   ConstrArgs ca = constr_args(ctx);
   const unsigned gb = (unsigned)ba.gp, gc = (unsigned)ca.gp;
   const double bb = body_bytes(N, C, mixed, planar), cb = constr_bytes(N, C, mixed, planar);
-  const FinalOut none = {};
   auto one_iteration = [&]() {
     {
       KTimer t(ctx, PCC_K_BODY_APPLY, bb);
       if (planar) {
         if (mixed)
-          k_body<float, true, false><<<gb, 256, 0, s>>>(ba, none);
+          k_body_coop<float, true><<<gb, 256, 0, s>>>(ba);
         else
-          k_body<double, true, false><<<gb, 256, 0, s>>>(ba, none);
+          k_body_coop<double, true><<<gb, 256, 0, s>>>(ba);
       } else {
         if (mixed)
-          k_body<float, false, false><<<gb, 256, 0, s>>>(ba, none);
+          k_body_coop<float, false><<<gb, 256, 0, s>>>(ba);
         else
-          k_body<double, false, false><<<gb, 256, 0, s>>>(ba, none);
+          k_body_coop<double, false><<<gb, 256, 0, s>>>(ba);
       }
     }
     {
