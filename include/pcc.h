@@ -180,6 +180,44 @@ pcc_status pcc_detect(pcc_ctx* ctx, double cutoff, pcc_constraints* out, int64_t
 pcc_status pcc_get_phase_stats(pcc_ctx* ctx, int64_t* bbpgd_iterations, int64_t* n_constraints,
                                int64_t max, int64_t* count);
 
+/* ---------------------------------------------------------------------------
+ * Batched ensembles: one context holds many independent colonies (same mode
+ * and precision; every other parameter per colony).  A batched step runs all
+ * colonies' ReLCP loops in lockstep with shared launches: every BBPGD
+ * iteration is one launch sequence over all colonies still solving, with a
+ * per-colony step size, iteration count and convergence flag on the device
+ * (a converged colony's blocks exit at once).  Each colony's results are the
+ * ones a context holding it alone produces, bit for bit.  Colonies are
+ * indexed 0..n-1 in the order they were set or added; removing one shifts the
+ * later ones down.  The single-colony entry points above require a context
+ * with exactly one colony.  No reference counterpart: the reference runs one
+ * colony per MPI job (Domain::run, Domain.cpp:121-143).
+ * ------------------------------------------------------------------------- */
+
+/* Replaces the context's colonies: bodies[q] and, if params is not NULL,
+ * params[q] (else the context's parameters) for q < n_colonies. */
+pcc_status pcc_batch_set_state(pcc_ctx* ctx, int64_t n_colonies, const pcc_bodies* bodies,
+                               const pcc_params* params);
+/* Appends one colony (state stays resident; *colony receives its index). */
+pcc_status pcc_batch_add(pcc_ctx* ctx, const pcc_bodies* bodies, const pcc_params* params,
+                         int64_t* colony);
+/* Removes colony `colony`; later colonies move down by one. */
+pcc_status pcc_batch_remove(pcc_ctx* ctx, int64_t colony);
+pcc_status pcc_batch_set_params(pcc_ctx* ctx, int64_t colony, const pcc_params* params);
+pcc_status pcc_batch_size(pcc_ctx* ctx, int64_t* n_colonies, int64_t* n_bodies);
+/* One timestep of every colony.  results[q] and statuses[q] (may be NULL)
+ * receive each colony's SolverSolution summary and solver status (the three
+ * solver failures of the reference are per colony; the call itself returns
+ * PCC_OK unless the device or the arguments fail). */
+pcc_status pcc_batch_step(pcc_ctx* ctx, int32_t iter, pcc_step_result* results,
+                          pcc_status* statuses);
+pcc_status pcc_batch_get_state(pcc_ctx* ctx, int64_t colony, pcc_bodies* bodies);
+/* Colony `colony`'s constraints; gidI / gidJ are indices within the colony. */
+pcc_status pcc_batch_get_constraints(pcc_ctx* ctx, int64_t colony, pcc_constraints* out,
+                                     int64_t* count);
+pcc_status pcc_batch_get_phase_stats(pcc_ctx* ctx, int64_t colony, int64_t* bbpgd_iterations,
+                                     int64_t* n_constraints, int64_t max, int64_t* count);
+
 /* Per-kernel GPU time accounting with HIP events on the context stream.
  * Kernel ids: see pcc_kernel_name(). */
 enum {

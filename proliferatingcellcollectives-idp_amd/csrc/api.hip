@@ -4,7 +4,13 @@
 // dispatches to the hard-model ReLCP loop (solveHardModel, Hard.cpp:91-193) or
 // the soft model (solveSoftPotential, Soft.cpp:21-114).  All device work runs
 // on the context's stream; the host only reads the handful of scalars that
-// steer the ReLCP loop (constraint count, max overlap, BBPGD flags).
+// steer the ReLCP loop (constraint counts, max overlaps, BBPGD flags).
+//
+// A context holds one colony (pcc_set_state) or a batch of independent
+// colonies (pcc_batch_*).  A batched step runs every colony's ReLCP loop in
+// lockstep: phase k detects, assembles and solves all colonies still in their
+// loop with the same launches; a colony leaves the loop exactly where it
+// would alone, and its results are the ones it would get alone.
 #include <stdlib.h>
 #include <string.h>
 
@@ -18,7 +24,6 @@
 
 namespace pcc {
 
-void detect_batch(pcc_ctx* ctx, double cutoff, int32_t iteration);
 void batch_to_host(pcc_ctx* ctx, pcc_constraints* out, int32_t iteration);
 void build_grid(pcc_ctx* ctx);
 
@@ -81,6 +86,52 @@ void reset_step(pcc_ctx* ctx) {
   BodyArrays b = body_arrays(ctx->body.p, N);
   if (N > 0) k_reset<<<(unsigned)((N + 255) / 256), 256, 0, ctx->stream>>>(N, b.ldot, ctx->nc.p);
   ctx->S.C = 0;
+  for (auto& I : ctx->inst) {
+    I.coff = 0;
+    I.C = 0;
+    I.m = 0;
+    I.bstart = 0;
+  }
+}
+
+// Instance records to the device (synchronises the stream, so the host copy
+// may change right after).
+void upload_instances(pcc_ctx* ctx) {
+  const int64_t B = (int64_t)ctx->inst.size();
+  ctx->inst_host.resize(B);
+  for (int64_t q = 0; q < B; ++q) {
+    const Instance& I = ctx->inst[q];
+    const pcc_params& p = I.p;
+    InstDev& d = ctx->inst_host[q];
+    d.boff = I.boff;
+    d.N = I.N;
+    d.coff = I.coff;
+    d.C = I.C;
+    d.G = I.G;
+    d.K = I.K;
+    d.lb = I.lb;
+    d.bb = I.bb;
+    d.bstart = I.bstart;
+    d.coff_new = I.coff_new_;
+    d.maxit = p.max_bbpgd_iterations;
+    d.dt = p.dt;
+    d.xi = p.xi;
+    d.itau = 1 / p.tau;
+    d.mlam = -p.lambda;
+    d.kcc = p.kcc;
+    d.tol = p.bbpgd_tol;
+    d.cutoff = I.cutoff;
+    d.warm = p.warm_start_factor;
+    d.origin = p.grid_origin;
+    d.h = p.linked_cell_size;
+    d.flags = I.flags;
+    d.pad = 0;
+  }
+  if (B == 0) return;
+  ctx->idev.ensure(B);
+  PCC_HIP(hipMemcpyAsync(ctx->idev.p, ctx->inst_host.data(), sizeof(InstDev) * B,
+                         hipMemcpyHostToDevice, ctx->stream));
+  PCC_HIP(hipStreamSynchronize(ctx->stream));
 }
 
 }  // namespace pcc
@@ -128,90 +179,349 @@ pcc_status check_params(const pcc_params* p, std::string* why) {
   return PCC_OK;
 }
 
-void copy_set_to_host(pcc_ctx* ctx, pcc_constraints* out) {
+// Copies colony q's constraints (its range of the colony-major set) to host
+// arrays, with body indices local to the colony.
+void copy_set_to_host(pcc_ctx* ctx, int64_t q, pcc_constraints* out) {
   const ConstraintSet& S = ctx->S;
-  const int64_t m = std::min<int64_t>(S.C, out->capacity);
+  const Instance& I = ctx->inst[q];
+  const int64_t m = std::min<int64_t>(I.C, out->capacity);
   if (m <= 0) return;
+  const int64_t o = I.coff;
   hipStream_t s = ctx->stream;
   auto cp = [&](void* dst, const void* src, size_t bytes) {
     if (dst) PCC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
   };
-  cp(out->gidI, S.I.p, sizeof(int32_t) * m);
-  cp(out->gidJ, S.J.p, sizeof(int32_t) * m);
-  cp(out->signed_distance, S.phi.p, sizeof(double) * m);
-  cp(out->normal, S.n.p, sizeof(double) * 3 * m);
-  cp(out->rPosI, S.rI.p, sizeof(double) * 3 * m);
-  cp(out->rPosJ, S.rJ.p, sizeof(double) * 3 * m);
-  cp(out->contact_point, S.p.p, sizeof(double) * 3 * m);
-  cp(out->stressI, S.sI.p, sizeof(double) * m);
-  cp(out->stressJ, S.sJ.p, sizeof(double) * m);
-  cp(out->gamma, S.gamma.p, sizeof(double) * m);
-  cp(out->gid, S.gid.p, sizeof(int32_t) * m);
-  cp(out->iteration, S.iter.p, sizeof(int32_t) * m);
+  cp(out->gidI, S.I.p + o, sizeof(int32_t) * m);
+  cp(out->gidJ, S.J.p + o, sizeof(int32_t) * m);
+  cp(out->signed_distance, S.phi.p + o, sizeof(double) * m);
+  cp(out->normal, S.n.p + 3 * o, sizeof(double) * 3 * m);
+  cp(out->rPosI, S.rI.p + 3 * o, sizeof(double) * 3 * m);
+  cp(out->rPosJ, S.rJ.p + 3 * o, sizeof(double) * 3 * m);
+  cp(out->contact_point, S.p.p + 3 * o, sizeof(double) * 3 * m);
+  cp(out->stressI, S.sI.p + o, sizeof(double) * m);
+  cp(out->stressJ, S.sJ.p + o, sizeof(double) * m);
+  cp(out->gamma, S.gamma.p + o, sizeof(double) * m);
+  cp(out->gid, S.gid.p + o, sizeof(int32_t) * m);
+  cp(out->iteration, S.iter.p + o, sizeof(int32_t) * m);
   PCC_HIP(hipStreamSynchronize(s));
+  const int32_t bo = (int32_t)I.boff;
+  if (bo != 0) {
+    for (int64_t c = 0; c < m; ++c) {
+      if (out->gidI) out->gidI[c] -= bo;
+      if (out->gidJ) out->gidJ[c] -= bo;
+    }
+  }
 }
 
-pcc_status hard_step(pcc_ctx* ctx, pcc_step_result* out) {
-  const pcc_params& p = ctx->params;
-  int32_t k = 0;
-  int64_t total_it = 0;
-  double res = 0.0, maxov = 0.0;
+void fill_result(const Instance& I, pcc_step_result* r) {
+  r->n_constraints = I.C;
+  r->constraint_iterations = I.k;
+  r->reserved = 0;
+  r->bbpgd_iterations = I.its;
+  r->residual = I.res;
+  r->max_overlap = I.maxov;
+}
+
+void begin_step(pcc_ctx* ctx) {
   reset_step(ctx);
-  ctx->planar = ctx->allow_planar && bodies_planar(ctx);
-  ctx->phase_iters.clear();
-  ctx->phase_C.clear();
-  pcc_status st = PCC_OK;
-  while (k < p.max_recursive_iterations) {  // Hard.cpp:113
-    const double cutoff = k == 0 ? p.cutoff_phase0 : p.cutoff_phaseK;  // Hard.cpp:121
-    detect_batch(ctx, cutoff, k);
-    append_batch(ctx, k);  // Hard.cpp:123
-    maxov = max_overlap(ctx);  // Hard.cpp:125-130
-    if (maxov <= p.relcp_tol && k > 0) break;  // Hard.cpp:132-134
-    build_incidence(ctx, true);  // createMatrices, Hard.cpp:136
-    int64_t its = 0;
-    st = bbpgd_solve(ctx, &its, &res);  // Hard.cpp:153
-    total_it += its;
-    ctx->phase_iters.push_back(its);
-    ctx->phase_C.push_back(ctx->S.C);
-    if (st != PCC_OK) {
-      ctx->last_error = "BBPGD did not converge";
-      break;
-    }
-    integrate_hard(ctx, its);  // Hard.cpp:160-170
-    ++k;
+  for (auto& I : ctx->inst) {
+    I.k = 0;
+    I.its = 0;
+    I.res = 0.0;
+    I.maxov = 0.0;
+    I.status = PCC_OK;
+    I.error.clear();
+    I.phase_iters.clear();
+    I.phase_C.clear();
+    I.active = true;
+    I.flags = kInstActive;
   }
-  if (st == PCC_OK) {
-    grow(ctx);  // Hard.cpp:185
-    if (k == p.max_recursive_iterations) {  // Hard.cpp:187-190
-      st = PCC_E_RELCP_MAX_ITERS;
-      ctx->last_error = "Maximum number of constraint iterations reached";
+}
+
+// solveHardModel (Hard.cpp:91-193) for every colony of the context.
+void hard_step(pcc_ctx* ctx) {
+  begin_step(ctx);
+  ctx->planar = ctx->allow_planar && bodies_planar(ctx);
+  std::vector<double> mo;
+  for (int32_t k = 0;; ++k) {
+    bool any = false;
+    for (auto& I : ctx->inst) {
+      if (I.active && k >= I.p.max_recursive_iterations) I.active = false;  // Hard.cpp:113
+      I.flags = I.active ? kInstActive : 0;
+      I.cutoff = k == 0 ? I.p.cutoff_phase0 : I.p.cutoff_phaseK;  // Hard.cpp:121
+      any = any || I.active;
+    }
+    if (!any) break;
+    upload_instances(ctx);
+    detect_batch(ctx);
+    append_batch(ctx, k);  // Hard.cpp:123
+    max_overlap_inst(ctx, mo);  // Hard.cpp:125-130
+    any = false;
+    for (size_t q = 0; q < ctx->inst.size(); ++q) {
+      Instance& I = ctx->inst[q];
+      if (!I.active) continue;
+      I.maxov = mo[q];
+      if (I.maxov <= I.p.relcp_tol && k > 0) I.active = false;  // Hard.cpp:132-134
+      I.flags = I.active ? kInstActive : 0;
+      any = any || I.active;
+    }
+    if (!any) break;
+    build_incidence(ctx, true);  // createMatrices, Hard.cpp:136
+    bbpgd_solve(ctx);            // Hard.cpp:153
+    for (auto& I : ctx->inst) {
+      if (!I.active) continue;
+      I.its += I.solve_its;
+      I.res = I.solve_res;
+      I.phase_iters.push_back(I.solve_its);
+      I.phase_C.push_back(I.C);
+      if (I.solve_failed) {
+        I.status = PCC_E_BBPGD_NOT_CONVERGED;
+        I.error = "BBPGD did not converge";
+        I.active = false;
+        I.flags = 0;
+      } else {
+        I.flags = kInstActive | kInstInteg;
+      }
+    }
+    upload_instances(ctx);
+    integrate_hard(ctx);  // Hard.cpp:160-170
+    for (auto& I : ctx->inst)
+      if (I.active) I.k = k + 1;
+  }
+  for (auto& I : ctx->inst) I.flags = I.status == PCC_OK ? kInstGrow : 0;
+  upload_instances(ctx);
+  grow(ctx);  // Hard.cpp:185
+  for (auto& I : ctx->inst) {
+    if (I.status == PCC_OK && I.k == I.p.max_recursive_iterations) {  // Hard.cpp:187-190
+      I.status = PCC_E_RELCP_MAX_ITERS;
+      I.error = "Maximum number of constraint iterations reached";
     }
   }
   PCC_HIP(hipStreamSynchronize(ctx->stream));
-  out->n_constraints = ctx->S.C;
-  out->constraint_iterations = k;
-  out->reserved = 0;
-  out->bbpgd_iterations = total_it;
-  out->residual = res;
-  out->max_overlap = maxov;
-  return st;
 }
 
-pcc_status soft_step(pcc_ctx* ctx, pcc_step_result* out) {
-  reset_step(ctx);
-  detect_batch(ctx, 0.0, 0);  // Soft.cpp:35
+// solveSoftPotential (Soft.cpp:21-114) for every colony of the context.
+void soft_step(pcc_ctx* ctx) {
+  begin_step(ctx);
+  for (auto& I : ctx->inst) I.cutoff = 0.0;  // Soft.cpp:35
+  upload_instances(ctx);
+  detect_batch(ctx);
   append_batch(ctx, 0);
-  const double maxov = max_overlap(ctx);  // Soft.cpp:37-41
+  std::vector<double> mo;
+  max_overlap_inst(ctx, mo);  // Soft.cpp:37-41
+  for (size_t q = 0; q < ctx->inst.size(); ++q) {
+    ctx->inst[q].maxov = mo[q];
+    ctx->inst[q].res = mo[q];
+  }
   build_incidence(ctx, false);
   soft_forces_and_integrate(ctx);
   PCC_HIP(hipStreamSynchronize(ctx->stream));
-  out->n_constraints = ctx->S.C;
-  out->constraint_iterations = 0;
-  out->reserved = 0;
-  out->bbpgd_iterations = 0;
-  out->residual = maxov;
-  out->max_overlap = maxov;
+}
+
+pcc_status run_step(pcc_ctx* ctx) {
+  if (ctx->params.mode == PCC_MODE_HARD)
+    hard_step(ctx);
+  else
+    soft_step(ctx);
   return PCC_OK;
+}
+
+pcc_status check_colony_params(pcc_ctx* ctx, const pcc_params* p, std::string* why) {
+  pcc_status s = check_params(p, why);
+  if (s != PCC_OK) return s;
+  if (p->mode != ctx->params.mode || p->precision != ctx->params.precision) {
+    *why = "a colony's mode and precision must be the context's";
+    return PCC_E_INVALID_ARGUMENT;
+  }
+  return PCC_OK;
+}
+
+pcc_status check_bodies(pcc_ctx* ctx, const pcc_bodies* b) {
+  if (!b || b->n < 0) return fail(ctx, PCC_E_INVALID_ARGUMENT, "invalid bodies");
+  if (b->n > 0 && (!b->pos || !b->quat || !b->length || !b->l0 || !b->diameter || !b->ldot))
+    return fail(ctx, PCC_E_INVALID_ARGUMENT, "missing body field");
+  return PCC_OK;
+}
+
+// Device arrays sized for N bodies in total (body state, outputs, colony map).
+void ensure_body_arrays(pcc_ctx* ctx, int64_t N) {
+  ctx->body.ensure(kBodyFields * N + 1);
+  ctx->F.ensure(6 * N + 6);
+  ctx->U.ensure(6 * N + 6);
+  ctx->sig.ensure(N + 1);
+  ctx->imp.ensure(N + 1);
+  ctx->ldot_new.ensure(N + 1);
+  ctx->nc.ensure(N + 1);
+  ctx->binst.ensure(N + 1);
+}
+
+// Colony of every body, from the colony ranges.
+void upload_binst(pcc_ctx* ctx) {
+  std::vector<int32_t> h(ctx->N);
+  for (size_t q = 0; q < ctx->inst.size(); ++q)
+    for (int64_t i = 0; i < ctx->inst[q].N; ++i) h[ctx->inst[q].boff + i] = (int32_t)q;
+  if (ctx->N > 0)
+    PCC_HIP(hipMemcpyAsync(ctx->binst.p, h.data(), sizeof(int32_t) * ctx->N, hipMemcpyHostToDevice,
+                           ctx->stream));
+  PCC_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+// Packs host bodies (interleaved vectors) into the field-major layout of a
+// context holding Ntot bodies, at body offset boff.
+void pack_bodies(const pcc_bodies* b, int64_t Ntot, int64_t boff, double* h) {
+  for (int64_t i = 0; i < b->n; ++i) {
+    const int64_t j = boff + i;
+    for (int k = 0; k < 3; ++k) h[(size_t)k * Ntot + j] = b->pos[3 * i + k];
+    for (int k = 0; k < 4; ++k) h[(size_t)(3 + k) * Ntot + j] = b->quat[4 * i + k];
+    h[(size_t)7 * Ntot + j] = b->length[i];
+    h[(size_t)8 * Ntot + j] = b->l0[i];
+    h[(size_t)9 * Ntot + j] = b->diameter[i];
+    h[(size_t)10 * Ntot + j] = b->ldot[i];
+  }
+}
+
+// Replaces the context's colonies (state uploaded, outputs zeroed).
+pcc_status set_colonies(pcc_ctx* ctx, int64_t B, const pcc_bodies* bodies,
+                        const pcc_params* params) {
+  if (B < 1) return fail(ctx, PCC_E_INVALID_ARGUMENT, "no colony");
+  int64_t Ntot = 0;
+  for (int64_t q = 0; q < B; ++q) {
+    pcc_status st = check_bodies(ctx, &bodies[q]);
+    if (st != PCC_OK) return st;
+    std::string why;
+    if (params) {
+      st = check_colony_params(ctx, &params[q], &why);
+      if (st != PCC_OK) return fail(ctx, st, why);
+    }
+    Ntot += bodies[q].n;
+  }
+  if (Ntot >= (int64_t)1 << 30) return fail(ctx, PCC_E_INVALID_ARGUMENT, "too many bodies");
+  return guarded(ctx, [&]() {
+    PCC_HIP(hipSetDevice(ctx->device));
+    std::vector<double> h((size_t)kBodyFields * Ntot);
+    ctx->inst.assign(B, Instance());
+    int64_t off = 0;
+    for (int64_t q = 0; q < B; ++q) {
+      Instance& I = ctx->inst[q];
+      I.p = params ? params[q] : ctx->params;
+      I.boff = off;
+      I.N = bodies[q].n;
+      pack_bodies(&bodies[q], Ntot, off, h.data());
+      off += I.N;
+    }
+    ctx->N = Ntot;
+    ensure_body_arrays(ctx, Ntot);
+    hipStream_t s = ctx->stream;
+    if (Ntot > 0)
+      PCC_HIP(hipMemcpyAsync(ctx->body.p, h.data(), sizeof(double) * h.size(),
+                             hipMemcpyHostToDevice, s));
+    PCC_HIP(hipMemsetAsync(ctx->F.p, 0, sizeof(double) * (6 * Ntot + 6), s));
+    PCC_HIP(hipMemsetAsync(ctx->U.p, 0, sizeof(double) * (6 * Ntot + 6), s));
+    PCC_HIP(hipMemsetAsync(ctx->sig.p, 0, sizeof(double) * (Ntot + 1), s));
+    PCC_HIP(hipMemsetAsync(ctx->imp.p, 0, sizeof(double) * (Ntot + 1), s));
+    PCC_HIP(hipMemsetAsync(ctx->nc.p, 0, sizeof(int32_t) * (Ntot + 1), s));
+    upload_binst(ctx);
+    upload_instances(ctx);
+    ctx->S.C = 0;
+    ctx->has_state = true;
+    ctx->has_snapshot = false;
+    return PCC_OK;
+  });
+}
+
+// Downloads colony q's state into host bodies (gid order within the colony).
+pcc_status get_colony_state(pcc_ctx* ctx, int64_t q, pcc_bodies* b) {
+  const Instance& I = ctx->inst[q];
+  if (b->n != I.N) return fail(ctx, PCC_E_INVALID_ARGUMENT, "body count mismatch");
+  return guarded(ctx, [&]() {
+    PCC_HIP(hipSetDevice(ctx->device));
+    const int64_t N = I.N, Nt = ctx->N, o = I.boff;
+    std::vector<double> h((size_t)kBodyFields * N), F(6 * N), U(6 * N), sg(N), im(N);
+    std::vector<int32_t> nc(N);
+    hipStream_t s = ctx->stream;
+    if (N > 0) {
+      for (int k = 0; k < kBodyFields; ++k)
+        PCC_HIP(hipMemcpyAsync(h.data() + (size_t)k * N, ctx->body.p + (size_t)k * Nt + o,
+                               sizeof(double) * N, hipMemcpyDeviceToHost, s));
+      PCC_HIP(hipMemcpyAsync(F.data(), ctx->F.p + 6 * o, sizeof(double) * 6 * N, hipMemcpyDeviceToHost, s));
+      PCC_HIP(hipMemcpyAsync(U.data(), ctx->U.p + 6 * o, sizeof(double) * 6 * N, hipMemcpyDeviceToHost, s));
+      PCC_HIP(hipMemcpyAsync(sg.data(), ctx->sig.p + o, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+      PCC_HIP(hipMemcpyAsync(im.data(), ctx->imp.p + o, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+      PCC_HIP(hipMemcpyAsync(nc.data(), ctx->nc.p + o, sizeof(int32_t) * N, hipMemcpyDeviceToHost, s));
+    }
+    PCC_HIP(hipStreamSynchronize(s));
+    for (int64_t i = 0; i < N; ++i) {
+      if (b->pos)
+        for (int k = 0; k < 3; ++k) b->pos[3 * i + k] = h[(size_t)k * N + i];
+      if (b->quat)
+        for (int k = 0; k < 4; ++k) b->quat[4 * i + k] = h[(size_t)(3 + k) * N + i];
+      if (b->length) b->length[i] = h[(size_t)7 * N + i];
+      if (b->l0) b->l0[i] = h[(size_t)8 * N + i];
+      if (b->diameter) b->diameter[i] = h[(size_t)9 * N + i];
+      if (b->ldot) b->ldot[i] = h[(size_t)10 * N + i];
+      for (int k = 0; k < 3; ++k) {
+        if (b->force) b->force[3 * i + k] = F[6 * i + k];
+        if (b->torque) b->torque[3 * i + k] = F[6 * i + 3 + k];
+        if (b->velocity) b->velocity[3 * i + k] = U[6 * i + k];
+        if (b->omega) b->omega[3 * i + k] = U[6 * i + 3 + k];
+      }
+      if (b->impedance) b->impedance[i] = im[i];
+      if (b->stress) b->stress[i] = sg[i];
+      if (b->num_constraints) b->num_constraints[i] = nc[i];
+    }
+    return PCC_OK;
+  });
+}
+
+// Moves body ranges (src, dst, count) of the current arrays into fresh arrays
+// for newN bodies; the rest of the fresh arrays is zero.
+struct Move {
+  int64_t src, dst, count;
+};
+
+template <class T>
+void move_rows(pcc_ctx* ctx, DArr<T>& a, int64_t oldN, int64_t newN, int width, bool field_major,
+               const std::vector<Move>& mv) {
+  DArr<T> fresh;
+  const int64_t rows = field_major ? width : 1;
+  const int64_t per = field_major ? 1 : width;
+  fresh.ensure(rows * per * newN + per + 1);
+  hipStream_t s = ctx->stream;
+  PCC_HIP(hipMemsetAsync(fresh.p, 0, sizeof(T) * (size_t)(rows * per * newN + per + 1), s));
+  for (int64_t r = 0; r < rows; ++r)
+    for (const Move& m : mv)
+      if (m.count > 0)
+        PCC_HIP(hipMemcpyAsync(fresh.p + r * newN * per + m.dst * per,
+                               a.p + r * oldN * per + m.src * per, sizeof(T) * m.count * per,
+                               hipMemcpyDeviceToDevice, s));
+  PCC_HIP(hipStreamSynchronize(s));
+  std::swap(a.p, fresh.p);
+  std::swap(a.cap, fresh.cap);
+}
+
+void repack(pcc_ctx* ctx, int64_t newN, const std::vector<Move>& mv) {
+  const int64_t oldN = ctx->N;
+  move_rows(ctx, ctx->body, oldN, newN, kBodyFields, true, mv);
+  move_rows(ctx, ctx->F, oldN, newN, 6, false, mv);
+  move_rows(ctx, ctx->U, oldN, newN, 6, false, mv);
+  move_rows(ctx, ctx->sig, oldN, newN, 1, false, mv);
+  move_rows(ctx, ctx->imp, oldN, newN, 1, false, mv);
+  move_rows(ctx, ctx->ldot_new, oldN, newN, 1, false, mv);
+  move_rows(ctx, ctx->nc, oldN, newN, 1, false, mv);
+  ctx->binst.release();
+  ctx->binst.ensure(newN + 1);
+  ctx->N = newN;
+  ctx->S.C = 0;
+  ctx->has_snapshot = false;
+  int64_t off = 0;
+  for (auto& I : ctx->inst) {
+    I.boff = off;
+    off += I.N;
+    I.C = 0;
+    I.coff = 0;
+  }
+  upload_binst(ctx);
+  upload_instances(ctx);
 }
 
 }  // namespace
@@ -269,10 +579,8 @@ pcc_status pcc_create(const pcc_params* params, int device, pcc_ctx** out) {
     PCC_HIP(hipSetDevice(device));
     PCC_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     PCC_HIP(hipHostMalloc((void**)&ctx->state_host, 4 * sizeof(SolverState), hipHostMallocDefault));
+    ctx->state_host_cap = 4;
     ctx->state.ensure(1);
-    ctx->ticket.ensure(1);
-    PCC_HIP(hipMemsetAsync(ctx->ticket.p, 0, sizeof(uint32_t) * ctx->ticket.cap, ctx->stream));
-    PCC_HIP(hipStreamSynchronize(ctx->stream));
     return PCC_OK;
   });
   if (s != PCC_OK) {
@@ -303,92 +611,22 @@ pcc_status pcc_set_params(pcc_ctx* ctx, const pcc_params* params) {
   pcc_status s = check_params(params, &why);
   if (s != PCC_OK) return fail(ctx, s, why);
   ctx->params = *params;
+  for (auto& I : ctx->inst) I.p = *params;
   return PCC_OK;
 }
 
 const char* pcc_last_error(const pcc_ctx* ctx) { return ctx ? ctx->last_error.c_str() : ""; }
 
 pcc_status pcc_set_state(pcc_ctx* ctx, const pcc_bodies* b) {
-  if (!ctx || !b || b->n < 0) return PCC_E_INVALID_ARGUMENT;
-  if (b->n > 0 && (!b->pos || !b->quat || !b->length || !b->l0 || !b->diameter || !b->ldot))
-    return fail(ctx, PCC_E_INVALID_ARGUMENT, "missing body field");
-  if (b->n >= (int64_t)1 << 30) return fail(ctx, PCC_E_INVALID_ARGUMENT, "too many bodies");
-  return guarded(ctx, [&]() {
-    PCC_HIP(hipSetDevice(ctx->device));
-    const int64_t N = b->n;
-    std::vector<double> h((size_t)kBodyFields * N);
-    for (int64_t i = 0; i < N; ++i) {
-      for (int k = 0; k < 3; ++k) h[(size_t)k * N + i] = b->pos[3 * i + k];
-      for (int k = 0; k < 4; ++k) h[(size_t)(3 + k) * N + i] = b->quat[4 * i + k];
-      h[(size_t)7 * N + i] = b->length[i];
-      h[(size_t)8 * N + i] = b->l0[i];
-      h[(size_t)9 * N + i] = b->diameter[i];
-      h[(size_t)10 * N + i] = b->ldot[i];
-    }
-    ctx->N = N;
-    ctx->body.ensure(kBodyFields * N + 1);
-    ctx->F.ensure(6 * N + 6);
-    ctx->U.ensure(6 * N + 6);
-    ctx->sig.ensure(N + 1);
-    ctx->imp.ensure(N + 1);
-    ctx->ldot_new.ensure(N + 1);
-    ctx->nc.ensure(N + 1);
-    if (N > 0)
-      PCC_HIP(hipMemcpyAsync(ctx->body.p, h.data(), sizeof(double) * h.size(),
-                             hipMemcpyHostToDevice, ctx->stream));
-    PCC_HIP(hipMemsetAsync(ctx->F.p, 0, sizeof(double) * (6 * N + 6), ctx->stream));
-    PCC_HIP(hipMemsetAsync(ctx->U.p, 0, sizeof(double) * (6 * N + 6), ctx->stream));
-    PCC_HIP(hipMemsetAsync(ctx->sig.p, 0, sizeof(double) * (N + 1), ctx->stream));
-    PCC_HIP(hipMemsetAsync(ctx->imp.p, 0, sizeof(double) * (N + 1), ctx->stream));
-    PCC_HIP(hipMemsetAsync(ctx->nc.p, 0, sizeof(int32_t) * (N + 1), ctx->stream));
-    PCC_HIP(hipStreamSynchronize(ctx->stream));
-    ctx->S.C = 0;
-    ctx->has_state = true;
-    ctx->has_snapshot = false;
-    return PCC_OK;
-  });
+  if (!ctx || !b) return PCC_E_INVALID_ARGUMENT;
+  return set_colonies(ctx, 1, b, nullptr);
 }
 
 pcc_status pcc_get_state(pcc_ctx* ctx, pcc_bodies* b) {
   if (!ctx || !b) return PCC_E_INVALID_ARGUMENT;
   if (!ctx->has_state) return fail(ctx, PCC_E_NO_STATE, "no state");
-  if (b->n != ctx->N) return fail(ctx, PCC_E_INVALID_ARGUMENT, "body count mismatch");
-  return guarded(ctx, [&]() {
-    PCC_HIP(hipSetDevice(ctx->device));
-    const int64_t N = ctx->N;
-    std::vector<double> h((size_t)kBodyFields * N), F(6 * N), U(6 * N), sg(N), im(N);
-    std::vector<int32_t> nc(N);
-    hipStream_t s = ctx->stream;
-    if (N > 0) {
-      PCC_HIP(hipMemcpyAsync(h.data(), ctx->body.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
-      PCC_HIP(hipMemcpyAsync(F.data(), ctx->F.p, sizeof(double) * 6 * N, hipMemcpyDeviceToHost, s));
-      PCC_HIP(hipMemcpyAsync(U.data(), ctx->U.p, sizeof(double) * 6 * N, hipMemcpyDeviceToHost, s));
-      PCC_HIP(hipMemcpyAsync(sg.data(), ctx->sig.p, sizeof(double) * N, hipMemcpyDeviceToHost, s));
-      PCC_HIP(hipMemcpyAsync(im.data(), ctx->imp.p, sizeof(double) * N, hipMemcpyDeviceToHost, s));
-      PCC_HIP(hipMemcpyAsync(nc.data(), ctx->nc.p, sizeof(int32_t) * N, hipMemcpyDeviceToHost, s));
-    }
-    PCC_HIP(hipStreamSynchronize(s));
-    for (int64_t i = 0; i < N; ++i) {
-      if (b->pos)
-        for (int k = 0; k < 3; ++k) b->pos[3 * i + k] = h[(size_t)k * N + i];
-      if (b->quat)
-        for (int k = 0; k < 4; ++k) b->quat[4 * i + k] = h[(size_t)(3 + k) * N + i];
-      if (b->length) b->length[i] = h[(size_t)7 * N + i];
-      if (b->l0) b->l0[i] = h[(size_t)8 * N + i];
-      if (b->diameter) b->diameter[i] = h[(size_t)9 * N + i];
-      if (b->ldot) b->ldot[i] = h[(size_t)10 * N + i];
-      for (int k = 0; k < 3; ++k) {
-        if (b->force) b->force[3 * i + k] = F[6 * i + k];
-        if (b->torque) b->torque[3 * i + k] = F[6 * i + 3 + k];
-        if (b->velocity) b->velocity[3 * i + k] = U[6 * i + k];
-        if (b->omega) b->omega[3 * i + k] = U[6 * i + 3 + k];
-      }
-      if (b->impedance) b->impedance[i] = im[i];
-      if (b->stress) b->stress[i] = sg[i];
-      if (b->num_constraints) b->num_constraints[i] = nc[i];
-    }
-    return PCC_OK;
-  });
+  if (ctx->inst.size() != 1) return fail(ctx, PCC_E_INVALID_ARGUMENT, "batched context: use pcc_batch_get_state");
+  return get_colony_state(ctx, 0, b);
 }
 
 pcc_status pcc_snapshot(pcc_ctx* ctx) {
@@ -425,24 +663,29 @@ pcc_status pcc_step(pcc_ctx* ctx, int32_t iter, pcc_step_result* out) {
   std::string why;
   pcc_status ps = check_params(&ctx->params, &why);
   if (ps != PCC_OK) return fail(ctx, ps, why);
+  if (ctx->inst.size() != 1) return fail(ctx, PCC_E_INVALID_ARGUMENT, "batched context: use pcc_batch_step");
   pcc_status s = guarded(ctx, [&]() {
     PCC_HIP(hipSetDevice(ctx->device));
-    if (ctx->params.mode == PCC_MODE_HARD) return hard_step(ctx, out);
-    return soft_step(ctx, out);
+    return run_step(ctx);
   });
   pcc_status t = guarded(ctx, [&]() {
     resolve_timing(ctx);
     return PCC_OK;
   });
-  return s != PCC_OK ? s : t;
+  if (s != PCC_OK) return s;
+  const Instance& I = ctx->inst[0];
+  fill_result(I, out);
+  if (I.status != PCC_OK) return fail(ctx, I.status, I.error);
+  return t;
 }
 
 pcc_status pcc_get_constraints(pcc_ctx* ctx, pcc_constraints* out, int64_t* count) {
   if (!ctx || !out) return PCC_E_INVALID_ARGUMENT;
-  if (count) *count = ctx->S.C;
+  if (ctx->inst.size() != 1) return fail(ctx, PCC_E_INVALID_ARGUMENT, "batched context: use pcc_batch_get_constraints");
+  if (count) *count = ctx->inst[0].C;
   return guarded(ctx, [&]() {
     PCC_HIP(hipSetDevice(ctx->device));
-    copy_set_to_host(ctx, out);
+    copy_set_to_host(ctx, 0, out);
     return PCC_OK;
   });
 }
@@ -451,9 +694,14 @@ pcc_status pcc_detect(pcc_ctx* ctx, double cutoff, pcc_constraints* out, int64_t
                       uint32_t* bucket_out, int32_t* perm_out) {
   if (!ctx) return PCC_E_INVALID_ARGUMENT;
   if (!ctx->has_state) return fail(ctx, PCC_E_NO_STATE, "no state");
+  if (ctx->inst.size() != 1) return fail(ctx, PCC_E_INVALID_ARGUMENT, "batched context");
   return guarded(ctx, [&]() {
     PCC_HIP(hipSetDevice(ctx->device));
-    detect_batch(ctx, cutoff, 0);
+    Instance& I = ctx->inst[0];
+    I.flags = kInstActive;
+    I.cutoff = cutoff;
+    upload_instances(ctx);
+    detect_batch(ctx);
     if (count) *count = ctx->batch_count;
     batch_to_host(ctx, out, 0);
     const int64_t N = ctx->N;
@@ -469,16 +717,150 @@ pcc_status pcc_detect(pcc_ctx* ctx, double cutoff, pcc_constraints* out, int64_t
   });
 }
 
+pcc_status pcc_batch_get_phase_stats(pcc_ctx* ctx, int64_t colony, int64_t* bbpgd_iterations,
+                                     int64_t* n_constraints, int64_t max, int64_t* count) {
+  if (!ctx) return PCC_E_INVALID_ARGUMENT;
+  if (colony < 0 || colony >= (int64_t)ctx->inst.size())
+    return fail(ctx, PCC_E_INVALID_ARGUMENT, "no such colony");
+  const Instance& I = ctx->inst[colony];
+  const int64_t n = (int64_t)I.phase_iters.size();
+  if (count) *count = n;
+  for (int64_t i = 0; i < n && i < max; ++i) {
+    if (bbpgd_iterations) bbpgd_iterations[i] = I.phase_iters[i];
+    if (n_constraints) n_constraints[i] = I.phase_C[i];
+  }
+  return PCC_OK;
+}
+
 pcc_status pcc_get_phase_stats(pcc_ctx* ctx, int64_t* bbpgd_iterations, int64_t* n_constraints,
                                int64_t max, int64_t* count) {
   if (!ctx) return PCC_E_INVALID_ARGUMENT;
-  const int64_t n = (int64_t)ctx->phase_iters.size();
-  if (count) *count = n;
-  for (int64_t i = 0; i < n && i < max; ++i) {
-    if (bbpgd_iterations) bbpgd_iterations[i] = ctx->phase_iters[i];
-    if (n_constraints) n_constraints[i] = ctx->phase_C[i];
+  if (ctx->inst.empty()) {
+    if (count) *count = 0;
+    return PCC_OK;
   }
+  return pcc_batch_get_phase_stats(ctx, 0, bbpgd_iterations, n_constraints, max, count);
+}
+
+pcc_status pcc_batch_set_state(pcc_ctx* ctx, int64_t n_colonies, const pcc_bodies* bodies,
+                               const pcc_params* params) {
+  if (!ctx || !bodies || n_colonies < 1) return PCC_E_INVALID_ARGUMENT;
+  return set_colonies(ctx, n_colonies, bodies, params);
+}
+
+pcc_status pcc_batch_size(pcc_ctx* ctx, int64_t* n_colonies, int64_t* n_bodies) {
+  if (!ctx) return PCC_E_INVALID_ARGUMENT;
+  if (n_colonies) *n_colonies = ctx->has_state ? (int64_t)ctx->inst.size() : 0;
+  if (n_bodies) *n_bodies = ctx->has_state ? ctx->N : 0;
   return PCC_OK;
+}
+
+pcc_status pcc_batch_step(pcc_ctx* ctx, int32_t iter, pcc_step_result* results,
+                          pcc_status* statuses) {
+  (void)iter;
+  if (!ctx || !results) return PCC_E_INVALID_ARGUMENT;
+  if (!ctx->has_state) return fail(ctx, PCC_E_NO_STATE, "no state");
+  std::string why;
+  pcc_status ps = check_params(&ctx->params, &why);
+  if (ps != PCC_OK) return fail(ctx, ps, why);
+  pcc_status s = guarded(ctx, [&]() {
+    PCC_HIP(hipSetDevice(ctx->device));
+    return run_step(ctx);
+  });
+  pcc_status t = guarded(ctx, [&]() {
+    resolve_timing(ctx);
+    return PCC_OK;
+  });
+  if (s != PCC_OK) return s;
+  for (size_t q = 0; q < ctx->inst.size(); ++q) {
+    fill_result(ctx->inst[q], &results[q]);
+    if (statuses) statuses[q] = ctx->inst[q].status;
+  }
+  return t;
+}
+
+pcc_status pcc_batch_get_state(pcc_ctx* ctx, int64_t colony, pcc_bodies* bodies) {
+  if (!ctx || !bodies) return PCC_E_INVALID_ARGUMENT;
+  if (!ctx->has_state) return fail(ctx, PCC_E_NO_STATE, "no state");
+  if (colony < 0 || colony >= (int64_t)ctx->inst.size())
+    return fail(ctx, PCC_E_INVALID_ARGUMENT, "no such colony");
+  return get_colony_state(ctx, colony, bodies);
+}
+
+pcc_status pcc_batch_get_constraints(pcc_ctx* ctx, int64_t colony, pcc_constraints* out,
+                                     int64_t* count) {
+  if (!ctx || !out) return PCC_E_INVALID_ARGUMENT;
+  if (colony < 0 || colony >= (int64_t)ctx->inst.size())
+    return fail(ctx, PCC_E_INVALID_ARGUMENT, "no such colony");
+  if (count) *count = ctx->inst[colony].C;
+  return guarded(ctx, [&]() {
+    PCC_HIP(hipSetDevice(ctx->device));
+    copy_set_to_host(ctx, colony, out);
+    return PCC_OK;
+  });
+}
+
+pcc_status pcc_batch_set_params(pcc_ctx* ctx, int64_t colony, const pcc_params* params) {
+  if (!ctx || !params) return PCC_E_INVALID_ARGUMENT;
+  if (colony < 0 || colony >= (int64_t)ctx->inst.size())
+    return fail(ctx, PCC_E_INVALID_ARGUMENT, "no such colony");
+  std::string why;
+  pcc_status s = check_colony_params(ctx, params, &why);
+  if (s != PCC_OK) return fail(ctx, s, why);
+  ctx->inst[colony].p = *params;
+  return PCC_OK;
+}
+
+pcc_status pcc_batch_add(pcc_ctx* ctx, const pcc_bodies* bodies, const pcc_params* params,
+                         int64_t* colony) {
+  if (!ctx || !bodies) return PCC_E_INVALID_ARGUMENT;
+  if (!ctx->has_state) {
+    pcc_status s = set_colonies(ctx, 1, bodies, params);
+    if (s == PCC_OK && colony) *colony = 0;
+    return s;
+  }
+  pcc_status st = check_bodies(ctx, bodies);
+  if (st != PCC_OK) return st;
+  std::string why;
+  if (params) {
+    st = check_colony_params(ctx, params, &why);
+    if (st != PCC_OK) return fail(ctx, st, why);
+  }
+  const int64_t oldN = ctx->N, n = bodies->n, newN = oldN + n;
+  if (newN >= (int64_t)1 << 30) return fail(ctx, PCC_E_INVALID_ARGUMENT, "too many bodies");
+  return guarded(ctx, [&]() {
+    PCC_HIP(hipSetDevice(ctx->device));
+    Instance I;
+    I.p = params ? *params : ctx->params;
+    I.N = n;
+    ctx->inst.push_back(I);
+    repack(ctx, newN, {{0, 0, oldN}});
+    // the new colony's fields
+    std::vector<double> h((size_t)kBodyFields * n);
+    pack_bodies(bodies, n, 0, h.data());
+    for (int k = 0; k < kBodyFields && n > 0; ++k)
+      PCC_HIP(hipMemcpyAsync(ctx->body.p + (size_t)k * newN + oldN, h.data() + (size_t)k * n,
+                             sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+    PCC_HIP(hipStreamSynchronize(ctx->stream));
+    if (colony) *colony = (int64_t)ctx->inst.size() - 1;
+    return PCC_OK;
+  });
+}
+
+pcc_status pcc_batch_remove(pcc_ctx* ctx, int64_t colony) {
+  if (!ctx) return PCC_E_INVALID_ARGUMENT;
+  if (colony < 0 || colony >= (int64_t)ctx->inst.size())
+    return fail(ctx, PCC_E_INVALID_ARGUMENT, "no such colony");
+  if (ctx->inst.size() == 1) return fail(ctx, PCC_E_INVALID_ARGUMENT, "cannot remove the last colony");
+  return guarded(ctx, [&]() {
+    PCC_HIP(hipSetDevice(ctx->device));
+    const Instance& I = ctx->inst[colony];
+    const int64_t oldN = ctx->N, newN = oldN - I.N;
+    const std::vector<Move> mv = {{0, 0, I.boff}, {I.boff + I.N, I.boff, oldN - I.boff - I.N}};
+    ctx->inst.erase(ctx->inst.begin() + colony);
+    repack(ctx, newN, mv);
+    return PCC_OK;
+  });
 }
 
 pcc_status pcc_set_timing(pcc_ctx* ctx, int enable) {

@@ -253,6 +253,13 @@ __host__ __device__ __forceinline__ uint32_t cell_hash(int32_t cx, int32_t cy, i
   return h;
 }
 
+// Per-colony salt of the cell hash: colonies of one context share the hashed
+// table without sharing cells.  Colony 0 gets 0, so a single colony hashes
+// exactly as above.
+__host__ __device__ __forceinline__ uint32_t inst_mix(int32_t b) {
+  return b == 0 ? 0u : cell_hash(b, 0x5bd1e995, 0x27d4eb2f);
+}
+
 // ---------------------------------------------------------------------------
 // Deterministic reductions.  A 64-lane butterfly (xor 32,16,...,1) leaves the
 // same value in every lane; a 256-thread block combines its four waves as

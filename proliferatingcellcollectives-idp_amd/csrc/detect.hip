@@ -15,8 +15,16 @@
 //                  tests every J > I of the same or an adjacent cell with the
 //                  reference's filters and exact FP64 segment-segment DCP;
 //                  pass 0 counts, pass 1 writes at the scanned offset
-//  5. k_segsort + radix sort: order = (contact x, I, J)
+//  5. k_segsort + radix sort: order = (contact x, I, J); with several colonies
+//                  in one context a second stable radix sort by colony makes
+//                  the order (colony, contact x, I, J)
+//
+// Colonies of one context share the hashed table: the colony index salts the
+// bucket hash and neighbour tests require the same colony, so each colony's
+// contact list is the one it would have alone.
+#include <algorithm>
 #include <cstring>
+#include <vector>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
@@ -27,18 +35,21 @@ namespace pcc {
 
 namespace {
 
-__global__ void k_cells(int64_t N, BodyArrays b, double origin, double h, uint32_t mask,
+__global__ void k_cells(int64_t N, BodyArrays b, const int32_t* __restrict__ binst,
+                        const InstDev* __restrict__ idev, uint32_t mask,
                         int32_t* __restrict__ cell, uint32_t* __restrict__ bucket,
                         int32_t* __restrict__ count) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
+  const int32_t inst = binst[i];
+  const double origin = idev[inst].origin, h = idev[inst].h;
   int32_t cx = cell_coord(b.px[i], origin, h);
   int32_t cy = cell_coord(b.py[i], origin, h);
   int32_t cz = cell_coord(b.pz[i], origin, h);
   cell[3 * i] = cx;
   cell[3 * i + 1] = cy;
   cell[3 * i + 2] = cz;
-  uint32_t bk = cell_hash(cx, cy, cz) & mask;
+  uint32_t bk = (cell_hash(cx, cy, cz) ^ inst_mix(inst)) & mask;
   bucket[i] = bk;
   atomicAdd(&count[bk], 1);
 }
@@ -77,10 +88,15 @@ enum { R_PHI = 0, R_N = 1, R_RI = 4, R_RJ = 7, R_P = 10, R_SI = 13, R_SJ = 14, R
 __global__ void __launch_bounds__(256)
 k_detect(int pass, int64_t N, BodyArrays b, const int32_t* __restrict__ cell,
          const int32_t* __restrict__ bstart, const int32_t* __restrict__ perm, uint32_t mask,
-         double cutoff, double warm, int32_t* __restrict__ cnt_off, int32_t* __restrict__ outI,
+         const int32_t* __restrict__ binst, const InstDev* __restrict__ idev,
+         int32_t* __restrict__ cnt_off, int32_t* __restrict__ outI,
          int32_t* __restrict__ outJ, double* __restrict__ rec) {
   int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (I >= N) return;
+  const int32_t inst = binst[I];
+  if (!(idev[inst].flags & kInstActive)) return;  // count stays 0
+  const double cutoff = idev[inst].cutoff, warm = idev[inst].warm;
+  const uint32_t salt = inst_mix(inst);
   const double xI[3] = {b.px[I], b.py[I], b.pz[I]};
   const double lenI = b.len[I], dI = b.diam[I];
   double tI[3];
@@ -100,13 +116,13 @@ k_detect(int pass, int64_t N, BodyArrays b, const int32_t* __restrict__ cell,
     for (int dy = -1; dy <= 1; ++dy)
       for (int dx = -1; dx <= 1; ++dx) {
         const int32_t nx = cx + dx, ny = cy + dy, nz = cz + dz;
-        const uint32_t bk = cell_hash(nx, ny, nz) & mask;
+        const uint32_t bk = (cell_hash(nx, ny, nz) ^ salt) & mask;
         const int32_t e1 = bstart[bk + 1];
         for (int32_t e = bstart[bk]; e < e1; ++e) {
           const int32_t J = perm[e];
           if (J <= I) continue;
           if (cell[3 * (int64_t)J] != nx || cell[3 * (int64_t)J + 1] != ny ||
-              cell[3 * (int64_t)J + 2] != nz)
+              cell[3 * (int64_t)J + 2] != nz || binst[J] != inst)
             continue;
           const double xJ[3] = {b.px[J], b.py[J], b.pz[J]};
           const double lenJ = b.len[J], dJ = b.diam[J];
@@ -186,11 +202,31 @@ __global__ void k_make_keys(int64_t m, const int32_t* __restrict__ idx,
   key[i] = order_key(rec[R_LEN * (int64_t)idx[i] + R_P]);
 }
 
+// Colony of each sorted batch record (key of the second, stable sort).
+__global__ void k_inst_keys(int64_t m, const int32_t* __restrict__ idx,
+                            const int32_t* __restrict__ rI_, const int32_t* __restrict__ binst,
+                            uint32_t* __restrict__ key) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  key[i] = (uint32_t)binst[rI_[idx[i]]];
+}
+
+// Batch offsets at the colony boundaries: colony b's records occupy
+// [off[boff_b], off[boff_b + N_b]) of the sorted batch.
+__global__ void k_batch_bounds(int64_t B, int64_t N, const InstDev* __restrict__ idev,
+                               const int32_t* __restrict__ off, int32_t* __restrict__ out) {
+  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > B) return;
+  out[b] = off[b < B ? idev[b].boff : N];
+}
+
 // Appends the sorted batch to the constraint set at offset C0 and builds the
 // Jacobian blocks of the new rows (Physics.cpp:118-154).
 __global__ void k_append(int64_t m, int64_t C0, int32_t iteration, const int32_t* __restrict__ idx,
                          const int32_t* __restrict__ rI_, const int32_t* __restrict__ rJ_,
-                         const double* __restrict__ rec, int32_t* __restrict__ SI,
+                         const double* __restrict__ rec, const int32_t* __restrict__ binst,
+                         const InstDev* __restrict__ idev, int32_t* __restrict__ Sinst,
+                         int32_t* __restrict__ SI,
                          int32_t* __restrict__ SJ, int32_t* __restrict__ Sgid,
                          int32_t* __restrict__ Siter, double* __restrict__ Sphi,
                          double* __restrict__ Sgam, double* __restrict__ Sn,
@@ -204,9 +240,11 @@ __global__ void k_append(int64_t m, int64_t C0, int32_t iteration, const int32_t
   const double* r = rec + R_LEN * (int64_t)src;
   const int64_t c = C0 + i;
   const int32_t I = rI_[src], J = rJ_[src];
+  const int32_t inst = binst[I];
   SI[c] = I;
   SJ[c] = J;
-  Sgid[c] = (int32_t)i;
+  Sinst[c] = inst;
+  Sgid[c] = (int32_t)(i - idev[inst].bstart);  // index within the colony's batch
   Siter[c] = iteration;
   Sphi[c] = r[R_PHI];
   Sgam[c] = r[R_G];
@@ -236,6 +274,26 @@ __global__ void k_append(int64_t m, int64_t C0, int32_t iteration, const int32_t
   }
   atomicAdd(&nc[I], 1);
   atomicAdd(&nc[J], 1);
+}
+
+// Instance-major merge: constraint g of the set (old entries first, then the
+// batch appended at C0) moves to its colony's new range, old constraints first.
+__global__ void k_regroup(int64_t total, int64_t C0, const InstDev* __restrict__ idev,
+                          const int32_t* __restrict__ inst_in, int64_t* __restrict__ dst) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  const InstDev& D = idev[inst_in[g]];
+  dst[g] = g < C0 ? g - D.coff + D.coff_new : D.coff_new + D.C + (g - C0 - D.bstart);
+}
+
+template <class T, int W>
+__global__ void k_scatter_rows(int64_t total, const int64_t* __restrict__ dst,
+                               const T* __restrict__ in, T* __restrict__ out) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  const int64_t d = dst[g];
+#pragma unroll
+  for (int k = 0; k < W; ++k) out[W * d + k] = in[W * g + k];
 }
 
 inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
@@ -280,8 +338,8 @@ void build_grid(pcc_ctx* ctx) {
   BodyArrays b = body_arrays(ctx->body.p, N);
   PCC_HIP(hipMemsetAsync(ctx->bcursor.p, 0, sizeof(int32_t) * (size_t)(T + 1), s));
   if (N > 0)
-    k_cells<<<nblk(N), 256, 0, s>>>(N, b, ctx->params.grid_origin, ctx->params.linked_cell_size,
-                                    (uint32_t)(T - 1), ctx->cell.p, ctx->bucket.p, ctx->bcursor.p);
+    k_cells<<<nblk(N), 256, 0, s>>>(N, b, ctx->binst.p, ctx->idev.p, (uint32_t)(T - 1),
+                                    ctx->cell.p, ctx->bucket.p, ctx->bcursor.p);
   excl_scan<int32_t>(ctx, ctx->bcursor.p, ctx->bstart.p, T + 1);
   PCC_HIP(hipMemsetAsync(ctx->bcursor.p, 0, sizeof(int32_t) * (size_t)(T + 1), s));
   if (N > 0) {
@@ -290,11 +348,12 @@ void build_grid(pcc_ctx* ctx) {
   }
 }
 
-// One detection batch; on return ctx->ridx[0..batch_count) lists the raw
-// records in (contact x, I, J) order.
-void detect_batch(pcc_ctx* ctx, double cutoff, int32_t iteration) {
-  (void)iteration;
+// One detection batch over the active colonies, each at the cutoff of its
+// instance record; on return ctx->ridx[0..batch_count) lists the raw records
+// in (colony, contact x, I, J) order and every colony's m / bstart is set.
+void detect_batch(pcc_ctx* ctx) {
   const int64_t N = ctx->N;
+  const int64_t B = (int64_t)ctx->inst.size();
   hipStream_t s = ctx->stream;
   KTimer tm(ctx, PCC_K_DETECT, 0.0);
   build_grid(ctx);
@@ -304,8 +363,8 @@ void detect_batch(pcc_ctx* ctx, double cutoff, int32_t iteration) {
   PCC_HIP(hipMemsetAsync(ctx->dcount.p, 0, sizeof(int32_t) * (size_t)(N + 1), s));
   if (N > 0)
     k_detect<<<nblk(N), 256, 0, s>>>(0, N, b, ctx->cell.p, ctx->bstart.p, ctx->perm.p, mask,
-                                     cutoff, ctx->params.warm_start_factor, ctx->dcount.p,
-                                     nullptr, nullptr, nullptr);
+                                     ctx->binst.p, ctx->idev.p, ctx->dcount.p, nullptr, nullptr,
+                                     nullptr);
   // offsets (in place into a second array)
   ctx->ridx2.ensure(N + 1);
   excl_scan<int32_t>(ctx, ctx->dcount.p, ctx->ridx2.p, N + 1);
@@ -324,9 +383,20 @@ void detect_batch(pcc_ctx* ctx, double cutoff, int32_t iteration) {
   ctx->ridx2.ensure(m + 1);
   ctx->rkey.ensure(m + 1);
   ctx->rkey2.ensure(m + 1);
+  // colony boundaries of the batch (a stable sort by colony keeps them)
+  ctx->bbounds.ensure(B + 1);
+  k_batch_bounds<<<nblk(B + 1), 256, 0, s>>>(B, N, ctx->idev.p, ctx->dcount.p, ctx->bbounds.p);
+  std::vector<int32_t> bounds(B + 1);
+  PCC_HIP(hipMemcpyAsync(bounds.data(), ctx->bbounds.p, sizeof(int32_t) * (B + 1),
+                         hipMemcpyDeviceToHost, s));
+  PCC_HIP(hipStreamSynchronize(s));
+  for (int64_t q = 0; q < B; ++q) {
+    ctx->inst[q].bstart = bounds[q];
+    ctx->inst[q].m = bounds[q + 1] - bounds[q];
+  }
   if (m == 0) return;
-  k_detect<<<nblk(N), 256, 0, s>>>(1, N, b, ctx->cell.p, ctx->bstart.p, ctx->perm.p, mask, cutoff,
-                                   ctx->params.warm_start_factor, ctx->dcount.p, ctx->rI_.p,
+  k_detect<<<nblk(N), 256, 0, s>>>(1, N, b, ctx->cell.p, ctx->bstart.p, ctx->perm.p, mask,
+                                   ctx->binst.p, ctx->idev.p, ctx->dcount.p, ctx->rI_.p,
                                    ctx->rJ_.p, ctx->rrec.p);
   k_segsort_by_J<<<nblk(N), 256, 0, s>>>(N, ctx->dcount.p, ctx->rJ_.p, ctx->ridx.p);
   k_make_keys<<<nblk(m), 256, 0, s>>>(m, ctx->ridx.p, ctx->rrec.p, ctx->rkey.p);
@@ -343,15 +413,30 @@ void detect_batch(pcc_ctx* ctx, double cutoff, int32_t iteration) {
   int64_t tc = ctx->ridx.cap;
   ctx->ridx.cap = ctx->ridx2.cap;
   ctx->ridx2.cap = tc;
+  if (B > 1) {
+    // (colony, contact x, I, J): stable LSD sort by colony on top
+    uint32_t* k0 = reinterpret_cast<uint32_t*>(ctx->rkey.p);
+    uint32_t* k1 = reinterpret_cast<uint32_t*>(ctx->rkey2.p);
+    k_inst_keys<<<nblk(m), 256, 0, s>>>(m, ctx->ridx.p, ctx->rI_.p, ctx->binst.p, k0);
+    int bits = 1;
+    while (((int64_t)1 << bits) < B) ++bits;
+    bytes = 0;
+    PCC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, k0, k1, ctx->ridx.p, ctx->ridx2.p,
+                                      (size_t)m, 0, bits, s));
+    ctx->sort_tmp.ensure((int64_t)bytes + 16);
+    PCC_HIP(rocprim::radix_sort_pairs(ctx->sort_tmp.p, bytes, k0, k1, ctx->ridx.p, ctx->ridx2.p,
+                                      (size_t)m, 0, bits, s));
+    std::swap(ctx->ridx.p, ctx->ridx2.p);
+    std::swap(ctx->ridx.cap, ctx->ridx2.cap);
+  }
 }
 
-static void ensure_set_capacity(pcc_ctx* ctx, int64_t need) {
-  ConstraintSet& S = ctx->S;
+static void ensure_set_capacity(ConstraintSet& S, int64_t need, hipStream_t s) {
   if (need <= S.cap) return;
   int64_t nc = S.cap > 0 ? S.cap : 4096;
   while (nc < need) nc *= 2;
-  hipStream_t s = ctx->stream;
   const int64_t keep = S.C;
+  S.inst.grow_keep(nc, keep, s);
   S.I.grow_keep(nc, keep, s);
   S.J.grow_keep(nc, keep, s);
   S.gid.grow_keep(nc, keep, s);
@@ -369,18 +454,83 @@ static void ensure_set_capacity(pcc_ctx* ctx, int64_t need) {
   S.cap = nc;
 }
 
+template <class T>
+static void swap_arr(DArr<T>& a, DArr<T>& b) {
+  std::swap(a.p, b.p);
+  std::swap(a.cap, b.cap);
+}
+
+// Appends the sorted batch; with several colonies the set is then regrouped
+// instance-major (each colony's old constraints, then its new ones).  The
+// instance records must hold the colonies' current ranges, m and bstart; on
+// return they hold the new ranges.
 void append_batch(pcc_ctx* ctx, int32_t iteration) {
   const int64_t m = ctx->batch_count;
+  const int64_t B = (int64_t)ctx->inst.size();
   ConstraintSet& S = ctx->S;
-  ensure_set_capacity(ctx, S.C + m + 1);
+  hipStream_t s = ctx->stream;
+  ensure_set_capacity(S, S.C + m + 1, s);
+  int64_t off = 0;
+  for (auto& q : ctx->inst) {
+    q.coff_new_ = off;
+    off += q.C + q.m;
+  }
+  upload_instances(ctx);
+  const int64_t C0 = S.C;
   if (m > 0) {
     KTimer tm(ctx, PCC_K_ASSEMBLE, 0.0);
-    k_append<<<nblk(m), 256, 0, ctx->stream>>>(
-        m, S.C, iteration, ctx->ridx.p, ctx->rI_.p, ctx->rJ_.p, ctx->rrec.p, S.I.p, S.J.p,
-        S.gid.p, S.iter.p, S.phi.p, S.gamma.p, S.n.p, S.rI.p, S.rJ.p, S.p.p, S.sI.p, S.sJ.p,
-        S.coef64.p, S.coef32.p, ctx->nc.p);
+    k_append<<<nblk(m), 256, 0, s>>>(
+        m, C0, iteration, ctx->ridx.p, ctx->rI_.p, ctx->rJ_.p, ctx->rrec.p, ctx->binst.p,
+        ctx->idev.p, S.inst.p, S.I.p, S.J.p, S.gid.p, S.iter.p, S.phi.p, S.gamma.p, S.n.p,
+        S.rI.p, S.rJ.p, S.p.p, S.sI.p, S.sJ.p, S.coef64.p, S.coef32.p, ctx->nc.p);
   }
-  S.C += m;
+  S.C = C0 + m;
+  if (B > 1 && m > 0) {
+    const int64_t T = S.C;
+    ConstraintSet& D = ctx->S2;
+    ensure_set_capacity(D, T + 1, s);
+    ctx->regroup_dst.ensure(T + 1);
+    int64_t* dst = ctx->regroup_dst.p;
+    const unsigned g = nblk(T);
+    k_regroup<<<g, 256, 0, s>>>(T, C0, ctx->idev.p, S.inst.p, dst);
+    k_scatter_rows<int32_t, 1><<<g, 256, 0, s>>>(T, dst, S.inst.p, D.inst.p);
+    k_scatter_rows<int32_t, 1><<<g, 256, 0, s>>>(T, dst, S.I.p, D.I.p);
+    k_scatter_rows<int32_t, 1><<<g, 256, 0, s>>>(T, dst, S.J.p, D.J.p);
+    k_scatter_rows<int32_t, 1><<<g, 256, 0, s>>>(T, dst, S.gid.p, D.gid.p);
+    k_scatter_rows<int32_t, 1><<<g, 256, 0, s>>>(T, dst, S.iter.p, D.iter.p);
+    k_scatter_rows<double, 1><<<g, 256, 0, s>>>(T, dst, S.phi.p, D.phi.p);
+    k_scatter_rows<double, 1><<<g, 256, 0, s>>>(T, dst, S.gamma.p, D.gamma.p);
+    k_scatter_rows<double, 1><<<g, 256, 0, s>>>(T, dst, S.sI.p, D.sI.p);
+    k_scatter_rows<double, 1><<<g, 256, 0, s>>>(T, dst, S.sJ.p, D.sJ.p);
+    k_scatter_rows<double, 3><<<g, 256, 0, s>>>(T, dst, S.n.p, D.n.p);
+    k_scatter_rows<double, 3><<<g, 256, 0, s>>>(T, dst, S.rI.p, D.rI.p);
+    k_scatter_rows<double, 3><<<g, 256, 0, s>>>(T, dst, S.rJ.p, D.rJ.p);
+    k_scatter_rows<double, 3><<<g, 256, 0, s>>>(T, dst, S.p.p, D.p.p);
+    k_scatter_rows<double, 12><<<g, 256, 0, s>>>(T, dst, S.coef64.p, D.coef64.p);
+    k_scatter_rows<float, 12><<<g, 256, 0, s>>>(T, dst, S.coef32.p, D.coef32.p);
+    swap_arr(S.inst, D.inst);
+    swap_arr(S.I, D.I);
+    swap_arr(S.J, D.J);
+    swap_arr(S.gid, D.gid);
+    swap_arr(S.iter, D.iter);
+    swap_arr(S.phi, D.phi);
+    swap_arr(S.gamma, D.gamma);
+    swap_arr(S.sI, D.sI);
+    swap_arr(S.sJ, D.sJ);
+    swap_arr(S.n, D.n);
+    swap_arr(S.rI, D.rI);
+    swap_arr(S.rJ, D.rJ);
+    swap_arr(S.p, D.p);
+    swap_arr(S.coef64, D.coef64);
+    swap_arr(S.coef32, D.coef32);
+    std::swap(S.cap, D.cap);
+    D.C = 0;
+  }
+  for (auto& q : ctx->inst) {
+    q.coff = q.coff_new_;
+    q.C += q.m;
+  }
+  upload_instances(ctx);
 }
 
 // Copies the current (sorted) detection batch to host arrays.

@@ -80,20 +80,44 @@ struct BodyArrays {
 };
 static constexpr int kBodyFields = 11;
 
-// Device-side solver scalars of one BBPGD solve.
+// Device-side solver scalars of one instance's BBPGD solve.
 struct SolverState {
   double alpha;
   double res;
   double maxov;
-  double pad0;
+  int32_t skip;       // instance takes no part in this solve
+  int32_t pad0;
   int64_t iter;       // BBPGD iteration about to run (0 = initial gradient)
   int32_t converged;  // residual <= tol reached at iteration `iter`
   int32_t failed;     // iteration == max
 };
 
+// Instance flags for the current ReLCP phase.
+enum { kInstActive = 1, kInstInteg = 2, kInstGrow = 4 };
+
+// One colony of a context as the kernels see it.  A context holds B >= 1
+// independent colonies; their bodies are concatenated (instance b owns bodies
+// [boff, boff + N)) and the constraint set is kept instance-major (instance b
+// owns constraints [coff, coff + C), in its own solver order), so every
+// per-instance sum and reduction has exactly the order of a context holding
+// that colony alone.
+struct InstDev {
+  int64_t boff, N;
+  int64_t coff, C;
+  int64_t G, K;      // BB reduction: G logical blocks of K tiles of 256 constraints
+  int64_t lb;        // first logical constraint block (= first partial slot)
+  int64_t bb;        // first logical body block
+  int64_t bstart;    // first record of the instance in the sorted detection batch
+  int64_t coff_new;  // constraint offset once the current batch is merged
+  int64_t maxit;
+  double dt, xi, itau, mlam, kcc, tol, cutoff, warm, origin, h;
+  int32_t flags;
+  int32_t pad;
+};
+
 // Constraint set of the current step (all ReLCP phases), FP64 SoA.
 struct ConstraintSet {
-  DArr<int32_t> I, J, gid, iter;
+  DArr<int32_t> I, J, gid, iter, inst;
   DArr<double> phi, gamma, n, rI, rJ, p, sI, sJ;  // n, rI, rJ, p hold 3*C
   // packed operator row per constraint: n[3], rI x n, rJ x -n, sI, sJ, pad
   DArr<double> coef64;                           // 12*C
@@ -115,6 +139,30 @@ struct Timing {
   double bytes[PCC_K_COUNT] = {0};
   std::vector<hipEvent_t> free_events;
   std::vector<Pending> pending;
+};
+
+// Host-side record of one colony.
+struct Instance {
+  pcc_params p;
+  int64_t boff = 0, N = 0;
+  int64_t coff = 0, C = 0;   // constraint range of the current step
+  int64_t m = 0, bstart = 0; // current detection batch
+  int64_t coff_new_ = 0;     // range offset once the batch is merged
+  double cutoff = 0.0;       // contact cutoff of the current detection
+  int64_t G = 1, K = 0, lb = 0, bb = 0, nbb = 0;
+  int32_t flags = 0;
+  // per-step outcome (SolverSolution, Hard.cpp:192)
+  bool active = false;
+  int32_t k = 0;
+  int64_t its = 0;
+  double res = 0.0, maxov = 0.0;
+  pcc_status status = PCC_OK;
+  std::string error;
+  std::vector<int64_t> phase_iters, phase_C;
+  // last BBPGD solve
+  int64_t solve_its = 0;
+  double solve_res = 0.0;
+  bool solve_failed = false;
 };
 
 }  // namespace pcc
@@ -167,9 +215,9 @@ struct pcc_ctx {
   pcc::DArr<double> ubuf;                  // 8 per body: U[6], dldot, pad
   pcc::DArr<double> mob;                   // 2 per body: 1/(xi l), 12/(xi l^3)
   pcc::DArr<double> partials;              // 4 * nblocks
-  pcc::DArr<uint32_t> ticket;
-  pcc::DArr<pcc::SolverState> state;
-  pcc::SolverState* state_host = nullptr;  // pinned, 4 slots
+  pcc::DArr<pcc::SolverState> state;       // B
+  pcc::SolverState* state_host = nullptr;  // pinned, 2*B slots
+  int64_t state_host_cap = 0;
   pcc::DArr<double> scratch_d;             // small reductions
 
   // soft
@@ -177,22 +225,36 @@ struct pcc_ctx {
   pcc::DArr<float> hertz32;                // 16 per constraint
 
   pcc::Timing timing;
-  std::vector<int64_t> phase_iters, phase_C;  // per solved ReLCP phase, last step
+
+  // colonies of this context (one for the single-colony entry points)
+  std::vector<pcc::Instance> inst;
+  std::vector<pcc::InstDev> inst_host;
+  pcc::DArr<pcc::InstDev> idev;            // B
+  pcc::DArr<int32_t> binst;                // N: instance of every body
+  pcc::DArr<int32_t> bmap;                 // logical body block -> instance
+  pcc::DArr<int32_t> cmap;                 // logical constraint block -> instance
+  pcc::DArr<int32_t> bbounds;              // B+1 detection-batch offsets
+  pcc::DArr<unsigned long long> maxbits;   // B: max overlap bits
+  pcc::ConstraintSet S2;                   // merge target (B > 1)
+  pcc::DArr<int64_t> regroup_dst;
+  int64_t nbb_total = 0, ncb_total = 0;
 };
 
 namespace pcc {
 
 // launchers (kernels in detect.hip / solve.hip / soft.hip)
-void detect_batch(pcc_ctx* ctx, double cutoff, int32_t iteration);  // fills ctx->ridx sorted
+void detect_batch(pcc_ctx* ctx);  // active colonies at their cutoffs; fills ctx->ridx sorted
 void append_batch(pcc_ctx* ctx, int32_t iteration);               // appends sorted batch to S
-double max_overlap(pcc_ctx* ctx);
 void build_incidence(pcc_ctx* ctx, bool records);
 bool bodies_planar(pcc_ctx* ctx);
-pcc_status bbpgd_solve(pcc_ctx* ctx, int64_t* iters, double* res);
-void integrate_hard(pcc_ctx* ctx, int64_t kfinal);
+void bbpgd_solve(pcc_ctx* ctx);
+void integrate_hard(pcc_ctx* ctx);
 void grow(pcc_ctx* ctx);
 void soft_forces_and_integrate(pcc_ctx* ctx);
 void reset_step(pcc_ctx* ctx);
+void upload_instances(pcc_ctx* ctx);
+void max_overlap_inst(pcc_ctx* ctx, std::vector<double>& out);
+void merge_batch(pcc_ctx* ctx);
 
 BodyArrays body_arrays(double* base, int64_t N);
 

@@ -7,7 +7,8 @@
 // CSR, i.e. a deterministic segmented sum with no float atomics; then
 // U = M F, the same integration as the hard model and the growth law
 // (Soft.cpp:97-111).  In mixed precision the pair forces and their sums run
-// in FP32 and the integration in FP64.
+// in FP32 and the integration in FP64.  Every colony of the context takes its
+// own kcc, xi, dt, lambda and tau from its instance record.
 #include "common.hpp"
 #include "internal.hpp"
 
@@ -23,9 +24,11 @@ __global__ void k_hertz(int64_t C, int mixed, const int32_t* __restrict__ I,
                         const double* __restrict__ n, const double* __restrict__ rI,
                         const double* __restrict__ rJ, const double* __restrict__ sI,
                         const double* __restrict__ sJ, const double* __restrict__ diam,
-                        double kcc, double* __restrict__ rec, float* __restrict__ rec32) {
+                        const int32_t* __restrict__ Sinst, const InstDev* __restrict__ idev,
+                        double* __restrict__ rec, float* __restrict__ rec32) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= C) return;
+  const double kcc = idev[Sinst[c]].kcc;
   const double overlap = -phi[c];
   double* r = rec + 16 * c;
   float* r32 = rec32 + 16 * c;
@@ -89,11 +92,14 @@ __global__ void k_hertz(int64_t C, int mixed, const int32_t* __restrict__ I,
 
 __global__ void k_soft_body(int64_t N, int mixed, BodyArrays b, const int32_t* __restrict__ rowptr,
                             const int32_t* __restrict__ inc, const double* __restrict__ rec,
-                            const float* __restrict__ rec32, double xi, double dt, double mlam,
-                            double itau, double* __restrict__ F_out, double* __restrict__ U_out,
-                            double* __restrict__ sig_out, double* __restrict__ imp_out) {
+                            const float* __restrict__ rec32, const int32_t* __restrict__ binst,
+                            const InstDev* __restrict__ idev, double* __restrict__ F_out,
+                            double* __restrict__ U_out, double* __restrict__ sig_out,
+                            double* __restrict__ imp_out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
+  const InstDev& D = idev[binst[i]];
+  const double xi = D.xi, dt = D.dt, mlam = D.mlam, itau = D.itau;
   double F[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double sig = 0.0;
   const int32_t e1 = rowptr[i + 1];
@@ -191,14 +197,13 @@ void soft_forces_and_integrate(pcc_ctx* ctx) {
     ConstraintSet& S = ctx->S;
     if (C > 0)
       k_hertz<<<nblk(C), 256, 0, s>>>(C, mixed ? 1 : 0, S.I.p, S.J.p, S.phi.p, S.n.p, S.rI.p,
-                                      S.rJ.p, S.sI.p, S.sJ.p, b.diam, ctx->params.kcc,
+                                      S.rJ.p, S.sI.p, S.sJ.p, b.diam, S.inst.p, ctx->idev.p,
                                       ctx->hertz.p, ctx->hertz32.p);
   }
   KTimer t(ctx, PCC_K_INTEGRATE, 0.0);
   if (N > 0)
     k_soft_body<<<nblk(N), 256, 0, s>>>(N, mixed ? 1 : 0, b, ctx->rowptr.p, ctx->inc.p,
-                                        ctx->hertz.p, ctx->hertz32.p, ctx->params.xi,
-                                        ctx->params.dt, -ctx->params.lambda, 1 / ctx->params.tau,
+                                        ctx->hertz.p, ctx->hertz32.p, ctx->binst.p, ctx->idev.p,
                                         ctx->F.p, ctx->U.p, ctx->sig.p, ctx->imp.p);
 }
 

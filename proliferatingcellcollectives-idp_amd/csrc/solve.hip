@@ -14,21 +14,25 @@
 //             ascending constraint order (PETSc's MatMultTranspose row order),
 //             applies mobility, impedance exp(-lambda sigma) and the growth
 //             rate, and writes U and ldot - ldot_prev.  No float atomics.
-//             (k_body, one thread per body, is the same sum and is used for
-//             the final write-back of F, U, sigma.)
+//             (k_body_final, one thread per body, is the same sum and is
+//             used for the final write-back of F, U, sigma.)
 //   k_constr  the same projection per constraint, D U and L dldot gathers,
 //             g = Phi + dt*phidot - dt*phidot_growth, BB dot-product and
 //             residual partials per block (fixed tile->block assignment and
 //             reduction tree, no atomics);
-//   k_finish  one block: reduces the partials in a fixed order and updates
-//             alpha / iteration / convergence on the device.
-// All three exit immediately once the device flags say the solve is done, so
-// the host can enqueue iterations speculatively.
+//   k_finish  one block per colony: reduces its partials in a fixed order and
+//             updates alpha / iteration / convergence on the device.
+// A context may hold many colonies (batched ensembles): every launch covers
+// all colonies of the current phase, each colony has its own solver state,
+// and its blocks exit immediately once its flags say its solve is done, so the
+// host can enqueue iterations speculatively until every colony has finished.
 //
 // Blocks are mapped XCD-aware: the hardware dispatches block h to XCD h % 8,
 // so logical block (h % 8) * (G/8) + h / 8 makes each XCD sweep one
 // contiguous range of bodies / constraints and keeps its gathers in its own L2.
+#include <algorithm>
 #include <cstring>
+#include <vector>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
@@ -171,10 +175,12 @@ __global__ void k_phase_init(int64_t C, const double* __restrict__ Sgam,
   r[1] = make_double2(g, p);
 }
 
-__global__ void k_mobility(int64_t N, const double* __restrict__ len, double xi,
+__global__ void k_mobility(int64_t N, const double* __restrict__ len,
+                           const int32_t* __restrict__ binst, const InstDev* __restrict__ idev,
                            double* __restrict__ mob) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
+  const double xi = idev[binst[i]].xi;
   double l = len[i];
   mob[2 * i] = 1.0 / (l * xi);               // Physics.cpp:203
   mob[2 * i + 1] = 12.0 / (l * l * l * xi);  // Physics.cpp:204
@@ -200,7 +206,11 @@ __global__ void k_planar_check(int64_t N, BodyArrays b, int32_t* __restrict__ no
 // results are bit-identical to the 3D kernels).
 // ---------------------------------------------------------------------------
 struct BodyArgs {
-  int64_t N, C, gp;    // gp: padded grid (multiple of 8)
+  int64_t N;           // bodies of the context
+  int64_t nbb, gp;     // logical body blocks (active colonies); padded grid (multiple of 8)
+  const int32_t* bmap; // logical body block -> colony
+  const int32_t* binst;
+  const InstDev* idev;
   const int32_t* rowptr;
   const void* rec;     // 3D: 8 CT per incidence (coef[6], s, index bits); 2D: 4 CT
   const int32_t* rc;   // 2D: constraint index per incidence
@@ -208,8 +218,7 @@ struct BodyArgs {
   const double* mob;
   const double* len;
   const double* ldot_prev;
-  double mlam, itau;
-  const SolverState* st;
+  const SolverState* st;  // one per colony
   double* ubuf;  // 3D: 8 per body (U[6], dldot, pad); 2D: 4 (ux, uy, wz, dldot)
 };
 
@@ -268,22 +277,20 @@ __device__ __forceinline__ void load_prec<float>(const void* base, int64_t e, do
   co[3] = a.w;
 }
 
-// Body apply.  FINAL = false: one BBPGD iteration (projection on the fly,
-// writes the velocity record).  FINAL = true: the workspace of the converged
-// iterate (HardModelGradient.cpp:89-121), writes F, U, sigma, impedance, ldot.
+// Workspace of the converged iterate (HardModelGradient.cpp:89-121), one
+// thread per body of the colonies being integrated: F = D^T (gamma -
+// gamma_old), U = M F, sigma = L^T gamma, impedance, ldot; the same
+// ascending-constraint sums as the iteration kernel.
 struct FinalOut {
   double *F, *U, *sig, *imp, *ld;
 };
 
-template <class CT, bool PLANAR, bool FINAL>
-__global__ void __launch_bounds__(256) k_body(BodyArgs a, FinalOut fo) {
-  const SolverState* st = a.st;
-  if (!FINAL && (st->converged | st->failed)) return;
-  const int64_t i = FINAL ? (int64_t)blockIdx.x * blockDim.x + threadIdx.x
-                          : xcd_block(blockIdx.x, a.gp) * blockDim.x + threadIdx.x;
+template <class CT, bool PLANAR>
+__global__ void __launch_bounds__(256) k_body_final(BodyArgs a, FinalOut fo) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.N) return;
-  const int64_t k = FINAL ? 1 : st->iter;
-  const double malpha = FINAL ? 0.0 : -st->alpha;
+  const InstDev& D = a.idev[a.binst[i]];
+  if (!(D.flags & kInstInteg)) return;
   double F[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double sig = 0.0;
   const int32_t e1 = a.rowptr[i + 1];
@@ -302,10 +309,8 @@ __global__ void __launch_bounds__(256) k_body(BodyArgs a, FinalOut fo) {
       load_rec<CT>(a.rec, e, co, c);
     }
     const double2* r = reinterpret_cast<const double2*>(a.it + IT_LEN * c);
-    const double2 yg = r[0];
-    const double gold = r[1].x;
-    const double gc = (FINAL || k == 0) ? yg.x : proj_step(yg.x, malpha, yg.y);
-    const double dg = gc - gold;
+    const double gc = r[0].x;
+    const double dg = gc - r[1].x;
     if (PLANAR) {
       F[0] += dg * co[0];
       F[1] += dg * co[1];
@@ -322,28 +327,16 @@ __global__ void __launch_bounds__(256) k_body(BodyArgs a, FinalOut fo) {
   for (int q = 0; q < 3; ++q) u[q] = (0.0 + mt * F[q]) + 0.0;
 #pragma unroll
   for (int q = 3; q < 6; ++q) u[q] = (0.0 + mr * F[q]) + 0.0;
-  const double imp = det_exp(sig * a.mlam);
-  const double ld = (a.len[i] * imp) * a.itau;
-  if (FINAL) {
+  const double imp = det_exp(sig * D.mlam);
+  const double ld = (a.len[i] * imp) * D.itau;
 #pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      fo.F[6 * i + q] = F[q];
-      fo.U[6 * i + q] = u[q];
-    }
-    fo.sig[i] = sig;
-    fo.imp[i] = imp;
-    fo.ld[i] = ld;
-  } else if (PLANAR) {
-    double2* o = reinterpret_cast<double2*>(a.ubuf + 4 * i);
-    o[0] = make_double2(u[0], u[1]);
-    o[1] = make_double2(u[5], ld - a.ldot_prev[i]);
-  } else {
-    double2* o = reinterpret_cast<double2*>(a.ubuf + 8 * i);
-    o[0] = make_double2(u[0], u[1]);
-    o[1] = make_double2(u[2], u[3]);
-    o[2] = make_double2(u[4], u[5]);
-    o[3] = make_double2(ld - a.ldot_prev[i], 0.0);
+  for (int q = 0; q < 6; ++q) {
+    fo.F[6 * i + q] = F[q];
+    fo.U[6 * i + q] = u[q];
   }
+  fo.sig[i] = sig;
+  fo.imp[i] = imp;
+  fo.ld[i] = ld;
 }
 
 // Body apply of one BBPGD iteration, block-cooperative: a block owns 256
@@ -352,19 +345,24 @@ __global__ void __launch_bounds__(256) k_body(BodyArgs a, FinalOut fo) {
 // iterate record (independent loads, full memory-level parallelism) and
 // writes the products dg*coef and gamma*s to LDS.  Phase 2: each thread adds
 // the products of its own body in ascending incidence order, so every sum has
-// exactly the order of the thread-per-body loop above.
+// exactly the order of the thread-per-body loop above.  Blocks never straddle
+// two colonies (each colony starts a new logical block).
 template <class CT, bool PLANAR>
 __global__ void __launch_bounds__(256) k_body_coop(BodyArgs a) {
   constexpr int NC = PLANAR ? 4 : 7;           // products per incidence
   constexpr int CHUNK = PLANAR ? 1024 : 512;   // incidences staged per pass
   __shared__ double prod[NC][CHUNK];
-  const SolverState* st = a.st;
-  if (st->converged | st->failed) return;
-  const int64_t b0 = xcd_block(blockIdx.x, a.gp) * PCC_BLOCK;
-  if (b0 >= a.N) return;
-  const int64_t b1 = b0 + PCC_BLOCK < a.N ? b0 + PCC_BLOCK : a.N;
+  const int64_t l = xcd_block(blockIdx.x, a.gp);
+  if (l >= a.nbb) return;
+  const int32_t inst = a.bmap[l];
+  const SolverState* st = a.st + inst;
+  if (st->skip | st->converged | st->failed) return;
+  const InstDev& D = a.idev[inst];
+  const int64_t bend = D.boff + D.N;
+  const int64_t b0 = D.boff + (l - D.bb) * PCC_BLOCK;
+  const int64_t b1 = b0 + PCC_BLOCK < bend ? b0 + PCC_BLOCK : bend;
   const int64_t i = b0 + threadIdx.x;
-  const bool active = i < a.N;
+  const bool active = i < b1;
   const int64_t k = st->iter;
   const double malpha = -st->alpha;
   const int32_t E0 = a.rowptr[b0], E1 = a.rowptr[b1];
@@ -428,8 +426,8 @@ __global__ void __launch_bounds__(256) k_body_coop(BodyArgs a) {
   for (int q = 0; q < 3; ++q) u[q] = (0.0 + mt * F[q]) + 0.0;
 #pragma unroll
   for (int q = 3; q < 6; ++q) u[q] = (0.0 + mr * F[q]) + 0.0;
-  const double imp = det_exp(sig * a.mlam);
-  const double ld = (a.len[i] * imp) * a.itau;
+  const double imp = det_exp(sig * D.mlam);
+  const double ld = (a.len[i] * imp) * D.itau;
   if (PLANAR) {
     double2* o = reinterpret_cast<double2*>(a.ubuf + 4 * i);
     o[0] = make_double2(u[0], u[1]);
@@ -444,17 +442,18 @@ __global__ void __launch_bounds__(256) k_body_coop(BodyArgs a) {
 }
 
 struct ConstrArgs {
-  int64_t C, G, K, gp;  // G logical blocks of K tiles of 256; gp padded grid
+  int64_t ncb, gp;      // logical constraint blocks (active colonies); padded grid
+  int64_t Gtot;         // stride of the partials arrays
+  const int32_t* cmap;  // logical constraint block -> colony
+  const InstDev* idev;
   const int32_t* I;
   const int32_t* J;
   const void* coef;     // 3D: 12 CT per constraint (n, rIxn, rJx-n, sI, sJ, pad); 2D: 6 CT
   double* it;
   double* gprev;        // lagging iterate saved on iterations % 10 == 0
   const double* ubuf;
-  double dt, tol;
-  int64_t maxit;
-  SolverState* st;
-  double* partials;     // 4 * G
+  SolverState* st;      // one per colony
+  double* partials;     // 4 * Gtot
 };
 
 template <class CT>
@@ -516,20 +515,29 @@ __device__ __forceinline__ void load_pcoef<float>(const void* base, int64_t c, d
   co[5] = d.y;
 }
 
+// A colony with Q tiles of 256 constraints gets G = min(Q, 2048) logical
+// blocks of K = ceil(Q / G) consecutive tiles; every thread accumulates its
+// lanes of those tiles in order, the block tree combines the threads, and
+// k_finish combines the G block partials in a fixed order.
 template <class CT, bool PLANAR>
 __global__ void __launch_bounds__(256) k_constr(ConstrArgs a) {
   __shared__ double lds[4];
-  const SolverState* st = a.st;
-  if (st->converged | st->failed) return;
-  const int64_t b = xcd_block(blockIdx.x, a.gp);
-  if (b >= a.G) return;
+  const int64_t l = xcd_block(blockIdx.x, a.gp);
+  if (l >= a.ncb) return;
+  const int32_t inst = a.cmap[l];
+  const SolverState* st = a.st + inst;
+  if (st->skip | st->converged | st->failed) return;
+  const InstDev& D = a.idev[inst];
+  const int64_t b = l - D.lb;
+  const int64_t K = D.K, cend = D.coff + D.C;
+  const double dt = D.dt;
   const int64_t k = st->iter;
   const double malpha = -st->alpha;
   const bool save_prev = k > 0 && k % 10 == 0;
   double v0 = 0.0, v1 = 0.0, v2 = 0.0, vr = 0.0;
-  for (int64_t j = 0; j < a.K; ++j) {
-    const int64_t c = (b * a.K + j) * PCC_BLOCK + threadIdx.x;
-    if (c >= a.C) break;
+  for (int64_t j = 0; j < K; ++j) {
+    const int64_t c = D.coff + (b * K + j) * PCC_BLOCK + threadIdx.x;
+    if (c >= cend) break;
     double2* r = reinterpret_cast<double2*>(a.it + IT_LEN * c);
     const double2 yg = r[0];  // (gamma_p, g_p)
     const double phi = r[1].y;
@@ -572,7 +580,7 @@ __global__ void __launch_bounds__(256) k_constr(ConstrArgs a) {
       pg += co[9] * i3.x;
       pg += co[10] * j3.x;
     }
-    const double g = phi + (a.dt * pd + (-a.dt) * pg);
+    const double g = phi + (dt * pd + (-dt) * pg);
     if (save_prev) a.gprev[c] = yg.x;
     r[0] = make_double2(gc, g);
     if (k > 0) {
@@ -591,27 +599,29 @@ __global__ void __launch_bounds__(256) k_constr(ConstrArgs a) {
   const double s2 = block_sum(v2, lds);
   const double rm = block_max(vr, lds);
   if (threadIdx.x == 0) {
-    a.partials[b] = s0;
-    a.partials[a.G + b] = s1;
-    a.partials[2 * a.G + b] = s2;
-    a.partials[3 * a.G + b] = rm;
+    a.partials[l] = s0;
+    a.partials[a.Gtot + l] = s1;
+    a.partials[2 * a.Gtot + l] = s2;
+    a.partials[3 * a.Gtot + l] = rm;
   }
 }
 
-// Reduction of the per-block partials in a fixed order and the BB step-size /
-// convergence update (BBPGD.cpp:49-116).  One block of PCC_BLOCK threads.
+// Reduction of a colony's block partials in a fixed order and its BB
+// step-size / convergence update (BBPGD.cpp:49-116).  One block of
+// PCC_BLOCK threads per colony.
 __global__ void __launch_bounds__(256) k_finish(ConstrArgs a) {
   __shared__ double lds[4];
-  SolverState* st = a.st;
-  if (st->converged | st->failed) return;
+  SolverState* st = a.st + blockIdx.x;
+  if (st->skip | st->converged | st->failed) return;
+  const InstDev& D = a.idev[blockIdx.x];
   const int64_t k = st->iter;
-  const int64_t G = a.G;
+  const int64_t G = D.G, o = D.lb, T = a.Gtot;
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, am = 0.0;
   for (int64_t i = threadIdx.x; i < G; i += PCC_BLOCK) {
-    a0 += a.partials[i];
-    a1 += a.partials[G + i];
-    a2 += a.partials[2 * G + i];
-    const double m = a.partials[3 * G + i];
+    a0 += a.partials[o + i];
+    a1 += a.partials[T + o + i];
+    a2 += a.partials[2 * T + o + i];
+    const double m = a.partials[3 * T + o + i];
     am = am < m ? m : am;
   }
   const double S0 = block_sum(a0, lds);
@@ -623,12 +633,12 @@ __global__ void __launch_bounds__(256) k_finish(ConstrArgs a) {
     st->res = R;
     st->alpha = 1.0 / R;
     st->iter = 1;
-    if (1 == a.maxit) st->failed = 1;
+    if (1 == D.maxit) st->failed = 1;
     return;
   }
   if (k % 10 == 0) {
     st->res = R;
-    if (R <= a.tol) {
+    if (R <= D.tol) {
       st->converged = 1;
       return;
     }
@@ -645,16 +655,20 @@ __global__ void __launch_bounds__(256) k_finish(ConstrArgs a) {
   if (fabs(den) < eps10) den += eps10;
   st->alpha = num / den;
   st->iter = k + 1;
-  if (k + 1 == a.maxit) st->failed = 1;
+  if (k + 1 == D.maxit) st->failed = 1;
 }
 
 // dC = G(q) U (Physics.cpp:220-265), Euler step and renormalisation
 // (Particle.cpp:37-57,110-120), write-back of ldot/impedance/stress
 // (ParticleManager.cpp:89-130).
-__global__ void k_integrate(int64_t N, BodyArrays b, double dt, const double* __restrict__ U,
+__global__ void k_integrate(int64_t N, BodyArrays b, const int32_t* __restrict__ binst,
+                            const InstDev* __restrict__ idev, const double* __restrict__ U,
                             const double* __restrict__ ld, double* __restrict__ ldot_field) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
+  const InstDev& D = idev[binst[i]];
+  if (!(D.flags & kInstInteg)) return;
+  const double dt = D.dt;
   const double* u = U + 6 * i;
   double q[4] = {b.qw[i], b.qx[i], b.qy[i], b.qz[i]};
   const double s = q[0], wx = q[1], wy = q[2], wz = q[3];
@@ -689,34 +703,51 @@ __global__ void k_integrate(int64_t N, BodyArrays b, double dt, const double* __
 // updateConstraintsFromSolution (Hard.cpp:49-89): gamma <- lagging iterate
 // (saved at the converging iteration), phi <- linearised gap of the last
 // evaluated iterate.
-__global__ void k_update_constraints(int64_t C, const double* __restrict__ it,
+__global__ void k_update_constraints(int64_t C, const int32_t* __restrict__ Sinst,
+                                     const InstDev* __restrict__ idev,
+                                     const double* __restrict__ it,
                                      const double* __restrict__ gprev,
                                      double* __restrict__ Sgam, double* __restrict__ Sphi) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= C) return;
+  if (!(idev[Sinst[c]].flags & kInstInteg)) return;
   Sgam[c] = gprev[c];
   Sphi[c] = it[IT_LEN * c + IT_G];
 }
 
-// max over the set of max(0, -phi) (Hard.cpp:125-130); max is order-free.
-__global__ void k_maxov(int64_t C, const double* __restrict__ phi, double* __restrict__ part) {
+// Per colony, max over its constraints of max(0, -phi) (Hard.cpp:125-130).
+// Max is order-free; non-negative doubles order like their bit patterns, so
+// an integer atomic max per colony is exact and deterministic.  The set is
+// colony-major: a block whose constraints share one colony reduces first.
+__global__ void k_maxov(int64_t C, const int32_t* __restrict__ Sinst,
+                        const double* __restrict__ phi, unsigned long long* __restrict__ bits) {
   __shared__ double lds[4];
-  double m = 0.0;
-  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < C;
-       c += (int64_t)gridDim.x * blockDim.x) {
+  const int64_t c0 = (int64_t)blockIdx.x * blockDim.x;
+  const int64_t c = c0 + threadIdx.x;
+  const int64_t cl = c0 + PCC_BLOCK - 1 < C ? c0 + PCC_BLOCK - 1 : C - 1;
+  double v = 0.0;
+  if (c < C) {
     const double p = phi[c];
-    const double v = p < 0 ? -p : 0;
-    m = m < v ? v : m;
+    v = p < 0 ? -p : 0.0;
   }
-  const double r = block_max(m, lds);
-  if (threadIdx.x == 0) part[blockIdx.x] = r;
+  const int32_t first = Sinst[c0];
+  if (first == Sinst[cl]) {
+    const double r = block_max(v, lds);
+    if (threadIdx.x == 0) atomicMax(&bits[first], (unsigned long long)__double_as_longlong(r));
+  } else if (c < C) {
+    atomicMax(&bits[Sinst[c]], (unsigned long long)__double_as_longlong(v));
+  }
 }
 
 // grow (ParticleManager.cpp:132-137 -> Particle.cpp:59-65)
-__global__ void k_grow(int64_t N, double dt, double* __restrict__ len,
+__global__ void k_grow(int64_t N, const int32_t* __restrict__ binst,
+                       const InstDev* __restrict__ idev, double* __restrict__ len,
                        const double* __restrict__ l0, const double* __restrict__ ldot) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
+  const InstDev& D = idev[binst[i]];
+  if (!(D.flags & kInstGrow)) return;
+  const double dt = D.dt;
   double l = len[i] + dt * ldot[i];
   if (l > 2 * l0[i]) l = 2 * l0[i];
   len[i] = l;
@@ -724,19 +755,20 @@ __global__ void k_grow(int64_t N, double dt, double* __restrict__ len,
 
 }  // namespace
 
-double max_overlap(pcc_ctx* ctx) {
-  const int64_t C = ctx->S.C;
-  if (C == 0) return 0.0;
-  const int nb = 256;
-  ctx->scratch_d.ensure(nb);
-  k_maxov<<<nb, 256, 0, ctx->stream>>>(C, ctx->S.phi.p, ctx->scratch_d.p);
-  double h[256];
-  PCC_HIP(hipMemcpyAsync(h, ctx->scratch_d.p, sizeof(double) * nb, hipMemcpyDeviceToHost,
-                         ctx->stream));
-  PCC_HIP(hipStreamSynchronize(ctx->stream));
-  double m = 0.0;
-  for (int i = 0; i < nb; ++i) m = m < h[i] ? h[i] : m;
-  return m;
+// Max overlap of every colony's current constraint set (Hard.cpp:125-130).
+void max_overlap_inst(pcc_ctx* ctx, std::vector<double>& out) {
+  const int64_t B = (int64_t)ctx->inst.size(), C = ctx->S.C;
+  out.assign(B, 0.0);
+  if (C == 0) return;
+  hipStream_t s = ctx->stream;
+  ctx->maxbits.ensure(B);
+  PCC_HIP(hipMemsetAsync(ctx->maxbits.p, 0, sizeof(unsigned long long) * B, s));
+  k_maxov<<<nblk(C), 256, 0, s>>>(C, ctx->S.inst.p, ctx->S.phi.p, ctx->maxbits.p);
+  std::vector<unsigned long long> h(B);
+  PCC_HIP(hipMemcpyAsync(h.data(), ctx->maxbits.p, sizeof(unsigned long long) * B,
+                         hipMemcpyDeviceToHost, s));
+  PCC_HIP(hipStreamSynchronize(s));
+  for (int64_t b = 0; b < B; ++b) memcpy(&out[b], &h[b], sizeof(double));
 }
 
 // 1 if every body lies in the z = 0 plane with an in-plane orientation.
@@ -788,14 +820,57 @@ void build_incidence(pcc_ctx* ctx, bool records) {
   }
 }
 
+// Logical block layout of the BBPGD kernels over the active colonies:
+// colony b gets ceil(N_b / 256) body blocks and G_b = min(Q_b, 2048) blocks
+// of K_b = ceil(Q_b / G_b) constraint tiles (Q_b = ceil(C_b / 256); a colony
+// with no constraints still gets one block, as a lone colony does).
+static void solver_layout(pcc_ctx* ctx) {
+  std::vector<int32_t> bm, cm;
+  int64_t lb = 0, bb = 0;
+  for (size_t q = 0; q < ctx->inst.size(); ++q) {
+    Instance& I = ctx->inst[q];
+    if (!(I.flags & kInstActive)) {
+      I.G = 1;
+      I.K = 0;
+      I.lb = 0;
+      I.bb = 0;
+      I.nbb = 0;
+      continue;
+    }
+    const int64_t Q = (I.C + PCC_BLOCK - 1) / PCC_BLOCK;
+    I.G = Q < kBbGridMax ? (Q > 0 ? Q : 1) : kBbGridMax;
+    I.K = (Q + I.G - 1) / I.G;
+    I.lb = lb;
+    lb += I.G;
+    cm.insert(cm.end(), (size_t)I.G, (int32_t)q);
+    I.nbb = (I.N + PCC_BLOCK - 1) / PCC_BLOCK;
+    I.bb = bb;
+    bb += I.nbb;
+    bm.insert(bm.end(), (size_t)I.nbb, (int32_t)q);
+  }
+  ctx->ncb_total = lb;
+  ctx->nbb_total = bb;
+  hipStream_t s = ctx->stream;
+  ctx->cmap.ensure(lb + 1);
+  ctx->bmap.ensure(bb + 1);
+  if (lb > 0)
+    PCC_HIP(hipMemcpyAsync(ctx->cmap.p, cm.data(), sizeof(int32_t) * lb, hipMemcpyHostToDevice, s));
+  if (bb > 0)
+    PCC_HIP(hipMemcpyAsync(ctx->bmap.p, bm.data(), sizeof(int32_t) * bb, hipMemcpyHostToDevice, s));
+  upload_instances(ctx);  // synchronises: the host vectors may go
+}
+
 static BodyArgs body_args(pcc_ctx* ctx) {
   BodyArgs a;
   const int64_t N = ctx->N;
   const bool mixed = ctx->params.precision == PCC_PREC_MIXED;
   BodyArrays b = body_arrays(ctx->body.p, N);
   a.N = N;
-  a.C = ctx->S.C;
-  a.gp = pad8(nblk(N > 0 ? N : 1));
+  a.nbb = ctx->nbb_total;
+  a.gp = std::max<int64_t>(pad8(ctx->nbb_total), kXcd);
+  a.bmap = ctx->bmap.p;
+  a.binst = ctx->binst.p;
+  a.idev = ctx->idev.p;
   a.rowptr = ctx->rowptr.p;
   a.rec = mixed ? (const void*)ctx->increc32.p : (const void*)ctx->increc.p;
   a.rc = ctx->incidx.p;
@@ -803,8 +878,6 @@ static BodyArgs body_args(pcc_ctx* ctx) {
   a.mob = ctx->mob.p;
   a.len = b.len;
   a.ldot_prev = b.ldot;
-  a.mlam = -ctx->params.lambda;
-  a.itau = 1 / ctx->params.tau;
   a.st = ctx->state.p;
   a.ubuf = ctx->ubuf.p;
   return a;
@@ -814,11 +887,11 @@ static ConstrArgs constr_args(pcc_ctx* ctx) {
   ConstrArgs a;
   ConstraintSet& S = ctx->S;
   const bool mixed = ctx->params.precision == PCC_PREC_MIXED;
-  const int64_t Q = (S.C + PCC_BLOCK - 1) / PCC_BLOCK;
-  a.C = S.C;
-  a.G = Q < kBbGridMax ? (Q > 0 ? Q : 1) : kBbGridMax;
-  a.K = (Q + a.G - 1) / a.G;
-  a.gp = pad8(a.G);
+  a.ncb = ctx->ncb_total;
+  a.gp = std::max<int64_t>(pad8(ctx->ncb_total), kXcd);
+  a.Gtot = std::max<int64_t>(ctx->ncb_total, 1);
+  a.cmap = ctx->cmap.p;
+  a.idev = ctx->idev.p;
   a.I = S.I.p;
   a.J = S.J.p;
   if (ctx->planar)
@@ -828,9 +901,6 @@ static ConstrArgs constr_args(pcc_ctx* ctx) {
   a.it = ctx->iter.p;
   a.gprev = ctx->gprev.p;
   a.ubuf = ctx->ubuf.p;
-  a.dt = ctx->params.dt;
-  a.tol = ctx->params.bbpgd_tol;
-  a.maxit = ctx->params.max_bbpgd_iterations;
   a.st = ctx->state.p;
   a.partials = ctx->partials.p;
   return a;
@@ -857,28 +927,60 @@ double constr_bytes(int64_t N, int64_t C, bool mixed, bool planar) {
   return C * (8 + row + 24 + 16) + N * ub;
 }
 
-// Runs BBPGD on the current phase operators.  Returns its status; *iters and
-// *res as BBPGDResult (BBPGD.h:13-16).
-pcc_status bbpgd_solve(pcc_ctx* ctx, int64_t* iters, double* res) {
+static void ensure_state_host(pcc_ctx* ctx, int64_t slots) {
+  if (ctx->state_host_cap >= slots) return;
+  if (ctx->state_host) PCC_HIP(hipHostFree(ctx->state_host));
+  ctx->state_host = nullptr;
+  ctx->state_host_cap = 0;
+  PCC_HIP(hipHostMalloc((void**)&ctx->state_host, sizeof(SolverState) * slots,
+                        hipHostMallocDefault));
+  ctx->state_host_cap = slots;
+}
+
+// Runs BBPGD for every active colony at once on the current phase operators
+// (BBPGD.cpp:32-127).  Each colony has its own step size, iteration count and
+// convergence flag on the device; a converged colony's blocks exit at once
+// while the others go on.  On return every active colony's solve_its /
+// solve_res / solve_failed are set (BBPGDResult, BBPGD.h:13-16).
+void bbpgd_solve(pcc_ctx* ctx) {
   const int64_t N = ctx->N, C = ctx->S.C;
+  const int64_t B = (int64_t)ctx->inst.size();
   const bool mixed = ctx->params.precision == PCC_PREC_MIXED;
   const bool planar = ctx->planar;
   hipStream_t s = ctx->stream;
+  solver_layout(ctx);
   ctx->iter.ensure(IT_LEN * (C + 1));
   ctx->gprev.ensure(C + 1);
   ctx->ubuf.ensure(8 * N + 8);
   ctx->mob.ensure(2 * N + 2);
-  ctx->partials.ensure(4 * kBbGridMax + 4);
+  ctx->partials.ensure(4 * std::max<int64_t>(ctx->ncb_total, 1) + 4);
+  ctx->state.ensure(B);
+  ensure_state_host(ctx, 2 * B);
   BodyArrays b = body_arrays(ctx->body.p, N);
   if (C > 0)
     k_phase_init<<<nblk(C), 256, 0, s>>>(C, ctx->S.gamma.p, ctx->S.phi.p, ctx->iter.p);
-  if (N > 0) k_mobility<<<nblk(N), 256, 0, s>>>(N, b.len, ctx->params.xi, ctx->mob.p);
-  SolverState init = {};
-  PCC_HIP(hipMemcpyAsync(ctx->state.p, &init, sizeof(SolverState), hipMemcpyHostToDevice, s));
+  if (N > 0)
+    k_mobility<<<nblk(N), 256, 0, s>>>(N, b.len, ctx->binst.p, ctx->idev.p, ctx->mob.p);
+  int64_t Nact = 0, Cact = 0;
+  {
+    std::vector<SolverState> init(B);
+    memset(init.data(), 0, sizeof(SolverState) * B);
+    for (int64_t q = 0; q < B; ++q) {
+      if (ctx->inst[q].flags & kInstActive) {
+        Nact += ctx->inst[q].N;
+        Cact += ctx->inst[q].C;
+      } else {
+        init[q].skip = 1;
+      }
+    }
+    PCC_HIP(hipMemcpyAsync(ctx->state.p, init.data(), sizeof(SolverState) * B,
+                           hipMemcpyHostToDevice, s));
+    PCC_HIP(hipStreamSynchronize(s));
+  }
   BodyArgs ba = body_args(ctx);
   ConstrArgs ca = constr_args(ctx);
   const unsigned gb = (unsigned)ba.gp, gc = (unsigned)ca.gp;
-  const double bb = body_bytes(N, C, mixed, planar), cb = constr_bytes(N, C, mixed, planar);
+  const double bb = body_bytes(Nact, Cact, mixed, planar), cb = constr_bytes(Nact, Cact, mixed, planar);
   auto one_iteration = [&]() {
     {
       KTimer t(ctx, PCC_K_BODY_APPLY, bb);
@@ -908,7 +1010,7 @@ pcc_status bbpgd_solve(pcc_ctx* ctx, int64_t* iters, double* res) {
           k_constr<double, false><<<gc, 256, 0, s>>>(ca);
       }
     }
-    k_finish<<<1, 256, 0, s>>>(ca);
+    k_finish<<<(unsigned)B, 256, 0, s>>>(ca);
   };
   one_iteration();  // k = 0
   const int chunk = 10;
@@ -920,28 +1022,36 @@ pcc_status bbpgd_solve(pcc_ctx* ctx, int64_t* iters, double* res) {
   bool done = false;
   while (!done) {
     for (int q = 0; q < chunk; ++q) one_iteration();
-    PCC_HIP(hipMemcpyAsync(&hs[j & 1], ctx->state.p, sizeof(SolverState), hipMemcpyDeviceToHost, s));
+    PCC_HIP(hipMemcpyAsync(hs + (j & 1) * B, ctx->state.p, sizeof(SolverState) * B,
+                           hipMemcpyDeviceToHost, s));
     PCC_HIP(hipEventRecord(ev[j & 1], s));
     if (j >= 1) {
       PCC_HIP(hipEventSynchronize(ev[(j - 1) & 1]));
-      const SolverState& h = hs[(j - 1) & 1];
-      if (h.converged || h.failed) done = true;
+      const SolverState* h = hs + ((j - 1) & 1) * B;
+      done = true;
+      for (int64_t q = 0; q < B && done; ++q)
+        if (!(h[q].skip | h[q].converged | h[q].failed)) done = false;
     }
     ++j;
   }
   PCC_HIP(hipStreamSynchronize(s));
   (void)hipEventDestroy(ev[0]);
   (void)hipEventDestroy(ev[1]);
-  SolverState fin;
-  PCC_HIP(hipMemcpy(&fin, ctx->state.p, sizeof(SolverState), hipMemcpyDeviceToHost));
-  *iters = fin.iter;
-  *res = fin.res;
-  return fin.failed ? PCC_E_BBPGD_NOT_CONVERGED : PCC_OK;
+  std::vector<SolverState> fin(B);
+  PCC_HIP(hipMemcpyAsync(fin.data(), ctx->state.p, sizeof(SolverState) * B, hipMemcpyDeviceToHost, s));
+  PCC_HIP(hipStreamSynchronize(s));
+  for (int64_t q = 0; q < B; ++q) {
+    Instance& I = ctx->inst[q];
+    if (!(I.flags & kInstActive)) continue;
+    I.solve_its = fin[q].iter;
+    I.solve_res = fin[q].res;
+    I.solve_failed = fin[q].failed != 0;
+  }
 }
 
-// Workspace of the converged iterate, move, write-back, constraint update.
-void integrate_hard(pcc_ctx* ctx, int64_t kf) {
-  (void)kf;
+// Workspace of the converged iterate, move, write-back, constraint update for
+// the colonies flagged kInstInteg.
+void integrate_hard(pcc_ctx* ctx) {
   const int64_t N = ctx->N, C = ctx->S.C;
   const bool mixed = ctx->params.precision == PCC_PREC_MIXED;
   hipStream_t s = ctx->stream;
@@ -953,27 +1063,29 @@ void integrate_hard(pcc_ctx* ctx, int64_t kf) {
     const unsigned g = nblk(N);
     if (ctx->planar) {
       if (mixed)
-        k_body<float, true, true><<<g, 256, 0, s>>>(ba, fo);
+        k_body_final<float, true><<<g, 256, 0, s>>>(ba, fo);
       else
-        k_body<double, true, true><<<g, 256, 0, s>>>(ba, fo);
+        k_body_final<double, true><<<g, 256, 0, s>>>(ba, fo);
     } else {
       if (mixed)
-        k_body<float, false, true><<<g, 256, 0, s>>>(ba, fo);
+        k_body_final<float, false><<<g, 256, 0, s>>>(ba, fo);
       else
-        k_body<double, false, true><<<g, 256, 0, s>>>(ba, fo);
+        k_body_final<double, false><<<g, 256, 0, s>>>(ba, fo);
     }
-    k_integrate<<<g, 256, 0, s>>>(N, b, ctx->params.dt, ctx->U.p, ctx->ldot_new.p, b.ldot);
+    k_integrate<<<g, 256, 0, s>>>(N, b, ctx->binst.p, ctx->idev.p, ctx->U.p, ctx->ldot_new.p,
+                                  b.ldot);
   }
   if (C > 0)
-    k_update_constraints<<<nblk(C), 256, 0, s>>>(C, ctx->iter.p, ctx->gprev.p, ctx->S.gamma.p,
-                                                 ctx->S.phi.p);
+    k_update_constraints<<<nblk(C), 256, 0, s>>>(C, ctx->S.inst.p, ctx->idev.p, ctx->iter.p,
+                                                 ctx->gprev.p, ctx->S.gamma.p, ctx->S.phi.p);
 }
 
+// Growth of the colonies flagged kInstGrow.
 void grow(pcc_ctx* ctx) {
   const int64_t N = ctx->N;
   if (N == 0) return;
   BodyArrays b = body_arrays(ctx->body.p, N);
-  k_grow<<<nblk(N), 256, 0, ctx->stream>>>(N, ctx->params.dt, b.len, b.l0, b.ldot);
+  k_grow<<<nblk(N), 256, 0, ctx->stream>>>(N, ctx->binst.p, ctx->idev.p, b.len, b.l0, b.ldot);
 }
 
 }  // namespace pcc

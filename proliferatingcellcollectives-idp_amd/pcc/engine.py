@@ -55,9 +55,27 @@ def lib():
         L.pcc_kernel_name.restype = C.c_char_p
         L.pcc_get_phase_stats.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                           C.c_int64, C.POINTER(C.c_int64)]
+        i64 = C.c_int64
+        L.pcc_batch_set_state.argtypes = [vp, i64, C.POINTER(T.BodiesC), C.POINTER(T.Params)]
+        L.pcc_batch_add.argtypes = [vp, C.POINTER(T.BodiesC), C.POINTER(T.Params),
+                                    C.POINTER(i64)]
+        L.pcc_batch_remove.argtypes = [vp, i64]
+        L.pcc_batch_set_params.argtypes = [vp, i64, C.POINTER(T.Params)]
+        L.pcc_batch_size.argtypes = [vp, C.POINTER(i64), C.POINTER(i64)]
+        L.pcc_batch_step.argtypes = [vp, C.c_int32, C.POINTER(T.StepResult),
+                                     C.POINTER(C.c_int)]
+        L.pcc_batch_get_state.argtypes = [vp, i64, C.POINTER(T.BodiesC)]
+        L.pcc_batch_get_constraints.argtypes = [vp, i64, C.POINTER(T.ConstraintsC),
+                                                C.POINTER(i64)]
+        L.pcc_batch_get_phase_stats.argtypes = [vp, i64, C.POINTER(i64), C.POINTER(i64), i64,
+                                                C.POINTER(i64)]
         for f in ("pcc_create", "pcc_set_params", "pcc_set_state", "pcc_get_state",
                   "pcc_snapshot", "pcc_restore", "pcc_step", "pcc_get_constraints",
-                  "pcc_detect", "pcc_set_timing", "pcc_get_timing", "pcc_get_phase_stats"):
+                  "pcc_detect", "pcc_set_timing", "pcc_get_timing", "pcc_get_phase_stats",
+                  "pcc_batch_set_state", "pcc_batch_add", "pcc_batch_remove",
+                  "pcc_batch_set_params", "pcc_batch_size", "pcc_batch_step",
+                  "pcc_batch_get_state", "pcc_batch_get_constraints",
+                  "pcc_batch_get_phase_stats"):
             getattr(L, f).restype = C.c_int
         _lib = L
     return _lib
@@ -66,7 +84,10 @@ def lib():
 EXPORTED = ("pcc_default_params", "pcc_create", "pcc_destroy", "pcc_set_params",
             "pcc_last_error", "pcc_status_string", "pcc_set_state", "pcc_get_state",
             "pcc_snapshot", "pcc_restore", "pcc_step", "pcc_get_constraints", "pcc_detect",
-            "pcc_set_timing", "pcc_get_timing", "pcc_kernel_name", "pcc_get_phase_stats")
+            "pcc_set_timing", "pcc_get_timing", "pcc_kernel_name", "pcc_get_phase_stats",
+            "pcc_batch_set_state", "pcc_batch_add", "pcc_batch_remove", "pcc_batch_set_params",
+            "pcc_batch_size", "pcc_batch_step", "pcc_batch_get_state",
+            "pcc_batch_get_constraints", "pcc_batch_get_phase_stats")
 
 
 class PccError(RuntimeError):
@@ -199,3 +220,96 @@ class ParticleManager:
                 "launches": int(n.value), "total_ms": float(ms.value),
                 "bytes_per_launch": float(by.value)}
         return out
+
+
+class ColonyBatch(ParticleManager):
+    """One libpcc context holding many independent colonies (same mode and
+    precision, every other parameter per colony).  `step()` advances all of
+    them with shared launches and returns one SolverSolution summary per
+    colony; each colony's results equal those of a ParticleManager holding it
+    alone.  Colony indices follow the order of set/add; `remove(q)` shifts the
+    later colonies down."""
+
+    def __init__(self, params: T.Params, device: int = 0):
+        super().__init__(params, device)
+        self.sizes = []
+
+    def set_state(self, colonies, params=None):
+        cs = [b.c_struct() for b in colonies]
+        arr = (T.BodiesC * len(cs))(*cs)
+        parr = None
+        if params is not None:
+            parr = (T.Params * len(params))(*[p.copy() for p in params])
+        self._check(lib().pcc_batch_set_state(self._ctx, len(cs), arr, parr))
+        self.sizes = [b.n for b in colonies]
+        self.n = sum(self.sizes)
+
+    def add(self, bodies: T.Bodies, params: T.Params = None) -> int:
+        s = bodies.c_struct()
+        idx = C.c_int64(-1)
+        pp = C.byref(params.copy()) if params is not None else None
+        self._check(lib().pcc_batch_add(self._ctx, C.byref(s), pp, C.byref(idx)))
+        self.sizes.append(bodies.n)
+        self.n = sum(self.sizes)
+        return int(idx.value)
+
+    def remove(self, q: int):
+        self._check(lib().pcc_batch_remove(self._ctx, int(q)))
+        del self.sizes[q]
+        self.n = sum(self.sizes)
+
+    def set_colony_params(self, q: int, params: T.Params):
+        self._check(lib().pcc_batch_set_params(self._ctx, int(q), C.byref(params.copy())))
+
+    def __len__(self):
+        return len(self.sizes)
+
+    def step(self, i: int = 0):
+        """One timestep of every colony: a list of SolverSolution dicts, each
+        with the colony's solver status under "status"."""
+        B = len(self.sizes)
+        res = (T.StepResult * B)()
+        sts = (C.c_int * B)()
+        self._check(lib().pcc_batch_step(self._ctx, int(i), res, sts))
+        out = []
+        for q in range(B):
+            d = res[q].as_dict()
+            d["status"] = int(sts[q])
+            out.append(d)
+        return out
+
+    def get_state(self, q: int) -> T.Bodies:
+        n = self.sizes[q]
+        b = T.Bodies(np.zeros((n, 3)), np.zeros((n, 4)), np.zeros(n), np.zeros(n), np.zeros(n),
+                     np.zeros(n))
+        s = b.c_struct()
+        self._check(lib().pcc_batch_get_state(self._ctx, int(q), C.byref(s)))
+        return b
+
+    def constraints(self, q: int) -> dict:
+        cnt = C.c_int64(0)
+        probe = T.Constraints(1)
+        ps = probe.c_struct()
+        ps.capacity = 0
+        self._check(lib().pcc_batch_get_constraints(self._ctx, int(q), C.byref(ps),
+                                                    C.byref(cnt)))
+        n = int(cnt.value)
+        cons = T.Constraints(max(n, 1))
+        cs = cons.c_struct()
+        self._check(lib().pcc_batch_get_constraints(self._ctx, int(q), C.byref(cs),
+                                                    C.byref(cnt)))
+        cons.count = n
+        return cons.trimmed()
+
+    def phase_stats(self, q: int = 0):
+        it = np.zeros(256, np.int64)
+        nc = np.zeros(256, np.int64)
+        cnt = C.c_int64(0)
+        self._check(lib().pcc_batch_get_phase_stats(
+            self._ctx, int(q), it.ctypes.data_as(C.POINTER(C.c_int64)),
+            nc.ctypes.data_as(C.POINTER(C.c_int64)), 256, C.byref(cnt)))
+        n = min(int(cnt.value), 256)
+        return list(zip(it[:n].tolist(), nc[:n].tolist()))
+
+    def detect(self, cutoff, capacity=None):
+        raise PccError(T.PCC_E_INVALID_ARGUMENT, "detect() needs a single-colony context")

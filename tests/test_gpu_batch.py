@@ -1,0 +1,161 @@
+"""Batched ensembles on the GPU: many colonies in one context, one launch
+sequence per BBPGD iteration with per-colony step sizes and convergence flags.
+
+The bar: every colony's results in the batch are the ones a context holding it
+alone produces, bit for bit (state, constraint set, SolverSolution summary and
+solver status), in FP64 and in mixed precision, for the hard and the soft
+model.  The single-colony path itself is pinned bit-exact to the FP64 oracle
+in test_gpu_parity.py.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from pcc import generators as G
+from pcc.abi_types import (MODE_SOFT, PCC_E_BBPGD_NOT_CONVERGED, PCC_E_RELCP_MAX_ITERS, PCC_OK,
+                           PREC_MIXED, reference_params)
+
+from test_gpu_parity import assert_same_constraints, assert_same_state, random_3d
+
+pytestmark = pytest.mark.gpu
+LN2 = math.log(2.0)
+
+
+def hard(**kw):
+    base = dict(dt=1e-3, tolerance=1e-6, tau=1 / LN2, cutoff_phase0=0.05, cutoff_phaseK=0.05)
+    base.update(kw)
+    return reference_params(**base)
+
+
+def alone(params, bodies, steps=1):
+    from pcc.engine import ParticleManager
+    pm = ParticleManager(params)
+    pm.set_state(bodies)
+    out = []
+    for k in range(steps):
+        out.append(pm.step(k, raise_on_error=False))
+    res = (out, pm.get_state(), pm.constraints(), pm.phase_stats())
+    pm.close()
+    return res
+
+
+def check_batch_equals_alone(ctx_params, colonies, params, steps=1):
+    from pcc.engine import ColonyBatch
+    want = [alone(p, b, steps) for p, b in zip(params, colonies)]
+    cb = ColonyBatch(ctx_params)
+    cb.set_state(colonies, params)
+    got_r = [cb.step(k) for k in range(steps)]
+    for q in range(len(colonies)):
+        print(f"  colony {q}: n={colonies[q].n} status={got_r[-1][q]['status']} "
+              f"C={got_r[-1][q]['n_constraints']} its={got_r[-1][q]['bbpgd_iterations']}",
+              flush=True)
+        w_r, w_state, w_cons, w_ph = want[q]
+        for k in range(steps):
+            assert got_r[k][q] == w_r[k], (q, k, got_r[k][q], w_r[k])
+        assert_same_state(cb.get_state(q), w_state)
+        assert_same_constraints(cb.constraints(q), w_cons)
+        assert cb.phase_stats(q) == w_ph
+    cb.close()
+
+
+SIZES = [37, 400, 1500, 90, 2600, 1]
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_batch_hard_equals_alone(precision):
+    """Colonies of uneven sizes with their own growth rates and tolerances."""
+    print(f"[batch-hard] precision={precision}", flush=True)
+    colonies = [G.packed_colony(n, seed=20 + i) for i, n in enumerate(SIZES)]
+    params = [hard(precision=precision, tau=1 / (LN2 * (0.5 + 0.2 * i)),
+                   tolerance=1e-6 if i % 2 == 0 else 1e-5) for i in range(len(SIZES))]
+    check_batch_equals_alone(params[0], colonies, params, steps=2)
+
+
+def test_batch_solver_failures_stay_per_colony():
+    """One colony exhausts BBPGD, one exhausts ReLCP phases, one is empty; the
+    others are unaffected and every status is the one it has alone."""
+    print("[batch-fail]", flush=True)
+    colonies = [G.packed_colony(800, seed=30), G.two_rods(0.9), G.two_rods(0.9),
+                G.Bodies(np.zeros((0, 3)), np.zeros((0, 4)), np.zeros(0)),
+                G.packed_colony(700, seed=31)]
+    params = [hard(), hard(max_bbpgd_iterations=5), hard(max_recursive_iterations=1, relcp_tol=1e-6),
+              hard(), hard(relcp_tol=1e-4)]
+    from pcc.engine import ColonyBatch
+    cb = ColonyBatch(params[0])
+    cb.set_state(colonies, params)
+    r = cb.step()
+    assert r[1]["status"] == PCC_E_BBPGD_NOT_CONVERGED
+    assert r[2]["status"] == PCC_E_RELCP_MAX_ITERS
+    assert r[0]["status"] == PCC_OK and r[3]["status"] == PCC_OK and r[4]["status"] == PCC_OK
+    cb.close()
+    check_batch_equals_alone(params[0], colonies, params)
+
+
+def test_batch_3d_and_planar_colonies():
+    """A 3D colony in the batch switches every colony to the 3D kernels, which
+    give the planar kernels' values."""
+    print("[batch-3d]", flush=True)
+    colonies = [G.dense_colony(3000, seed=40), random_3d(300, 41), G.packed_colony(500, seed=42)]
+    params = [hard(), hard(relcp_tol=1e-3, max_recursive_iterations=8), hard()]
+    check_batch_equals_alone(params[0], colonies, params)
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_batch_soft_equals_alone(precision):
+    print(f"[batch-soft] precision={precision}", flush=True)
+    colonies = [G.packed_colony(n, seed=50 + i) for i, n in enumerate([300, 2000, 40])]
+    params = [reference_params(mode=MODE_SOFT, precision=precision, dt=1e-3 / 30, tolerance=1e-6,
+                               tau=1 / (LN2 * (0.6 + 0.3 * i)), kcc=4.0 / 3.0 * 1e4)
+              for i in range(3)]
+    check_batch_equals_alone(params[0], colonies, params, steps=3)
+
+
+def test_batch_matches_oracle_fp64():
+    """Straight to the oracle as well: a batched FP64 colony is bit-identical
+    to the FP64 restatement."""
+    print("[batch-oracle]", flush=True)
+    colonies = [G.packed_colony(600, seed=60), G.random_disc(500, seed=61)]
+    params = [hard(), hard(tau=1.0)]
+    from pcc.engine import ColonyBatch
+    cb = ColonyBatch(params[0])
+    cb.set_state(colonies, params)
+    r = cb.step()
+    for q in range(2):
+        st, res, want_b, want_c, ph, err = oracle.step(params[q], colonies[q])
+        assert r[q]["status"] == st
+        for k in res:
+            assert r[q][k] == res[k], (q, k)
+        assert_same_state(cb.get_state(q), want_b)
+        assert_same_constraints(cb.constraints(q), want_c)
+    cb.close()
+
+
+def test_batch_add_remove_keeps_resident_state():
+    """Continuous batching: colonies join and leave between steps; the others'
+    device-resident state carries on as if they ran alone."""
+    print("[batch-add-remove]", flush=True)
+    from pcc.engine import ColonyBatch, ParticleManager
+    p = hard(precision=PREC_MIXED)
+    a, b, c = G.packed_colony(700, seed=70), G.packed_colony(300, seed=71), G.packed_colony(1200, seed=72)
+    ref = {}
+    for name, bod, steps in (("a", a, 3), ("b", b, 1), ("c", c, 2)):
+        pm = ParticleManager(p)
+        pm.set_state(bod)
+        for k in range(steps):
+            pm.step(k)
+        ref[name] = pm.get_state()
+        pm.close()
+    cb = ColonyBatch(p)
+    cb.set_state([a, b], [p, p])
+    cb.step(0)  # a: 1, b: 1
+    assert_same_state(cb.get_state(1), ref["b"])
+    cb.remove(1)  # b leaves
+    qc = cb.add(c, p)  # c joins
+    assert qc == 1
+    cb.step(1)  # a: 2, c: 1
+    cb.step(2)  # a: 3, c: 2
+    assert_same_state(cb.get_state(0), ref["a"])
+    assert_same_state(cb.get_state(1), ref["c"])
+    cb.close()
