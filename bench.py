@@ -1,6 +1,6 @@
 """Benchmark of the collision pipeline's hard-model step on one MI355X.
 
-Workload (default): one hard-model timestep of a 300,000-cell colony:
+Headline workload (default): one hard-model timestep of 300,000-cell colonies:
 lengths U[1,2], diameter 0.5, nematic domains with directions U[0,pi) packed
 at area fraction ~0.85 (R ~ 279) with ~1e-3..1e-2 overlaps, contact cutoff
 0.05 in every ReLCP phase, BBPGD and ReLCP tolerance 1e-6, dt = 1e-3,
@@ -10,10 +10,18 @@ not used: ~16% of their contacts are crossing rods whose normal is the
 arbitrary (1,0,0) of ArrayMath::normalize, and BBPGD then does not reach 1e-6
 (3M iterations at 5k and 20k cells).
 
+`--instances B` colonies (seeds seed..seed+B-1) are stepped together by the
+batched mixed-precision engine (one context, shared launches, captured HIP
+graphs); the plain FP64 single-colony path steps the same colonies one after
+another.  `value` is the batched mixed path's cell-steps/s.
+
 A "step" = restore the initial state from a device-resident snapshot (a
-device-to-device copy) + pcc_step.  The state is resident in HBM before the
-timed region starts.  `value` is the mixed-precision path's cell-steps/s; the
-plain-FP64 path is measured beside it on identical inputs.
+device-to-device copy) + the step call.  The state is resident in HBM before
+the timed region starts.
+
+`--workload ensemble`: 128 independent colonies with log-uniform sizes in
+[500, 50000] (~1.5M cells) stepped 50 times in mixed precision by one batched
+context; the state carries on from step to step.
 
 Prints ONE JSON line.  --gpus N > 1 exits: this build runs on one GPU.
 """
@@ -42,35 +50,75 @@ def params_for(precision):
                             precision=precision, max_bbpgd_iterations=2_000_000)
 
 
-def run_gpu(bodies, precision, steps, warmup):
+def run_single(colonies, precision, steps, warmup):
+    """The single-colony path: one context per colony, colonies one after
+    another; per-kernel timing pass on the first colony."""
     from pcc.engine import ParticleManager
-    pm = ParticleManager(params_for(precision), device=0)
-    pm.set_state(bodies)
-    pm.snapshot()
-    results = []
+    total, lat, results, phases, tim = 0.0, [], [], None, None
+    for q, bodies in enumerate(colonies):
+        pm = ParticleManager(params_for(precision), device=0)
+        pm.set_state(bodies)
+        pm.snapshot()
+        for i in range(warmup):
+            pm.restore()
+            pm.step(i)
+        for i in range(steps):
+            a = time.perf_counter()
+            pm.restore()
+            r = pm.step(i)
+            dt = time.perf_counter() - a
+            lat.append(dt)
+            total += dt
+            results.append(r)
+        if q == 0:
+            phases = pm.phase_stats()
+            pm.set_timing(True)
+            pm.restore()
+            pm.step(0)
+            tim = pm.timing()
+            pm.set_timing(False)
+        pm.close()
+    return {"total_s": total, "lat": lat, "results": results, "timing": tim, "phases": phases}
+
+
+def run_batched(colonies, precision, steps, warmup, timing=True, restore=True):
+    """All colonies in one batched context; one latency per batch step."""
+    from pcc.engine import ColonyBatch
+    cb = ColonyBatch(params_for(precision), device=0)
+    cb.set_state(colonies, [params_for(precision)] * len(colonies))
+    if restore:
+        cb.snapshot()
     for i in range(warmup):
-        pm.restore()
-        results.append(pm.step(i))
-    lat = []
+        if restore:
+            cb.restore()
+        cb.step(i)
+    lat, results = [], []
     t0 = time.perf_counter()
     for i in range(steps):
         a = time.perf_counter()
-        pm.restore()
-        r = pm.step(i)
+        if restore:
+            cb.restore()
+        results.append(cb.step(i))
         lat.append(time.perf_counter() - a)
-        results.append(r)
     total = time.perf_counter() - t0
-    # kernel timing pass: one more step with HIP events around every launch of
-    # the BBPGD kernels, on the context stream
-    phases = pm.phase_stats()
-    pm.set_timing(True)
-    pm.restore()
-    pm.step(0)
-    tim = pm.timing()
-    pm.set_timing(False)
-    pm.close()
-    last = results[-1]
-    return {"total_s": total, "lat": lat, "result": last, "timing": tim, "phases": phases}
+    phases = [cb.phase_stats(q) for q in range(len(colonies))]
+    tim = None
+    if timing:
+        # per-kernel pass: HIP events around every BBPGD launch on the context
+        # stream (plain launches, as event records cannot sit in a graph)
+        cb.set_timing(True)
+        if restore:
+            cb.restore()
+        cb.step(0)
+        tim = cb.timing()
+        cb.set_timing(False)
+    cb.close()
+    return {"total_s": total, "lat": lat, "results": results, "timing": tim, "phases": phases}
+
+
+def latency_stats(lat):
+    a = np.array(lat) * 1e3
+    return float(np.percentile(a, 50)), float(np.percentile(a, 99))
 
 
 def roofline(tim):
@@ -131,44 +179,110 @@ def cpu_baseline(bodies, phases, sample_caps=(10, 110)):
     }
 
 
+def ensemble_colonies(n_colonies, seed, lo=500, hi=50_000):
+    u = np.array(G.uniform(seed + 977, 0, n_colonies))
+    sizes = np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo))).astype(int)
+    return [G.packed_colony(int(n), seed=seed + 1000 + i) for i, n in enumerate(sizes)]
+
+
+def main_ensemble(args):
+    cols = ensemble_colonies(args.colonies, args.seed)
+    ncell = sum(c.n for c in cols)
+    print(f"[bench] ensemble: {len(cols)} colonies, {ncell} cells, sizes "
+          f"{min(c.n for c in cols)}..{max(c.n for c in cols)}", file=sys.stderr, flush=True)
+    steps = args.steps if args.steps_given else 50
+    r = run_batched(cols, PREC_MIXED, steps, args.warmup, timing=True, restore=False)
+    p50, p99 = latency_stats(r["lat"])
+    its = [sum(x["bbpgd_iterations"] for x in step) for step in r["results"]]
+    rf = roofline(r["timing"])
+    line = {
+        "metric": "cell-steps/s, batched ensemble of independent colonies; p50/p99 step latency",
+        "value": ncell * steps / r["total_s"],
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["total_s"] / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": (f"{len(cols)} independent hard-model colony packings, sizes "
+                                f"log-uniform in [500, 50000] ({ncell} cells), mixed precision, "
+                                f"{steps} consecutive steps in one batched context with captured "
+                                f"BBPGD graphs and per-colony convergence"),
+                   "n_cells": ncell, "colonies": len(cols)},
+        "p50_step_ms": p50,
+        "p99_step_ms": p99,
+        "bbpgd_iterations_per_step": {"min": int(min(its)), "max": int(max(its))},
+        "roofline": {"bound": "hbm", "kernel": rf["kernel"], "achieved": rf["gbs"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": rf["gbs"] / HBM_PEAK_GBS,
+                     "traffic": None, "avg_launch_us": rf["avg_us"],
+                     "algorithmic_bytes_per_launch": rf["bytes"]},
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=300_000)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--instances", type=int, default=4)
+    ap.add_argument("--colonies", type=int, default=128)
+    ap.add_argument("--workload", choices=["colony300k", "ensemble"], default="colony300k")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["both", "fp64", "mixed"], default="both")
     args = ap.parse_args()
+    args.steps_given = args.steps is not None
+    if args.steps is None:
+        args.steps = 3
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported",
               file=sys.stderr)
         sys.exit(2)
-    bodies = G.dense_colony(args.n, seed=args.seed)
-    print(f"[bench] n={bodies.n} R={G.disc_radius(bodies):.1f} "
+    if args.workload == "ensemble":
+        return main_ensemble(args)
+    cols = [G.dense_colony(args.n, seed=args.seed + q) for q in range(args.instances)]
+    bodies = cols[0]
+    ncell = sum(c.n for c in cols)
+    print(f"[bench] {len(cols)} x n={bodies.n} R={G.disc_radius(bodies):.1f} "
           f"area_fraction={G.area_fraction(bodies):.3f}", file=sys.stderr, flush=True)
     out = {}
-    for prec, name in ((PREC_FP64, "fp64"), (PREC_MIXED, "mixed")):
-        if args.only not in ("both", name):
-            continue
-        r = run_gpu(bodies, prec, args.steps, args.warmup)
-        lat = np.array(r["lat"]) * 1e3
-        out[name] = {
-            "cell_steps_per_s": bodies.n * args.steps / r["total_s"],
-            "ms_per_step": r["total_s"] / args.steps * 1e3,
-            "p50_ms": float(np.percentile(lat, 50)),
-            "p99_ms": float(np.percentile(lat, 99)),
-            "bbpgd_iterations": r["result"]["bbpgd_iterations"],
-            "n_constraints": r["result"]["n_constraints"],
-            "residual": r["result"]["residual"],
+    if args.only in ("both", "fp64"):
+        r = run_single(cols, PREC_FP64, args.steps, args.warmup)
+        p50, p99 = latency_stats(r["lat"])
+        out["fp64"] = {
+            "cell_steps_per_s": ncell * args.steps / r["total_s"],
+            "ms_per_step": r["total_s"] / (args.steps * len(cols)) * 1e3,
+            "p50_ms": p50, "p99_ms": p99,
+            "bbpgd_iterations": [x["bbpgd_iterations"] for x in r["results"][::args.steps]],
+            "n_constraints": [x["n_constraints"] for x in r["results"][::args.steps]],
             "roofline": roofline(r["timing"]),
-            "kernels": r["timing"],
             "phases": r["phases"],
         }
-        print(f"[bench] {name}: {out[name]['ms_per_step']:.1f} ms/step, "
-              f"{out[name]['bbpgd_iterations']} BBPGD iterations, "
-              f"C={out[name]['n_constraints']}", file=sys.stderr, flush=True)
+        print(f"[bench] fp64 single-colony path: {out['fp64']['ms_per_step']:.1f} ms per colony-step, "
+              f"iterations {out['fp64']['bbpgd_iterations']}", file=sys.stderr, flush=True)
+    if args.only in ("both", "mixed"):
+        r = run_batched(cols, PREC_MIXED, args.steps, args.warmup)
+        p50, p99 = latency_stats(r["lat"])
+        last = r["results"][-1]
+        out["mixed"] = {
+            "cell_steps_per_s": ncell * args.steps / r["total_s"],
+            "ms_per_step": r["total_s"] / args.steps * 1e3,
+            "p50_ms": p50, "p99_ms": p99,
+            "bbpgd_iterations": [x["bbpgd_iterations"] for x in last],
+            "n_constraints": [x["n_constraints"] for x in last],
+            "roofline": roofline(r["timing"]),
+            "kernels": r["timing"],
+            "phases": r["phases"][0],
+        }
+        print(f"[bench] mixed batched ({len(cols)} colonies): {out['mixed']['ms_per_step']:.1f} ms "
+              f"per batch step, iterations {out['mixed']['bbpgd_iterations']}",
+              file=sys.stderr, flush=True)
     head = out.get("mixed") or out.get("fp64")
     rf = head["roofline"]
     line = {
@@ -185,14 +299,15 @@ def main():
         "dtype": "f32+f64" if "mixed" in out else "f64",
         "data": "synthetic",
         "config": {
-            "workload": (f"hard-model single step (full ReLCP), {bodies.n}-cell colony packing "
-                         f"(nematic domains, area fraction {G.area_fraction(bodies):.3f}, "
-                         f"R={G.disc_radius(bodies):.0f}, seed {args.seed}), lengths U[1,2], "
-                         f"d=0.5, cutoff 0.05, BBPGD/ReLCP tol 1e-6, dt 1e-3"),
+            "workload": (f"hard-model single step (full ReLCP) of {len(cols)} independent "
+                         f"{bodies.n}-cell colony packings (nematic domains, area fraction "
+                         f"{G.area_fraction(bodies):.3f}, R={G.disc_radius(bodies):.0f}, seeds "
+                         f"{args.seed}..{args.seed + len(cols) - 1}) batched in one context; "
+                         f"lengths U[1,2], d=0.5, cutoff 0.05, BBPGD/ReLCP tol 1e-6, dt 1e-3"),
             "n_cells": bodies.n,
+            "instances": len(cols),
             "precision": "mixed (FP32 bulk applies, FP64 reductions/integration)"
             if "mixed" in out else "fp64",
-            "instances": 1,
         },
         "p50_step_ms": head["p50_ms"],
         "p99_step_ms": head["p99_ms"],

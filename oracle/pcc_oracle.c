@@ -560,13 +560,15 @@ double oracle_block_sum(const double* v, int64_t n) {
 }
 
 /* Three BB sums at once: sum(dg*dg), sum(dg*dp), sum(dp*dp), in the GPU's
- * order: the constraints form Q = ceil(C/256) tiles of 256; G = min(Q, 2048)
- * logical blocks each own K = ceil(Q/G) consecutive tiles; thread t of a
- * block accumulates element t of its tiles in tile order starting from 0.0;
- * then the block tree per block and the partial reduction over blocks. */
+ * order: the constraints, taken in solver order ord[] (stably sorted by the
+ * lower body index I, the layout the GPU solver keeps them in), form
+ * Q = ceil(C/256) tiles of 256; G = min(Q, 2048) logical blocks each own
+ * K = ceil(Q/G) consecutive tiles; thread t of a block accumulates element t
+ * of its tiles in tile order starting from 0.0; then the block tree per block
+ * and the partial reduction over blocks. */
 #define ORACLE_BB_GMAX 2048
 static void bb_sums(const double* gi, const double* gp, const double* ri, const double* rp,
-                    int64_t C, double* Sgg, double* Sgp, double* Spp) {
+                    const int64_t* ord, int64_t C, double* Sgg, double* Sgp, double* Spp) {
   const int64_t Q = (C + 255) / 256;
   const int64_t G = Q < ORACLE_BB_GMAX ? (Q > 0 ? Q : 1) : ORACLE_BB_GMAX;
   const int64_t K = (Q + G - 1) / G;
@@ -577,8 +579,9 @@ static void bb_sums(const double* gi, const double* gp, const double* ri, const 
     for (int t = 0; t < 256; ++t) {
       double a0 = 0.0, a1 = 0.0, a2 = 0.0;
       for (int64_t j = 0; j < K; ++j) {
-        int64_t i = (b * K + j) * 256 + t;
-        if (i >= C) break;
+        int64_t pos = (b * K + j) * 256 + t;
+        if (pos >= C) break;
+        const int64_t i = ord[pos];
         double dgam = gi[i] - gp[i];
         double dphi = ri[i] - rp[i];
         a0 += dgam * dgam;
@@ -643,6 +646,7 @@ typedef struct phase_t {
   double* ldot_prev;
   int64_t* rowptr;  /* body -> incidences (ascending constraint index) */
   int64_t* inc;     /* 2*c + side */
+  int64_t* ord;     /* solver order: constraints stably sorted by I */
   /* workspace (Workspace.h:5-20) */
   double* dgam;
   double* F;        /* 6N */
@@ -667,6 +671,7 @@ static void phase_free(phase_t* ph) {
   free(ph->ldot_prev);
   free(ph->rowptr);
   free(ph->inc);
+  free(ph->ord);
   free(ph->dgam);
   free(ph->F);
   free(ph->U);
@@ -745,6 +750,14 @@ static void phase_build(phase_t* ph, const cset_t* S, const pcc_bodies* b, const
     ph->inc[cur[ph->I[c]]++] = 2 * c;
     ph->inc[cur[ph->J[c]]++] = 2 * c + 1;
   }
+  /* solver order: counting sort of the constraints by I, stable (ties keep
+   * ascending constraint index) */
+  ph->ord = ALLOC(int64_t, C);
+  int64_t* cnt = (int64_t*)calloc((size_t)N + 1, sizeof(int64_t));
+  for (int64_t c = 0; c < C; ++c) cnt[ph->I[c] + 1]++;
+  for (int64_t i = 0; i < N; ++i) cnt[i + 1] += cnt[i];
+  for (int64_t c = 0; c < C; ++c) ph->ord[cnt[ph->I[c]]++] = c;
+  free(cnt);
   free(cur);
   ph->F = ALLOC(double, 6 * N);
   ph->U = ALLOC(double, 6 * N);
@@ -856,7 +869,7 @@ static pcc_status bbpgd(phase_t* ph, double* Gamma, double tol, int64_t maxit, d
     }
     if (cap > 0 && it >= cap) break;
     double Sgg, Sgp, Spp; /* :91-104 */
-    bb_sums(yi, yp, gi, gp, C, &Sgg, &Sgp, &Spp);
+    bb_sums(yi, yp, gi, gp, ph->ord, C, &Sgg, &Sgp, &Spp);
     double num, den;
     if (it % 2 == 0) {
       num = Sgg;

@@ -575,6 +575,8 @@ pcc_status pcc_create(const pcc_params* params, int device, pcc_ctx** out) {
   ctx->device = device;
   const char* planar_env = getenv("PCC_PLANAR");
   ctx->allow_planar = !(planar_env && planar_env[0] == '0');
+  const char* graph_env = getenv("PCC_GRAPH");
+  ctx->use_graphs = !(graph_env && graph_env[0] == '0');
   pcc_status s = guarded(ctx, [&]() {
     PCC_HIP(hipSetDevice(device));
     PCC_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));

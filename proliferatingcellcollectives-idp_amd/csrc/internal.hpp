@@ -176,6 +176,7 @@ struct pcc_ctx {
   bool has_state = false;
   bool planar = false;        // current step uses the 2D monolayer kernels
   bool allow_planar = true;   // PCC_PLANAR=0 in the environment forces the 3D kernels
+  bool use_graphs = true;     // BBPGD chunks as captured HIP graphs (PCC_GRAPH=0: plain launches)
 
   // body state + snapshot + outputs
   pcc::DArr<double> body;      // kBodyFields * N
@@ -206,11 +207,16 @@ struct pcc_ctx {
   pcc::DArr<int32_t> rowptr, inccur, inc;  // N+1, N, 2C
   pcc::DArr<double> increc;                // per incidence: 8 (3D) or 4 (planar) doubles
   pcc::DArr<float> increc32;               // per incidence: 8 or 4 floats
-  pcc::DArr<int32_t> incidx;               // planar: constraint index per incidence
-  pcc::DArr<double> prow;                  // planar operator rows, 6 per constraint
+  pcc::DArr<int32_t> incidx;               // planar: solver position per incidence
+  pcc::DArr<double> gold_e;                // gamma_old per incidence
+  pcc::DArr<int32_t> sord, spos;           // solver order and its inverse
+  pcc::DArr<int32_t> skey, skey2, sval;    // sort scratch
+  pcc::DArr<int32_t> sI, sJ;               // solver order
+  pcc::DArr<double> sphi;                  // solver order
+  pcc::DArr<double> prow;                  // operator rows, solver order (6 planar, 12 3D)
   pcc::DArr<float> prow32;
   pcc::DArr<int32_t> flag;                 // small device flags
-  pcc::DArr<double> iter;                  // 4 per constraint: iterate record
+  pcc::DArr<double> iter;                  // 2 per constraint: (gamma, g), solver order
   pcc::DArr<double> gprev;                 // lagging iterate at convergence checks
   pcc::DArr<double> ubuf;                  // 8 per body: U[6], dldot, pad
   pcc::DArr<double> mob;                   // 2 per body: 1/(xi l), 12/(xi l^3)
@@ -259,6 +265,10 @@ void merge_batch(pcc_ctx* ctx);
 BodyArrays body_arrays(double* base, int64_t N);
 
 // timing helpers
+inline void add_timing_bytes(pcc_ctx* ctx, int kernel, double bytes) {
+  if (ctx->timing.enabled) ctx->timing.bytes[kernel] += bytes;
+}
+
 struct KTimer {
   pcc_ctx* ctx;
   int k;

@@ -44,12 +44,17 @@ namespace pcc {
 void exclusive_scan_i32(pcc_ctx* ctx, const int32_t* in, int32_t* out, int64_t n);
 void seg_sort_i32(pcc_ctx* ctx, int64_t nseg, const int32_t* start, int32_t* v);
 
-// Per-constraint iterate record (4 doubles, 32 bytes): gamma, g of the last
-// evaluated iterate, gamma_old, Phi.  The constraint kernel overwrites
-// (gamma, g) in place; the lagging iterate the reference returns (BBPGD.cpp:124)
-// is only needed where convergence can be declared (iteration % 10 == 0), so
-// it is saved to gprev on those iterations.
-enum { IT_GAM = 0, IT_G = 1, IT_GOLD = 2, IT_PHI = 3, IT_LEN = 4 };
+// Solver layout.  Inside a BBPGD solve the per-constraint arrays are kept in
+// "solver order": the constraint set stably sorted by the lower body index I
+// (sord maps solver position -> constraint, spos the inverse).  A block of
+// consecutive bodies then finds the constraints it gathers from in a few short
+// runs of memory instead of one run per ReLCP phase.  Solver-order arrays:
+// I, J, the operator row, Phi, the iterate record (gamma, g) of the last
+// evaluated iterate (16 bytes, overwritten in place by the constraint kernel)
+// and the lagging iterate gprev that the reference returns (BBPGD.cpp:124),
+// saved on iterations % 10 == 0.  Each body still sums its incidences in
+// ascending constraint order; gamma_old (fixed within a phase) travels with
+// the incidence records.
 static constexpr int kBbGridMax = 2048;
 static constexpr int kXcd = 8;
 
@@ -87,55 +92,73 @@ __global__ void k_inc_scatter(int64_t C, const int32_t* __restrict__ I,
   inc[rowptr[b] + atomicAdd(&cur[b], 1)] = (int32_t)(2 * c + 1);
 }
 
-// Incidence records, body-major: coefficients of the D row on this body's
-// six DOFs ([n, rI x n] for side I, [-n, rJ x -n] for side J), the stress
-// coefficient, and the constraint index, in FP64 and FP32 copies.
+// Solver order: positions of the constraints stably sorted by I.
+__global__ void k_iota(int64_t n, int32_t* __restrict__ v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = (int32_t)i;
+}
+
+__global__ void k_invert(int64_t n, const int32_t* __restrict__ ord, int32_t* __restrict__ pos) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) pos[ord[p]] = (int32_t)p;
+}
+
+// Solver-order copies of the phase's constraint data, and the initial iterate
+// record (gamma_0, 0).  Row layout: planar (n_x, n_y, tauI_z, tauJ_z, sI, sJ)
+// (the 2D monolayer operator: every body has z = 0 and q = (w, 0, 0, z), so
+// n_z, r_z, the x/y torque components and U_z, omega_x, omega_y are exact
+// zeros); 3D the packed row (n, rI x n, rJ x -n, sI, sJ, pad).
+template <bool PLANAR>
+__global__ void k_solver_rows(int64_t C, const int32_t* __restrict__ ord,
+                              const int32_t* __restrict__ I, const int32_t* __restrict__ J,
+                              const double* __restrict__ phi, const double* __restrict__ gam,
+                              const double* __restrict__ coef, int32_t* __restrict__ sI,
+                              int32_t* __restrict__ sJ, double* __restrict__ sphi,
+                              double2* __restrict__ it, double* __restrict__ row,
+                              float* __restrict__ row32) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= C) return;
+  const int64_t c = ord[p];
+  sI[p] = I[c];
+  sJ[p] = J[c];
+  sphi[p] = phi[c];
+  it[p] = make_double2(gam[c], 0.0);
+  const double* r = coef + 12 * c;
+  if (PLANAR) {
+    const double v[6] = {r[0], r[1], r[5], r[8], r[9], r[10]};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      row[6 * p + k] = v[k];
+      row32[6 * p + k] = (float)v[k];
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      row[12 * p + k] = r[k];
+      row32[12 * p + k] = (float)r[k];
+    }
+  }
+}
+
+// Incidence records, body-major: the coefficients of the D row on this body's
+// DOFs ([n, rI x n] for side I, [-n, rJ x -n] for side J; planar (c_x, c_y,
+// tau_z)), the stress coefficient, the constraint's solver position (3D: in
+// the last slot; planar: in rc) and gamma_old, in FP64 and FP32 copies.
+template <bool PLANAR>
 __global__ void k_inc_records(int64_t E, const int32_t* __restrict__ inc,
-                              const double* __restrict__ coef, double* __restrict__ rec,
-                              float* __restrict__ rec32) {
+                              const double* __restrict__ coef, const int32_t* __restrict__ spos,
+                              const double* __restrict__ gam, double* __restrict__ rec,
+                              float* __restrict__ rec32, int32_t* __restrict__ rc,
+                              double* __restrict__ gold) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E) return;
   const int32_t v = inc[e];
   const int64_t c = v >> 1;
   const int side = v & 1;
   const double* row = coef + 12 * c;
-  double co[8];
-  if (side == 0) {
-#pragma unroll
-    for (int k = 0; k < 6; ++k) co[k] = row[k];
-    co[6] = row[9];
-  } else {
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      co[k] = -row[k];
-      co[3 + k] = row[6 + k];
-    }
-    co[6] = row[10];
-  }
-  co[7] = __longlong_as_double((long long)c);
-  double* r = rec + 8 * e;
-#pragma unroll
-  for (int k = 0; k < 8; ++k) r[k] = co[k];
-  float* r32 = rec32 + 8 * e;
-#pragma unroll
-  for (int k = 0; k < 7; ++k) r32[k] = (float)co[k];
-  r32[7] = __int_as_float((int)c);
-}
-
-// Planar (2D monolayer) operator: every body has z = 0 and q = (w, 0, 0, z),
-// so n_z, r_z, the x/y torque components and U_z, omega_x, omega_y are exact
-// zeros.  Incidence record (c_x, c_y, tau_z, s) plus the constraint index;
-// constraint row (n_x, n_y, tauI_z, tauJ_z, sI, sJ).
-__global__ void k_planar_records(int64_t E, int64_t C, const int32_t* __restrict__ inc,
-                                 const double* __restrict__ coef, double* __restrict__ rec,
-                                 float* __restrict__ rec32, int32_t* __restrict__ rc,
-                                 double* __restrict__ prow, float* __restrict__ prow32) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < E) {
-    const int32_t v = inc[e];
-    const int64_t c = v >> 1;
-    const int side = v & 1;
-    const double* row = coef + 12 * c;
+  const int32_t pos = spos[c];
+  gold[e] = gam[c];
+  if (PLANAR) {
     double co[4];
     if (side == 0) {
       co[0] = row[0];
@@ -152,27 +175,30 @@ __global__ void k_planar_records(int64_t E, int64_t C, const int32_t* __restrict
     reinterpret_cast<double2*>(rec)[2 * e + 1] = make_double2(co[2], co[3]);
     reinterpret_cast<float4*>(rec32)[e] =
         make_float4((float)co[0], (float)co[1], (float)co[2], (float)co[3]);
-    rc[e] = (int32_t)c;
-  }
-  if (e < C) {
-    const double* row = coef + 12 * e;
-    const double p[6] = {row[0], row[1], row[5], row[8], row[9], row[10]};
+    rc[e] = pos;
+  } else {
+    double co[8];
+    if (side == 0) {
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      prow[6 * e + k] = p[k];
-      prow32[6 * e + k] = (float)p[k];
+      for (int k = 0; k < 6; ++k) co[k] = row[k];
+      co[6] = row[9];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        co[k] = -row[k];
+        co[3 + k] = row[6 + k];
+      }
+      co[6] = row[10];
     }
+    co[7] = __longlong_as_double((long long)pos);
+    double* r = rec + 8 * e;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) r[k] = co[k];
+    float* r32 = rec32 + 8 * e;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) r32[k] = (float)co[k];
+    r32[7] = __int_as_float(pos);
   }
-}
-
-__global__ void k_phase_init(int64_t C, const double* __restrict__ Sgam,
-                             const double* __restrict__ Sphi, double* __restrict__ it) {
-  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= C) return;
-  const double g = Sgam[c], p = Sphi[c];
-  double2* r = reinterpret_cast<double2*>(it + IT_LEN * c);
-  r[0] = make_double2(g, 0.0);
-  r[1] = make_double2(g, p);
 }
 
 __global__ void k_mobility(int64_t N, const double* __restrict__ len,
@@ -214,7 +240,8 @@ struct BodyArgs {
   const int32_t* rowptr;
   const void* rec;     // 3D: 8 CT per incidence (coef[6], s, index bits); 2D: 4 CT
   const int32_t* rc;   // 2D: constraint index per incidence
-  double* it;          // iterate records, IT_LEN per constraint
+  const double* gold;  // gamma_old per incidence
+  double* it;          // iterate records (gamma, g), solver order
   const double* mob;
   const double* len;
   const double* ldot_prev;
@@ -308,9 +335,8 @@ __global__ void __launch_bounds__(256) k_body_final(BodyArgs a, FinalOut fo) {
     } else {
       load_rec<CT>(a.rec, e, co, c);
     }
-    const double2* r = reinterpret_cast<const double2*>(a.it + IT_LEN * c);
-    const double gc = r[0].x;
-    const double dg = gc - r[1].x;
+    const double gc = reinterpret_cast<const double2*>(a.it)[c].x;
+    const double dg = gc - a.gold[e];
     if (PLANAR) {
       F[0] += dg * co[0];
       F[1] += dg * co[1];
@@ -370,37 +396,57 @@ __global__ void __launch_bounds__(256) k_body_coop(BodyArgs a) {
   const int32_t e_end = active ? a.rowptr[i + 1] : 0;
   double F[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double sig = 0.0;
+  constexpr int PER = CHUNK / PCC_BLOCK;  // incidences per lane and chunk
   for (int32_t base = E0; base < E1; base += CHUNK) {
     const int32_t lim = base + CHUNK < E1 ? base + CHUNK : E1;
-    for (int32_t e = base + (int32_t)threadIdx.x; e < lim; e += PCC_BLOCK) {
-      double co[7];
-      int64_t c;
-      if (PLANAR) {
-        double p[4];
-        load_prec<CT>(a.rec, e, p);
-        co[0] = p[0];
-        co[1] = p[1];
-        co[5] = p[2];
-        co[6] = p[3];
-        c = a.rc[e];
-      } else {
-        load_rec<CT>(a.rec, e, co, c);
+    // Phase 1 in three sweeps so that every lane has all of its loads in
+    // flight at once: records, then the iterate-record gathers, then products.
+    double co[PER][7];
+    int64_t cc[PER];
+    bool ok[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int32_t e = base + (int32_t)threadIdx.x + j * PCC_BLOCK;
+      ok[j] = e < lim;
+      cc[j] = 0;
+      if (ok[j]) {
+        if (PLANAR) {
+          double p[4];
+          load_prec<CT>(a.rec, e, p);
+          co[j][0] = p[0];
+          co[j][1] = p[1];
+          co[j][5] = p[2];
+          co[j][6] = p[3];
+          cc[j] = a.rc[e];
+        } else {
+          load_rec<CT>(a.rec, e, co[j], cc[j]);
+        }
       }
-      const double2* r = reinterpret_cast<const double2*>(a.it + IT_LEN * c);
-      const double2 yg = r[0];
-      const double gold = r[1].x;
-      const double gc = (k == 0) ? yg.x : proj_step(yg.x, malpha, yg.y);
-      const double dg = gc - gold;
-      const int q = e - base;
+    }
+    double2 yg[PER];
+    double gold[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      if (ok[j]) {
+        yg[j] = reinterpret_cast<const double2*>(a.it)[cc[j]];
+        gold[j] = a.gold[base + (int32_t)threadIdx.x + j * PCC_BLOCK];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      if (!ok[j]) continue;
+      const double gc = (k == 0) ? yg[j].x : proj_step(yg[j].x, malpha, yg[j].y);
+      const double dg = gc - gold[j];
+      const int q = (int)threadIdx.x + j * PCC_BLOCK;
       if (PLANAR) {
-        prod[0][q] = dg * co[0];
-        prod[1][q] = dg * co[1];
-        prod[2][q] = dg * co[5];
-        prod[3][q] = gc * co[6];
+        prod[0][q] = dg * co[j][0];
+        prod[1][q] = dg * co[j][1];
+        prod[2][q] = dg * co[j][5];
+        prod[3][q] = gc * co[j][6];
       } else {
 #pragma unroll
-        for (int m = 0; m < 6; ++m) prod[m][q] = dg * co[m];
-        prod[6][q] = gc * co[6];
+        for (int m = 0; m < 6; ++m) prod[m][q] = dg * co[j][m];
+        prod[6][q] = gc * co[j][6];
       }
     }
     __syncthreads();
@@ -449,6 +495,7 @@ struct ConstrArgs {
   const int32_t* I;
   const int32_t* J;
   const void* coef;     // 3D: 12 CT per constraint (n, rIxn, rJx-n, sI, sJ, pad); 2D: 6 CT
+  const double* phi;    // Phi, solver order
   double* it;
   double* gprev;        // lagging iterate saved on iterations % 10 == 0
   const double* ubuf;
@@ -456,69 +503,63 @@ struct ConstrArgs {
   double* partials;     // 4 * Gtot
 };
 
-template <class CT>
-__device__ __forceinline__ void load_coef(const void* base, int64_t c, double co[11]);
+// Operator row of one constraint as stored (CT), kept unconverted in
+// registers: planar (nx, ny, tauI_z, tauJ_z, sI, sJ); 3D (n, rI x n,
+// rJ x -n, sI, sJ).
+template <class CT, bool PLANAR>
+struct Row {
+  CT v[PLANAR ? 6 : 12];
+};
 
-template <>
-__device__ __forceinline__ void load_coef<double>(const void* base, int64_t c, double co[11]) {
-  const double2* r = reinterpret_cast<const double2*>(base) + 6 * c;
+template <class CT, bool PLANAR>
+__device__ __forceinline__ Row<CT, PLANAR> load_row(const void* base, int64_t c) {
+  Row<CT, PLANAR> r;
+  constexpr int W = PLANAR ? 6 : 12;
+  if constexpr (sizeof(CT) == 8) {
+    const double2* p = reinterpret_cast<const double2*>(base) + (W / 2) * c;
 #pragma unroll
-  for (int q = 0; q < 5; ++q) {
-    const double2 v = r[q];
-    co[2 * q] = v.x;
-    co[2 * q + 1] = v.y;
+    for (int q = 0; q < W / 2; ++q) {
+      const double2 v = p[q];
+      r.v[2 * q] = v.x;
+      r.v[2 * q + 1] = v.y;
+    }
+  } else if constexpr (PLANAR) {
+    const float2* p = reinterpret_cast<const float2*>(base) + 3 * c;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const float2 v = p[q];
+      r.v[2 * q] = v.x;
+      r.v[2 * q + 1] = v.y;
+    }
+  } else {
+    const float4* p = reinterpret_cast<const float4*>(base) + 3 * c;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const float4 v = p[q];
+      r.v[4 * q] = v.x;
+      r.v[4 * q + 1] = v.y;
+      r.v[4 * q + 2] = v.z;
+      r.v[4 * q + 3] = v.w;
+    }
   }
-  co[10] = r[5].x;
+  return r;
 }
 
-template <>
-__device__ __forceinline__ void load_coef<float>(const void* base, int64_t c, double co[11]) {
-  const float4* r = reinterpret_cast<const float4*>(base) + 3 * c;
-  const float4 a = r[0], b = r[1], d = r[2];
-  co[0] = a.x;
-  co[1] = a.y;
-  co[2] = a.z;
-  co[3] = a.w;
-  co[4] = b.x;
-  co[5] = b.y;
-  co[6] = b.z;
-  co[7] = b.w;
-  co[8] = d.x;
-  co[9] = d.y;
-  co[10] = d.z;
-}
-
-template <class CT>
-__device__ __forceinline__ void load_pcoef(const void* base, int64_t c, double co[6]);
-
-template <>
-__device__ __forceinline__ void load_pcoef<double>(const void* base, int64_t c, double co[6]) {
-  const double2* r = reinterpret_cast<const double2*>(base) + 3 * c;
-  const double2 a = r[0], b = r[1], d = r[2];
-  co[0] = a.x;
-  co[1] = a.y;
-  co[2] = b.x;
-  co[3] = b.y;
-  co[4] = d.x;
-  co[5] = d.y;
-}
-
-template <>
-__device__ __forceinline__ void load_pcoef<float>(const void* base, int64_t c, double co[6]) {
-  const float2* r = reinterpret_cast<const float2*>(base) + 3 * c;
-  const float2 a = r[0], b = r[1], d = r[2];
-  co[0] = a.x;
-  co[1] = a.y;
-  co[2] = b.x;
-  co[3] = b.y;
-  co[4] = d.x;
-  co[5] = d.y;
-}
+// Inputs of one constraint of a tile.
+template <class CT, bool PLANAR>
+struct TileIn {
+  int32_t I, J;
+  double2 yg;  // (gamma_p, g_p)
+  double phi;
+  Row<CT, PLANAR> row;
+};
 
 // A colony with Q tiles of 256 constraints gets G = min(Q, 2048) logical
 // blocks of K = ceil(Q / G) consecutive tiles; every thread accumulates its
 // lanes of those tiles in order, the block tree combines the threads, and
-// k_finish combines the G block partials in a fixed order.
+// k_finish combines the G block partials in a fixed order.  The tile loop is
+// software-pipelined: tile j+1's indices, row and iterate record are loaded
+// while tile j's body-velocity gathers are in flight.
 template <class CT, bool PLANAR>
 __global__ void __launch_bounds__(256) k_constr(ConstrArgs a) {
   __shared__ double lds[4];
@@ -534,55 +575,65 @@ __global__ void __launch_bounds__(256) k_constr(ConstrArgs a) {
   const int64_t k = st->iter;
   const double malpha = -st->alpha;
   const bool save_prev = k > 0 && k % 10 == 0;
+  constexpr int UW = PLANAR ? 4 : 8;  // doubles per body velocity record
+  auto fetch = [&](int64_t c, TileIn<CT, PLANAR>& t) {
+    t.I = a.I[c];
+    t.J = a.J[c];
+    t.yg = reinterpret_cast<const double2*>(a.it)[c];
+    t.phi = a.phi[c];
+    t.row = load_row<CT, PLANAR>(a.coef, c);
+  };
   double v0 = 0.0, v1 = 0.0, v2 = 0.0, vr = 0.0;
-  for (int64_t j = 0; j < K; ++j) {
-    const int64_t c = D.coff + (b * K + j) * PCC_BLOCK + threadIdx.x;
-    if (c >= cend) break;
-    double2* r = reinterpret_cast<double2*>(a.it + IT_LEN * c);
-    const double2 yg = r[0];  // (gamma_p, g_p)
-    const double phi = r[1].y;
+  int64_t c = D.coff + b * K * PCC_BLOCK + threadIdx.x;
+  bool ok = K > 0 && c < cend;
+  TileIn<CT, PLANAR> cur;
+  if (ok) fetch(c, cur);
+  for (int64_t j = 0; j < K && ok; ++j) {
+    const double2* UI = reinterpret_cast<const double2*>(a.ubuf + UW * (int64_t)cur.I);
+    const double2* UJ = reinterpret_cast<const double2*>(a.ubuf + UW * (int64_t)cur.J);
+    double2 ui[UW / 2], uj[UW / 2];
+#pragma unroll
+    for (int q = 0; q < UW / 2; ++q) {
+      ui[q] = UI[q];
+      uj[q] = UJ[q];
+    }
+    const int64_t cn = c + PCC_BLOCK;
+    const bool okn = j + 1 < K && cn < cend;
+    TileIn<CT, PLANAR> nxt;
+    if (okn) fetch(cn, nxt);
+    const double2 yg = cur.yg;
     const double gc = (k == 0) ? yg.x : proj_step(yg.x, malpha, yg.y);
     double pd = 0.0;  // D U_total, columns of body I (lower gid) first
     double pg = 0.0;  // L ldot_diff
+    const CT* co = cur.row.v;
     if (PLANAR) {
-      double co[6];
-      load_pcoef<CT>(a.coef, c, co);
-      const double2* UI = reinterpret_cast<const double2*>(a.ubuf + 4 * (int64_t)a.I[c]);
-      const double2* UJ = reinterpret_cast<const double2*>(a.ubuf + 4 * (int64_t)a.J[c]);
-      const double2 i0 = UI[0], i1 = UI[1], j0 = UJ[0], j1 = UJ[1];
-      pd += co[0] * i0.x;
-      pd += co[1] * i0.y;
-      pd += co[2] * i1.x;
-      pd += (-co[0]) * j0.x;
-      pd += (-co[1]) * j0.y;
-      pd += co[3] * j1.x;
-      pg += co[4] * i1.y;
-      pg += co[5] * j1.y;
+      pd += (double)co[0] * ui[0].x;
+      pd += (double)co[1] * ui[0].y;
+      pd += (double)co[2] * ui[1].x;
+      pd += (-(double)co[0]) * uj[0].x;
+      pd += (-(double)co[1]) * uj[0].y;
+      pd += (double)co[3] * uj[1].x;
+      pg += (double)co[4] * ui[1].y;
+      pg += (double)co[5] * uj[1].y;
     } else {
-      double co[11];
-      load_coef<CT>(a.coef, c, co);
-      const double2* UI = reinterpret_cast<const double2*>(a.ubuf + 8 * (int64_t)a.I[c]);
-      const double2* UJ = reinterpret_cast<const double2*>(a.ubuf + 8 * (int64_t)a.J[c]);
-      const double2 i0 = UI[0], i1 = UI[1], i2 = UI[2], i3 = UI[3];
-      const double2 j0 = UJ[0], j1 = UJ[1], j2 = UJ[2], j3 = UJ[3];
-      pd += co[0] * i0.x;
-      pd += co[1] * i0.y;
-      pd += co[2] * i1.x;
-      pd += co[3] * i1.y;
-      pd += co[4] * i2.x;
-      pd += co[5] * i2.y;
-      pd += (-co[0]) * j0.x;
-      pd += (-co[1]) * j0.y;
-      pd += (-co[2]) * j1.x;
-      pd += co[6] * j1.y;
-      pd += co[7] * j2.x;
-      pd += co[8] * j2.y;
-      pg += co[9] * i3.x;
-      pg += co[10] * j3.x;
+      pd += (double)co[0] * ui[0].x;
+      pd += (double)co[1] * ui[0].y;
+      pd += (double)co[2] * ui[1].x;
+      pd += (double)co[3] * ui[1].y;
+      pd += (double)co[4] * ui[2].x;
+      pd += (double)co[5] * ui[2].y;
+      pd += (-(double)co[0]) * uj[0].x;
+      pd += (-(double)co[1]) * uj[0].y;
+      pd += (-(double)co[2]) * uj[1].x;
+      pd += (double)co[6] * uj[1].y;
+      pd += (double)co[7] * uj[2].x;
+      pd += (double)co[8] * uj[2].y;
+      pg += (double)co[9] * ui[3].x;
+      pg += (double)co[10] * uj[3].x;
     }
-    const double g = phi + (dt * pd + (-dt) * pg);
+    const double g = cur.phi + (dt * pd + (-dt) * pg);
     if (save_prev) a.gprev[c] = yg.x;
-    r[0] = make_double2(gc, g);
+    reinterpret_cast<double2*>(a.it)[c] = make_double2(gc, g);
     if (k > 0) {
       const double dgam = gc - yg.x;
       const double dphi = g - yg.y;
@@ -593,6 +644,9 @@ __global__ void __launch_bounds__(256) k_constr(ConstrArgs a) {
     const double pj = gc > 0 ? g : (g < 0.0 ? g : 0.0);
     const double ar = fabs(pj);
     vr = vr < ar ? ar : vr;
+    c = cn;
+    ok = okn;
+    if (okn) cur = nxt;
   }
   const double s0 = block_sum(v0, lds);
   const double s1 = block_sum(v1, lds);
@@ -702,17 +756,19 @@ __global__ void k_integrate(int64_t N, BodyArrays b, const int32_t* __restrict__
 
 // updateConstraintsFromSolution (Hard.cpp:49-89): gamma <- lagging iterate
 // (saved at the converging iteration), phi <- linearised gap of the last
-// evaluated iterate.
+// evaluated iterate; read at the constraint's solver position.
 __global__ void k_update_constraints(int64_t C, const int32_t* __restrict__ Sinst,
                                      const InstDev* __restrict__ idev,
-                                     const double* __restrict__ it,
+                                     const int32_t* __restrict__ spos,
+                                     const double2* __restrict__ it,
                                      const double* __restrict__ gprev,
                                      double* __restrict__ Sgam, double* __restrict__ Sphi) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= C) return;
   if (!(idev[Sinst[c]].flags & kInstInteg)) return;
-  Sgam[c] = gprev[c];
-  Sphi[c] = it[IT_LEN * c + IT_G];
+  const int32_t p = spos[c];
+  Sgam[c] = gprev[p];
+  Sphi[c] = it[p].y;
 }
 
 // Per colony, max over its constraints of max(0, -phi) (Hard.cpp:125-130).
@@ -803,21 +859,55 @@ void build_incidence(pcc_ctx* ctx, bool records) {
                                         ctx->inc.p);
   seg_sort_i32(ctx, N, ctx->rowptr.p, ctx->inc.p);
   if (!records) return;
-  if (ctx->planar) {
-    ctx->increc.ensure(4 * (2 * C + 1));
-    ctx->increc32.ensure(4 * (2 * C + 1));
-    ctx->incidx.ensure(2 * C + 1);
-    ctx->prow.ensure(6 * (C + 1));
-    ctx->prow32.ensure(6 * (C + 1));
-    k_planar_records<<<nblk(2 * C), 256, 0, s>>>(2 * C, C, ctx->inc.p, ctx->S.coef64.p,
-                                                 ctx->increc.p, ctx->increc32.p, ctx->incidx.p,
-                                                 ctx->prow.p, ctx->prow32.p);
-  } else {
-    ctx->increc.ensure(8 * (2 * C + 1));
-    ctx->increc32.ensure(8 * (2 * C + 1));
-    k_inc_records<<<nblk(2 * C), 256, 0, s>>>(2 * C, ctx->inc.p, ctx->S.coef64.p, ctx->increc.p,
-                                              ctx->increc32.p);
-  }
+  // solver order: stable radix sort of the constraints by I
+  ctx->sord.ensure(C + 1);
+  ctx->spos.ensure(C + 1);
+  ctx->skey.ensure(C + 1);
+  ctx->sval.ensure(C + 1);
+  PCC_HIP(hipMemcpyAsync(ctx->skey.p, ctx->S.I.p, sizeof(int32_t) * C, hipMemcpyDeviceToDevice, s));
+  k_iota<<<nblk(C), 256, 0, s>>>(C, ctx->sval.p);
+  int bits = 1;
+  while (((int64_t)1 << bits) < N) ++bits;
+  ctx->skey2.ensure(C + 1);
+  size_t bytes = 0;
+  PCC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, ctx->skey.p, ctx->skey2.p, ctx->sval.p,
+                                    ctx->sord.p, (size_t)C, 0, bits, s));
+  ctx->sort_tmp.ensure((int64_t)bytes + 16);
+  PCC_HIP(rocprim::radix_sort_pairs(ctx->sort_tmp.p, bytes, ctx->skey.p, ctx->skey2.p,
+                                    ctx->sval.p, ctx->sord.p, (size_t)C, 0, bits, s));
+  k_invert<<<nblk(C), 256, 0, s>>>(C, ctx->sord.p, ctx->spos.p);
+  const int W = ctx->planar ? 6 : 12;
+  ctx->sI.ensure(C + 1);
+  ctx->sJ.ensure(C + 1);
+  ctx->sphi.ensure(C + 1);
+  ctx->iter.ensure(2 * (C + 1));
+  ctx->gprev.ensure(C + 1);
+  ctx->prow.ensure(W * (C + 1));
+  ctx->prow32.ensure(W * (C + 1));
+  ConstraintSet& S = ctx->S;
+  if (ctx->planar)
+    k_solver_rows<true><<<nblk(C), 256, 0, s>>>(C, ctx->sord.p, S.I.p, S.J.p, S.phi.p,
+                                                S.gamma.p, S.coef64.p, ctx->sI.p, ctx->sJ.p,
+                                                ctx->sphi.p, (double2*)ctx->iter.p, ctx->prow.p,
+                                                ctx->prow32.p);
+  else
+    k_solver_rows<false><<<nblk(C), 256, 0, s>>>(C, ctx->sord.p, S.I.p, S.J.p, S.phi.p,
+                                                 S.gamma.p, S.coef64.p, ctx->sI.p, ctx->sJ.p,
+                                                 ctx->sphi.p, (double2*)ctx->iter.p, ctx->prow.p,
+                                                 ctx->prow32.p);
+  const int RW = ctx->planar ? 4 : 8;
+  ctx->increc.ensure(RW * (2 * C + 1));
+  ctx->increc32.ensure(RW * (2 * C + 1));
+  ctx->incidx.ensure(2 * C + 1);
+  ctx->gold_e.ensure(2 * C + 1);
+  if (ctx->planar)
+    k_inc_records<true><<<nblk(2 * C), 256, 0, s>>>(2 * C, ctx->inc.p, S.coef64.p, ctx->spos.p,
+                                                    S.gamma.p, ctx->increc.p, ctx->increc32.p,
+                                                    ctx->incidx.p, ctx->gold_e.p);
+  else
+    k_inc_records<false><<<nblk(2 * C), 256, 0, s>>>(2 * C, ctx->inc.p, S.coef64.p, ctx->spos.p,
+                                                     S.gamma.p, ctx->increc.p, ctx->increc32.p,
+                                                     ctx->incidx.p, ctx->gold_e.p);
 }
 
 // Logical block layout of the BBPGD kernels over the active colonies:
@@ -874,6 +964,7 @@ static BodyArgs body_args(pcc_ctx* ctx) {
   a.rowptr = ctx->rowptr.p;
   a.rec = mixed ? (const void*)ctx->increc32.p : (const void*)ctx->increc.p;
   a.rc = ctx->incidx.p;
+  a.gold = ctx->gold_e.p;
   a.it = ctx->iter.p;
   a.mob = ctx->mob.p;
   a.len = b.len;
@@ -892,12 +983,11 @@ static ConstrArgs constr_args(pcc_ctx* ctx) {
   a.Gtot = std::max<int64_t>(ctx->ncb_total, 1);
   a.cmap = ctx->cmap.p;
   a.idev = ctx->idev.p;
-  a.I = S.I.p;
-  a.J = S.J.p;
-  if (ctx->planar)
-    a.coef = mixed ? (const void*)ctx->prow32.p : (const void*)ctx->prow.p;
-  else
-    a.coef = mixed ? (const void*)S.coef32.p : (const void*)S.coef64.p;
+  (void)S;
+  a.I = ctx->sI.p;
+  a.J = ctx->sJ.p;
+  a.coef = mixed ? (const void*)ctx->prow32.p : (const void*)ctx->prow.p;
+  a.phi = ctx->sphi.p;
   a.it = ctx->iter.p;
   a.gprev = ctx->gprev.p;
   a.ubuf = ctx->ubuf.p;
@@ -949,30 +1039,19 @@ void bbpgd_solve(pcc_ctx* ctx) {
   const bool planar = ctx->planar;
   hipStream_t s = ctx->stream;
   solver_layout(ctx);
-  ctx->iter.ensure(IT_LEN * (C + 1));
-  ctx->gprev.ensure(C + 1);
   ctx->ubuf.ensure(8 * N + 8);
   ctx->mob.ensure(2 * N + 2);
   ctx->partials.ensure(4 * std::max<int64_t>(ctx->ncb_total, 1) + 4);
   ctx->state.ensure(B);
   ensure_state_host(ctx, 2 * B);
   BodyArrays b = body_arrays(ctx->body.p, N);
-  if (C > 0)
-    k_phase_init<<<nblk(C), 256, 0, s>>>(C, ctx->S.gamma.p, ctx->S.phi.p, ctx->iter.p);
   if (N > 0)
     k_mobility<<<nblk(N), 256, 0, s>>>(N, b.len, ctx->binst.p, ctx->idev.p, ctx->mob.p);
-  int64_t Nact = 0, Cact = 0;
   {
     std::vector<SolverState> init(B);
     memset(init.data(), 0, sizeof(SolverState) * B);
-    for (int64_t q = 0; q < B; ++q) {
-      if (ctx->inst[q].flags & kInstActive) {
-        Nact += ctx->inst[q].N;
-        Cact += ctx->inst[q].C;
-      } else {
-        init[q].skip = 1;
-      }
-    }
+    for (int64_t q = 0; q < B; ++q)
+      if (!(ctx->inst[q].flags & kInstActive)) init[q].skip = 1;
     PCC_HIP(hipMemcpyAsync(ctx->state.p, init.data(), sizeof(SolverState) * B,
                            hipMemcpyHostToDevice, s));
     PCC_HIP(hipStreamSynchronize(s));
@@ -980,10 +1059,9 @@ void bbpgd_solve(pcc_ctx* ctx) {
   BodyArgs ba = body_args(ctx);
   ConstrArgs ca = constr_args(ctx);
   const unsigned gb = (unsigned)ba.gp, gc = (unsigned)ca.gp;
-  const double bb = body_bytes(Nact, Cact, mixed, planar), cb = constr_bytes(Nact, Cact, mixed, planar);
   auto one_iteration = [&]() {
     {
-      KTimer t(ctx, PCC_K_BODY_APPLY, bb);
+      KTimer t(ctx, PCC_K_BODY_APPLY, 0.0);
       if (planar) {
         if (mixed)
           k_body_coop<float, true><<<gb, 256, 0, s>>>(ba);
@@ -997,7 +1075,7 @@ void bbpgd_solve(pcc_ctx* ctx) {
       }
     }
     {
-      KTimer t(ctx, PCC_K_CONSTR_APPLY, cb);
+      KTimer t(ctx, PCC_K_CONSTR_APPLY, 0.0);
       if (planar) {
         if (mixed)
           k_constr<float, true><<<gc, 256, 0, s>>>(ca);
@@ -1014,6 +1092,25 @@ void bbpgd_solve(pcc_ctx* ctx) {
   };
   one_iteration();  // k = 0
   const int chunk = 10;
+  // The chunk of `chunk` iterations is launched as one captured HIP graph
+  // (kernel nodes only; the state read-back below stays outside it), unless
+  // per-kernel event timing is on.
+  hipGraphExec_t exec = nullptr;
+  if (ctx->use_graphs && !ctx->timing.enabled) {
+    hipGraph_t graph = nullptr;
+    PCC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (int q = 0; q < chunk; ++q) one_iteration();
+    PCC_HIP(hipStreamEndCapture(s, &graph));
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    PCC_HIP(e);
+  }
+  struct ExecGuard {
+    hipGraphExec_t& e;
+    ~ExecGuard() {
+      if (e) (void)hipGraphExecDestroy(e);
+    }
+  } guard{exec};
   SolverState* hs = ctx->state_host;
   hipEvent_t ev[2];
   PCC_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
@@ -1021,7 +1118,10 @@ void bbpgd_solve(pcc_ctx* ctx) {
   int64_t j = 0;
   bool done = false;
   while (!done) {
-    for (int q = 0; q < chunk; ++q) one_iteration();
+    if (exec)
+      PCC_HIP(hipGraphLaunch(exec, s));
+    else
+      for (int q = 0; q < chunk; ++q) one_iteration();
     PCC_HIP(hipMemcpyAsync(hs + (j & 1) * B, ctx->state.p, sizeof(SolverState) * B,
                            hipMemcpyDeviceToHost, s));
     PCC_HIP(hipEventRecord(ev[j & 1], s));
@@ -1046,6 +1146,11 @@ void bbpgd_solve(pcc_ctx* ctx) {
     I.solve_its = fin[q].iter;
     I.solve_res = fin[q].res;
     I.solve_failed = fin[q].failed != 0;
+    // algorithmic bytes of the launches in which this colony's blocks worked:
+    // iterations 0..iter (the converging one included), or 0..iter-1 on failure
+    const double worked = (double)(fin[q].iter + (I.solve_failed ? 0 : 1));
+    add_timing_bytes(ctx, PCC_K_BODY_APPLY, worked * body_bytes(I.N, I.C, mixed, planar));
+    add_timing_bytes(ctx, PCC_K_CONSTR_APPLY, worked * constr_bytes(I.N, I.C, mixed, planar));
   }
 }
 
@@ -1076,8 +1181,9 @@ void integrate_hard(pcc_ctx* ctx) {
                                   b.ldot);
   }
   if (C > 0)
-    k_update_constraints<<<nblk(C), 256, 0, s>>>(C, ctx->S.inst.p, ctx->idev.p, ctx->iter.p,
-                                                 ctx->gprev.p, ctx->S.gamma.p, ctx->S.phi.p);
+    k_update_constraints<<<nblk(C), 256, 0, s>>>(C, ctx->S.inst.p, ctx->idev.p, ctx->spos.p,
+                                                 (const double2*)ctx->iter.p, ctx->gprev.p,
+                                                 ctx->S.gamma.p, ctx->S.phi.p);
 }
 
 // Growth of the colonies flagged kInstGrow.

@@ -159,3 +159,29 @@ def test_batch_add_remove_keeps_resident_state():
     assert_same_state(cb.get_state(0), ref["a"])
     assert_same_state(cb.get_state(1), ref["c"])
     cb.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_graph_replays_equal_plain_launches(precision, monkeypatch):
+    """BBPGD chunks run as captured HIP graphs by default; every replay must
+    give what plain launches give (PCC_GRAPH=0), step after step, with the
+    host's eager copies and synchronisations between the replays."""
+    print(f"[graph] precision={precision}", flush=True)
+    from pcc.engine import ColonyBatch
+    colonies = [G.packed_colony(n, seed=80 + i) for i, n in enumerate([900, 150, 2200])]
+    params = [hard(precision=precision, tau=1 / (LN2 * (0.7 + 0.3 * i))) for i in range(3)]
+    runs = []
+    for env in ("1", "0"):
+        monkeypatch.setenv("PCC_GRAPH", env)
+        cb = ColonyBatch(params[0])
+        cb.set_state(colonies, params)
+        steps = []
+        for k in range(4):
+            r = cb.step(k)
+            steps.append((r, [cb.get_state(q) for q in range(3)]))
+        cb.close()
+        runs.append(steps)
+    for (ra, sa), (rb, sb) in zip(runs[0], runs[1]):
+        assert ra == rb
+        for q in range(3):
+            assert_same_state(sa[q], sb[q])
