@@ -23,6 +23,11 @@ the timed region starts.
 [500, 50000] (~1.5M cells) stepped 50 times in mixed precision by one batched
 context; the state carries on from step to step.
 
+`--workload serving`: colony-growth requests with Poisson arrivals (20/s for
+10 s), 1k-100k cells each, hard or soft 50/50, growth rate U[0.5,1.5] ln2,
+200 steps each, served by the continuous-batching loop (pcc.serving) under an
+HBM budget; p50/p99 are over every request's per-step latencies.
+
 Prints ONE JSON line.  --gpus N > 1 exits: this build runs on one GPU.
 """
 import argparse
@@ -114,6 +119,24 @@ def run_batched(colonies, precision, steps, warmup, timing=True, restore=True):
         cb.set_timing(False)
     cb.close()
     return {"total_s": total, "lat": lat, "results": results, "timing": tim, "phases": phases}
+
+
+PMC_SUMMARY = os.path.join(REPO, "profiles", "bbpgd_mixed_300k_colony_pmc.txt")
+
+
+def traffic_from_profile(kernel, instances):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE and WRITE_SIZE passes, each scaled
+    by the factor that maps a copy of known size onto its true bytes), or
+    None when no summary for this configuration is committed."""
+    if not os.path.exists(PMC_SUMMARY):
+        return None
+    for line in open(PMC_SUMMARY):
+        f = line.split()
+        if len(f) == 4 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel \
+                and f[2] == f"instances={instances}":
+            return float(f[3])
+    return None
 
 
 def latency_stats(lat):
@@ -224,6 +247,71 @@ def main_ensemble(args):
     print(json.dumps(line))
 
 
+def serving_requests(rate, duration, seed, steps, lo=1_000, hi=100_000):
+    """Poisson arrivals at `rate`/s over `duration` s; colony sizes
+    log-uniform in [lo, hi]; hard or soft 50/50; growth rate U[0.5,1.5] ln2."""
+    from pcc.abi_types import MODE_SOFT
+    from pcc.serving import Request
+    rng = np.random.default_rng(seed)
+    t, reqs = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration:
+            break
+        n = int(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+        rate_g = rng.uniform(0.5, 1.5) * math.log(2)
+        soft = rng.uniform() < 0.5
+        if soft:
+            p = reference_params(mode=MODE_SOFT, dt=1e-3 / 30, tolerance=1e-6, tau=1 / rate_g,
+                                 kcc=4.0 / 3.0 * 1e4, precision=PREC_MIXED)
+        else:
+            p = params_for(PREC_MIXED).replace(tau=1 / rate_g)
+        reqs.append(Request(len(reqs), G.packed_colony(n, seed=seed * 100_000 + len(reqs)), p,
+                            steps, arrival=t))
+    return reqs
+
+
+def main_serving(args):
+    from pcc.serving import ColonyServer
+    reqs = serving_requests(args.rate, args.duration, args.seed, args.serve_steps)
+    ncell = sum(r.bodies.n for r in reqs)
+    print(f"[bench] serving: {len(reqs)} requests over {args.duration:.0f} s, {ncell} cells, "
+          f"{sum(r.params.mode == 1 for r in reqs)} soft", file=sys.stderr, flush=True)
+    srv = ColonyServer(int(args.hbm_budget_gb * 2**30), precision=PREC_MIXED)
+    st = srv.run(reqs, time_limit=args.duration + args.serve_limit,
+                 progress=lambda m: print(f"[serving] {m}", file=sys.stderr, flush=True))
+    srv.close()
+    waits = [r.admitted - r.arrival for r in srv.results.values()]
+    line = {
+        "metric": "cell-steps/s, continuous-batching colony server; p50/p99 per-step latency",
+        "value": st["cell_steps_per_s"],
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.serve_steps,
+        "warmup": 0,
+        "ms_per_step": st["p50_step_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": (f"Poisson arrivals at {args.rate:g} requests/s for "
+                                f"{args.duration:g} s, colonies of 1k-100k cells (log-uniform), "
+                                f"hard or soft 50/50, growth rate U[0.5,1.5] ln2, "
+                                f"{args.serve_steps} steps each, HBM budget "
+                                f"{args.hbm_budget_gb:g} GiB, mixed precision"),
+                   "requests": len(reqs), "n_cells": ncell},
+        "p50_step_ms": st["p50_step_ms"],
+        "p99_step_ms": st["p99_step_ms"],
+        "serving": {k: st[k] for k in ("completed", "cell_steps", "wall_s", "batch_steps",
+                                       "peak_reserved_bytes", "peak_device_bytes",
+                                       "budget_bytes")},
+        "admission_wait_s": {"p50": float(np.percentile(waits, 50)) if waits else 0.0,
+                             "max": float(max(waits)) if waits else 0.0},
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -231,9 +319,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=300_000)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--instances", type=int, default=4)
+    ap.add_argument("--instances", type=int, default=2)
     ap.add_argument("--colonies", type=int, default=128)
-    ap.add_argument("--workload", choices=["colony300k", "ensemble"], default="colony300k")
+    ap.add_argument("--workload", choices=["colony300k", "ensemble", "serving"],
+                    default="colony300k")
+    ap.add_argument("--rate", type=float, default=20.0)
+    ap.add_argument("--duration", type=float, default=10.0)
+    ap.add_argument("--serve-steps", type=int, default=200)
+    ap.add_argument("--hbm-budget-gb", type=float, default=128.0)
+    ap.add_argument("--serve-limit", type=float, default=120.0,
+                    help="seconds of serving after the last arrival before the run stops")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["both", "fp64", "mixed"], default="both")
     args = ap.parse_args()
@@ -246,6 +341,10 @@ def main():
         sys.exit(2)
     if args.workload == "ensemble":
         return main_ensemble(args)
+    if args.workload == "serving":
+        return main_serving(args)
+    from pcc.engine import copy_bandwidth
+    copy_gbs = copy_bandwidth(1 << 30, 10)
     cols = [G.dense_colony(args.n, seed=args.seed + q) for q in range(args.instances)]
     bodies = cols[0]
     ncell = sum(c.n for c in cols)
@@ -318,11 +417,24 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": rf["gbs"] / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic_from_profile(rf["kernel"], len(cols)),
             "avg_launch_us": rf["avg_us"],
             "algorithmic_bytes_per_launch": rf["bytes"],
+            "achievable_copy_gbs": copy_gbs,
+            "frac_of_achievable": rf["gbs"] / copy_gbs,
         },
     }
+    if "mixed" in out:
+        k = out["mixed"]["kernels"]
+        line["kernels"] = {}
+        for name in ("body_apply", "constraint_apply"):
+            t = k[name]
+            if t["launches"]:
+                avg = t["total_ms"] / t["launches"] / 1e3
+                line["kernels"][name] = {"launches": t["launches"], "avg_us": avg * 1e6,
+                                         "algorithmic_bytes_per_launch": t["bytes_per_launch"],
+                                         "gbs": t["bytes_per_launch"] / avg / 1e9,
+                                         "frac": t["bytes_per_launch"] / avg / 1e9 / HBM_PEAK_GBS}
     if "fp64" in out and "mixed" in out:
         line["fp64"] = {k: out["fp64"][k] for k in ("cell_steps_per_s", "ms_per_step", "p50_ms",
                                                     "p99_ms", "bbpgd_iterations", "n_constraints",

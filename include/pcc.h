@@ -201,7 +201,9 @@ pcc_status pcc_batch_set_state(pcc_ctx* ctx, int64_t n_colonies, const pcc_bodie
 /* Appends one colony (state stays resident; *colony receives its index). */
 pcc_status pcc_batch_add(pcc_ctx* ctx, const pcc_bodies* bodies, const pcc_params* params,
                          int64_t* colony);
-/* Removes colony `colony`; later colonies move down by one. */
+/* Removes colony `colony`; later colonies move down by one.  Removing the
+ * last colony leaves the context without state (pcc_batch_add starts a new
+ * batch). */
 pcc_status pcc_batch_remove(pcc_ctx* ctx, int64_t colony);
 pcc_status pcc_batch_set_params(pcc_ctx* ctx, int64_t colony, const pcc_params* params);
 pcc_status pcc_batch_size(pcc_ctx* ctx, int64_t* n_colonies, int64_t* n_bodies);
@@ -217,6 +219,16 @@ pcc_status pcc_batch_get_constraints(pcc_ctx* ctx, int64_t colony, pcc_constrain
                                      int64_t* count);
 pcc_status pcc_batch_get_phase_stats(pcc_ctx* ctx, int64_t colony, int64_t* bbpgd_iterations,
                                      int64_t* n_constraints, int64_t max, int64_t* count);
+
+/* Device memory held by the context (every device buffer it has allocated),
+ * for admission against an HBM budget.  No reference counterpart. */
+pcc_status pcc_device_bytes(pcc_ctx* ctx, int64_t* bytes);
+
+/* Achievable device copy bandwidth (bytes read + written per second / 1e9)
+ * of a grid-stride copy of `bytes` bytes, averaged over `reps` launches: the
+ * practical ceiling beside the published HBM peak.  No reference
+ * counterpart. */
+pcc_status pcc_measure_copy_bandwidth(int device, int64_t bytes, int reps, double* gbs);
 
 /* Per-kernel GPU time accounting with HIP events on the context stream.
  * Kernel ids: see pcc_kernel_name(). */

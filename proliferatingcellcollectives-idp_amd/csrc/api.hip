@@ -853,7 +853,14 @@ pcc_status pcc_batch_remove(pcc_ctx* ctx, int64_t colony) {
   if (!ctx) return PCC_E_INVALID_ARGUMENT;
   if (colony < 0 || colony >= (int64_t)ctx->inst.size())
     return fail(ctx, PCC_E_INVALID_ARGUMENT, "no such colony");
-  if (ctx->inst.size() == 1) return fail(ctx, PCC_E_INVALID_ARGUMENT, "cannot remove the last colony");
+  if (ctx->inst.size() == 1) {
+    ctx->inst.clear();
+    ctx->N = 0;
+    ctx->S.C = 0;
+    ctx->has_state = false;
+    ctx->has_snapshot = false;
+    return PCC_OK;
+  }
   return guarded(ctx, [&]() {
     PCC_HIP(hipSetDevice(ctx->device));
     const Instance& I = ctx->inst[colony];
@@ -892,6 +899,31 @@ pcc_status pcc_get_timing(pcc_ctx* ctx, int kernel, int64_t* launches, double* t
       *bytes_per_launch = T.launches[kernel] ? T.bytes[kernel] / (double)T.launches[kernel] : 0.0;
     return PCC_OK;
   });
+}
+
+pcc_status pcc_device_bytes(pcc_ctx* ctx, int64_t* bytes) {
+  if (!ctx || !bytes) return PCC_E_INVALID_ARGUMENT;
+  int64_t t = 0;
+  auto add = [&](const auto& a) { t += a.cap * (int64_t)sizeof(*a.p); };
+  auto add_set = [&](const ConstraintSet& S) {
+    add(S.I); add(S.J); add(S.gid); add(S.iter); add(S.inst); add(S.phi); add(S.gamma);
+    add(S.n); add(S.rI); add(S.rJ); add(S.p); add(S.sI); add(S.sJ); add(S.coef64);
+    add(S.coef32);
+  };
+  add(ctx->body); add(ctx->snap); add(ctx->F); add(ctx->U); add(ctx->sig); add(ctx->imp);
+  add(ctx->ldot_new); add(ctx->nc); add(ctx->cell); add(ctx->bucket); add(ctx->perm);
+  add(ctx->bstart); add(ctx->bcursor); add(ctx->dcount); add(ctx->rI_); add(ctx->rJ_);
+  add(ctx->rrec); add(ctx->ridx); add(ctx->ridx2); add(ctx->rkey); add(ctx->rkey2);
+  add(ctx->sort_tmp); add_set(ctx->S); add_set(ctx->S2); add(ctx->rowptr); add(ctx->inccur);
+  add(ctx->inc); add(ctx->increc); add(ctx->increc32); add(ctx->incidx); add(ctx->gold_e);
+  add(ctx->sord); add(ctx->spos); add(ctx->skey); add(ctx->skey2); add(ctx->sval); add(ctx->sI);
+  add(ctx->sJ); add(ctx->sphi); add(ctx->prow); add(ctx->prow32); add(ctx->flag);
+  add(ctx->iter); add(ctx->gprev); add(ctx->ubuf); add(ctx->mob); add(ctx->partials);
+  add(ctx->state); add(ctx->scratch_d); add(ctx->hertz); add(ctx->hertz32); add(ctx->idev);
+  add(ctx->binst); add(ctx->bmap); add(ctx->cmap); add(ctx->bbounds); add(ctx->maxbits);
+  add(ctx->regroup_dst);
+  *bytes = t;
+  return PCC_OK;
 }
 
 const char* pcc_kernel_name(int kernel) {

@@ -69,13 +69,16 @@ def lib():
                                                 C.POINTER(i64)]
         L.pcc_batch_get_phase_stats.argtypes = [vp, i64, C.POINTER(i64), C.POINTER(i64), i64,
                                                 C.POINTER(i64)]
+        L.pcc_device_bytes.argtypes = [vp, C.POINTER(i64)]
+        L.pcc_measure_copy_bandwidth.argtypes = [C.c_int, i64, C.c_int, C.POINTER(C.c_double)]
         for f in ("pcc_create", "pcc_set_params", "pcc_set_state", "pcc_get_state",
                   "pcc_snapshot", "pcc_restore", "pcc_step", "pcc_get_constraints",
                   "pcc_detect", "pcc_set_timing", "pcc_get_timing", "pcc_get_phase_stats",
                   "pcc_batch_set_state", "pcc_batch_add", "pcc_batch_remove",
                   "pcc_batch_set_params", "pcc_batch_size", "pcc_batch_step",
                   "pcc_batch_get_state", "pcc_batch_get_constraints",
-                  "pcc_batch_get_phase_stats"):
+                  "pcc_batch_get_phase_stats", "pcc_device_bytes",
+                  "pcc_measure_copy_bandwidth"):
             getattr(L, f).restype = C.c_int
         _lib = L
     return _lib
@@ -87,7 +90,17 @@ EXPORTED = ("pcc_default_params", "pcc_create", "pcc_destroy", "pcc_set_params",
             "pcc_set_timing", "pcc_get_timing", "pcc_kernel_name", "pcc_get_phase_stats",
             "pcc_batch_set_state", "pcc_batch_add", "pcc_batch_remove", "pcc_batch_set_params",
             "pcc_batch_size", "pcc_batch_step", "pcc_batch_get_state",
-            "pcc_batch_get_constraints", "pcc_batch_get_phase_stats")
+            "pcc_batch_get_constraints", "pcc_batch_get_phase_stats", "pcc_device_bytes",
+            "pcc_measure_copy_bandwidth")
+
+
+def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 10, device: int = 0) -> float:
+    """Achievable device copy bandwidth in GB/s (bytes read + written)."""
+    g = C.c_double(0)
+    st = lib().pcc_measure_copy_bandwidth(int(device), int(nbytes), int(reps), C.byref(g))
+    if st != T.PCC_OK:
+        raise PccError(st, "pcc_measure_copy_bandwidth failed")
+    return float(g.value)
 
 
 class PccError(RuntimeError):
@@ -147,6 +160,12 @@ class ParticleManager:
         s = b.c_struct()
         self._check(lib().pcc_get_state(self._ctx, C.byref(s)))
         return b
+
+    def device_bytes(self) -> int:
+        """Device memory held by this context."""
+        b = C.c_int64(0)
+        self._check(lib().pcc_device_bytes(self._ctx, C.byref(b)))
+        return int(b.value)
 
     def snapshot(self):
         self._check(lib().pcc_snapshot(self._ctx))

@@ -1,0 +1,53 @@
+"""The continuous-batching server on the GPU: colonies join and leave the
+hard and the soft batch between steps while others are mid-run, and every
+result is bit-identical to a context that runs the same request alone."""
+import math
+
+import pytest
+
+from pcc import abi_types as T
+from pcc import generators as G
+
+from test_gpu_parity import assert_same_state
+
+pytestmark = pytest.mark.gpu
+LN2 = math.log(2.0)
+
+
+def request_params(mode, rate):
+    if mode == T.MODE_SOFT:
+        return T.reference_params(mode=T.MODE_SOFT, dt=1e-3 / 30, tolerance=1e-6, tau=1 / rate,
+                                  kcc=4.0 / 3.0 * 1e4, precision=T.PREC_MIXED)
+    return T.reference_params(dt=1e-3, tolerance=1e-6, tau=1 / rate, cutoff_phase0=0.05,
+                              cutoff_phaseK=0.05, precision=T.PREC_MIXED)
+
+
+def test_served_results_equal_alone():
+    from pcc.engine import ParticleManager
+    from pcc.serving import ColonyServer, Request
+    spec = [(600, 0, 4, 0.0), (300, 1, 3, 0.0), (900, 0, 2, 0.0), (150, 1, 5, 0.0),
+            (400, 0, 3, 0.0), (1200, 1, 2, 0.0), (250, 0, 6, 0.0)]
+    reqs = []
+    for rid, (n, mode, steps, arr) in enumerate(spec):
+        rate = (0.5 + 0.15 * rid) * LN2
+        reqs.append(Request(rid, G.packed_colony(n, seed=90 + rid), request_params(mode, rate),
+                            steps, arr))
+    # room for ~1500 cells at a time: later requests wait and then join
+    # batches whose other colonies are mid-run
+    srv = ColonyServer(1500 * 10_000, precision=T.PREC_MIXED, bytes_per_cell=10_000)
+    st = srv.run(reqs)
+    srv.close()
+    print(f"[serving] {st}", flush=True)
+    assert st["completed"] == len(reqs)
+    assert st["peak_reserved_bytes"] <= 1500 * 10_000
+    waited = [r for r in srv.results.values() if r.admitted > 0.0]
+    assert waited, "no request had to wait for memory"
+    for r in reqs:
+        res = srv.results[r.rid]
+        assert res.statuses == [T.PCC_OK] * r.steps
+        pm = ParticleManager(r.params)
+        pm.set_state(r.bodies)
+        for k in range(r.steps):
+            pm.step(k)
+        assert_same_state(res.state, pm.get_state())
+        pm.close()

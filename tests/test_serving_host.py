@@ -1,0 +1,111 @@
+"""Host logic of the continuous-batching server (no GPU): admission against
+the HBM budget, first come first served, joining running batches, retirement
+after exactly the requested steps.  The batched context is replaced by an
+in-memory stand-in that advances each colony's lengths by one per step."""
+import time
+
+import numpy as np
+import pytest
+
+from pcc import abi_types as T
+from pcc import generators as G
+from pcc import serving
+
+
+class FakeBatch:
+    instances = []
+
+    def __init__(self, params, device=0):
+        self.mode = params.mode
+        self.cols = []
+        self.events = []
+        FakeBatch.instances.append(self)
+
+    def add(self, bodies, params):
+        assert params.mode == self.mode
+        self.events.append(("add", len(self.cols), tuple(c[1] for c in self.cols)))
+        self.cols.append([bodies.copy(), 0])
+        return len(self.cols) - 1
+
+    def remove(self, q):
+        del self.cols[q]
+
+    def step(self, i):
+        time.sleep(0.002)  # a device step takes time
+        for c in self.cols:
+            c[0].length = c[0].length + 1.0
+            c[1] += 1
+        return [{"status": 0, "n_constraints": 0, "constraint_iterations": 0,
+                 "bbpgd_iterations": 0, "residual": 0.0, "max_overlap": 0.0}
+                for _ in self.cols]
+
+    def get_state(self, q):
+        return self.cols[q][0]
+
+    def device_bytes(self):
+        return 1000 * sum(c[0].n for c in self.cols)
+
+    def close(self):
+        pass
+
+
+@pytest.fixture(autouse=True)
+def fake_batch(monkeypatch):
+    FakeBatch.instances = []
+    monkeypatch.setattr(serving, "ColonyBatch", FakeBatch)
+
+
+def make_requests(spec):
+    reqs = []
+    for rid, (n, mode, steps, arrival) in enumerate(spec):
+        p = T.reference_params(mode=mode, precision=T.PREC_MIXED)
+        reqs.append(serving.Request(rid, G.packed_colony(n, seed=rid), p, steps, arrival))
+    return reqs
+
+
+def test_every_request_runs_exactly_its_steps():
+    reqs = make_requests([(100, 0, 3, 0.0), (200, 1, 5, 0.0), (150, 0, 2, 0.0), (80, 1, 4, 0.0)])
+    srv = serving.ColonyServer(10**12, bytes_per_cell=1000)
+    st = srv.run(reqs)
+    assert st["completed"] == 4
+    for r in reqs:
+        res = srv.results[r.rid]
+        assert len(res.step_ms) == r.steps and res.statuses == [0] * r.steps
+        assert np.array_equal(res.state.length, r.bodies.length + r.steps)
+        assert res.mode == r.params.mode
+    # one batch per model
+    assert sorted(b.mode for b in FakeBatch.instances) == [0, 1]
+
+
+def test_admission_respects_budget_and_order():
+    # budget for 300 cells at 1000 B/cell: the requests must take turns
+    reqs = make_requests([(200, 0, 3, 0.0), (100, 0, 5, 0.0), (250, 0, 2, 0.0), (50, 0, 1, 0.0)])
+    srv = serving.ColonyServer(300 * 1000, bytes_per_cell=1000)
+    st = srv.run(reqs)
+    assert st["completed"] == 4
+    assert st["peak_reserved_bytes"] <= 300 * 1000
+    r = srv.results
+    # first come, first served: request 2 (250 cells) waits for 0 and 1 to
+    # leave although 3 (50 cells) would fit earlier
+    assert r[2].admitted >= r[0].finished and r[2].admitted >= r[1].finished
+    assert r[3].admitted >= r[2].admitted
+    for q in reqs:
+        assert np.array_equal(r[q.rid].state.length, q.bodies.length + q.steps)
+
+
+def test_late_arrival_joins_a_running_batch():
+    reqs = make_requests([(100, 0, 40, 0.0), (100, 0, 3, 0.02)])
+    srv = serving.ColonyServer(10**12, bytes_per_cell=1000)
+    srv.run(reqs)
+    adds = FakeBatch.instances[0].events
+    # the second colony was added while the first had already stepped
+    assert len(adds) == 2 and adds[1][2][0] > 0
+    assert np.array_equal(srv.results[1].state.length, reqs[1].bodies.length + 3)
+
+
+def test_oversized_request_runs_alone():
+    reqs = make_requests([(500, 0, 2, 0.0), (100, 0, 2, 0.0)])
+    srv = serving.ColonyServer(100 * 1000, bytes_per_cell=1000)
+    st = srv.run(reqs)
+    assert st["completed"] == 2
+    assert srv.results[1].admitted >= srv.results[0].finished
