@@ -19,7 +19,8 @@ A "step" = restore the initial state from a device-resident snapshot (a
 device-to-device copy) + the step call.  The state is resident in HBM before
 the timed region starts.
 
-`--workload ensemble`: 128 independent colonies with log-uniform sizes in
+`--workload ensemble`: 128 independent colonies (the same nematic-domain
+packings, standing in for grown colonies) with log-uniform sizes in
 [500, 50000] (~1.5M cells) stepped 50 times in mixed precision by one batched
 context; the state carries on from step to step.
 
@@ -205,7 +206,7 @@ def cpu_baseline(bodies, phases, sample_caps=(10, 110)):
 def ensemble_colonies(n_colonies, seed, lo=500, hi=50_000):
     u = np.array(G.uniform(seed + 977, 0, n_colonies))
     sizes = np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo))).astype(int)
-    return [G.packed_colony(int(n), seed=seed + 1000 + i) for i, n in enumerate(sizes)]
+    return [G.dense_colony(int(n), seed=seed + 1000 + i) for i, n in enumerate(sizes)]
 
 
 def main_ensemble(args):
@@ -231,8 +232,9 @@ def main_ensemble(args):
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": (f"{len(cols)} independent hard-model colony packings, sizes "
-                                f"log-uniform in [500, 50000] ({ncell} cells), mixed precision, "
+        "config": {"workload": (f"{len(cols)} independent hard-model colony packings (nematic "
+                                f"domains, area fraction ~0.85), sizes log-uniform in "
+                                f"[500, 50000] ({ncell} cells), mixed precision, "
                                 f"{steps} consecutive steps in one batched context with captured "
                                 f"BBPGD graphs and per-colony convergence"),
                    "n_cells": ncell, "colonies": len(cols)},
@@ -266,7 +268,7 @@ def serving_requests(rate, duration, seed, steps, lo=1_000, hi=100_000):
                                  kcc=4.0 / 3.0 * 1e4, precision=PREC_MIXED)
         else:
             p = params_for(PREC_MIXED).replace(tau=1 / rate_g)
-        reqs.append(Request(len(reqs), G.packed_colony(n, seed=seed * 100_000 + len(reqs)), p,
+        reqs.append(Request(len(reqs), G.dense_colony(n, seed=seed * 100_000 + len(reqs)), p,
                             steps, arrival=t))
     return reqs
 

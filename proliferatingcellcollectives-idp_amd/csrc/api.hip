@@ -27,6 +27,11 @@ namespace pcc {
 void batch_to_host(pcc_ctx* ctx, pcc_constraints* out, int32_t iteration);
 void build_grid(pcc_ctx* ctx);
 
+std::mutex& device_mutex() {
+  static std::mutex m;
+  return m;
+}
+
 BodyArrays body_arrays(double* base, int64_t N) {
   BodyArrays b;
   double** f[kBodyFields] = {&b.px, &b.py, &b.pz, &b.qw, &b.qx, &b.qy,
@@ -44,6 +49,7 @@ KTimer::KTimer(pcc_ctx* c, int kernel, double bytes) : ctx(c), k(kernel) {
       e = T.free_events.back();
       T.free_events.pop_back();
     } else {
+      DeviceLock lock(device_mutex());
       PCC_HIP(hipEventCreate(&e));
     }
     return e;
@@ -579,6 +585,7 @@ pcc_status pcc_create(const pcc_params* params, int device, pcc_ctx** out) {
   ctx->use_graphs = !(graph_env && graph_env[0] == '0');
   pcc_status s = guarded(ctx, [&]() {
     PCC_HIP(hipSetDevice(device));
+    DeviceLock lock(device_mutex());
     PCC_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     PCC_HIP(hipHostMalloc((void**)&ctx->state_host, 4 * sizeof(SolverState), hipHostMallocDefault));
     ctx->state_host_cap = 4;
@@ -597,13 +604,16 @@ void pcc_destroy(pcc_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  for (auto& p : ctx->timing.pending) {
-    (void)hipEventDestroy(p.a);
-    (void)hipEventDestroy(p.b);
+  {
+    DeviceLock lock(device_mutex());
+    for (auto& p : ctx->timing.pending) {
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    for (auto e : ctx->timing.free_events) (void)hipEventDestroy(e);
+    if (ctx->state_host) (void)hipHostFree(ctx->state_host);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   }
-  for (auto e : ctx->timing.free_events) (void)hipEventDestroy(e);
-  if (ctx->state_host) (void)hipHostFree(ctx->state_host);
-  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;  // DArr members free their device memory
 }
 

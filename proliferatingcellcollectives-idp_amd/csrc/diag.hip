@@ -6,7 +6,7 @@
 // known size against which hardware traffic counters are calibrated.
 #include <hip/hip_runtime.h>
 
-#include "pcc.h"
+#include "internal.hpp"
 
 namespace {
 
@@ -23,6 +23,7 @@ extern "C" pcc_status pcc_measure_copy_bandwidth(int device, int64_t bytes, int 
                                                  double* gbs) {
   if (!gbs || bytes < 1024 || reps < 1) return PCC_E_INVALID_ARGUMENT;
   if (hipSetDevice(device) != hipSuccess) return PCC_E_DEVICE;
+  pcc::DeviceLock lock(pcc::device_mutex());
   const int64_t n = bytes / 32;  // double4 elements
   double4* a = nullptr;
   double4* b = nullptr;

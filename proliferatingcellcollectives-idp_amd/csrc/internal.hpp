@@ -4,12 +4,21 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "pcc.h"
 
 namespace pcc {
+
+// One lock for the whole library.  HIP refuses device-wide runtime calls
+// (allocation, freeing, device synchronisation, pinned host memory, stream
+// and event creation) on every thread while any stream of the process is being
+// captured into a graph, so graph captures and those calls take turns.  A
+// capture holds the lock only while it records one chunk of kernel launches.
+std::mutex& device_mutex();
+using DeviceLock = std::lock_guard<std::mutex>;
 
 struct HipError {
   hipError_t err;
@@ -41,6 +50,7 @@ struct DArr {
     if (n <= cap) return;
     int64_t nc = cap > 0 ? cap : 1024;
     while (nc < n) nc *= 2;
+    DeviceLock lock(device_mutex());
     T* q = nullptr;
     hipError_t e = hipMalloc(&q, sizeof(T) * (size_t)nc);
     if (e != hipSuccess) throw StatusError{PCC_E_OUT_OF_MEMORY, "hipMalloc failed"};
@@ -56,6 +66,7 @@ struct DArr {
     if (n <= cap) return;
     int64_t nc = cap > 0 ? cap : 1024;
     while (nc < n) nc *= 2;
+    DeviceLock lock(device_mutex());
     T* q = nullptr;
     hipError_t e = hipMalloc(&q, sizeof(T) * (size_t)nc);
     if (e != hipSuccess) throw StatusError{PCC_E_OUT_OF_MEMORY, "hipMalloc failed"};
@@ -68,7 +79,10 @@ struct DArr {
     cap = nc;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      DeviceLock lock(device_mutex());
+      (void)hipFree(p);
+    }
     p = nullptr;
     cap = 0;
   }

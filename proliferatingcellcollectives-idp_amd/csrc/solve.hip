@@ -1019,6 +1019,7 @@ double constr_bytes(int64_t N, int64_t C, bool mixed, bool planar) {
 
 static void ensure_state_host(pcc_ctx* ctx, int64_t slots) {
   if (ctx->state_host_cap >= slots) return;
+  DeviceLock lock(device_mutex());
   if (ctx->state_host) PCC_HIP(hipHostFree(ctx->state_host));
   ctx->state_host = nullptr;
   ctx->state_host_cap = 0;
@@ -1097,6 +1098,7 @@ void bbpgd_solve(pcc_ctx* ctx) {
   // per-kernel event timing is on.
   hipGraphExec_t exec = nullptr;
   if (ctx->use_graphs && !ctx->timing.enabled) {
+    DeviceLock lock(device_mutex());
     hipGraph_t graph = nullptr;
     PCC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     for (int q = 0; q < chunk; ++q) one_iteration();
@@ -1108,13 +1110,19 @@ void bbpgd_solve(pcc_ctx* ctx) {
   struct ExecGuard {
     hipGraphExec_t& e;
     ~ExecGuard() {
-      if (e) (void)hipGraphExecDestroy(e);
+      if (e) {
+        DeviceLock lock(device_mutex());
+        (void)hipGraphExecDestroy(e);
+      }
     }
   } guard{exec};
   SolverState* hs = ctx->state_host;
   hipEvent_t ev[2];
-  PCC_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-  PCC_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  {
+    DeviceLock lock(device_mutex());
+    PCC_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    PCC_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  }
   int64_t j = 0;
   bool done = false;
   while (!done) {
@@ -1135,8 +1143,11 @@ void bbpgd_solve(pcc_ctx* ctx) {
     ++j;
   }
   PCC_HIP(hipStreamSynchronize(s));
-  (void)hipEventDestroy(ev[0]);
-  (void)hipEventDestroy(ev[1]);
+  {
+    DeviceLock lock(device_mutex());
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+  }
   std::vector<SolverState> fin(B);
   PCC_HIP(hipMemcpyAsync(fin.data(), ctx->state.p, sizeof(SolverState) * B, hipMemcpyDeviceToHost, s));
   PCC_HIP(hipStreamSynchronize(s));

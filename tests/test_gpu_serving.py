@@ -51,3 +51,62 @@ def test_served_results_equal_alone():
             pm.step(k)
         assert_same_state(res.state, pm.get_state())
         pm.close()
+
+
+def test_concurrent_contexts_allocate_during_graph_capture():
+    """One thread steps a hard batch (every BBPGD phase captures a graph)
+    while another keeps growing soft contexts (device allocations, frees and
+    synchronisations): both must run to the end and give their solo results."""
+    import threading
+    from pcc.engine import ColonyBatch, ParticleManager
+    hp = request_params(T.MODE_HARD, LN2)
+    sp = request_params(T.MODE_SOFT, LN2)
+    hard_cols = [G.dense_colony(n, seed=120 + i) for i, n in enumerate([3000, 1200, 5000])]
+    soft_cols = [G.dense_colony(n, seed=130 + i) for i, n in enumerate([500, 1500, 4000, 9000])]
+    out, errors = {}, []
+
+    def hard_worker():
+        try:
+            cb = ColonyBatch(hp)
+            cb.set_state(hard_cols, [hp] * 3)
+            for k in range(6):
+                cb.step(k)
+            out["hard"] = [cb.get_state(q) for q in range(3)]
+            cb.close()
+        except Exception as e:  # reported below
+            errors.append(repr(e))
+
+    def soft_worker():
+        try:
+            res = []
+            for b in soft_cols:  # growing sizes: every context allocates afresh
+                pm = ParticleManager(sp)
+                pm.set_state(b)
+                for k in range(3):
+                    pm.step(k)
+                res.append(pm.get_state())
+                pm.close()
+            out["soft"] = res
+        except Exception as e:
+            errors.append(repr(e))
+
+    th = [threading.Thread(target=hard_worker), threading.Thread(target=soft_worker)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errors, errors
+    cb = ColonyBatch(hp)
+    cb.set_state(hard_cols, [hp] * 3)
+    for k in range(6):
+        cb.step(k)
+    for q in range(3):
+        assert_same_state(out["hard"][q], cb.get_state(q))
+    cb.close()
+    for b, got in zip(soft_cols, out["soft"]):
+        pm = ParticleManager(sp)
+        pm.set_state(b)
+        for k in range(3):
+            pm.step(k)
+        assert_same_state(got, pm.get_state())
+        pm.close()
