@@ -220,6 +220,34 @@ pcc_status pcc_batch_get_constraints(pcc_ctx* ctx, int64_t colony, pcc_constrain
 pcc_status pcc_batch_get_phase_stats(pcc_ctx* ctx, int64_t colony, int64_t* bbpgd_iterations,
                                      int64_t* n_constraints, int64_t max, int64_t* count);
 
+/* Cell division and gid reassignment between steps, on the device
+ * (ParticleManager::divideParticles / Particle::divide, ParticleManager.cpp:139-148,
+ * Particle.cpp:122-150, then Domain::assignGlobalIDs, Domain.cpp:492-508):
+ * every cell with length >= 2 l0 splits into l1 = U[1-length_noise,
+ * 1+length_noise] L/2 and l2 = L - l1 at x -/+ (l/2) t (the reference draws
+ * length_noise 0.02 from an unseeded std::random_device); each child is then
+ * turned in-plane about z by an angle drawn from N(0, angle_noise) (a seeded
+ * Irwin-Hall approximation; angle_noise <= 0.2).  Draws depend on (seed, step,
+ * index of the parent within its colony) only.  Cells are then renumbered by
+ * ascending x (ties keep parents first, right children after, in parent
+ * order), also when no cell divides, as the reference renumbers before every
+ * step.  The per-body outputs of the last step are reset to zero.
+ * pcc_divide: single-colony context, *n_bodies receives the new count;
+ * pcc_batch_divide: seeds[q] per colony, sizes[q] receives the new counts. */
+pcc_status pcc_divide(pcc_ctx* ctx, uint64_t seed, int32_t step, double length_noise,
+                      double angle_noise, int64_t* n_bodies);
+pcc_status pcc_batch_divide(pcc_ctx* ctx, const uint64_t* seeds, int32_t step,
+                            double length_noise, double angle_noise, int64_t* sizes);
+
+/* Autotuner state.  The first BBPGD solve with at least 1e5 constraints
+ * times every staging size of the body kernel (incidences per LDS pass:
+ * 512/1024/2048 planar, 256/512/1024 3D; the sums do not depend on it) on its
+ * own operators and keeps the fastest for the context (PCC_AUTOTUNE=0 in the
+ * environment keeps the default).  *body_chunk: the size in use; *tuned: 1
+ * once tuned; variant_us[3] (may be NULL): the measured launch times.  No
+ * reference counterpart. */
+pcc_status pcc_get_tuning(pcc_ctx* ctx, int32_t* body_chunk, int32_t* tuned, double* variant_us);
+
 /* Device memory held by the context (every device buffer it has allocated),
  * for admission against an HBM budget.  No reference counterpart. */
 pcc_status pcc_device_bytes(pcc_ctx* ctx, int64_t* bytes);

@@ -1150,3 +1150,160 @@ pcc_status oracle_step(const pcc_params* p, pcc_bodies* b, int32_t iter, pcc_ste
   set_err(err, err_len, "Invalid mode");
   return PCC_E_INVALID_MODE;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Division and gid reassignment (outside the step)                          */
+/* ------------------------------------------------------------------------ */
+/* splitmix64 finaliser (Steele, Lea, Flood 2014) */
+static inline uint64_t split_mix(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+/* Draw k in [0, 1) of parent p (index within its colony) at (seed, step). */
+static inline double div_uniform(uint64_t seed, int32_t step, int64_t p, int k) {
+  const uint64_t key = split_mix(split_mix(seed) ^ (uint64_t)(uint32_t)step);
+  const uint64_t z = split_mix(key ^ ((uint64_t)p * 32u + (uint64_t)k));
+  return (double)(z >> 11) * 0x1.0p-53;
+}
+
+/* Approximately N(0, 1): Irwin-Hall sum of twelve uniforms minus 6. */
+static inline double div_normal(uint64_t seed, int32_t step, int64_t p, int k0) {
+  double s = 0.0;
+  for (int k = 0; k < 12; ++k) s += div_uniform(seed, step, p, k0 + k);
+  return s - 6.0;
+}
+
+/* cos and sin of a half-angle |x| <= 0.6 by their Taylor series to x^21
+ * (truncation below 1e-25), Horner in x^2 with correctly rounded 1/n!. */
+static inline void half_angle_cs(double x, double* c, double* s) {
+  const double y = x * x;
+  double pc = -0x1.0ce396db7f853p-70;        /* -1/22! */
+  pc = pc * y + 0x1.e542ba4020225p-62;       /* 1/20! */
+  pc = pc * y - 0x1.6827863b97d97p-53;       /* 1/18! */
+  pc = pc * y + 0x1.ae7f3e733b81fp-45;       /* 1/16! */
+  pc = pc * y - 0x1.93974a8c07c9dp-37;       /* 1/14! */
+  pc = pc * y + 0x1.1eed8eff8d898p-29;       /* 1/12! */
+  pc = pc * y - 0x1.27e4fb7789f5cp-22;       /* 1/10! */
+  pc = pc * y + 0x1.a01a01a01a01ap-16;       /* 1/8! */
+  pc = pc * y - 0x1.6c16c16c16c17p-10;       /* 1/6! */
+  pc = pc * y + 0x1.5555555555555p-5;        /* 1/4! */
+  pc = pc * y - 0.5;
+  *c = pc * y + 1.0;
+  double ps = 0x1.71b8ef6dcf572p-66;         /* 1/21! */
+  ps = ps * y - 0x1.2f49b46814157p-57;       /* 1/19! */
+  ps = ps * y + 0x1.952c77030ad4ap-49;       /* 1/17! */
+  ps = ps * y - 0x1.ae7f3e733b81fp-41;       /* 1/15! */
+  ps = ps * y + 0x1.6124613a86d09p-33;       /* 1/13! */
+  ps = ps * y - 0x1.ae64567f544e4p-26;       /* 1/11! */
+  ps = ps * y + 0x1.71de3a556c734p-19;       /* 1/9! */
+  ps = ps * y - 0x1.a01a01a01a01ap-13;       /* 1/7! */
+  ps = ps * y + 0x1.1111111111111p-7;        /* 1/5! */
+  ps = ps * y - 0x1.5555555555555p-3;        /* 1/3! */
+  ps = ps * y + 1.0;
+  *s = ps * x;
+}
+
+/* q <- normalize((cos(d/2), 0, 0, sin(d/2)) * q): an in-plane turn by d
+ * about the z axis, renormalised as Particle.cpp:110-120 does. */
+static void turn_z(double q[4], double d) {
+  double c, s;
+  half_angle_cs(0.5 * d, &c, &s);
+  double r[4];
+  r[0] = c * q[0] - s * q[3];
+  r[1] = c * q[1] - s * q[2];
+  r[2] = c * q[2] + s * q[1];
+  r[3] = c * q[3] + s * q[0];
+  const double norm = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2] + r[3] * r[3]);
+  for (int k = 0; k < 4; ++k) q[k] = r[k] / norm;
+}
+
+typedef struct {
+  uint64_t key;
+  int64_t idx;
+} xkey_t;
+
+static uint64_t order_key_host(double x) {
+  x = x + 0.0;
+  uint64_t u;
+  memcpy(&u, &x, sizeof(u));
+  return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+
+static int xkey_cmp(const void* a, const void* b) {
+  const xkey_t* x = (const xkey_t*)a;
+  const xkey_t* y = (const xkey_t*)b;
+  if (x->key != y->key) return x->key < y->key ? -1 : 1;
+  return x->idx < y->idx ? -1 : (x->idx > y->idx);
+}
+
+/* Particle::divide (Particle.cpp:122-150) for every cell with length >= 2 l0,
+ * then gid reassignment by ascending x (Domain::assignGlobalIDs,
+ * Domain.cpp:492-508; ties keep the staging order).  l1 = U[1-a, 1+a] L/2,
+ * l2 = L - l1, centres x -/+ (l/2) t; each child then turns in-plane by
+ * angle_noise * N(0,1).  Staging order: every parent's left child in its
+ * place, right children appended in parent order. */
+int64_t oracle_divide(int64_t n, int64_t cap, double* pos, double* quat, double* length,
+                      double* l0, double* diameter, double* ldot, uint64_t seed, int32_t step,
+                      double length_noise, double angle_noise) {
+  int64_t d = 0;
+  for (int64_t i = 0; i < n; ++i) d += length[i] >= 2 * l0[i];
+  const int64_t m = n + d;
+  if (m > cap) return -1;
+  int64_t r = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    if (!(length[i] >= 2 * l0[i])) continue;
+    const double L = length[i];
+    const double half = L / 2;
+    const double lo = half * (1.0 - length_noise), hi = half * (1.0 + length_noise);
+    const double l1 = lo + div_uniform(seed, step, i, 0) * (hi - lo);
+    const double l2 = L - l1;
+    double t[3];
+    oracle_direction(quat + 4 * i, t);
+    const int64_t j = n + r++;
+    double ql[4], qr[4];
+    for (int k = 0; k < 4; ++k) ql[k] = qr[k] = quat[4 * i + k];
+    turn_z(ql, angle_noise * div_normal(seed, step, i, 1));
+    turn_z(qr, angle_noise * div_normal(seed, step, i, 13));
+    for (int k = 0; k < 3; ++k) {
+      const double x = pos[3 * i + k];
+      pos[3 * j + k] = x + (l2 / 2) * t[k];
+      pos[3 * i + k] = x - (l1 / 2) * t[k];
+    }
+    for (int k = 0; k < 4; ++k) {
+      quat[4 * i + k] = ql[k];
+      quat[4 * j + k] = qr[k];
+    }
+    length[i] = l1;
+    length[j] = l2;
+    l0[j] = l0[i];
+    diameter[j] = diameter[i];
+    ldot[j] = ldot[i];
+  }
+  /* gids by ascending x */
+  xkey_t* ord = (xkey_t*)malloc(sizeof(xkey_t) * (size_t)(m > 0 ? m : 1));
+  for (int64_t i = 0; i < m; ++i) {
+    ord[i].key = order_key_host(pos[3 * i]);
+    ord[i].idx = i;
+  }
+  qsort(ord, (size_t)m, sizeof(xkey_t), xkey_cmp);
+  double* tmp = (double*)malloc(sizeof(double) * 4 * (size_t)(m > 0 ? m : 1));
+#define PERMUTE(arr, w)                                                           \
+  do {                                                                            \
+    for (int64_t i = 0; i < m; ++i)                                               \
+      for (int k = 0; k < (w); ++k) tmp[(w) * i + k] = (arr)[(w) * ord[i].idx + k]; \
+    memcpy((arr), tmp, sizeof(double) * (w) * (size_t)m);                        \
+  } while (0)
+  PERMUTE(pos, 3);
+  PERMUTE(quat, 4);
+  PERMUTE(length, 1);
+  PERMUTE(l0, 1);
+  PERMUTE(diameter, 1);
+  PERMUTE(ldot, 1);
+#undef PERMUTE
+  free(tmp);
+  free(ord);
+  return m;
+}

@@ -27,8 +27,8 @@ namespace pcc {
 void batch_to_host(pcc_ctx* ctx, pcc_constraints* out, int32_t iteration);
 void build_grid(pcc_ctx* ctx);
 
-std::mutex& device_mutex() {
-  static std::mutex m;
+std::recursive_mutex& device_mutex() {
+  static std::recursive_mutex m;
   return m;
 }
 
@@ -583,11 +583,16 @@ pcc_status pcc_create(const pcc_params* params, int device, pcc_ctx** out) {
   ctx->allow_planar = !(planar_env && planar_env[0] == '0');
   const char* graph_env = getenv("PCC_GRAPH");
   ctx->use_graphs = !(graph_env && graph_env[0] == '0');
+  const char* tune_env = getenv("PCC_AUTOTUNE");
+  ctx->autotune = !(tune_env && tune_env[0] == '0');
   pcc_status s = guarded(ctx, [&]() {
     PCC_HIP(hipSetDevice(device));
-    DeviceLock lock(device_mutex());
-    PCC_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-    PCC_HIP(hipHostMalloc((void**)&ctx->state_host, 4 * sizeof(SolverState), hipHostMallocDefault));
+    {
+      DeviceLock lock(device_mutex());
+      PCC_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+      PCC_HIP(hipHostMalloc((void**)&ctx->state_host, 4 * sizeof(SolverState),
+                            hipHostMallocDefault));
+    }
     ctx->state_host_cap = 4;
     ctx->state.ensure(1);
     return PCC_OK;
@@ -812,6 +817,50 @@ pcc_status pcc_batch_get_constraints(pcc_ctx* ctx, int64_t colony, pcc_constrain
   });
 }
 
+static pcc_status divide_impl(pcc_ctx* ctx, const uint64_t* seeds, int32_t step,
+                              double length_noise, double angle_noise) {
+  if (!ctx->has_state) return fail(ctx, PCC_E_NO_STATE, "no state");
+  if (!(length_noise >= 0.0 && length_noise < 1.0) || !(angle_noise >= 0.0 && angle_noise <= 0.2))
+    return fail(ctx, PCC_E_INVALID_ARGUMENT, "division noise out of range");
+  return guarded(ctx, [&]() {
+    PCC_HIP(hipSetDevice(ctx->device));
+    divide_colonies(ctx, seeds, step, length_noise, angle_noise);
+    {
+      const int64_t N = ctx->N;
+      ensure_body_arrays(ctx, N);
+      hipStream_t s = ctx->stream;
+      PCC_HIP(hipMemsetAsync(ctx->F.p, 0, sizeof(double) * (6 * N + 6), s));
+      PCC_HIP(hipMemsetAsync(ctx->U.p, 0, sizeof(double) * (6 * N + 6), s));
+      PCC_HIP(hipMemsetAsync(ctx->sig.p, 0, sizeof(double) * (N + 1), s));
+      PCC_HIP(hipMemsetAsync(ctx->imp.p, 0, sizeof(double) * (N + 1), s));
+      PCC_HIP(hipMemsetAsync(ctx->ldot_new.p, 0, sizeof(double) * (N + 1), s));
+      PCC_HIP(hipMemsetAsync(ctx->nc.p, 0, sizeof(int32_t) * (N + 1), s));
+      upload_binst(ctx);
+      upload_instances(ctx);
+    }
+    return PCC_OK;
+  });
+}
+
+pcc_status pcc_divide(pcc_ctx* ctx, uint64_t seed, int32_t step, double length_noise,
+                      double angle_noise, int64_t* n_bodies) {
+  if (!ctx) return PCC_E_INVALID_ARGUMENT;
+  if (ctx->has_state && ctx->inst.size() != 1)
+    return fail(ctx, PCC_E_INVALID_ARGUMENT, "batched context: use pcc_batch_divide");
+  pcc_status st = divide_impl(ctx, &seed, step, length_noise, angle_noise);
+  if (st == PCC_OK && n_bodies) *n_bodies = ctx->N;
+  return st;
+}
+
+pcc_status pcc_batch_divide(pcc_ctx* ctx, const uint64_t* seeds, int32_t step,
+                            double length_noise, double angle_noise, int64_t* sizes) {
+  if (!ctx || !seeds) return PCC_E_INVALID_ARGUMENT;
+  pcc_status st = divide_impl(ctx, seeds, step, length_noise, angle_noise);
+  if (st == PCC_OK && sizes)
+    for (size_t q = 0; q < ctx->inst.size(); ++q) sizes[q] = ctx->inst[q].N;
+  return st;
+}
+
 pcc_status pcc_batch_set_params(pcc_ctx* ctx, int64_t colony, const pcc_params* params) {
   if (!ctx || !params) return PCC_E_INVALID_ARGUMENT;
   if (colony < 0 || colony >= (int64_t)ctx->inst.size())
@@ -909,6 +958,15 @@ pcc_status pcc_get_timing(pcc_ctx* ctx, int kernel, int64_t* launches, double* t
       *bytes_per_launch = T.launches[kernel] ? T.bytes[kernel] / (double)T.launches[kernel] : 0.0;
     return PCC_OK;
   });
+}
+
+pcc_status pcc_get_tuning(pcc_ctx* ctx, int32_t* body_chunk, int32_t* tuned, double* variant_us) {
+  if (!ctx) return PCC_E_INVALID_ARGUMENT;
+  if (body_chunk) *body_chunk = body_chunk_of(ctx->body_variant, ctx->planar);
+  if (tuned) *tuned = ctx->body_tuned ? 1 : 0;
+  if (variant_us)
+    for (int v = 0; v < kBodyVariants; ++v) variant_us[v] = ctx->body_variant_ms[v] * 1e3;
+  return PCC_OK;
 }
 
 pcc_status pcc_device_bytes(pcc_ctx* ctx, int64_t* bytes) {

@@ -17,8 +17,9 @@ namespace pcc {
 // and event creation) on every thread while any stream of the process is being
 // captured into a graph, so graph captures and those calls take turns.  A
 // capture holds the lock only while it records one chunk of kernel launches.
-std::mutex& device_mutex();
-using DeviceLock = std::lock_guard<std::mutex>;
+// Recursive: a holder may call into code that takes it again.
+std::recursive_mutex& device_mutex();
+using DeviceLock = std::lock_guard<std::recursive_mutex>;
 
 struct HipError {
   hipError_t err;
@@ -93,6 +94,8 @@ struct BodyArrays {
   double *px, *py, *pz, *qw, *qx, *qy, *qz, *len, *l0, *diam, *ldot;
 };
 static constexpr int kBodyFields = 11;
+static constexpr int kBodyVariants = 3;
+int body_chunk_of(int variant, bool planar);
 
 // Device-side solver scalars of one instance's BBPGD solve.
 struct SolverState {
@@ -191,6 +194,11 @@ struct pcc_ctx {
   bool planar = false;        // current step uses the 2D monolayer kernels
   bool allow_planar = true;   // PCC_PLANAR=0 in the environment forces the 3D kernels
   bool use_graphs = true;     // BBPGD chunks as captured HIP graphs (PCC_GRAPH=0: plain launches)
+  // autotuner (PCC_AUTOTUNE=0: off): body-kernel staging size
+  bool autotune = true;
+  bool body_tuned = false;
+  int body_variant = 1;
+  float body_variant_ms[3] = {0.f, 0.f, 0.f};
 
   // body state + snapshot + outputs
   pcc::DArr<double> body;      // kBodyFields * N
@@ -272,6 +280,8 @@ void integrate_hard(pcc_ctx* ctx);
 void grow(pcc_ctx* ctx);
 void soft_forces_and_integrate(pcc_ctx* ctx);
 void reset_step(pcc_ctx* ctx);
+void divide_colonies(pcc_ctx* ctx, const uint64_t* seeds, int32_t step, double length_noise,
+                     double angle_noise);
 void upload_instances(pcc_ctx* ctx);
 void max_overlap_inst(pcc_ctx* ctx, std::vector<double>& out);
 void merge_batch(pcc_ctx* ctx);

@@ -56,6 +56,7 @@ void seg_sort_i32(pcc_ctx* ctx, int64_t nseg, const int32_t* start, int32_t* v);
 // ascending constraint order; gamma_old (fixed within a phase) travels with
 // the incidence records.
 static constexpr int kBbGridMax = 2048;
+static constexpr int64_t kTuneMinConstraints = 100000;
 static constexpr int kXcd = 8;
 
 namespace {
@@ -373,10 +374,11 @@ __global__ void __launch_bounds__(256) k_body_final(BodyArgs a, FinalOut fo) {
 // the products of its own body in ascending incidence order, so every sum has
 // exactly the order of the thread-per-body loop above.  Blocks never straddle
 // two colonies (each colony starts a new logical block).
-template <class CT, bool PLANAR>
+// CHUNK (incidences staged per pass) does not change any sum and is picked by
+// the autotuner.
+template <class CT, bool PLANAR, int CHUNK>
 __global__ void __launch_bounds__(256) k_body_coop(BodyArgs a) {
-  constexpr int NC = PLANAR ? 4 : 7;           // products per incidence
-  constexpr int CHUNK = PLANAR ? 1024 : 512;   // incidences staged per pass
+  constexpr int NC = PLANAR ? 4 : 7;  // products per incidence
   __shared__ double prod[NC][CHUNK];
   const int64_t l = xcd_block(blockIdx.x, a.gp);
   if (l >= a.nbb) return;
@@ -1017,6 +1019,37 @@ double constr_bytes(int64_t N, int64_t C, bool mixed, bool planar) {
   return C * (8 + row + 24 + 16) + N * ub;
 }
 
+// Body-kernel staging sizes the autotuner chooses from (incidences per pass):
+// planar 512 / 1024 / 2048, 3D 256 / 512 / 1024.
+static void launch_body(int variant, bool planar, bool mixed, unsigned grid, hipStream_t s,
+                        const BodyArgs& ba) {
+#define PCC_BODY(CT, PL, CH) k_body_coop<CT, PL, CH><<<grid, 256, 0, s>>>(ba)
+  if (planar) {
+    if (mixed) {
+      if (variant == 0) PCC_BODY(float, true, 512);
+      else if (variant == 1) PCC_BODY(float, true, 1024);
+      else PCC_BODY(float, true, 2048);
+    } else {
+      if (variant == 0) PCC_BODY(double, true, 512);
+      else if (variant == 1) PCC_BODY(double, true, 1024);
+      else PCC_BODY(double, true, 2048);
+    }
+  } else {
+    if (mixed) {
+      if (variant == 0) PCC_BODY(float, false, 256);
+      else if (variant == 1) PCC_BODY(float, false, 512);
+      else PCC_BODY(float, false, 1024);
+    } else {
+      if (variant == 0) PCC_BODY(double, false, 256);
+      else if (variant == 1) PCC_BODY(double, false, 512);
+      else PCC_BODY(double, false, 1024);
+    }
+  }
+#undef PCC_BODY
+}
+
+int body_chunk_of(int variant, bool planar) { return (planar ? 512 : 256) << variant; }
+
 static void ensure_state_host(pcc_ctx* ctx, int64_t slots) {
   if (ctx->state_host_cap >= slots) return;
   DeviceLock lock(device_mutex());
@@ -1060,20 +1093,11 @@ void bbpgd_solve(pcc_ctx* ctx) {
   BodyArgs ba = body_args(ctx);
   ConstrArgs ca = constr_args(ctx);
   const unsigned gb = (unsigned)ba.gp, gc = (unsigned)ca.gp;
+  auto body = [&](int variant) { launch_body(variant, planar, mixed, gb, s, ba); };
   auto one_iteration = [&]() {
     {
       KTimer t(ctx, PCC_K_BODY_APPLY, 0.0);
-      if (planar) {
-        if (mixed)
-          k_body_coop<float, true><<<gb, 256, 0, s>>>(ba);
-        else
-          k_body_coop<double, true><<<gb, 256, 0, s>>>(ba);
-      } else {
-        if (mixed)
-          k_body_coop<float, false><<<gb, 256, 0, s>>>(ba);
-        else
-          k_body_coop<double, false><<<gb, 256, 0, s>>>(ba);
-      }
+      body(ctx->body_variant);
     }
     {
       KTimer t(ctx, PCC_K_CONSTR_APPLY, 0.0);
@@ -1092,6 +1116,39 @@ void bbpgd_solve(pcc_ctx* ctx) {
     k_finish<<<(unsigned)B, 256, 0, s>>>(ca);
   };
   one_iteration();  // k = 0
+  if (ctx->autotune && !ctx->body_tuned && C >= kTuneMinConstraints && !ctx->timing.enabled) {
+    // Autotuner: time every staging size of the body kernel on this solve's
+    // operators.  The kernel only rewrites the velocity records of the
+    // iteration about to run, identically in every variant, so timing it
+    // changes no state.
+    float best = 0.f;
+    hipEvent_t e0, e1;
+    {
+      DeviceLock lock(device_mutex());
+      PCC_HIP(hipEventCreate(&e0));
+      PCC_HIP(hipEventCreate(&e1));
+    }
+    for (int v = 0; v < kBodyVariants; ++v) {
+      body(v);  // warm
+      PCC_HIP(hipEventRecord(e0, s));
+      for (int r = 0; r < 3; ++r) body(v);
+      PCC_HIP(hipEventRecord(e1, s));
+      PCC_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      PCC_HIP(hipEventElapsedTime(&ms, e0, e1));
+      ctx->body_variant_ms[v] = ms / 3;
+      if (v == 0 || ms < best) {
+        best = ms;
+        ctx->body_variant = v;
+      }
+    }
+    {
+      DeviceLock lock(device_mutex());
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+    }
+    ctx->body_tuned = true;
+  }
   const int chunk = 10;
   // The chunk of `chunk` iterations is launched as one captured HIP graph
   // (kernel nodes only; the state read-back below stays outside it), unless

@@ -70,6 +70,12 @@ def lib():
         L.pcc_batch_get_phase_stats.argtypes = [vp, i64, C.POINTER(i64), C.POINTER(i64), i64,
                                                 C.POINTER(i64)]
         L.pcc_device_bytes.argtypes = [vp, C.POINTER(i64)]
+        L.pcc_get_tuning.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                     C.POINTER(C.c_double)]
+        L.pcc_divide.argtypes = [vp, C.c_uint64, C.c_int32, C.c_double, C.c_double,
+                                 C.POINTER(i64)]
+        L.pcc_batch_divide.argtypes = [vp, C.POINTER(C.c_uint64), C.c_int32, C.c_double,
+                                       C.c_double, C.POINTER(i64)]
         L.pcc_measure_copy_bandwidth.argtypes = [C.c_int, i64, C.c_int, C.POINTER(C.c_double)]
         for f in ("pcc_create", "pcc_set_params", "pcc_set_state", "pcc_get_state",
                   "pcc_snapshot", "pcc_restore", "pcc_step", "pcc_get_constraints",
@@ -78,7 +84,8 @@ def lib():
                   "pcc_batch_set_params", "pcc_batch_size", "pcc_batch_step",
                   "pcc_batch_get_state", "pcc_batch_get_constraints",
                   "pcc_batch_get_phase_stats", "pcc_device_bytes",
-                  "pcc_measure_copy_bandwidth"):
+                  "pcc_measure_copy_bandwidth", "pcc_divide", "pcc_batch_divide",
+                  "pcc_get_tuning"):
             getattr(L, f).restype = C.c_int
         _lib = L
     return _lib
@@ -91,7 +98,7 @@ EXPORTED = ("pcc_default_params", "pcc_create", "pcc_destroy", "pcc_set_params",
             "pcc_batch_set_state", "pcc_batch_add", "pcc_batch_remove", "pcc_batch_set_params",
             "pcc_batch_size", "pcc_batch_step", "pcc_batch_get_state",
             "pcc_batch_get_constraints", "pcc_batch_get_phase_stats", "pcc_device_bytes",
-            "pcc_measure_copy_bandwidth")
+            "pcc_measure_copy_bandwidth", "pcc_divide", "pcc_batch_divide", "pcc_get_tuning")
 
 
 def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 10, device: int = 0) -> float:
@@ -161,11 +168,30 @@ class ParticleManager:
         self._check(lib().pcc_get_state(self._ctx, C.byref(s)))
         return b
 
+    def tuning(self) -> dict:
+        """Autotuner state: body-kernel staging size in use, whether it was
+        tuned, and the measured time of each candidate (us)."""
+        ch, tu = C.c_int32(0), C.c_int32(0)
+        us = (C.c_double * 3)()
+        self._check(lib().pcc_get_tuning(self._ctx, C.byref(ch), C.byref(tu), us))
+        return {"body_chunk": int(ch.value), "tuned": bool(tu.value),
+                "variant_us": [float(x) for x in us]}
+
     def device_bytes(self) -> int:
         """Device memory held by this context."""
         b = C.c_int64(0)
         self._check(lib().pcc_device_bytes(self._ctx, C.byref(b)))
         return int(b.value)
+
+    def divide(self, seed: int, step: int, length_noise: float = 0.0,
+               angle_noise: float = 0.01) -> int:
+        """Divides every cell with length >= 2 l0 on the device and renumbers
+        the cells by x; returns the new cell count."""
+        n = C.c_int64(0)
+        self._check(lib().pcc_divide(self._ctx, C.c_uint64(seed & 0xFFFFFFFFFFFFFFFF), int(step),
+                                     float(length_noise), float(angle_noise), C.byref(n)))
+        self.n = int(n.value)
+        return self.n
 
     def snapshot(self):
         self._check(lib().pcc_snapshot(self._ctx))
@@ -332,3 +358,15 @@ class ColonyBatch(ParticleManager):
 
     def detect(self, cutoff, capacity=None):
         raise PccError(T.PCC_E_INVALID_ARGUMENT, "detect() needs a single-colony context")
+
+    def divide(self, seeds, step: int, length_noise: float = 0.0, angle_noise: float = 0.01):
+        """Divides the cells of every colony (seeds[q] per colony) and
+        renumbers each colony by x; returns the new colony sizes."""
+        B = len(self.sizes)
+        sd = (C.c_uint64 * B)(*[int(x) & 0xFFFFFFFFFFFFFFFF for x in seeds])
+        out = (C.c_int64 * B)()
+        self._check(lib().pcc_batch_divide(self._ctx, sd, int(step), float(length_noise),
+                                           float(angle_noise), out))
+        self.sizes = [int(x) for x in out]
+        self.n = sum(self.sizes)
+        return list(self.sizes)

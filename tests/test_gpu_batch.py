@@ -185,3 +185,25 @@ def test_graph_replays_equal_plain_launches(precision, monkeypatch):
         assert ra == rb
         for q in range(3):
             assert_same_state(sa[q], sb[q])
+
+
+def test_autotuned_body_kernel_gives_the_same_bits(monkeypatch):
+    """The autotuner picks the body kernel's staging size on the first solve
+    with >= 1e5 constraints; every candidate sums in the same order, so a
+    tuned step equals an untuned one bit for bit (FP64)."""
+    from pcc.engine import ParticleManager
+    print("[autotune]", flush=True)
+    b = G.dense_colony(45000, seed=160)
+    p = hard()
+    runs = []
+    for env in ("1", "0"):
+        monkeypatch.setenv("PCC_AUTOTUNE", env)
+        pm = ParticleManager(p)
+        pm.set_state(b)
+        r = pm.step()
+        runs.append((r, pm.get_state(), pm.tuning()))
+        pm.close()
+    print(f"  tuning: {runs[0][2]}", flush=True)
+    assert runs[0][2]["tuned"] and not runs[1][2]["tuned"]
+    assert runs[0][0] == runs[1][0]
+    assert_same_state(runs[0][1], runs[1][1])

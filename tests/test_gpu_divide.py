@@ -1,0 +1,84 @@
+"""Device-side division and gid reassignment (pcc_divide / pcc_batch_divide)
+against the oracle's restatement: bit-exact, alone and batched, and a growth
+run that stays on the device from one cell, stepped and divided in turn,
+compared with the oracle at every step."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from pcc import generators as G
+from pcc.abi_types import reference_params
+
+pytestmark = pytest.mark.gpu
+LN2 = math.log(2.0)
+FIELDS = ("pos", "quat", "length", "l0", "diameter", "ldot")
+
+
+def same_bodies(a, b):
+    assert a.n == b.n
+    for f in FIELDS:
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
+
+
+def with_dividers(b, every=3):
+    b = b.copy()
+    b.length[::every] = 2 * b.l0[::every]
+    return b
+
+
+@pytest.mark.parametrize("length_noise,angle_noise", [(0.0, 0.01), (0.02, 0.01), (0.02, 0.0)])
+def test_divide_matches_oracle(length_noise, angle_noise):
+    from pcc.engine import ParticleManager
+    b = with_dividers(G.dense_colony(3000, seed=140))
+    print(f"[divide] n={b.n} noise=({length_noise}, {angle_noise})", flush=True)
+    pm = ParticleManager(reference_params())
+    pm.set_state(b)
+    n = pm.divide(seed=77, step=12, length_noise=length_noise, angle_noise=angle_noise)
+    want = oracle.divide(b, 77, 12, length_noise, angle_noise)
+    assert n == want.n == b.n + len(b.length[::3])
+    same_bodies(pm.get_state(), want)
+    pm.close()
+
+
+def test_batch_divide_equals_alone():
+    from pcc.engine import ColonyBatch
+    cols = [with_dividers(G.dense_colony(n, seed=150 + i), every=2 + i)
+            for i, n in enumerate([800, 2500, 60])]
+    seeds = [5, 6, 7]
+    p = reference_params()
+    cb = ColonyBatch(p)
+    cb.set_state(cols, [p] * 3)
+    sizes = cb.divide(seeds, step=4, length_noise=0.02, angle_noise=0.01)
+    for q in range(3):
+        want = oracle.divide(cols[q], seeds[q], 4, 0.02, 0.01)
+        assert sizes[q] == want.n
+        same_bodies(cb.get_state(q), want)
+    cb.close()
+
+
+def test_device_resident_growth_from_one_cell():
+    """Config-1 style: one cell grows at rate ln2, divides at 2 l0 (exact
+    halves, N(0, 0.01) turns), FP64 hard model; step and divide alternate on
+    the device and the state equals the oracle's after every step."""
+    from pcc.engine import ParticleManager
+    p = reference_params(dt=1e-2, tolerance=1e-6, tau=1 / LN2, cutoff_phase0=0.05,
+                         cutoff_phaseK=0.05)
+    b = G.single_cell()
+    pm = ParticleManager(p)
+    pm.set_state(b)
+    want = b
+    k = 0
+    while want.n < 40 and k < 800:
+        st, res, want, _, _, err = oracle.step(p, want, k)
+        r = pm.step(k)
+        assert r["bbpgd_iterations"] == res["bbpgd_iterations"], k
+        want = oracle.divide(want, 0, k, 0.0, 0.01)
+        n = pm.divide(0, k, 0.0, 0.01)
+        assert n == want.n, k
+        k += 1
+    print(f"[growth] {k} steps -> {want.n} cells", flush=True)
+    assert want.n >= 40
+    same_bodies(pm.get_state(), want)
+    pm.close()
