@@ -19,10 +19,11 @@ A "step" = restore the initial state from a device-resident snapshot (a
 device-to-device copy) + the step call.  The state is resident in HBM before
 the timed region starts.
 
-`--workload ensemble`: 128 independent colonies (the same nematic-domain
-packings, standing in for grown colonies) with log-uniform sizes in
+`--workload ensemble`: 128 independent colony packings (disc-shaped nematic
+domains, standing in for grown colonies) with log-uniform sizes in
 [500, 50000] (~1.5M cells) stepped 50 times in mixed precision by one batched
-context; the state carries on from step to step.
+context; the state carries on from step to step.  The untimed warm-up step
+relaxes the packings' initial overlaps.
 
 `--workload serving`: colony-growth requests with Poisson arrivals (20/s for
 10 s), 1k-100k cells each, hard or soft 50/50, growth rate U[0.5,1.5] ln2,
@@ -87,8 +88,9 @@ def run_single(colonies, precision, steps, warmup):
     return {"total_s": total, "lat": lat, "results": results, "timing": tim, "phases": phases}
 
 
-def run_batched(colonies, precision, steps, warmup, timing=True, restore=True):
-    """All colonies in one batched context; one latency per batch step."""
+def run_batched(colonies, precision, steps, warmup, timing=True, restore=True, report=0):
+    """All colonies in one batched context; one latency per batch step
+    (a progress line every `report` steps when report > 0)."""
     from pcc.engine import ColonyBatch
     cb = ColonyBatch(params_for(precision), device=0)
     cb.set_state(colonies, [params_for(precision)] * len(colonies))
@@ -106,6 +108,9 @@ def run_batched(colonies, precision, steps, warmup, timing=True, restore=True):
             cb.restore()
         results.append(cb.step(i))
         lat.append(time.perf_counter() - a)
+        if report and (i + 1) % report == 0:
+            print(f"[bench] step {i + 1}/{steps}: {lat[-1] * 1e3:.1f} ms", file=sys.stderr,
+                  flush=True)
     total = time.perf_counter() - t0
     phases = [cb.phase_stats(q) for q in range(len(colonies))]
     tim = None
@@ -206,7 +211,7 @@ def cpu_baseline(bodies, phases, sample_caps=(10, 110)):
 def ensemble_colonies(n_colonies, seed, lo=500, hi=50_000):
     u = np.array(G.uniform(seed + 977, 0, n_colonies))
     sizes = np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo))).astype(int)
-    return [G.dense_colony(int(n), seed=seed + 1000 + i) for i, n in enumerate(sizes)]
+    return [G.packed_colony(int(n), seed=seed + 1000 + i) for i, n in enumerate(sizes)]
 
 
 def main_ensemble(args):
@@ -215,7 +220,7 @@ def main_ensemble(args):
     print(f"[bench] ensemble: {len(cols)} colonies, {ncell} cells, sizes "
           f"{min(c.n for c in cols)}..{max(c.n for c in cols)}", file=sys.stderr, flush=True)
     steps = args.steps if args.steps_given else 50
-    r = run_batched(cols, PREC_MIXED, steps, args.warmup, timing=True, restore=False)
+    r = run_batched(cols, PREC_MIXED, steps, args.warmup, timing=True, restore=False, report=5)
     p50, p99 = latency_stats(r["lat"])
     its = [sum(x["bbpgd_iterations"] for x in step) for step in r["results"]]
     rf = roofline(r["timing"])
@@ -232,9 +237,9 @@ def main_ensemble(args):
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": (f"{len(cols)} independent hard-model colony packings (nematic "
-                                f"domains, area fraction ~0.85), sizes log-uniform in "
-                                f"[500, 50000] ({ncell} cells), mixed precision, "
+        "config": {"workload": (f"{len(cols)} independent hard-model colony packings (disc "
+                                f"nematic domains, relaxed by the warm-up step), sizes "
+                                f"log-uniform in [500, 50000] ({ncell} cells), mixed precision, "
                                 f"{steps} consecutive steps in one batched context with captured "
                                 f"BBPGD graphs and per-colony convergence"),
                    "n_cells": ncell, "colonies": len(cols)},

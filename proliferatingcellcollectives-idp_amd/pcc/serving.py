@@ -12,10 +12,14 @@ context per model (`engine.ColonyBatch`).  Between batch steps it
     downloaded and the colony leaves the batch, freeing its share of the
     budget for the next request.
 
-The hard and the soft batch are stepped from two host threads, so their work
-overlaps on the device (each context owns its own HIP stream).  Every colony's
-result equals the result of a context that runs it alone: batched colonies
-share launches, never data.
+Colonies are also split by size into buckets, one batched context per
+(model, bucket): a batched step runs its colonies' ReLCP phases and BBPGD
+solves in lockstep, so the slowest colony of a batch sets that batch's step
+time, and bucketing keeps small colonies from waiting on large ones.  All
+batches are stepped from their own host threads, so their work overlaps on
+the device (each context owns its own HIP stream).  Every colony's result
+equals the result of a context that runs it alone: batched colonies share
+launches, never data.
 
 The reference runs one colony per MPI job (Domain::run, Domain.cpp:121-143);
 this loop has no counterpart there.
@@ -24,17 +28,17 @@ import collections
 import threading
 import time
 from dataclasses import dataclass, field
-from typing import Dict, List, Optional
+from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 
 from . import abi_types as T
 from .engine import ColonyBatch
 
-# Device bytes per cell assumed for admission until a batch has been measured
-# (body state and outputs, constraint set, solver-order and incidence arrays,
-# detection scratch; dense 2D packings hold ~10 constraints per cell over the
-# ReLCP phases of a step).
+# Device bytes per cell charged against the budget at admission (body state
+# and outputs, constraint set, solver-order and incidence arrays, detection
+# scratch; dense 2D packings hold ~10 constraints per cell over the ReLCP
+# phases of a step).  The run reports the measured peak beside it.
 DEFAULT_BYTES_PER_CELL = 12_000
 
 
@@ -71,34 +75,48 @@ class _Member:
 
 
 class ColonyServer:
-    """Continuous-batching server over one batched context per model."""
+    """Continuous-batching server: one worker thread per (model, size bucket)
+    batch steps its colonies back to back; the caller's thread admits
+    arrivals first come first served against the HBM budget and hands them to
+    the worker of their batch, which adds them between two of its steps."""
 
     def __init__(self, hbm_budget_bytes: int, precision: int = T.PREC_MIXED, device: int = 0,
-                 bytes_per_cell: Optional[float] = None, overlap: bool = True):
+                 bytes_per_cell: Optional[float] = None, overlap: bool = True,
+                 bucket_edges: Sequence[int] = (4_000, 20_000)):
         self.budget = int(hbm_budget_bytes)
         self.precision = precision
         self.device = device
-        self.overlap = overlap
+        self.overlap = overlap  # False: one worker steps every batch in turn
         self.bytes_per_cell = float(bytes_per_cell or DEFAULT_BYTES_PER_CELL)
-        self._fixed_rate = bytes_per_cell is not None
-        self.batches: Dict[int, Optional[ColonyBatch]] = {T.MODE_HARD: None, T.MODE_SOFT: None}
-        self.members: Dict[int, List[_Member]] = {T.MODE_HARD: [], T.MODE_SOFT: []}
+        self.bucket_edges = tuple(sorted(bucket_edges))
+        self.keys = [(m, b) for m in (T.MODE_HARD, T.MODE_SOFT)
+                     for b in range(len(self.bucket_edges) + 1)]
+        self.batches: Dict[Tuple[int, int], Optional[ColonyBatch]] = {k: None for k in self.keys}
+        self.members: Dict[Tuple[int, int], List[_Member]] = {k: [] for k in self.keys}
+        self.handoff: Dict[Tuple[int, int], List[_Member]] = {k: [] for k in self.keys}
         self.queue: collections.deque = collections.deque()
         self.results: Dict[int, Result] = {}
         self.peak_reserved = 0
         self.peak_device_bytes = 0
         self.batch_steps = 0
-        self._peak_cells = 0
+        self.cell_steps = 0
+        self._reserved = 0
+        self._lock = threading.Condition()
+        self._stop = False
+        self._errors: List[str] = []
+        self._t0 = 0.0
+        self._first_step = None
+        self._last_step = None
 
-    # -- admission -----------------------------------------------------
+    # -- admission (caller's thread) -------------------------------------
     def footprint(self, n_cells: int) -> int:
         return int(self.bytes_per_cell * max(n_cells, 1))
 
     def reserved(self) -> int:
-        return sum(self.footprint(m.req.bodies.n) for ms in self.members.values() for m in ms)
+        return self._reserved
 
-    def device_bytes(self) -> int:
-        return sum(b.device_bytes() for b in self.batches.values() if b is not None)
+    def _bucket(self, n_cells: int) -> int:
+        return sum(n_cells >= e for e in self.bucket_edges)
 
     def submit(self, req: Request):
         if req.params.mode not in (T.MODE_HARD, T.MODE_SOFT):
@@ -109,128 +127,153 @@ class ColonyServer:
 
     def _admit(self, now: float) -> int:
         n = 0
-        while self.queue:
-            req = self.queue[0]
-            need = self.footprint(req.bodies.n)
-            if self.reserved() + need > self.budget and self.reserved() > 0:
-                break  # first come, first served: the head waits for memory
-            self.queue.popleft()
-            mode = req.params.mode
-            if self.batches[mode] is None:
-                self.batches[mode] = ColonyBatch(req.params, device=self.device)
-            self.batches[mode].add(req.bodies, req.params)
-            self.members[mode].append(_Member(req, now))
-            self.peak_reserved = max(self.peak_reserved, self.reserved())
-            n += 1
+        with self._lock:
+            while self.queue:
+                req = self.queue[0]
+                need = self.footprint(req.bodies.n)
+                if self._reserved + need > self.budget and self._reserved > 0:
+                    break  # first come, first served: the head waits for memory
+                self.queue.popleft()
+                self._reserved += need
+                self.peak_reserved = max(self.peak_reserved, self._reserved)
+                key = (req.params.mode, self._bucket(req.bodies.n))
+                self.handoff[key].append(_Member(req, now))
+                n += 1
+            if n:
+                self._lock.notify_all()
         return n
 
-    # -- stepping ------------------------------------------------------
-    def _step_mode(self, mode: int, out: dict):
-        b = self.batches[mode]
+    def _resident(self) -> int:
+        return sum(len(v) for v in self.members.values()) + \
+            sum(len(v) for v in self.handoff.values())
+
+    # -- one batch (its worker thread) -----------------------------------
+    def _serve_once(self, key) -> bool:
+        """Adds handed-off colonies, steps the batch once, retires finished
+        colonies.  Returns False when the batch had nothing to do."""
+        with self._lock:
+            new = self.handoff[key]
+            self.handoff[key] = []
+        ms = self.members[key]
+        for m in new:
+            if self.batches[key] is None:
+                self.batches[key] = ColonyBatch(m.req.params, device=self.device)
+            self.batches[key].add(m.req.bodies, m.req.params)
+            ms.append(m)
+        if not ms:
+            return False
+        b = self.batches[key]
         t = time.perf_counter()
-        r = b.step(self.batch_steps)
-        out[mode] = (r, (time.perf_counter() - t) * 1e3)
-
-    def _step_all(self) -> dict:
-        modes = [m for m in (T.MODE_HARD, T.MODE_SOFT) if self.members[m]]
-        out = {}
-        if self.overlap and len(modes) > 1:
-            th = [threading.Thread(target=self._step_mode, args=(m, out)) for m in modes]
-            for x in th:
-                x.start()
-            for x in th:
-                x.join()
-        else:
-            for m in modes:
-                self._step_mode(m, out)
-        missing = [m for m in modes if m not in out]
-        if missing:
-            raise RuntimeError(f"batch step failed for modes {missing}")
-        self.batch_steps += 1
-        return out
-
-    def _retire(self, now: float):
-        for mode, ms in self.members.items():
-            b = self.batches[mode]
-            for q in range(len(ms) - 1, -1, -1):
-                m = ms[q]
-                if m.done < m.req.steps:
-                    continue
-                state = b.get_state(q)
-                b.remove(q)
-                del ms[q]
+        res = b.step(self.batch_steps)
+        t1 = time.perf_counter()
+        dt_ms = (t1 - t) * 1e3
+        cells = 0
+        for q, m in enumerate(ms):
+            m.done += 1
+            m.step_ms.append(dt_ms)
+            m.statuses.append(res[q]["status"])
+            cells += m.req.bodies.n
+        dev = b.device_bytes()
+        done = []
+        for q in range(len(ms) - 1, -1, -1):
+            m = ms[q]
+            if m.done < m.req.steps:
+                continue
+            state = b.get_state(q)
+            b.remove(q)
+            del ms[q]
+            done.append((m, state))
+        now = time.perf_counter() - self._t0
+        with self._lock:
+            self.batch_steps += 1
+            self.cell_steps += cells
+            self.peak_device_bytes = max(self.peak_device_bytes, dev)
+            if self._first_step is None or t < self._first_step:
+                self._first_step = t
+            self._last_step = t1 if self._last_step is None else max(self._last_step, t1)
+            for m, state in done:
+                self._reserved -= self.footprint(m.req.bodies.n)
                 self.results[m.req.rid] = Result(
-                    m.req.rid, mode, m.req.bodies.n, m.req.steps, state, m.step_ms, m.statuses,
-                    m.req.arrival, m.admitted, now)
+                    m.req.rid, key[0], m.req.bodies.n, m.req.steps, state, m.step_ms,
+                    m.statuses, m.req.arrival, m.admitted, now)
+            self._lock.notify_all()
+        return True
 
-    def _learn_rate(self):
-        # Device buffers keep their high-water size after colonies leave, so
-        # the bytes-per-cell rate is only learned when the resident cell count
-        # reaches a new peak (the buffers were sized for exactly that load).
-        if self._fixed_rate:
-            return
-        cells = sum(m.req.bodies.n for ms in self.members.values() for m in ms)
-        if cells > self._peak_cells:
-            self._peak_cells = cells
-            self.bytes_per_cell = max(self.bytes_per_cell, self.device_bytes() / cells)
+    def _worker(self, keys):
+        try:
+            while True:
+                busy = False
+                for key in keys:
+                    if self._stop:
+                        return
+                    busy = self._serve_once(key) or busy
+                if not busy:
+                    with self._lock:
+                        if self._stop:
+                            return
+                        if not any(self.handoff[k] for k in keys):
+                            self._lock.wait(0.02)
+        except Exception as e:  # surfaced by run()
+            with self._lock:
+                self._errors.append(f"{keys}: {e!r}")
+                self._lock.notify_all()
 
     def run(self, requests: List[Request], time_limit: Optional[float] = None,
             progress=None, progress_every: float = 5.0) -> dict:
         """Serves `requests` (arrival times relative to the start) to
         completion, or until `time_limit` seconds have passed; returns the
         run's statistics.  Results are in self.results.  `progress(msg)`, if
-        given, receives a status line every `progress_every` seconds of
-        stepping."""
+        given, receives a status line every `progress_every` seconds."""
         pending = sorted(requests, key=lambda r: r.arrival)
         i = 0
-        t0 = time.perf_counter()
-        busy_cell_steps = 0
-        first_step = None
-        last_step = None
+        self._t0 = time.perf_counter()
+        groups = [[k] for k in self.keys] if self.overlap else [list(self.keys)]
+        workers = [threading.Thread(target=self._worker, args=(g,)) for g in groups]
+        for w in workers:
+            w.start()
         next_report = progress_every
-        while True:
-            now = time.perf_counter() - t0
-            while i < len(pending) and pending[i].arrival <= now:
-                self.submit(pending[i])
-                i += 1
-            self._admit(now)
-            active = sum(len(ms) for ms in self.members.values())
-            if active == 0:
-                if i >= len(pending) and not self.queue:
-                    break
-                if i < len(pending):
-                    time.sleep(max(0.0, min(pending[i].arrival - now, 0.05)))
-                continue
-            if time_limit is not None and now > time_limit:
-                break
-            if first_step is None:
-                first_step = time.perf_counter()
-            out = self._step_all()
-            last_step = time.perf_counter()
-            for mode, (res, ms_) in out.items():
-                for q, m in enumerate(self.members[mode]):
-                    m.done += 1
-                    m.step_ms.append(ms_)
-                    m.statuses.append(res[q]["status"])
-                    busy_cell_steps += m.req.bodies.n
-            self.peak_device_bytes = max(self.peak_device_bytes, self.device_bytes())
-            self._learn_rate()
-            self._retire(time.perf_counter() - t0)
-            if progress is not None and time.perf_counter() - t0 >= next_report:
-                next_report += progress_every
-                progress(f"t={time.perf_counter() - t0:.1f}s batch_steps={self.batch_steps} "
-                         f"resident={active} queued={len(self.queue)} "
-                         f"completed={len(self.results)}/{len(requests)} "
-                         f"cell_steps={busy_cell_steps}")
-        wall = (last_step - first_step) if first_step is not None else 0.0
+        try:
+            while True:
+                now = time.perf_counter() - self._t0
+                while i < len(pending) and pending[i].arrival <= now:
+                    self.submit(pending[i])
+                    i += 1
+                self._admit(now)
+                with self._lock:
+                    if self._errors:
+                        raise RuntimeError("batch worker failed: " + "; ".join(self._errors))
+                    idle = i >= len(pending) and not self.queue and self._resident() == 0
+                    if idle:
+                        break
+                    if time_limit is not None and now > time_limit:
+                        break
+                    if progress is not None and now >= next_report:
+                        next_report += progress_every
+                        progress(f"t={now:.1f}s batch_steps={self.batch_steps} "
+                                 f"resident={self._resident()} queued={len(self.queue)} "
+                                 f"completed={len(self.results)}/{len(requests)} "
+                                 f"cell_steps={self.cell_steps}")
+                    wait = 0.05
+                    if i < len(pending):
+                        wait = max(0.0, min(wait, pending[i].arrival - now))
+                    self._lock.wait(wait)
+        finally:
+            with self._lock:
+                self._stop = True
+                self._lock.notify_all()
+            for w in workers:
+                w.join()
+        if self._errors:
+            raise RuntimeError("batch worker failed: " + "; ".join(self._errors))
+        wall = (self._last_step - self._first_step) if self._first_step is not None else 0.0
         lat = np.array([x for r in self.results.values() for x in r.step_ms]
                        + [x for ms in self.members.values() for m in ms for x in m.step_ms])
         return {
             "requests": len(requests),
             "completed": len(self.results),
-            "cell_steps": int(busy_cell_steps),
+            "cell_steps": int(self.cell_steps),
             "wall_s": wall,
-            "cell_steps_per_s": busy_cell_steps / wall if wall > 0 else 0.0,
+            "cell_steps_per_s": self.cell_steps / wall if wall > 0 else 0.0,
             "p50_step_ms": float(np.percentile(lat, 50)) if len(lat) else 0.0,
             "p99_step_ms": float(np.percentile(lat, 99)) if len(lat) else 0.0,
             "batch_steps": self.batch_steps,
@@ -240,7 +283,7 @@ class ColonyServer:
         }
 
     def close(self):
-        for b in self.batches.values():
+        for k, b in self.batches.items():
             if b is not None:
                 b.close()
-        self.batches = {T.MODE_HARD: None, T.MODE_SOFT: None}
+            self.batches[k] = None

@@ -73,8 +73,19 @@ def test_every_request_runs_exactly_its_steps():
         assert len(res.step_ms) == r.steps and res.statuses == [0] * r.steps
         assert np.array_equal(res.state.length, r.bodies.length + r.steps)
         assert res.mode == r.params.mode
-    # one batch per model
+    # one batch per (model, size bucket) in use: all sizes here fall in the
+    # smallest bucket
     assert sorted(b.mode for b in FakeBatch.instances) == [0, 1]
+
+
+def test_size_buckets_get_their_own_batches():
+    reqs = make_requests([(100, 0, 2, 0.0), (5000, 0, 2, 0.0), (25000, 0, 2, 0.0),
+                          (150, 0, 2, 0.0)])
+    srv = serving.ColonyServer(10**12, bytes_per_cell=1000, bucket_edges=(4000, 20000))
+    st = srv.run(reqs)
+    assert st["completed"] == 4
+    assert len(FakeBatch.instances) == 3
+    assert sorted(len(b.events) for b in FakeBatch.instances) == [1, 1, 2]
 
 
 def test_admission_respects_budget_and_order():
@@ -97,7 +108,7 @@ def test_late_arrival_joins_a_running_batch():
     reqs = make_requests([(100, 0, 40, 0.0), (100, 0, 3, 0.02)])
     srv = serving.ColonyServer(10**12, bytes_per_cell=1000)
     srv.run(reqs)
-    adds = FakeBatch.instances[0].events
+    adds = FakeBatch.instances[0].events  # both colonies fall in the same bucket
     # the second colony was added while the first had already stepped
     assert len(adds) == 2 and adds[1][2][0] > 0
     assert np.array_equal(srv.results[1].state.length, reqs[1].bodies.length + 3)
