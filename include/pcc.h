@@ -248,6 +248,20 @@ pcc_status pcc_batch_divide(pcc_ctx* ctx, const uint64_t* seeds, int32_t step,
  * reference counterpart. */
 pcc_status pcc_get_tuning(pcc_ctx* ctx, int32_t* body_chunk, int32_t* tuned, double* variant_us);
 
+/* CFL time-step adaptation between steps (Domain::adaptDt, Domain.cpp:69-118),
+ * on the device: the median over the colony's cells of |v| + ldot after the
+ * last step (mean of the two middle values for an even count), then
+ * dt' = clamp(dt (1 - alpha) + alpha cfl_factor tolerance / median,
+ * 0.9 dt, 1.1 dt); dt is kept when the median is 0.  The reference uses
+ * cfl_factor 0.5 (-cfl_factor), alpha 0.01 and its single `tolerance`.
+ * pcc_adapt_dt updates the context's dt; pcc_batch_adapt_dt every colony's,
+ * with the colony's relcp_tol as the tolerance.  *dt / dts[q] receive the new
+ * step, *median / medians[q] the median speed (either may be NULL). */
+pcc_status pcc_adapt_dt(pcc_ctx* ctx, double cfl_factor, double tolerance, double alpha,
+                        double* dt, double* median);
+pcc_status pcc_batch_adapt_dt(pcc_ctx* ctx, double cfl_factor, double alpha, double* dts,
+                              double* medians);
+
 /* Device memory held by the context (every device buffer it has allocated),
  * for admission against an HBM budget.  No reference counterpart. */
 pcc_status pcc_device_bytes(pcc_ctx* ctx, int64_t* bytes);

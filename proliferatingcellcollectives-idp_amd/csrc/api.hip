@@ -861,6 +861,47 @@ pcc_status pcc_batch_divide(pcc_ctx* ctx, const uint64_t* seeds, int32_t step,
   return st;
 }
 
+pcc_status pcc_adapt_dt(pcc_ctx* ctx, double cfl_factor, double tolerance, double alpha,
+                        double* dt, double* median) {
+  if (!ctx) return PCC_E_INVALID_ARGUMENT;
+  if (!ctx->has_state) return fail(ctx, PCC_E_NO_STATE, "no state");
+  if (ctx->inst.size() != 1)
+    return fail(ctx, PCC_E_INVALID_ARGUMENT, "batched context: use pcc_batch_adapt_dt");
+  if (!(alpha >= 0.0 && alpha <= 1.0) || !(cfl_factor > 0.0) || !(tolerance > 0.0))
+    return fail(ctx, PCC_E_INVALID_ARGUMENT, "invalid time-step adaptation parameter");
+  return guarded(ctx, [&]() {
+    PCC_HIP(hipSetDevice(ctx->device));
+    std::vector<double> m;
+    colony_median_speeds(ctx, m);
+    const double nd = adapted_dt(ctx->params.dt, m[0], cfl_factor, tolerance, alpha);
+    ctx->params.dt = nd;
+    ctx->inst[0].p.dt = nd;
+    if (dt) *dt = nd;
+    if (median) *median = m[0];
+    return PCC_OK;
+  });
+}
+
+pcc_status pcc_batch_adapt_dt(pcc_ctx* ctx, double cfl_factor, double alpha, double* dts,
+                              double* medians) {
+  if (!ctx) return PCC_E_INVALID_ARGUMENT;
+  if (!ctx->has_state) return fail(ctx, PCC_E_NO_STATE, "no state");
+  if (!(alpha >= 0.0 && alpha <= 1.0) || !(cfl_factor > 0.0))
+    return fail(ctx, PCC_E_INVALID_ARGUMENT, "invalid time-step adaptation parameter");
+  return guarded(ctx, [&]() {
+    PCC_HIP(hipSetDevice(ctx->device));
+    std::vector<double> m;
+    colony_median_speeds(ctx, m);
+    for (size_t q = 0; q < ctx->inst.size(); ++q) {
+      pcc_params& p = ctx->inst[q].p;
+      p.dt = adapted_dt(p.dt, m[q], cfl_factor, p.relcp_tol, alpha);
+      if (dts) dts[q] = p.dt;
+      if (medians) medians[q] = m[q];
+    }
+    return PCC_OK;
+  });
+}
+
 pcc_status pcc_batch_set_params(pcc_ctx* ctx, int64_t colony, const pcc_params* params) {
   if (!ctx || !params) return PCC_E_INVALID_ARGUMENT;
   if (colony < 0 || colony >= (int64_t)ctx->inst.size())

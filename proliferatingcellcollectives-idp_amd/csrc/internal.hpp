@@ -282,6 +282,8 @@ void soft_forces_and_integrate(pcc_ctx* ctx);
 void reset_step(pcc_ctx* ctx);
 void divide_colonies(pcc_ctx* ctx, const uint64_t* seeds, int32_t step, double length_noise,
                      double angle_noise);
+void colony_median_speeds(pcc_ctx* ctx, std::vector<double>& median);
+double adapted_dt(double dt, double median, double cfl_factor, double tolerance, double alpha);
 void upload_instances(pcc_ctx* ctx);
 void max_overlap_inst(pcc_ctx* ctx, std::vector<double>& out);
 void merge_batch(pcc_ctx* ctx);

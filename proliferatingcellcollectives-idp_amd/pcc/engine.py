@@ -72,6 +72,9 @@ def lib():
         L.pcc_device_bytes.argtypes = [vp, C.POINTER(i64)]
         L.pcc_get_tuning.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                      C.POINTER(C.c_double)]
+        dpt = C.POINTER(C.c_double)
+        L.pcc_adapt_dt.argtypes = [vp, C.c_double, C.c_double, C.c_double, dpt, dpt]
+        L.pcc_batch_adapt_dt.argtypes = [vp, C.c_double, C.c_double, dpt, dpt]
         L.pcc_divide.argtypes = [vp, C.c_uint64, C.c_int32, C.c_double, C.c_double,
                                  C.POINTER(i64)]
         L.pcc_batch_divide.argtypes = [vp, C.POINTER(C.c_uint64), C.c_int32, C.c_double,
@@ -85,7 +88,7 @@ def lib():
                   "pcc_batch_get_state", "pcc_batch_get_constraints",
                   "pcc_batch_get_phase_stats", "pcc_device_bytes",
                   "pcc_measure_copy_bandwidth", "pcc_divide", "pcc_batch_divide",
-                  "pcc_get_tuning"):
+                  "pcc_get_tuning", "pcc_adapt_dt", "pcc_batch_adapt_dt"):
             getattr(L, f).restype = C.c_int
         _lib = L
     return _lib
@@ -98,7 +101,8 @@ EXPORTED = ("pcc_default_params", "pcc_create", "pcc_destroy", "pcc_set_params",
             "pcc_batch_set_state", "pcc_batch_add", "pcc_batch_remove", "pcc_batch_set_params",
             "pcc_batch_size", "pcc_batch_step", "pcc_batch_get_state",
             "pcc_batch_get_constraints", "pcc_batch_get_phase_stats", "pcc_device_bytes",
-            "pcc_measure_copy_bandwidth", "pcc_divide", "pcc_batch_divide", "pcc_get_tuning")
+            "pcc_measure_copy_bandwidth", "pcc_divide", "pcc_batch_divide", "pcc_get_tuning",
+            "pcc_adapt_dt", "pcc_batch_adapt_dt")
 
 
 def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 10, device: int = 0) -> float:
@@ -167,6 +171,16 @@ class ParticleManager:
         s = b.c_struct()
         self._check(lib().pcc_get_state(self._ctx, C.byref(s)))
         return b
+
+    def adapt_dt(self, cfl_factor: float = 0.5, tolerance: float = None,
+                 alpha: float = 0.01):
+        """Domain::adaptDt on the device; returns (new dt, median speed)."""
+        tol = self.params.relcp_tol if tolerance is None else tolerance
+        dt, med = C.c_double(0), C.c_double(0)
+        self._check(lib().pcc_adapt_dt(self._ctx, float(cfl_factor), float(tol), float(alpha),
+                                       C.byref(dt), C.byref(med)))
+        self.params.dt = float(dt.value)
+        return float(dt.value), float(med.value)
 
     def tuning(self) -> dict:
         """Autotuner state: body-kernel staging size in use, whether it was
@@ -358,6 +372,15 @@ class ColonyBatch(ParticleManager):
 
     def detect(self, cutoff, capacity=None):
         raise PccError(T.PCC_E_INVALID_ARGUMENT, "detect() needs a single-colony context")
+
+    def adapt_dt(self, cfl_factor: float = 0.5, alpha: float = 0.01):
+        """Domain::adaptDt for every colony (its relcp_tol as the tolerance);
+        returns (dts, medians)."""
+        B = len(self.sizes)
+        dts, med = (C.c_double * B)(), (C.c_double * B)()
+        self._check(lib().pcc_batch_adapt_dt(self._ctx, float(cfl_factor), float(alpha), dts,
+                                             med))
+        return [float(x) for x in dts], [float(x) for x in med]
 
     def divide(self, seeds, step: int, length_noise: float = 0.0, angle_noise: float = 0.01):
         """Divides the cells of every colony (seeds[q] per colony) and

@@ -82,3 +82,51 @@ def test_device_resident_growth_from_one_cell():
     assert want.n >= 40
     same_bodies(pm.get_state(), want)
     pm.close()
+
+
+def expected_dt(state, dt, cfl_factor, tol, alpha):
+    """Domain::adaptDt (Domain.cpp:69-118) restated with numpy: median of
+    |v| + ldot (mean of the two middle values for an even count), smoothed
+    CFL step, clamped to [0.9 dt, 1.1 dt]."""
+    u = state.velocity
+    v = np.sqrt(u[:, 0] * u[:, 0] + u[:, 1] * u[:, 1] + u[:, 2] * u[:, 2]) + state.ldot
+    s = np.sort(v)
+    n = len(s)
+    med = float(s[n // 2]) if n % 2 else 0.5 * (float(s[n // 2 - 1]) + float(s[n // 2]))
+    if med == 0.0:
+        return dt, med
+    nd = dt * (1.0 - alpha) + alpha * ((cfl_factor * tol) / med)
+    lo, hi = 0.9 * dt, 1.1 * dt
+    return (lo if nd < lo else hi if hi < nd else nd), med
+
+
+@pytest.mark.parametrize("n,tol", [(2000, 1e-3), (2001, 1e-6), (1, 1e-2)])
+def test_adapt_dt_matches_reference_rule(n, tol):
+    from pcc.engine import ParticleManager
+    p = reference_params(dt=1e-3, tolerance=tol, tau=1 / LN2, cutoff_phase0=0.05,
+                         cutoff_phaseK=0.05)
+    b = G.dense_colony(n, seed=170) if n > 1 else G.single_cell()
+    pm = ParticleManager(p)
+    pm.set_state(b)
+    pm.step(0)
+    want_dt, want_med = expected_dt(pm.get_state(), p.dt, 0.5, tol, 0.01)
+    dt, med = pm.adapt_dt(0.5, tol, 0.01)
+    print(f"[adapt_dt] n={n} median={med:.6g} dt={dt:.6g}", flush=True)
+    assert med == want_med and dt == want_dt
+    pm.step(1)  # the next step runs with the new dt
+    pm.close()
+
+
+def test_batch_adapt_dt_per_colony():
+    from pcc.engine import ColonyBatch, ParticleManager
+    cols = [G.dense_colony(n, seed=180 + i) for i, n in enumerate([1500, 301, 2400])]
+    ps = [reference_params(dt=1e-3 * (1 + i), tolerance=1e-3, tau=1 / LN2, cutoff_phase0=0.05,
+                           cutoff_phaseK=0.05) for i in range(3)]
+    cb = ColonyBatch(ps[0])
+    cb.set_state(cols, ps)
+    cb.step(0)
+    dts, meds = cb.adapt_dt(0.5, 0.01)
+    for q in range(3):
+        want_dt, want_med = expected_dt(cb.get_state(q), ps[q].dt, 0.5, ps[q].relcp_tol, 0.01)
+        assert meds[q] == want_med and dts[q] == want_dt
+    cb.close()
