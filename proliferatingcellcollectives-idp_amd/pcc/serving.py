@@ -151,15 +151,17 @@ class ColonyServer:
     def _serve_once(self, key) -> bool:
         """Adds handed-off colonies, steps the batch once, retires finished
         colonies.  Returns False when the batch had nothing to do."""
+        ms = self.members[key]
         with self._lock:
+            # moved in one critical section so that run() never sees a colony
+            # in neither list
             new = self.handoff[key]
             self.handoff[key] = []
-        ms = self.members[key]
+            ms.extend(new)
         for m in new:
             if self.batches[key] is None:
                 self.batches[key] = ColonyBatch(m.req.params, device=self.device)
             self.batches[key].add(m.req.bodies, m.req.params)
-            ms.append(m)
         if not ms:
             return False
         b = self.batches[key]
@@ -181,8 +183,7 @@ class ColonyServer:
                 continue
             state = b.get_state(q)
             b.remove(q)
-            del ms[q]
-            done.append((m, state))
+            done.append((q, m, state))
         now = time.perf_counter() - self._t0
         with self._lock:
             self.batch_steps += 1
@@ -191,7 +192,8 @@ class ColonyServer:
             if self._first_step is None or t < self._first_step:
                 self._first_step = t
             self._last_step = t1 if self._last_step is None else max(self._last_step, t1)
-            for m, state in done:
+            for q, m, state in done:  # descending q
+                del ms[q]
                 self._reserved -= self.footprint(m.req.bodies.n)
                 self.results[m.req.rid] = Result(
                     m.req.rid, key[0], m.req.bodies.n, m.req.steps, state, m.step_ms,
