@@ -292,6 +292,7 @@ class ColonyBatch(ParticleManager):
     def __init__(self, params: T.Params, device: int = 0):
         super().__init__(params, device)
         self.sizes = []
+        self.colony_params = []  # host copy of every colony's parameters
 
     def set_state(self, colonies, params=None):
         cs = [b.c_struct() for b in colonies]
@@ -301,6 +302,8 @@ class ColonyBatch(ParticleManager):
             parr = (T.Params * len(params))(*[p.copy() for p in params])
         self._check(lib().pcc_batch_set_state(self._ctx, len(cs), arr, parr))
         self.sizes = [b.n for b in colonies]
+        self.colony_params = [(params[q] if params is not None else self.params).copy()
+                              for q in range(len(cs))]
         self.n = sum(self.sizes)
 
     def add(self, bodies: T.Bodies, params: T.Params = None) -> int:
@@ -309,16 +312,19 @@ class ColonyBatch(ParticleManager):
         pp = C.byref(params.copy()) if params is not None else None
         self._check(lib().pcc_batch_add(self._ctx, C.byref(s), pp, C.byref(idx)))
         self.sizes.append(bodies.n)
+        self.colony_params.append((params if params is not None else self.params).copy())
         self.n = sum(self.sizes)
         return int(idx.value)
 
     def remove(self, q: int):
         self._check(lib().pcc_batch_remove(self._ctx, int(q)))
         del self.sizes[q]
+        del self.colony_params[q]
         self.n = sum(self.sizes)
 
     def set_colony_params(self, q: int, params: T.Params):
         self._check(lib().pcc_batch_set_params(self._ctx, int(q), C.byref(params.copy())))
+        self.colony_params[q] = params.copy()
 
     def __len__(self):
         return len(self.sizes)
@@ -380,6 +386,8 @@ class ColonyBatch(ParticleManager):
         dts, med = (C.c_double * B)(), (C.c_double * B)()
         self._check(lib().pcc_batch_adapt_dt(self._ctx, float(cfl_factor), float(alpha), dts,
                                              med))
+        for q in range(B):
+            self.colony_params[q].dt = float(dts[q])
         return [float(x) for x in dts], [float(x) for x in med]
 
     def divide(self, seeds, step: int, length_noise: float = 0.0, angle_noise: float = 0.01):
