@@ -134,6 +134,25 @@ typedef struct pcc_step_result {
   double max_overlap;
 } pcc_step_result;
 
+/* One BBPGD iteration of a traced step: BBPGDStep (logger/BBPGDLogger.h:7-17)
+ * as collected at BBPGD.cpp:57-80, with the energy terms of
+ * HardModelGradient::energy (HardModelGradient.cpp:160-203) at the iterate:
+ *   linear    = gamma . Phi
+ *   quadratic = dt/2 (D gamma) . M (D gamma)
+ *   growth    = dt/(lambda tau) sum_i l_i exp(-lambda (L gamma)_i)
+ *   total     = linear + quadratic + growth. */
+typedef struct pcc_bbpgd_record {
+  int64_t step;                /* BBPGD iteration (0 = the initial gradient) */
+  int64_t recursive_iteration; /* ReLCP phase of the solve */
+  int64_t total_constraints;   /* constraints of the phase */
+  double residual;             /* projected-gradient residual at the iterate */
+  double step_size;            /* BB step that produced the iterate (1/res at step 0) */
+  double linear;
+  double quadratic;
+  double growth;
+  double total;
+} pcc_bbpgd_record;
+
 typedef struct pcc_ctx pcc_ctx;
 
 /* Reference defaults (code/cpp/src/util/CLIParser.h:167-194,232) with the
@@ -261,6 +280,21 @@ pcc_status pcc_adapt_dt(pcc_ctx* ctx, double cfl_factor, double tolerance, doubl
                         double* dt, double* median);
 pcc_status pcc_batch_adapt_dt(pcc_ctx* ctx, double cfl_factor, double alpha, double* dts,
                               double* medians);
+
+/* BBPGD trace (the reference's bbpgd_logger, Hard.cpp:107-116,181-183, and
+ * BBPGD.cpp:57-80).  While enabled, every hard step records, per colony, one
+ * pcc_bbpgd_record per BBPGD iteration of every ReLCP phase (up to
+ * `capacity` records per colony and step; later ones are counted, not
+ * stored), and, as in the reference with a logger attached, checks the
+ * residual at every iteration instead of every tenth, so a traced solve may
+ * stop earlier than an untraced one.  The energy terms are summed on the
+ * device in a fixed order.  enable = 0 turns it off (capacity ignored). */
+pcc_status pcc_set_trace(pcc_ctx* ctx, int enable, int64_t capacity);
+/* Records of the last step for colony `colony` (0 for a single-colony
+ * context): copies min(count, capacity, trace capacity) records to `out` (may
+ * be NULL with capacity 0); *count = the iterations traced in that step. */
+pcc_status pcc_get_trace(pcc_ctx* ctx, int64_t colony, pcc_bbpgd_record* out, int64_t capacity,
+                         int64_t* count);
 
 /* Device memory held by the context (every device buffer it has allocated),
  * for admission against an HBM budget.  No reference counterpart. */

@@ -15,7 +15,7 @@ _PKG = os.path.join(os.path.dirname(_HERE), "proliferatingcellcollectives-idp_am
 if _PKG not in sys.path:
     sys.path.insert(0, _PKG)
 
-from pcc.abi_types import (Bodies, Constraints, Params, StepResult,  # noqa: E402
+from pcc.abi_types import (BBPGD_RECORD, Bodies, Constraints, Params, StepResult,  # noqa: E402
                            reference_params)
 
 _libs = {}
@@ -56,6 +56,10 @@ def lib(fast: bool = False):
         L.oracle_threads.restype = C.c_int
         L.oracle_phase_times.argtypes = [C.POINTER(C.c_double), C.c_int64]
         L.oracle_phase_times.restype = C.c_int64
+        L.oracle_set_trace.argtypes = [C.c_void_p, C.c_int64]
+        L.oracle_trace_count.restype = C.c_int64
+        L.oracle_phase0_energy.argtypes = [C.POINTER(Params), C.c_void_p, dp, dp, dp]
+        L.oracle_phase0_energy.restype = C.c_int64
         _libs[name] = L
     return _libs[name]
 
@@ -170,6 +174,41 @@ def step(params: Params, bodies: Bodies, iter_=0, capacity=None, fast=False, pha
         err.value.decode()
 
 
+def traced_step(params: Params, bodies: Bodies, iter_=0, capacity=100000, fast=False):
+    """step() with the BBPGD trace on: (status, result, bodies, constraints,
+    phase iterations, error, trace records, iterations traced)."""
+    L = lib(fast)
+    buf = np.zeros(capacity, BBPGD_RECORD)
+    cap_c = 64 * max(bodies.n, 1) + 64
+    while True:  # step() re-runs itself when its constraint buffer is short
+        L.oracle_set_trace(buf.ctypes.data_as(C.c_void_p), capacity)
+        try:
+            out = step(params, bodies, iter_, capacity=cap_c, fast=fast)
+            n = L.oracle_trace_count()
+        finally:
+            L.oracle_set_trace(None, 0)
+        if out[1]["n_constraints"] <= cap_c:
+            break
+        cap_c = out[1]["n_constraints"]
+    return out + (buf[:min(n, capacity)].copy(), n)
+
+
+def phase0_energy(params: Params, bodies: Bodies, gamma=None):
+    """(C, E[4], g) of phase 0 at `bodies`: E = (linear, quadratic, growth,
+    total) and g the BBPGD gradient at gamma (None: C only)."""
+    bs = bodies.c_struct()
+    L = lib()
+    C_ = L.oracle_phase0_energy(C.byref(params), C.byref(bs), None, None, None)
+    if gamma is None:
+        return C_, None, None
+    gam = np.ascontiguousarray(gamma, dtype=np.float64)
+    assert len(gam) == C_
+    E = np.zeros(4)
+    g = np.zeros(max(C_, 1))
+    L.oracle_phase0_energy(C.byref(params), C.byref(bs), _d(gam), _d(E), _d(g))
+    return C_, E, g[:C_]
+
+
 def phase_times(fast=False):
     """Wall time of each ReLCP phase of the last hard step."""
     out = np.zeros(64)
@@ -182,4 +221,5 @@ def threads(fast=False):
 
 
 __all__ = ["Bodies", "Params", "reference_params", "dcp", "direction", "normalize", "exp",
+           "traced_step", "phase0_energy",
            "block_sum", "grid", "detect", "step", "lib", "threads"]

@@ -839,8 +839,114 @@ static double residual(const double* g, const double* gam, int64_t C) {
   return res;
 }
 
+/* HardModelGradient::energy (HardModelGradient.cpp:160-203) at gam:
+ * out = {gamma.Phi, dt/2 (D gamma).M(D gamma), dt/(lambda tau) sum l exp(-lambda
+ * L gamma), their sum}.  Summed in the GPU's order: gamma.Phi over the
+ * constraints in solver order in the blocked tree of bb_sums; per body the six
+ * products F_k U_k in k order and l exp(-lambda sigma), then blocks of 256
+ * bodies through block256 and the block partials through reduce_partials.
+ * F = D^T gamma and sigma = L^T gamma in ascending incidence order, as in
+ * gradient(). */
+static void energy(const phase_t* ph, const double* gam, double out[4]) {
+  const int64_t N = ph->N, C = ph->C;
+  const int64_t Q = (C + 255) / 256;
+  const int64_t G = Q < ORACLE_BB_GMAX ? (Q > 0 ? Q : 1) : ORACLE_BB_GMAX;
+  const int64_t K = (Q + G - 1) / G;
+  double* part = ALLOC(double, G);
+  for (int64_t b = 0; b < G; ++b) {
+    double x[256];
+    for (int t = 0; t < 256; ++t) {
+      double a = 0.0;
+      for (int64_t j = 0; j < K; ++j) {
+        int64_t pos = (b * K + j) * 256 + t;
+        if (pos >= C) break;
+        const int64_t i = ph->ord[pos];
+        a += gam[i] * ph->phi[i];
+      }
+      x[t] = a;
+    }
+    part[b] = block256(x);
+  }
+  const double term1 = reduce_partials(part, G);
+  free(part);
+  const int64_t nb = (N + 255) / 256;
+  double* p2 = ALLOC(double, nb);
+  double* p3 = ALLOC(double, nb);
+  const double mlam = -ph->lambda;
+  for (int64_t b = 0; b < nb; ++b) {
+    double x2[256], x3[256];
+    for (int t = 0; t < 256; ++t) {
+      const int64_t i = b * 256 + t;
+      x2[t] = 0.0;
+      x3[t] = 0.0;
+      if (i >= N) continue;
+      double F[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      double sig = 0.0;
+      for (int64_t e = ph->rowptr[i]; e < ph->rowptr[i + 1]; ++e) {
+        const int64_t c = ph->inc[e] >> 1;
+        const int side = (int)(ph->inc[e] & 1);
+        const double* Dv = ph->D + 12 * c + 6 * side;
+        for (int k = 0; k < 6; ++k) F[k] += gam[c] * Dv[k];
+        sig += gam[c] * (side ? ph->sJ[c] : ph->sI[c]);
+      }
+      double q = 0.0;
+      for (int k = 0; k < 6; ++k) {
+        const double m = k < 3 ? ph->mt[i] : ph->mr[i];
+        const double u = (0.0 + m * F[k]) + 0.0;
+        q += F[k] * u;
+      }
+      x2[t] = q;
+      x3[t] = ph->len[i] * oracle_exp(sig * mlam);
+    }
+    p2[b] = block256(x2);
+    p3[b] = block256(x3);
+  }
+  double term2 = reduce_partials(p2, nb);
+  double term3 = reduce_partials(p3, nb);
+  free(p2);
+  free(p3);
+  term2 *= (ph->dt / 2.0);
+  term3 *= (ph->dt / (ph->lambda * ph->tau));
+  out[0] = term1;
+  out[1] = term2;
+  out[2] = term3;
+  out[3] = term1 + term2 + term3;
+}
+
+/* BBPGD trace (BBPGDLogger; collected at BBPGD.cpp:57-60,73-80). */
+static pcc_bbpgd_record* g_trace = NULL;
+static int64_t g_trace_cap = 0, g_trace_n = 0, g_trace_phase = 0;
+
+void oracle_set_trace(pcc_bbpgd_record* buf, int64_t capacity) {
+  g_trace = buf;
+  g_trace_cap = buf ? capacity : 0;
+  g_trace_n = 0;
+}
+int64_t oracle_trace_count(void) { return g_trace_n; }
+
+static void trace_collect(const phase_t* ph, const double* gam, int64_t step, double res,
+                          double alpha) {
+  if (g_trace_n < g_trace_cap) {
+    double E[4];
+    energy(ph, gam, E);
+    pcc_bbpgd_record* r = g_trace + g_trace_n;
+    r->step = step;
+    r->recursive_iteration = g_trace_phase;
+    r->total_constraints = ph->C;
+    r->residual = res;
+    r->step_size = alpha;
+    r->linear = E[0];
+    r->quadratic = E[1];
+    r->growth = E[2];
+    r->total = E[3];
+  }
+  ++g_trace_n;
+}
+
 /* BBPGD (BBPGD.cpp:32-127).  On return Gamma holds the lagging iterate (Q3),
- * g_last the gradient of the last evaluated iterate (the workspace). */
+ * g_last the gradient of the last evaluated iterate (the workspace).  While
+ * a trace buffer is set, the residual is checked at every iteration
+ * (BBPGD.cpp:73, `|| bbpgd_logger`) and every iteration is recorded. */
 static pcc_status bbpgd(phase_t* ph, double* Gamma, double tol, int64_t maxit, double* g_last,
                         int64_t* iters, double* res_out) {
   const int64_t C = ph->C;
@@ -852,6 +958,8 @@ static pcc_status bbpgd(phase_t* ph, double* Gamma, double tol, int64_t maxit, d
   gradient(ph, yp, gp);             /* :49 */
   double res = residual(gp, yp, C); /* :51 */
   double alpha = 1.0 / res;         /* :52 */
+  const int tracing = g_trace != NULL;
+  if (tracing) trace_collect(ph, yp, 0, res, alpha); /* :57-60 */
   int64_t it = 1;
   pcc_status st = PCC_OK;
   int64_t cap = g_iteration_cap;
@@ -863,8 +971,9 @@ static pcc_status bbpgd(phase_t* ph, double* Gamma, double tol, int64_t maxit, d
       yi[c] = v < 0.0 ? 0.0 : v;
     }
     gradient(ph, yi, gi); /* :70 */
-    if (it % 10 == 0) {   /* :73-88 */
+    if (it % 10 == 0 || tracing) { /* :73-88 */
       res = residual(gi, yi, C);
+      if (tracing) trace_collect(ph, yi, it, res, alpha);
       if (res <= tol) break;
     }
     if (cap > 0 && it >= cap) break;
@@ -982,6 +1091,7 @@ static pcc_status hard_step(const pcc_params* p, pcc_bodies* b, pcc_step_result*
     double* g_last = ALLOC(double, S.C);
     for (int64_t c = 0; c < S.C; ++c) Gamma[c] = S.r[c].gamma;
     int64_t its = 0;
+    g_trace_phase = k; /* set_recursive_iteration, Hard.cpp:114-116 */
     st = bbpgd(&ph, Gamma, p->bbpgd_tol, p->max_bbpgd_iterations, g_last, &its, &res); /* :153 */
     if (phase_iters && k < max_phase) phase_iters[k] = its;
     if (st != PCC_OK) {
@@ -1133,6 +1243,29 @@ static pcc_status soft_step(const pcc_params* p, pcc_bodies* b, pcc_step_result*
   free(sig);
   free(recs);
   return PCC_OK;
+}
+
+int64_t oracle_phase0_energy(const pcc_params* p, const pcc_bodies* b, const double* gamma,
+                             double E[4], double* g) {
+  body_view bv = {b->pos, b->quat, b->length, b->diameter};
+  cset_t S = {0, 0, NULL, NULL, NULL};
+  rec_t* recs = NULL;
+  int64_t m = detect_records(&bv, b->n, p->cutoff_phase0, p->linked_cell_size, p->grid_origin,
+                             p->warm_start_factor, 0, &recs);
+  cset_append(&S, recs, m, 0);
+  free(recs);
+  if (gamma) {
+    phase_t ph;
+    memset(&ph, 0, sizeof(ph));
+    phase_build(&ph, &S, b, p);
+    if (E) energy(&ph, gamma, E);
+    if (g) gradient(&ph, gamma, g);
+    phase_free(&ph);
+  }
+  free(S.r);
+  free(S.gid);
+  free(S.iter);
+  return m;
 }
 
 pcc_status oracle_step(const pcc_params* p, pcc_bodies* b, int32_t iter, pcc_step_result* result,

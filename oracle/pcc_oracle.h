@@ -86,6 +86,22 @@ void oracle_set_iteration_cap(int64_t iteration_cap);
  * returns the number of phases. */
 int64_t oracle_phase_times(double* out, int64_t max);
 
+/* BBPGD trace of the following hard steps (BBPGDLogger; BBPGD.cpp:57-80):
+ * one record per BBPGD iteration of every phase, up to `capacity`; the
+ * residual is checked at every iteration while a buffer is set.  NULL turns
+ * it off.  oracle_trace_count(): iterations traced since the last set. */
+void oracle_set_trace(pcc_bbpgd_record* buf, int64_t capacity);
+int64_t oracle_trace_count(void);
+
+/* Phase 0 of a hard step at the given state, for tests of the energy: detects
+ * with cutoff_phase0, builds the operators (gamma_old = the warm start,
+ * ldot_prev = b->ldot) and, if gamma (one value per constraint) is non-NULL,
+ * writes E = HardModelGradient::energy (HardModelGradient.cpp:160-203) and
+ * g = HardModelGradient::gradient (:89-121) at gamma (either may be NULL).
+ * Returns the number of constraints. */
+int64_t oracle_phase0_energy(const pcc_params* p, const pcc_bodies* b, const double* gamma,
+                             double E[4], double* g);
+
 /* Deterministic blocked sum used for the BB dot products (exposed for tests). */
 double oracle_block_sum(const double* v, int64_t n);
 

@@ -130,8 +130,11 @@ void upload_instances(pcc_ctx* ctx) {
     d.warm = p.warm_start_factor;
     d.origin = p.grid_origin;
     d.h = p.linked_cell_size;
+    d.lam = p.lambda;
+    d.tau = p.tau;
+    d.phase = I.k;
     d.flags = I.flags;
-    d.pad = 0;
+    d.check = ctx->trace_on ? 1 : 10;
   }
   if (B == 0) return;
   ctx->idev.ensure(B);
@@ -230,6 +233,14 @@ void fill_result(const Instance& I, pcc_step_result* r) {
 
 void begin_step(pcc_ctx* ctx) {
   reset_step(ctx);
+  const int64_t B = (int64_t)ctx->inst.size();
+  ctx->trace_B = 0;
+  if (ctx->trace_on && B > 0) {
+    ctx->trace.ensure(B * std::max<int64_t>(ctx->trace_cap, 1));
+    ctx->trace_n.ensure(B);
+    PCC_HIP(hipMemsetAsync(ctx->trace_n.p, 0, sizeof(int64_t) * B, ctx->stream));
+    ctx->trace_B = B;
+  }
   for (auto& I : ctx->inst) {
     I.k = 0;
     I.its = 0;
@@ -1010,6 +1021,39 @@ pcc_status pcc_get_tuning(pcc_ctx* ctx, int32_t* body_chunk, int32_t* tuned, dou
   return PCC_OK;
 }
 
+pcc_status pcc_set_trace(pcc_ctx* ctx, int enable, int64_t capacity) {
+  if (!ctx) return PCC_E_INVALID_ARGUMENT;
+  if (enable && capacity < 0)
+    return fail(ctx, PCC_E_INVALID_ARGUMENT, "trace capacity must be >= 0");
+  ctx->trace_on = enable != 0;
+  ctx->trace_cap = enable ? capacity : 0;
+  return PCC_OK;
+}
+
+pcc_status pcc_get_trace(pcc_ctx* ctx, int64_t colony, pcc_bbpgd_record* out, int64_t capacity,
+                         int64_t* count) {
+  if (!ctx || !count || capacity < 0 || (capacity > 0 && !out))
+    return PCC_E_INVALID_ARGUMENT;
+  *count = 0;
+  if (ctx->trace_B == 0) return PCC_OK;  // the last step was not traced
+  if (colony < 0 || colony >= ctx->trace_B)
+    return fail(ctx, PCC_E_INVALID_ARGUMENT, "colony out of range of the traced step");
+  return guarded(ctx, [&]() {
+    int64_t n = 0;
+    PCC_HIP(hipMemcpyAsync(&n, ctx->trace_n.p + colony, sizeof(int64_t), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    PCC_HIP(hipStreamSynchronize(ctx->stream));
+    const int64_t m = std::min(std::min(n, capacity), ctx->trace_cap);
+    if (m > 0) {
+      PCC_HIP(hipMemcpyAsync(out, ctx->trace.p + colony * std::max<int64_t>(ctx->trace_cap, 1),
+                             sizeof(pcc_bbpgd_record) * m, hipMemcpyDeviceToHost, ctx->stream));
+      PCC_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    *count = n;
+    return PCC_OK;
+  });
+}
+
 pcc_status pcc_device_bytes(pcc_ctx* ctx, int64_t* bytes) {
   if (!ctx || !bytes) return PCC_E_INVALID_ARGUMENT;
   int64_t t = 0;
@@ -1030,7 +1074,7 @@ pcc_status pcc_device_bytes(pcc_ctx* ctx, int64_t* bytes) {
   add(ctx->iter); add(ctx->gprev); add(ctx->ubuf); add(ctx->mob); add(ctx->partials);
   add(ctx->state); add(ctx->scratch_d); add(ctx->hertz); add(ctx->hertz32); add(ctx->idev);
   add(ctx->binst); add(ctx->bmap); add(ctx->cmap); add(ctx->bbounds); add(ctx->maxbits);
-  add(ctx->regroup_dst);
+  add(ctx->regroup_dst); add(ctx->trace); add(ctx->trace_n); add(ctx->epart);
   *bytes = t;
   return PCC_OK;
 }

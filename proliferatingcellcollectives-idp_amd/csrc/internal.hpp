@@ -128,8 +128,10 @@ struct InstDev {
   int64_t coff_new;  // constraint offset once the current batch is merged
   int64_t maxit;
   double dt, xi, itau, mlam, kcc, tol, cutoff, warm, origin, h;
+  double lam, tau;   // for the energy's growth term dt / (lambda tau)
+  int64_t phase;     // ReLCP phase of the current solve
   int32_t flags;
-  int32_t pad;
+  int32_t check;     // residual checked every `check` iterations (10; 1 while tracing)
 };
 
 // Constraint set of the current step (all ReLCP phases), FP64 SoA.
@@ -242,7 +244,14 @@ struct pcc_ctx {
   pcc::DArr<double> gprev;                 // lagging iterate at convergence checks
   pcc::DArr<double> ubuf;                  // 8 per body: U[6], dldot, pad
   pcc::DArr<double> mob;                   // 2 per body: 1/(xi l), 12/(xi l^3)
-  pcc::DArr<double> partials;              // 4 * nblocks
+  pcc::DArr<double> partials;              // 4 * nblocks (5 while tracing)
+  // BBPGD trace: per colony `trace_cap` records and a count per step
+  bool trace_on = false;
+  int64_t trace_cap = 0;
+  int64_t trace_B = 0;  // colonies traced in the last step (0: last step untraced)
+  pcc::DArr<pcc_bbpgd_record> trace;
+  pcc::DArr<int64_t> trace_n;
+  pcc::DArr<double> epart;                 // energy partials per body block (2 * nbb)
   pcc::DArr<pcc::SolverState> state;       // B
   pcc::SolverState* state_host = nullptr;  // pinned, 2*B slots
   int64_t state_host_cap = 0;

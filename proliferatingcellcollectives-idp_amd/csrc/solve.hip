@@ -502,7 +502,14 @@ struct ConstrArgs {
   double* gprev;        // lagging iterate saved on iterations % 10 == 0
   const double* ubuf;
   SolverState* st;      // one per colony
-  double* partials;     // 4 * Gtot
+  double* partials;     // 4 * Gtot (5 * Gtot while tracing: gamma.Phi)
+  // BBPGD trace (all null / 0 when off)
+  int32_t trace;
+  int64_t trace_cap;
+  pcc_bbpgd_record* trace_buf;  // trace_cap records per colony
+  int64_t* trace_n;             // records per colony this step
+  const double* epart;          // energy partials: [nbb] F.MF, [nbb] l exp(-lambda sigma)
+  int64_t nbb;
 };
 
 // Operator row of one constraint as stored (CT), kept unconverted in
@@ -576,7 +583,7 @@ __global__ void __launch_bounds__(256) k_constr(ConstrArgs a) {
   const double dt = D.dt;
   const int64_t k = st->iter;
   const double malpha = -st->alpha;
-  const bool save_prev = k > 0 && k % 10 == 0;
+  const bool save_prev = k > 0 && k % D.check == 0;
   constexpr int UW = PLANAR ? 4 : 8;  // doubles per body velocity record
   auto fetch = [&](int64_t c, TileIn<CT, PLANAR>& t) {
     t.I = a.I[c];
@@ -585,7 +592,7 @@ __global__ void __launch_bounds__(256) k_constr(ConstrArgs a) {
     t.phi = a.phi[c];
     t.row = load_row<CT, PLANAR>(a.coef, c);
   };
-  double v0 = 0.0, v1 = 0.0, v2 = 0.0, vr = 0.0;
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0, vr = 0.0, ve = 0.0;
   int64_t c = D.coff + b * K * PCC_BLOCK + threadIdx.x;
   bool ok = K > 0 && c < cend;
   TileIn<CT, PLANAR> cur;
@@ -646,6 +653,7 @@ __global__ void __launch_bounds__(256) k_constr(ConstrArgs a) {
     const double pj = gc > 0 ? g : (g < 0.0 ? g : 0.0);
     const double ar = fabs(pj);
     vr = vr < ar ? ar : vr;
+    if (a.trace) ve += gc * cur.phi;  // energy: gamma . Phi
     c = cn;
     ok = okn;
     if (okn) cur = nxt;
@@ -659,6 +667,72 @@ __global__ void __launch_bounds__(256) k_constr(ConstrArgs a) {
     a.partials[a.Gtot + l] = s1;
     a.partials[2 * a.Gtot + l] = s2;
     a.partials[3 * a.Gtot + l] = rm;
+  }
+  if (a.trace) {
+    const double se = block_sum(ve, lds);
+    if (threadIdx.x == 0) a.partials[4 * a.Gtot + l] = se;
+  }
+}
+
+// Energy of the iterate just evaluated (HardModelGradient.cpp:160-203), body
+// part, while tracing: per body F = D^T gamma and sigma = L^T gamma in
+// ascending incidence order, U = M F, F.U (six products in order) and
+// l exp(-lambda sigma); per logical body block their block sums.
+template <class CT, bool PLANAR>
+__global__ void __launch_bounds__(256) k_energy_body(BodyArgs a, double* __restrict__ epart) {
+  __shared__ double lds[4];
+  const int64_t l = xcd_block(blockIdx.x, a.gp);
+  if (l >= a.nbb) return;
+  const int32_t inst = a.bmap[l];
+  const SolverState* st = a.st + inst;
+  if (st->skip | st->converged | st->failed) return;
+  const InstDev& D = a.idev[inst];
+  const int64_t i = D.boff + (l - D.bb) * PCC_BLOCK + threadIdx.x;
+  double q2 = 0.0, q3 = 0.0;
+  if (i < D.boff + D.N) {
+    double F[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double sig = 0.0;
+    const int32_t e1 = a.rowptr[i + 1];
+    for (int32_t e = a.rowptr[i]; e < e1; ++e) {
+      double co[7];
+      int64_t c;
+      if (PLANAR) {
+        double p[4];
+        load_prec<CT>(a.rec, e, p);
+        co[0] = p[0];
+        co[1] = p[1];
+        co[5] = p[2];
+        co[6] = p[3];
+        c = a.rc[e];
+      } else {
+        load_rec<CT>(a.rec, e, co, c);
+      }
+      const double gc = reinterpret_cast<const double2*>(a.it)[c].x;
+      if (PLANAR) {
+        F[0] += gc * co[0];
+        F[1] += gc * co[1];
+        F[5] += gc * co[5];
+      } else {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) F[q] += gc * co[q];
+      }
+      sig += gc * co[6];
+    }
+    const double mt = a.mob[2 * i], mr = a.mob[2 * i + 1];
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const double u = (0.0 + (q < 3 ? mt : mr) * F[q]) + 0.0;
+      s += F[q] * u;
+    }
+    q2 = s;
+    q3 = a.len[i] * det_exp(sig * D.mlam);
+  }
+  const double s2 = block_sum(q2, lds);
+  const double s3 = block_sum(q3, lds);
+  if (threadIdx.x == 0) {
+    epart[l] = s2;
+    epart[a.nbb + l] = s3;
   }
 }
 
@@ -684,16 +758,52 @@ __global__ void __launch_bounds__(256) k_finish(ConstrArgs a) {
   const double S1 = block_sum(a1, lds);
   const double S2 = block_sum(a2, lds);
   const double R = block_max(am, lds);
+  // BBPGD trace record of this iterate (BBPGD.cpp:57-60,76-80): energy terms
+  // from the gamma.Phi block partials and the colony's body-block partials,
+  // each combined in the fixed strided order above
+  double E1 = 0.0, E2 = 0.0, E3 = 0.0;
+  if (a.trace) {
+    double a4 = 0.0, a5 = 0.0, a6 = 0.0;
+    for (int64_t i = threadIdx.x; i < G; i += PCC_BLOCK) a4 += a.partials[4 * T + o + i];
+    const int64_t nb = (D.N + PCC_BLOCK - 1) / PCC_BLOCK;
+    for (int64_t i = threadIdx.x; i < nb; i += PCC_BLOCK) {
+      a5 += a.epart[D.bb + i];
+      a6 += a.epart[a.nbb + D.bb + i];
+    }
+    E1 = block_sum(a4, lds);
+    E2 = block_sum(a5, lds);
+    E3 = block_sum(a6, lds);
+  }
   if (threadIdx.x != 0) return;
+  auto record = [&](double res, double alpha) {
+    const int64_t n = a.trace_n[blockIdx.x];
+    if (n < a.trace_cap) {
+      pcc_bbpgd_record* r = a.trace_buf + (int64_t)blockIdx.x * a.trace_cap + n;
+      const double t2 = E2 * (D.dt / 2.0);
+      const double t3 = E3 * (D.dt / (D.lam * D.tau));
+      r->step = k;
+      r->recursive_iteration = D.phase;
+      r->total_constraints = D.C;
+      r->residual = res;
+      r->step_size = alpha;
+      r->linear = E1;
+      r->quadratic = t2;
+      r->growth = t3;
+      r->total = E1 + t2 + t3;
+    }
+    a.trace_n[blockIdx.x] = n + 1;
+  };
   if (k == 0) {  // initial gradient: res = residual(g, Gamma), alpha = 1/res
     st->res = R;
     st->alpha = 1.0 / R;
     st->iter = 1;
+    if (a.trace) record(R, st->alpha);
     if (1 == D.maxit) st->failed = 1;
     return;
   }
-  if (k % 10 == 0) {
+  if (k % D.check == 0) {
     st->res = R;
+    if (a.trace) record(R, st->alpha);
     if (R <= D.tol) {
       st->converged = 1;
       return;
@@ -995,6 +1105,12 @@ static ConstrArgs constr_args(pcc_ctx* ctx) {
   a.ubuf = ctx->ubuf.p;
   a.st = ctx->state.p;
   a.partials = ctx->partials.p;
+  a.trace = ctx->trace_on ? 1 : 0;
+  a.trace_cap = ctx->trace_on ? ctx->trace_cap : 0;
+  a.trace_buf = ctx->trace_on ? ctx->trace.p : nullptr;
+  a.trace_n = ctx->trace_on ? ctx->trace_n.p : nullptr;
+  a.epart = ctx->trace_on ? ctx->epart.p : nullptr;
+  a.nbb = ctx->nbb_total;
   return a;
 }
 
@@ -1075,7 +1191,8 @@ void bbpgd_solve(pcc_ctx* ctx) {
   solver_layout(ctx);
   ctx->ubuf.ensure(8 * N + 8);
   ctx->mob.ensure(2 * N + 2);
-  ctx->partials.ensure(4 * std::max<int64_t>(ctx->ncb_total, 1) + 4);
+  ctx->partials.ensure((ctx->trace_on ? 5 : 4) * std::max<int64_t>(ctx->ncb_total, 1) + 4);
+  if (ctx->trace_on) ctx->epart.ensure(2 * ctx->nbb_total + 2);
   ctx->state.ensure(B);
   ensure_state_host(ctx, 2 * B);
   BodyArrays b = body_arrays(ctx->body.p, N);
@@ -1111,6 +1228,20 @@ void bbpgd_solve(pcc_ctx* ctx) {
           k_constr<float, false><<<gc, 256, 0, s>>>(ca);
         else
           k_constr<double, false><<<gc, 256, 0, s>>>(ca);
+      }
+    }
+    if (ctx->trace_on) {
+      double* ep = ctx->epart.p;
+      if (planar) {
+        if (mixed)
+          k_energy_body<float, true><<<gb, 256, 0, s>>>(ba, ep);
+        else
+          k_energy_body<double, true><<<gb, 256, 0, s>>>(ba, ep);
+      } else {
+        if (mixed)
+          k_energy_body<float, false><<<gb, 256, 0, s>>>(ba, ep);
+        else
+          k_energy_body<double, false><<<gb, 256, 0, s>>>(ba, ep);
       }
     }
     k_finish<<<(unsigned)B, 256, 0, s>>>(ca);

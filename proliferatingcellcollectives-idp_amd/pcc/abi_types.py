@@ -155,6 +155,15 @@ class StepResult(C.Structure):
         }
 
 
+# pcc_bbpgd_record: BBPGDStep (logger/BBPGDLogger.h:7-17) with the energy
+# terms of HardModelGradient::energy at the iterate
+BBPGD_RECORD = np.dtype([
+    ("step", "<i8"), ("recursive_iteration", "<i8"), ("total_constraints", "<i8"),
+    ("residual", "<f8"), ("step_size", "<f8"), ("linear", "<f8"), ("quadratic", "<f8"),
+    ("growth", "<f8"), ("total", "<f8"),
+])
+
+
 def _ptr(a, ct):
     if a is None:
         return C.cast(None, C.POINTER(ct))

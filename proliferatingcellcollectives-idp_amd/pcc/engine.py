@@ -80,6 +80,8 @@ def lib():
         L.pcc_batch_divide.argtypes = [vp, C.POINTER(C.c_uint64), C.c_int32, C.c_double,
                                        C.c_double, C.POINTER(i64)]
         L.pcc_measure_copy_bandwidth.argtypes = [C.c_int, i64, C.c_int, C.POINTER(C.c_double)]
+        L.pcc_set_trace.argtypes = [vp, C.c_int, i64]
+        L.pcc_get_trace.argtypes = [vp, i64, vp, i64, C.POINTER(i64)]
         for f in ("pcc_create", "pcc_set_params", "pcc_set_state", "pcc_get_state",
                   "pcc_snapshot", "pcc_restore", "pcc_step", "pcc_get_constraints",
                   "pcc_detect", "pcc_set_timing", "pcc_get_timing", "pcc_get_phase_stats",
@@ -88,7 +90,8 @@ def lib():
                   "pcc_batch_get_state", "pcc_batch_get_constraints",
                   "pcc_batch_get_phase_stats", "pcc_device_bytes",
                   "pcc_measure_copy_bandwidth", "pcc_divide", "pcc_batch_divide",
-                  "pcc_get_tuning", "pcc_adapt_dt", "pcc_batch_adapt_dt"):
+                  "pcc_get_tuning", "pcc_adapt_dt", "pcc_batch_adapt_dt", "pcc_set_trace",
+                  "pcc_get_trace"):
             getattr(L, f).restype = C.c_int
         _lib = L
     return _lib
@@ -102,7 +105,7 @@ EXPORTED = ("pcc_default_params", "pcc_create", "pcc_destroy", "pcc_set_params",
             "pcc_batch_size", "pcc_batch_step", "pcc_batch_get_state",
             "pcc_batch_get_constraints", "pcc_batch_get_phase_stats", "pcc_device_bytes",
             "pcc_measure_copy_bandwidth", "pcc_divide", "pcc_batch_divide", "pcc_get_tuning",
-            "pcc_adapt_dt", "pcc_batch_adapt_dt")
+            "pcc_adapt_dt", "pcc_batch_adapt_dt", "pcc_set_trace", "pcc_get_trace")
 
 
 def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 10, device: int = 0) -> float:
@@ -190,6 +193,24 @@ class ParticleManager:
         self._check(lib().pcc_get_tuning(self._ctx, C.byref(ch), C.byref(tu), us))
         return {"body_chunk": int(ch.value), "tuned": bool(tu.value),
                 "variant_us": [float(x) for x in us]}
+
+    def set_trace(self, enable: bool = True, capacity: int = 100000):
+        """BBPGD trace of the following steps (the reference's bbpgd_logger):
+        one record per iteration of every phase, residual checked every
+        iteration while on."""
+        self._check(lib().pcc_set_trace(self._ctx, int(bool(enable)), int(capacity)))
+        self._trace_cap = int(capacity) if enable else 0
+
+    def trace(self, q: int = 0):
+        """(records, iterations traced) of the last step for colony q:
+        records is a numpy array of dtype abi_types.BBPGD_RECORD."""
+        cnt = C.c_int64(0)
+        self._check(lib().pcc_get_trace(self._ctx, int(q), None, 0, C.byref(cnt)))
+        n = min(int(cnt.value), getattr(self, "_trace_cap", 0))  # records stored
+        buf = np.zeros(max(n, 1), T.BBPGD_RECORD)
+        self._check(lib().pcc_get_trace(self._ctx, int(q), buf.ctypes.data_as(C.c_void_p), n,
+                                        C.byref(cnt)))
+        return buf[:n].copy(), int(cnt.value)
 
     def device_bytes(self) -> int:
         """Device memory held by this context."""

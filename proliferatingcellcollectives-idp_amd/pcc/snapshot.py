@@ -15,7 +15,7 @@ Checkpoint (`save_checkpoint` / `load_checkpoint`)
     parameters.
 
 VTU / PVTU (`VTKDataLogger`, `ParticleLogger`, `ConstraintLogger`,
-`SimulationLogger`, `VTKStateLoader`)
+`SimulationLogger`, `BBPGDLogger`, `VTKStateLoader`)
     Files laid out as the reference writes them (code/cpp/src/logger/VTK.h:
     180-313 for the directory layout and the XML, ParticleLogger.cpp:61-74,
     ConstraintLogger.cpp, SimulationLogger.cpp:14-29 for the array names), so
@@ -545,6 +545,30 @@ class ConstraintLogger:
         L.add_point_data("gamma", cons["gamma"].astype(np.float64))
         L.add_point_data("rank", np.full(n, L.rank, np.int32))
         L.write()
+
+
+class BBPGDLogger:
+    """BBPGDLogger::log (BBPGDLogger.cpp): one rank-0 VTU per traced BBPGD
+    iteration, its record as field data (step, recursive_iteration,
+    total_constraints, residual, step_size, linear, quadratic, growth, total),
+    under the reference's "logs/bbpgdtrace_<k>" names (Hard.cpp:110).
+    `records` is what ParticleManager.trace() returns."""
+
+    FIELDS = ("step", "recursive_iteration", "total_constraints", "residual", "step_size",
+              "linear", "quadratic", "growth", "total")
+
+    def __init__(self, output_directory: str = "logs", base_filename: str = "bbpgdtrace",
+                 step: int = 0, precision: str = "reference"):
+        self.logger = VTKDataLogger(output_directory, base_filename, True, False, step, 0, 1,
+                                    precision)
+
+    def log(self, records):
+        for r in records:
+            for f in self.FIELDS:
+                v = r[f]
+                self.logger.add_field_data(f, int(v) if np.issubdtype(type(v), np.integer)
+                                           else float(v))
+            self.logger.write()
 
 
 # SimulationStep (SimulationLogger.h:7-30): name -> integer-valued

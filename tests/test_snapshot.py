@@ -275,3 +275,16 @@ def test_constraint_logger_arrays(tmp_path):
                      "rank"]
     n = int(root.find("UnstructuredGrid/Piece").attrib["NumberOfPoints"])
     assert n == len(cons["gidI"]) > 0
+
+
+def test_bbpgd_logger_writes_one_file_per_iteration(tmp_path):
+    p = T.reference_params(dt=1e-2, tolerance=1e-4)
+    rec = oracle.traced_step(p, G.dense_colony(200, seed=13))[6]
+    S.BBPGDLogger(str(tmp_path)).log(rec[:7])
+    assert sorted(os.listdir(tmp_path)) == ["bbpgdtrace_%07d.pvtu" % k for k in range(7)] + \
+        ["data"]
+    L = S.VTKStateLoader(str(tmp_path))
+    fd = L.read_field_data(str(tmp_path / "bbpgdtrace_0000003.pvtu"))
+    assert list(fd) == list(S.BBPGDLogger.FIELDS)
+    assert fd["step"] == 3 and fd["total_constraints"] == rec["total_constraints"][3]
+    assert fd["residual"] == float("%g" % rec["residual"][3])
