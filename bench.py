@@ -61,7 +61,7 @@ def run_single(colonies, precision, steps, warmup):
     """The single-colony path: one context per colony, colonies one after
     another; per-kernel timing pass on the first colony."""
     from pcc.engine import ParticleManager
-    total, lat, results, phases, tim = 0.0, [], [], None, None
+    total, lat, results, phases, tim, tuning = 0.0, [], [], None, None, None
     for q, bodies in enumerate(colonies):
         pm = ParticleManager(params_for(precision), device=0)
         pm.set_state(bodies)
@@ -79,13 +79,15 @@ def run_single(colonies, precision, steps, warmup):
             results.append(r)
         if q == 0:
             phases = pm.phase_stats()
+            tuning = pm.tuning()
             pm.set_timing(True)
             pm.restore()
             pm.step(0)
             tim = pm.timing()
             pm.set_timing(False)
         pm.close()
-    return {"total_s": total, "lat": lat, "results": results, "timing": tim, "phases": phases}
+    return {"total_s": total, "lat": lat, "results": results, "timing": tim, "phases": phases,
+            "tuning": tuning}
 
 
 def run_batched(colonies, precision, steps, warmup, timing=True, restore=True, report=0):
@@ -113,6 +115,7 @@ def run_batched(colonies, precision, steps, warmup, timing=True, restore=True, r
                   flush=True)
     total = time.perf_counter() - t0
     phases = [cb.phase_stats(q) for q in range(len(colonies))]
+    tuning = cb.tuning()
     tim = None
     if timing:
         # per-kernel pass: HIP events around every BBPGD launch on the context
@@ -124,7 +127,8 @@ def run_batched(colonies, precision, steps, warmup, timing=True, restore=True, r
         tim = cb.timing()
         cb.set_timing(False)
     cb.close()
-    return {"total_s": total, "lat": lat, "results": results, "timing": tim, "phases": phases}
+    return {"total_s": total, "lat": lat, "results": results, "timing": tim, "phases": phases,
+            "tuning": tuning}
 
 
 PMC_SUMMARY = os.path.join(REPO, "profiles", "bbpgd_mixed_300k_colony_pmc.txt")
@@ -369,6 +373,7 @@ def main():
             "n_constraints": [x["n_constraints"] for x in r["results"][::args.steps]],
             "roofline": roofline(r["timing"]),
             "phases": r["phases"],
+            "tuning": r["tuning"],
         }
         print(f"[bench] fp64 single-colony path: {out['fp64']['ms_per_step']:.1f} ms per colony-step, "
               f"iterations {out['fp64']['bbpgd_iterations']}", file=sys.stderr, flush=True)
@@ -385,6 +390,7 @@ def main():
             "roofline": roofline(r["timing"]),
             "kernels": r["timing"],
             "phases": r["phases"][0],
+            "tuning": r["tuning"],
         }
         print(f"[bench] mixed batched ({len(cols)} colonies): {out['mixed']['ms_per_step']:.1f} ms "
               f"per batch step, iterations {out['mixed']['bbpgd_iterations']}",
@@ -449,6 +455,8 @@ def main():
         line["mixed_over_fp64"] = out["mixed"]["cell_steps_per_s"] / out["fp64"]["cell_steps_per_s"]
         line["mixed_bbpgd_iterations"] = out["mixed"]["bbpgd_iterations"]
     line["phases"] = {k: v["phases"] for k, v in out.items()}
+    # body-kernel variant picked by the autotuner, with every candidate's time
+    line["tuning"] = {k: v["tuning"] for k, v in out.items()}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(bodies, (out.get("fp64") or head)["phases"])
     print(json.dumps(line))

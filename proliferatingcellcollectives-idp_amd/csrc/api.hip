@@ -596,6 +596,14 @@ pcc_status pcc_create(const pcc_params* params, int device, pcc_ctx** out) {
   ctx->use_graphs = !(graph_env && graph_env[0] == '0');
   const char* tune_env = getenv("PCC_AUTOTUNE");
   ctx->autotune = !(tune_env && tune_env[0] == '0');
+  // PCC_BODY_VARIANT=v (0..PCC_BODY_VARIANTS-1) fixes the body-kernel variant
+  if (const char* v = getenv("PCC_BODY_VARIANT")) {
+    const int x = atoi(v);
+    if (v[0] >= '0' && v[0] <= '9' && x >= 0 && x < kBodyVariants) {
+      ctx->body_variant = x;
+      ctx->autotune = false;
+    }
+  }
   pcc_status s = guarded(ctx, [&]() {
     PCC_HIP(hipSetDevice(device));
     {
@@ -1012,9 +1020,11 @@ pcc_status pcc_get_timing(pcc_ctx* ctx, int kernel, int64_t* launches, double* t
   });
 }
 
-pcc_status pcc_get_tuning(pcc_ctx* ctx, int32_t* body_chunk, int32_t* tuned, double* variant_us) {
+pcc_status pcc_get_tuning(pcc_ctx* ctx, int32_t* body_chunk, int32_t* body_threads,
+                          int32_t* tuned, double* variant_us) {
   if (!ctx) return PCC_E_INVALID_ARGUMENT;
   if (body_chunk) *body_chunk = body_chunk_of(ctx->body_variant, ctx->planar);
+  if (body_threads) *body_threads = body_threads_of(ctx->body_variant);
   if (tuned) *tuned = ctx->body_tuned ? 1 : 0;
   if (variant_us)
     for (int v = 0; v < kBodyVariants; ++v) variant_us[v] = ctx->body_variant_ms[v] * 1e3;
@@ -1068,7 +1078,7 @@ pcc_status pcc_device_bytes(pcc_ctx* ctx, int64_t* bytes) {
   add(ctx->bstart); add(ctx->bcursor); add(ctx->dcount); add(ctx->rI_); add(ctx->rJ_);
   add(ctx->rrec); add(ctx->ridx); add(ctx->ridx2); add(ctx->rkey); add(ctx->rkey2);
   add(ctx->sort_tmp); add_set(ctx->S); add_set(ctx->S2); add(ctx->rowptr); add(ctx->inccur);
-  add(ctx->inc); add(ctx->increc); add(ctx->increc32); add(ctx->incidx); add(ctx->gold_e);
+  add(ctx->inc); add(ctx->increc); add(ctx->increc32); add(ctx->incidx); add(ctx->gold_e); add(ctx->fold);
   add(ctx->sord); add(ctx->spos); add(ctx->skey); add(ctx->skey2); add(ctx->sval); add(ctx->sI);
   add(ctx->sJ); add(ctx->sphi); add(ctx->prow); add(ctx->prow32); add(ctx->flag);
   add(ctx->iter); add(ctx->gprev); add(ctx->ubuf); add(ctx->mob); add(ctx->partials);

@@ -94,8 +94,9 @@ struct BodyArrays {
   double *px, *py, *pz, *qw, *qx, *qy, *qz, *len, *l0, *diam, *ldot;
 };
 static constexpr int kBodyFields = 11;
-static constexpr int kBodyVariants = 3;
+static constexpr int kBodyVariants = PCC_BODY_VARIANTS;
 int body_chunk_of(int variant, bool planar);
+int body_threads_of(int variant);
 
 // Device-side solver scalars of one instance's BBPGD solve.
 struct SolverState {
@@ -200,7 +201,7 @@ struct pcc_ctx {
   bool autotune = true;
   bool body_tuned = false;
   int body_variant = 1;
-  float body_variant_ms[3] = {0.f, 0.f, 0.f};
+  float body_variant_ms[PCC_BODY_VARIANTS] = {};
 
   // body state + snapshot + outputs
   pcc::DArr<double> body;      // kBodyFields * N
@@ -233,6 +234,7 @@ struct pcc_ctx {
   pcc::DArr<float> increc32;               // per incidence: 8 or 4 floats
   pcc::DArr<int32_t> incidx;               // planar: solver position per incidence
   pcc::DArr<double> gold_e;                // gamma_old per incidence
+  pcc::DArr<double> fold;                  // mixed: D^T gamma_old per body (6, planar 3)
   pcc::DArr<int32_t> sord, spos;           // solver order and its inverse
   pcc::DArr<int32_t> skey, skey2, sval;    // sort scratch
   pcc::DArr<int32_t> sI, sJ;               // solver order

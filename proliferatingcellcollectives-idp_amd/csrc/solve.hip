@@ -241,7 +241,8 @@ struct BodyArgs {
   const int32_t* rowptr;
   const void* rec;     // 3D: 8 CT per incidence (coef[6], s, index bits); 2D: 4 CT
   const int32_t* rc;   // 2D: constraint index per incidence
-  const double* gold;  // gamma_old per incidence
+  const double* gold;  // gamma_old per incidence (FP64 kernels)
+  const double* fold;  // mixed kernels: D^T gamma_old per body (3D: 6, planar: x, y, z-torque)
   double* it;          // iterate records (gamma, g), solver order
   const double* mob;
   const double* len;
@@ -305,6 +306,59 @@ __device__ __forceinline__ void load_prec<float>(const void* base, int64_t e, do
   co[3] = a.w;
 }
 
+// Mixed precision: F = D^T (gamma - gamma_old) is evaluated as D^T gamma
+// minus the per-body sum D^T gamma_old, which is fixed within a phase, so the
+// iteration reads 3 or 6 doubles per body instead of gamma_old per incidence.
+// Both sums use the same FP32 coefficients and FP64 accumulation.
+__device__ __forceinline__ void unfold(const double* __restrict__ fold, int64_t i, double F[6],
+                                       bool planar) {
+  if (planar) {
+    const double* f = fold + 3 * i;
+    F[0] -= f[0];
+    F[1] -= f[1];
+    F[5] -= f[2];
+  } else {
+    const double* f = fold + 6 * i;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) F[q] -= f[q];
+  }
+}
+
+// D^T gamma_old per body with the FP32 incidence coefficients, in ascending
+// incidence order (once per phase, mixed precision only).
+template <bool PLANAR>
+__global__ void k_fold(int64_t N, const int32_t* __restrict__ rowptr, const void* __restrict__ rec,
+                       const double* __restrict__ gold, double* __restrict__ fold) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double F[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const int32_t e1 = rowptr[i + 1];
+  for (int32_t e = rowptr[i]; e < e1; ++e) {
+    const double g = gold[e];
+    if (PLANAR) {
+      double p[4];
+      load_prec<float>(rec, e, p);
+      F[0] += g * p[0];
+      F[1] += g * p[1];
+      F[5] += g * p[2];
+    } else {
+      double co[7];
+      int64_t c;
+      load_rec<float>(rec, e, co, c);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) F[q] += g * co[q];
+    }
+  }
+  if (PLANAR) {
+    fold[3 * i] = F[0];
+    fold[3 * i + 1] = F[1];
+    fold[3 * i + 2] = F[5];
+  } else {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) fold[6 * i + q] = F[q];
+  }
+}
+
 // Workspace of the converged iterate (HardModelGradient.cpp:89-121), one
 // thread per body of the colonies being integrated: F = D^T (gamma -
 // gamma_old), U = M F, sigma = L^T gamma, impedance, ldot; the same
@@ -315,6 +369,7 @@ struct FinalOut {
 
 template <class CT, bool PLANAR>
 __global__ void __launch_bounds__(256) k_body_final(BodyArgs a, FinalOut fo) {
+  constexpr bool FOLD = sizeof(CT) == 4;  // mixed: gamma_old enters as the per-body sum
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.N) return;
   const InstDev& D = a.idev[a.binst[i]];
@@ -337,7 +392,7 @@ __global__ void __launch_bounds__(256) k_body_final(BodyArgs a, FinalOut fo) {
       load_rec<CT>(a.rec, e, co, c);
     }
     const double gc = reinterpret_cast<const double2*>(a.it)[c].x;
-    const double dg = gc - a.gold[e];
+    const double dg = FOLD ? gc : gc - a.gold[e];
     if (PLANAR) {
       F[0] += dg * co[0];
       F[1] += dg * co[1];
@@ -348,6 +403,7 @@ __global__ void __launch_bounds__(256) k_body_final(BodyArgs a, FinalOut fo) {
     }
     sig += gc * co[6];
   }
+  if (FOLD) unfold(a.fold, i, F, PLANAR);
   const double mt = a.mob[2 * i], mr = a.mob[2 * i + 1];
   double u[6];
 #pragma unroll
@@ -371,14 +427,17 @@ __global__ void __launch_bounds__(256) k_body_final(BodyArgs a, FinalOut fo) {
 // lane takes incidences of the chunk in turn, gathers the constraint's
 // iterate record (independent loads, full memory-level parallelism) and
 // writes the products dg*coef and gamma*s to LDS.  Phase 2: each thread adds
-// the products of its own body in ascending incidence order, so every sum has
-// exactly the order of the thread-per-body loop above.  Blocks never straddle
-// two colonies (each colony starts a new logical block).
-// CHUNK (incidences staged per pass) does not change any sum and is picked by
-// the autotuner.
-template <class CT, bool PLANAR, int CHUNK>
-__global__ void __launch_bounds__(256) k_body_coop(BodyArgs a) {
+// the products of its own bodies (256 / TPB of them) in ascending incidence
+// order, so every sum has exactly the order of the thread-per-body loop
+// above.  Blocks never straddle two colonies (each colony starts a new
+// logical block).  CHUNK (incidences staged per pass) and TPB (threads per
+// block: 256, or 128 so that a 600k-body grid fits the device in one round)
+// do not change any sum and are picked by the autotuner.
+template <class CT, bool PLANAR, int CHUNK, int TPB>
+__global__ void __launch_bounds__(TPB) k_body_coop(BodyArgs a) {
   constexpr int NC = PLANAR ? 4 : 7;  // products per incidence
+  constexpr bool FOLD = sizeof(CT) == 4;  // mixed: gamma_old enters as the per-body sum
+  constexpr int R = PCC_BLOCK / TPB;      // bodies per thread
   __shared__ double prod[NC][CHUNK];
   const int64_t l = xcd_block(blockIdx.x, a.gp);
   if (l >= a.nbb) return;
@@ -389,16 +448,22 @@ __global__ void __launch_bounds__(256) k_body_coop(BodyArgs a) {
   const int64_t bend = D.boff + D.N;
   const int64_t b0 = D.boff + (l - D.bb) * PCC_BLOCK;
   const int64_t b1 = b0 + PCC_BLOCK < bend ? b0 + PCC_BLOCK : bend;
-  const int64_t i = b0 + threadIdx.x;
-  const bool active = i < b1;
   const int64_t k = st->iter;
   const double malpha = -st->alpha;
   const int32_t E0 = a.rowptr[b0], E1 = a.rowptr[b1];
-  int32_t e_my = active ? a.rowptr[i] : 0;
-  const int32_t e_end = active ? a.rowptr[i + 1] : 0;
-  double F[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  double sig = 0.0;
-  constexpr int PER = CHUNK / PCC_BLOCK;  // incidences per lane and chunk
+  int32_t e_my[R], e_end[R];
+  double F[R][6], sig[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int64_t i = b0 + threadIdx.x + r * TPB;
+    const bool act = i < b1;
+    e_my[r] = act ? a.rowptr[i] : 0;
+    e_end[r] = act ? a.rowptr[i + 1] : 0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) F[r][q] = 0.0;
+    sig[r] = 0.0;
+  }
+  constexpr int PER = CHUNK / TPB;  // incidences per lane and chunk
   for (int32_t base = E0; base < E1; base += CHUNK) {
     const int32_t lim = base + CHUNK < E1 ? base + CHUNK : E1;
     // Phase 1 in three sweeps so that every lane has all of its loads in
@@ -408,7 +473,7 @@ __global__ void __launch_bounds__(256) k_body_coop(BodyArgs a) {
     bool ok[PER];
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-      const int32_t e = base + (int32_t)threadIdx.x + j * PCC_BLOCK;
+      const int32_t e = base + (int32_t)threadIdx.x + j * TPB;
       ok[j] = e < lim;
       cc[j] = 0;
       if (ok[j]) {
@@ -431,15 +496,15 @@ __global__ void __launch_bounds__(256) k_body_coop(BodyArgs a) {
     for (int j = 0; j < PER; ++j) {
       if (ok[j]) {
         yg[j] = reinterpret_cast<const double2*>(a.it)[cc[j]];
-        gold[j] = a.gold[base + (int32_t)threadIdx.x + j * PCC_BLOCK];
+        if (!FOLD) gold[j] = a.gold[base + (int32_t)threadIdx.x + j * TPB];
       }
     }
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
       if (!ok[j]) continue;
       const double gc = (k == 0) ? yg[j].x : proj_step(yg[j].x, malpha, yg[j].y);
-      const double dg = gc - gold[j];
-      const int q = (int)threadIdx.x + j * PCC_BLOCK;
+      const double dg = FOLD ? gc : gc - gold[j];
+      const int q = (int)threadIdx.x + j * TPB;
       if (PLANAR) {
         prod[0][q] = dg * co[j][0];
         prod[1][q] = dg * co[j][1];
@@ -452,40 +517,48 @@ __global__ void __launch_bounds__(256) k_body_coop(BodyArgs a) {
       }
     }
     __syncthreads();
-    for (; e_my < e_end && e_my < lim; ++e_my) {
-      const int q = e_my - base;
-      if (PLANAR) {
-        F[0] += prod[0][q];
-        F[1] += prod[1][q];
-        F[5] += prod[2][q];
-        sig += prod[3][q];
-      } else {
 #pragma unroll
-        for (int m = 0; m < 6; ++m) F[m] += prod[m][q];
-        sig += prod[6][q];
+    for (int r = 0; r < R; ++r) {
+      for (; e_my[r] < e_end[r] && e_my[r] < lim; ++e_my[r]) {
+        const int q = e_my[r] - base;
+        if (PLANAR) {
+          F[r][0] += prod[0][q];
+          F[r][1] += prod[1][q];
+          F[r][5] += prod[2][q];
+          sig[r] += prod[3][q];
+        } else {
+#pragma unroll
+          for (int m = 0; m < 6; ++m) F[r][m] += prod[m][q];
+          sig[r] += prod[6][q];
+        }
       }
     }
     __syncthreads();
   }
-  if (!active) return;
-  const double mt = a.mob[2 * i], mr = a.mob[2 * i + 1];
-  double u[6];
 #pragma unroll
-  for (int q = 0; q < 3; ++q) u[q] = (0.0 + mt * F[q]) + 0.0;
+  for (int r = 0; r < R; ++r) {
+    const int64_t i = b0 + threadIdx.x + r * TPB;
+    if (i >= b1) continue;
+    if (FOLD) unfold(a.fold, i, F[r], PLANAR);
+    const double mt = a.mob[2 * i], mr = a.mob[2 * i + 1];
+    double u[6];
 #pragma unroll
-  for (int q = 3; q < 6; ++q) u[q] = (0.0 + mr * F[q]) + 0.0;
-  const double imp = det_exp(sig * D.mlam);
-  const double ld = (a.len[i] * imp) * D.itau;
-  if (PLANAR) {
-    double2* o = reinterpret_cast<double2*>(a.ubuf + 4 * i);
-    o[0] = make_double2(u[0], u[1]);
-    o[1] = make_double2(u[5], ld - a.ldot_prev[i]);
-  } else {
-    double2* o = reinterpret_cast<double2*>(a.ubuf + 8 * i);
-    o[0] = make_double2(u[0], u[1]);
-    o[1] = make_double2(u[2], u[3]);
-    o[2] = make_double2(u[4], u[5]);
-    o[3] = make_double2(ld - a.ldot_prev[i], 0.0);
+    for (int q = 0; q < 3; ++q) u[q] = (0.0 + mt * F[r][q]) + 0.0;
+#pragma unroll
+    for (int q = 3; q < 6; ++q) u[q] = (0.0 + mr * F[r][q]) + 0.0;
+    const double imp = det_exp(sig[r] * D.mlam);
+    const double ld = (a.len[i] * imp) * D.itau;
+    if (PLANAR) {
+      double2* o = reinterpret_cast<double2*>(a.ubuf + 4 * i);
+      o[0] = make_double2(u[0], u[1]);
+      o[1] = make_double2(u[5], ld - a.ldot_prev[i]);
+    } else {
+      double2* o = reinterpret_cast<double2*>(a.ubuf + 8 * i);
+      o[0] = make_double2(u[0], u[1]);
+      o[1] = make_double2(u[2], u[3]);
+      o[2] = make_double2(u[4], u[5]);
+      o[3] = make_double2(ld - a.ldot_prev[i], 0.0);
+    }
   }
 }
 
@@ -955,6 +1028,28 @@ bool bodies_planar(pcc_ctx* ctx) {
 // Body-incidence CSR over the whole constraint set, each body's incidences in
 // ascending constraint order, plus the per-incidence coefficient records (3D
 // or planar, FP64 and FP32) and, planar, the compact constraint rows.
+// Mixed precision: D^T gamma_old per body for the phase (rowptr, and for
+// C > 0 the FP32 records and gamma_old per incidence, must be current).
+static void fold_gamma_old(pcc_ctx* ctx) {
+  const int64_t N = ctx->N;
+  if (ctx->params.precision != PCC_PREC_MIXED || N == 0) return;
+  ctx->fold.ensure(6 * N + 6);
+  hipStream_t s = ctx->stream;
+  if (ctx->planar)
+    k_fold<true><<<nblk(N), 256, 0, s>>>(N, ctx->rowptr.p, ctx->increc32.p, ctx->gold_e.p,
+                                         ctx->fold.p);
+  else
+    k_fold<false><<<nblk(N), 256, 0, s>>>(N, ctx->rowptr.p, ctx->increc32.p, ctx->gold_e.p,
+                                          ctx->fold.p);
+}
+
+// The mixed body kernels read 3 or 6 doubles of D^T gamma_old per body.
+static void check_fold(pcc_ctx* ctx) {
+  if (ctx->params.precision != PCC_PREC_MIXED) return;
+  if (ctx->N > 0 && (ctx->fold.p == nullptr || ctx->fold.cap < 6 * ctx->N))
+    throw StatusError{PCC_E_DEVICE, "internal: gamma_old sums missing for the mixed kernels"};
+}
+
 void build_incidence(pcc_ctx* ctx, bool records) {
   const int64_t N = ctx->N, C = ctx->S.C;
   hipStream_t s = ctx->stream;
@@ -966,7 +1061,10 @@ void build_incidence(pcc_ctx* ctx, bool records) {
   if (C > 0) k_inc_count<<<nblk(C), 256, 0, s>>>(C, ctx->S.I.p, ctx->S.J.p, ctx->inccur.p);
   exclusive_scan_i32(ctx, ctx->inccur.p, ctx->rowptr.p, N + 1);
   PCC_HIP(hipMemsetAsync(ctx->inccur.p, 0, sizeof(int32_t) * (size_t)(N + 1), s));
-  if (C == 0) return;
+  if (C == 0) {
+    if (records) fold_gamma_old(ctx);  // every body's sum is 0
+    return;
+  }
   k_inc_scatter<<<nblk(C), 256, 0, s>>>(C, ctx->S.I.p, ctx->S.J.p, ctx->rowptr.p, ctx->inccur.p,
                                         ctx->inc.p);
   seg_sort_i32(ctx, N, ctx->rowptr.p, ctx->inc.p);
@@ -1020,6 +1118,7 @@ void build_incidence(pcc_ctx* ctx, bool records) {
     k_inc_records<false><<<nblk(2 * C), 256, 0, s>>>(2 * C, ctx->inc.p, S.coef64.p, ctx->spos.p,
                                                      S.gamma.p, ctx->increc.p, ctx->increc32.p,
                                                      ctx->incidx.p, ctx->gold_e.p);
+  fold_gamma_old(ctx);
 }
 
 // Logical block layout of the BBPGD kernels over the active colonies:
@@ -1077,6 +1176,7 @@ static BodyArgs body_args(pcc_ctx* ctx) {
   a.rec = mixed ? (const void*)ctx->increc32.p : (const void*)ctx->increc.p;
   a.rc = ctx->incidx.p;
   a.gold = ctx->gold_e.p;
+  a.fold = ctx->fold.p;
   a.it = ctx->iter.p;
   a.mob = ctx->mob.p;
   a.len = b.len;
@@ -1120,11 +1220,14 @@ static ConstrArgs constr_args(pcc_ctx* ctx) {
 double body_bytes(int64_t N, int64_t C, bool mixed, bool planar) {
   const double sc = mixed ? 4 : 8;
   // per incidence: record (8 or 4 coefficient slots [+ 4-byte index]) +
-  // the iterate record's (gamma, g, gamma_old)
-  // per body: rowptr, mobility (2), length, ldot_prev, velocity record out
+  // the iterate record's (gamma, g) + gamma_old (FP64 only)
+  // per body: rowptr, mobility (2), length, ldot_prev, velocity record out,
+  // and in mixed precision D^T gamma_old (6 or 3 doubles)
   const double rec = planar ? 4 * sc + 4 : 8 * sc;
   const double ub = planar ? 32 : 64;
-  return 2.0 * C * (rec + 24) + N * (4 + 16 + 16 + ub);
+  const double gold_e = mixed ? 0 : 8;
+  const double fold = mixed ? (planar ? 24 : 48) : 0;
+  return 2.0 * C * (rec + 16 + gold_e) + N * (4 + 16 + 16 + ub + fold);
 }
 double constr_bytes(int64_t N, int64_t C, bool mixed, bool planar) {
   const double sc = mixed ? 4 : 8;
@@ -1135,36 +1238,38 @@ double constr_bytes(int64_t N, int64_t C, bool mixed, bool planar) {
   return C * (8 + row + 24 + 16) + N * ub;
 }
 
-// Body-kernel staging sizes the autotuner chooses from (incidences per pass):
-// planar 512 / 1024 / 2048, 3D 256 / 512 / 1024.
-static void launch_body(int variant, bool planar, bool mixed, unsigned grid, hipStream_t s,
-                        const BodyArgs& ba) {
-#define PCC_BODY(CT, PL, CH) k_body_coop<CT, PL, CH><<<grid, 256, 0, s>>>(ba)
-  if (planar) {
-    if (mixed) {
-      if (variant == 0) PCC_BODY(float, true, 512);
-      else if (variant == 1) PCC_BODY(float, true, 1024);
-      else PCC_BODY(float, true, 2048);
-    } else {
-      if (variant == 0) PCC_BODY(double, true, 512);
-      else if (variant == 1) PCC_BODY(double, true, 1024);
-      else PCC_BODY(double, true, 2048);
-    }
-  } else {
-    if (mixed) {
-      if (variant == 0) PCC_BODY(float, false, 256);
-      else if (variant == 1) PCC_BODY(float, false, 512);
-      else PCC_BODY(float, false, 1024);
-    } else {
-      if (variant == 0) PCC_BODY(double, false, 256);
-      else if (variant == 1) PCC_BODY(double, false, 512);
-      else PCC_BODY(double, false, 1024);
-    }
+// Body-kernel variants the autotuner chooses from: staging size (incidences
+// per pass: planar 512 / 1024 / 2048, 3D 256 / 512 / 1024) times threads per
+// block (variants 0-2: 256, one body each; 3-5: 128, two bodies each).
+template <class CT, bool PL, int C0>
+static void launch_body_t(int variant, unsigned grid, hipStream_t s, const BodyArgs& ba) {
+  switch (variant) {
+    case 0: k_body_coop<CT, PL, C0, 256><<<grid, 256, 0, s>>>(ba); break;
+    case 1: k_body_coop<CT, PL, 2 * C0, 256><<<grid, 256, 0, s>>>(ba); break;
+    case 2: k_body_coop<CT, PL, 4 * C0, 256><<<grid, 256, 0, s>>>(ba); break;
+    case 3: k_body_coop<CT, PL, C0, 128><<<grid, 128, 0, s>>>(ba); break;
+    case 4: k_body_coop<CT, PL, 2 * C0, 128><<<grid, 128, 0, s>>>(ba); break;
+    default: k_body_coop<CT, PL, 4 * C0, 128><<<grid, 128, 0, s>>>(ba); break;
   }
-#undef PCC_BODY
 }
 
-int body_chunk_of(int variant, bool planar) { return (planar ? 512 : 256) << variant; }
+static void launch_body(int variant, bool planar, bool mixed, unsigned grid, hipStream_t s,
+                        const BodyArgs& ba) {
+  if (planar) {
+    if (mixed)
+      launch_body_t<float, true, 512>(variant, grid, s, ba);
+    else
+      launch_body_t<double, true, 512>(variant, grid, s, ba);
+  } else {
+    if (mixed)
+      launch_body_t<float, false, 256>(variant, grid, s, ba);
+    else
+      launch_body_t<double, false, 256>(variant, grid, s, ba);
+  }
+}
+
+int body_chunk_of(int variant, bool planar) { return (planar ? 512 : 256) << (variant % 3); }
+int body_threads_of(int variant) { return variant < 3 ? 256 : 128; }
 
 static void ensure_state_host(pcc_ctx* ctx, int64_t slots) {
   if (ctx->state_host_cap >= slots) return;
@@ -1207,6 +1312,7 @@ void bbpgd_solve(pcc_ctx* ctx) {
                            hipMemcpyHostToDevice, s));
     PCC_HIP(hipStreamSynchronize(s));
   }
+  check_fold(ctx);
   BodyArgs ba = body_args(ctx);
   ConstrArgs ca = constr_args(ctx);
   const unsigned gb = (unsigned)ba.gp, gc = (unsigned)ca.gp;
@@ -1362,6 +1468,7 @@ void integrate_hard(pcc_ctx* ctx) {
   BodyArrays b = body_arrays(ctx->body.p, N);
   KTimer t(ctx, PCC_K_INTEGRATE, 0.0);
   if (N > 0) {
+    check_fold(ctx);
     BodyArgs ba = body_args(ctx);
     const FinalOut fo = {ctx->F.p, ctx->U.p, ctx->sig.p, ctx->imp.p, ctx->ldot_new.p};
     const unsigned g = nblk(N);

@@ -207,3 +207,31 @@ def test_autotuned_body_kernel_gives_the_same_bits(monkeypatch):
     assert runs[0][2]["tuned"] and not runs[1][2]["tuned"]
     assert runs[0][0] == runs[1][0]
     assert_same_state(runs[0][1], runs[1][1])
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_every_body_kernel_variant_gives_the_same_bits(precision, monkeypatch):
+    """Each staging size and threads-per-block variant (forced with
+    PCC_BODY_VARIANT) sums every body in the same order: planar and 3D steps
+    equal the default variant's bit for bit."""
+    from pcc.engine import BODY_VARIANTS, ParticleManager
+    for name, b, p in (
+            ("planar", G.dense_colony(6000, seed=161), hard(precision=precision)),
+            # a hard 3D packing, capped: failing solves must agree bit for bit too
+            ("3d", random_3d(300, 41), hard(precision=precision, max_bbpgd_iterations=3000,
+                                            max_recursive_iterations=4))):
+        runs = []
+        for v in range(BODY_VARIANTS):
+            monkeypatch.setenv("PCC_BODY_VARIANT", str(v))
+            pm = ParticleManager(p)
+            pm.set_state(b)
+            r = pm.step(raise_on_error=False)
+            t = pm.tuning()
+            runs.append((r, pm.get_state(), pm.constraints()))
+            pm.close()
+            print(f"  [{name} variant {v}] chunk={t['body_chunk']} threads={t['body_threads']} "
+                  f"its={r['bbpgd_iterations']}", flush=True)
+        for r, st, cons in runs[1:]:
+            assert r == runs[0][0]
+            assert_same_state(st, runs[0][1])
+            assert_same_constraints(cons, runs[0][2])
