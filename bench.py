@@ -288,11 +288,31 @@ def main_serving(args):
     ncell = sum(r.bodies.n for r in reqs)
     print(f"[bench] serving: {len(reqs)} requests over {args.duration:.0f} s, {ncell} cells, "
           f"{sum(r.params.mode == 1 for r in reqs)} soft", file=sys.stderr, flush=True)
-    srv = ColonyServer(int(args.hbm_budget_gb * 2**30), precision=PREC_MIXED)
+    srv = ColonyServer(int(args.hbm_budget_gb * 2**30), precision=PREC_MIXED,
+                       warm_steps=args.warm_steps)
     st = srv.run(reqs, time_limit=args.duration + args.serve_limit,
                  progress=lambda m: print(f"[serving] {m}", file=sys.stderr, flush=True))
     srv.close()
     waits = [r.admitted - r.arrival for r in srv.results.values()]
+    # per-step latency of completed requests, split into warm-up and steady steps
+    warm = [x for r in srv.results.values() for x in r.step_ms[:args.warm_steps]]
+    steady = [x for r in srv.results.values() for x in r.step_ms[args.warm_steps:]]
+    grown = [r.state.n / r.n_cells for r in srv.results.values() if r.n_cells]
+    # every recorded step (completed and still-resident requests) per model
+    per_mode = {0: [], 1: []}
+    for r in srv.results.values():
+        per_mode[r.mode].extend(r.step_ms)
+    for key, ms in srv.members.items():
+        for m in ms:
+            per_mode[key[0]].extend(m.step_ms)
+    modes = {}
+    for mode, name in ((0, "hard"), (1, "soft")):
+        x = per_mode[mode]
+        done = [r for r in srv.results.values() if r.mode == mode]
+        modes[name] = {"requests": sum(q.params.mode == mode for q in reqs),
+                       "completed": len(done), "steps_recorded": len(x),
+                       "p50_step_ms": float(np.percentile(x, 50)) if x else 0.0,
+                       "p99_step_ms": float(np.percentile(x, 99)) if x else 0.0}
     line = {
         "metric": "cell-steps/s, continuous-batching colony server; p50/p99 per-step latency",
         "value": st["cell_steps_per_s"],
@@ -309,9 +329,11 @@ def main_serving(args):
         "config": {"workload": (f"Poisson arrivals at {args.rate:g} requests/s for "
                                 f"{args.duration:g} s, colonies of 1k-100k cells (log-uniform), "
                                 f"hard or soft 50/50, growth rate U[0.5,1.5] ln2, "
-                                f"{args.serve_steps} steps each, HBM budget "
-                                f"{args.hbm_budget_gb:g} GiB, mixed precision"),
-                   "requests": len(reqs), "n_cells": ncell},
+                                f"{args.serve_steps} steps each (divide, then step), HBM "
+                                f"budget {args.hbm_budget_gb:g} GiB, mixed precision"),
+                   "requests": len(reqs), "n_cells": ncell, "warm_steps": args.warm_steps,
+                   "division": "device, seeded per request and step, exact halves, "
+                               "N(0, 0.01 rad) turns"},
         "p50_step_ms": st["p50_step_ms"],
         "p99_step_ms": st["p99_step_ms"],
         "serving": {k: st[k] for k in ("completed", "cell_steps", "wall_s", "batch_steps",
@@ -319,6 +341,14 @@ def main_serving(args):
                                        "budget_bytes")},
         "admission_wait_s": {"p50": float(np.percentile(waits, 50)) if waits else 0.0,
                              "max": float(max(waits)) if waits else 0.0},
+        "completed_step_ms": {
+            "warm_up": {"p50": float(np.percentile(warm, 50)) if warm else 0.0,
+                        "p99": float(np.percentile(warm, 99)) if warm else 0.0},
+            "steady": {"p50": float(np.percentile(steady, 50)) if steady else 0.0,
+                       "p99": float(np.percentile(steady, 99)) if steady else 0.0}},
+        "growth_factor": {"mean": float(np.mean(grown)) if grown else 0.0,
+                          "max": float(max(grown)) if grown else 0.0},
+        "per_model": modes,
     }
     print(json.dumps(line))
 
@@ -328,6 +358,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warm-steps", type=int, default=2,
+                    help="serving: steps a new colony runs in the warm-up batch")
     ap.add_argument("--n", type=int, default=300_000)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--instances", type=int, default=2)

@@ -1,12 +1,18 @@
 """Serving loop for colony-growth requests: continuous batching on one GPU.
 
 A request is a colony's initial state, its parameters (hard or soft model,
-growth rate, ...) and a number of timesteps.  The server keeps one batched
-context per model (`engine.ColonyBatch`).  Between batch steps it
+growth rate, ...) and a number of timesteps.  A timestep is the reference's
+time-loop body (Domain::run, Domain.cpp:121-143): divide the cells that have
+reached twice their rest length and renumber by x, on the device
+(pcc_batch_divide, seeded by the request id and the colony's own step count,
+so a colony's run does not depend on what it is batched with), then step.
+The server keeps one batched context per model (`engine.ColonyBatch`).
+Between batch steps it
 
   * admits waiting requests, first come first served, while the estimated
     device footprint of the resident colonies plus the newcomer stays within
-    an HBM budget: an admitted colony joins its model's batch and its state
+    an HBM budget (sized for the most cells the colony can reach by division
+    in its steps): an admitted colony joins its model's batch and its state
     stays resident in HBM until its last step;
   * retires colonies that have run all their steps: the final state is
     downloaded and the colony leaves the batch, freeing its share of the
@@ -15,11 +21,16 @@ context per model (`engine.ColonyBatch`).  Between batch steps it
 Colonies are also split by size into buckets, one batched context per
 (model, bucket): a batched step runs its colonies' ReLCP phases and BBPGD
 solves in lockstep, so the slowest colony of a batch sets that batch's step
-time, and bucketing keeps small colonies from waiting on large ones.  All
-batches are stepped from their own host threads, so their work overlaps on
-the device (each context owns its own HIP stream).  Every colony's result
-equals the result of a context that runs it alone: batched colonies share
-launches, never data.
+time, and bucketing keeps small colonies from waiting on large ones.  For the
+same reason a newly admitted colony first runs `warm_steps` steps in a warm-up
+batch of its (model, bucket): the first steps of a fresh colony relax its
+initial overlaps and can take orders of magnitude more BBPGD iterations than
+the steps after them, and only colonies in the same situation wait for them.
+It then moves, state unchanged, to the steady batch.  All batches are stepped
+from their own host threads, so their work overlaps on the device (each
+context owns its own HIP stream).  Every colony's result equals the result of
+a context that runs it alone: batched colonies share launches, never data,
+and a move between batches carries the full state.
 
 The reference runs one colony per MPI job (Domain::run, Domain.cpp:121-143);
 this loop has no counterpart there.
@@ -51,6 +62,29 @@ class Request:
     arrival: float = 0.0  # seconds after the start of the run
 
 
+def division_seed(rid: int, step: int) -> int:
+    """Seed of the divisions before a request's step `step` (the device
+    generator is keyed by seed, step argument 0 and parent index)."""
+    return ((int(rid) & 0xFFFFFFFF) << 32) | (int(step) & 0xFFFFFFFF)
+
+
+def max_cells(req: "Request", length_noise: float = 0.0) -> int:
+    """Bound on a colony's cell count over `steps` steps of growth with
+    division.  Each step multiplies a length by at most 1 + dt/tau (the
+    impedance exp(-lambda sigma) is <= 1 since stresses are >= 0) and
+    division conserves length, so the total length stays below
+    L0 (1 + dt/tau)^steps; every cell is at least as long as the shortest
+    initial cell or the shortest child (half of >= 2 l0, less the length
+    noise), so N <= total length / that minimum."""
+    b = req.bodies
+    if b.n == 0:
+        return 0
+    total = float(np.sum(b.length)) * (1.0 + req.params.dt / req.params.tau) ** req.steps
+    shortest = min(float(np.min(b.length)),
+                   float(np.min(b.l0)) * max(1.0 - 2.0 * length_noise, 0.05))
+    return max(b.n, int(np.ceil(total / shortest * (1.0 + 1e-9))) + 1)
+
+
 @dataclass
 class Result:
     rid: int
@@ -72,28 +106,37 @@ class _Member:
     done: int = 0
     step_ms: List[float] = field(default_factory=list)
     statuses: List[int] = field(default_factory=list)
+    state: Optional[T.Bodies] = None  # current state when moved between batches
 
 
 class ColonyServer:
-    """Continuous-batching server: one worker thread per (model, size bucket)
-    batch steps its colonies back to back; the caller's thread admits
+    """Continuous-batching server: one worker thread per (model, size bucket,
+    stage) batch steps its colonies back to back; the caller's thread admits
     arrivals first come first served against the HBM budget and hands them to
-    the worker of their batch, which adds them between two of its steps."""
+    the warm-up batch of their (model, bucket) (or straight to the steady
+    batch when warm_steps is 0), whose worker adds them between two of its
+    steps."""
 
     def __init__(self, hbm_budget_bytes: int, precision: int = T.PREC_MIXED, device: int = 0,
                  bytes_per_cell: Optional[float] = None, overlap: bool = True,
-                 bucket_edges: Sequence[int] = (4_000, 20_000)):
+                 bucket_edges: Sequence[int] = (4_000, 20_000), warm_steps: int = 2,
+                 divide: bool = True, length_noise: float = 0.0, angle_noise: float = 0.01):
         self.budget = int(hbm_budget_bytes)
         self.precision = precision
         self.device = device
         self.overlap = overlap  # False: one worker steps every batch in turn
         self.bytes_per_cell = float(bytes_per_cell or DEFAULT_BYTES_PER_CELL)
         self.bucket_edges = tuple(sorted(bucket_edges))
-        self.keys = [(m, b) for m in (T.MODE_HARD, T.MODE_SOFT)
-                     for b in range(len(self.bucket_edges) + 1)]
-        self.batches: Dict[Tuple[int, int], Optional[ColonyBatch]] = {k: None for k in self.keys}
-        self.members: Dict[Tuple[int, int], List[_Member]] = {k: [] for k in self.keys}
-        self.handoff: Dict[Tuple[int, int], List[_Member]] = {k: [] for k in self.keys}
+        self.warm_steps = int(warm_steps)
+        self.divide = bool(divide)  # False: cells grow but never divide
+        self.length_noise = float(length_noise)
+        self.angle_noise = float(angle_noise)
+        stages = (0, 1) if self.warm_steps > 0 else (1,)  # 0: warm-up, 1: steady
+        self.keys = [(m, b, g) for m in (T.MODE_HARD, T.MODE_SOFT)
+                     for b in range(len(self.bucket_edges) + 1) for g in stages]
+        self.batches: Dict[Tuple[int, int, int], Optional[ColonyBatch]] = {k: None for k in self.keys}
+        self.members: Dict[Tuple[int, int, int], List[_Member]] = {k: [] for k in self.keys}
+        self.handoff: Dict[Tuple[int, int, int], List[_Member]] = {k: [] for k in self.keys}
         self.queue: collections.deque = collections.deque()
         self.results: Dict[int, Result] = {}
         self.peak_reserved = 0
@@ -111,6 +154,11 @@ class ColonyServer:
     # -- admission (caller's thread) -------------------------------------
     def footprint(self, n_cells: int) -> int:
         return int(self.bytes_per_cell * max(n_cells, 1))
+
+    def reservation(self, req: Request) -> int:
+        """Device bytes reserved for a request from admission to retirement."""
+        return self.footprint(max_cells(req, self.length_noise) if self.divide
+                              else req.bodies.n)
 
     def reserved(self) -> int:
         return self._reserved
@@ -130,13 +178,14 @@ class ColonyServer:
         with self._lock:
             while self.queue:
                 req = self.queue[0]
-                need = self.footprint(req.bodies.n)
+                need = self.reservation(req)
                 if self._reserved + need > self.budget and self._reserved > 0:
                     break  # first come, first served: the head waits for memory
                 self.queue.popleft()
                 self._reserved += need
                 self.peak_reserved = max(self.peak_reserved, self._reserved)
-                key = (req.params.mode, self._bucket(req.bodies.n))
+                key = (req.params.mode, self._bucket(req.bodies.n),
+                       0 if self.warm_steps > 0 else 1)
                 self.handoff[key].append(_Member(req, now))
                 n += 1
             if n:
@@ -161,11 +210,16 @@ class ColonyServer:
         for m in new:
             if self.batches[key] is None:
                 self.batches[key] = ColonyBatch(m.req.params, device=self.device)
-            self.batches[key].add(m.req.bodies, m.req.params)
+            self.batches[key].add(m.state if m.state is not None else m.req.bodies,
+                                  m.req.params)
+            m.state = None
         if not ms:
             return False
         b = self.batches[key]
         t = time.perf_counter()
+        if self.divide:
+            b.divide([division_seed(m.req.rid, m.done) for m in ms], 0, self.length_noise,
+                     self.angle_noise)
         res = b.step(self.batch_steps)
         t1 = time.perf_counter()
         dt_ms = (t1 - t) * 1e3
@@ -174,16 +228,20 @@ class ColonyServer:
             m.done += 1
             m.step_ms.append(dt_ms)
             m.statuses.append(res[q]["status"])
-            cells += m.req.bodies.n
+            cells += b.sizes[q]
         dev = b.device_bytes()
-        done = []
+        gone = []  # (q, member, final state or None if moved), q descending
         for q in range(len(ms) - 1, -1, -1):
             m = ms[q]
             if m.done < m.req.steps:
+                if key[2] == 0 and m.done >= self.warm_steps:
+                    m.state = b.get_state(q)  # warm-up over: on to the steady batch
+                    b.remove(q)
+                    gone.append((q, m, None))
                 continue
             state = b.get_state(q)
             b.remove(q)
-            done.append((q, m, state))
+            gone.append((q, m, state))
         now = time.perf_counter() - self._t0
         with self._lock:
             self.batch_steps += 1
@@ -192,9 +250,12 @@ class ColonyServer:
             if self._first_step is None or t < self._first_step:
                 self._first_step = t
             self._last_step = t1 if self._last_step is None else max(self._last_step, t1)
-            for q, m, state in done:  # descending q
+            for q, m, state in gone:  # descending q
                 del ms[q]
-                self._reserved -= self.footprint(m.req.bodies.n)
+                if state is None:
+                    self.handoff[(key[0], key[1], 1)].append(m)
+                    continue
+                self._reserved -= self.reservation(m.req)
                 self.results[m.req.rid] = Result(
                     m.req.rid, key[0], m.req.bodies.n, m.req.steps, state, m.step_ms,
                     m.statuses, m.req.arrival, m.admitted, now)

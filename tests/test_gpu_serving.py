@@ -1,6 +1,8 @@
 """The continuous-batching server on the GPU: colonies join and leave the
-hard and the soft batch between steps while others are mid-run, and every
-result is bit-identical to a context that runs the same request alone."""
+hard and the soft batches between steps while others are mid-run, divide
+before every step, move from the warm-up to the steady batch, and every
+result is bit-identical to a context that runs the same request alone
+(divide with the request's seed, then step)."""
 import math
 
 import pytest
@@ -24,22 +26,24 @@ def request_params(mode, rate):
 
 def test_served_results_equal_alone():
     from pcc.engine import ParticleManager
-    from pcc.serving import ColonyServer, Request
+    from pcc.serving import ColonyServer, Request, division_seed
     spec = [(600, 0, 4, 0.0), (300, 1, 3, 0.0), (900, 0, 2, 0.0), (150, 1, 5, 0.0),
             (400, 0, 3, 0.0), (1200, 1, 2, 0.0), (250, 0, 6, 0.0)]
     reqs = []
     for rid, (n, mode, steps, arr) in enumerate(spec):
         rate = (0.5 + 0.15 * rid) * LN2
-        reqs.append(Request(rid, G.packed_colony(n, seed=90 + rid), request_params(mode, rate),
-                            steps, arr))
-    # room for ~1500 cells at a time: later requests wait and then join
+        b = G.packed_colony(n, seed=90 + rid)
+        b.length[b.length > 1.9] = 2.0  # these divide before the first step
+        reqs.append(Request(rid, b, request_params(mode, rate), steps, arr))
+    # room for ~2500 reserved cells at a time (a reservation covers the cells
+    # division can add): later requests wait and then join
     # batches whose other colonies are mid-run
-    srv = ColonyServer(1500 * 10_000, precision=T.PREC_MIXED, bytes_per_cell=10_000)
+    srv = ColonyServer(2500 * 10_000, precision=T.PREC_MIXED, bytes_per_cell=10_000)
     st = srv.run(reqs)
     srv.close()
     print(f"[serving] {st}", flush=True)
     assert st["completed"] == len(reqs)
-    assert st["peak_reserved_bytes"] <= 1500 * 10_000
+    assert st["peak_reserved_bytes"] <= 2500 * 10_000
     waited = [r for r in srv.results.values() if r.admitted > 0.0]
     assert waited, "no request had to wait for memory"
     for r in reqs:
@@ -48,7 +52,9 @@ def test_served_results_equal_alone():
         pm = ParticleManager(r.params)
         pm.set_state(r.bodies)
         for k in range(r.steps):
+            pm.divide(division_seed(r.rid, k), 0, 0.0, 0.01)
             pm.step(k)
+        assert res.state.n > r.bodies.n  # cells divided
         assert_same_state(res.state, pm.get_state())
         pm.close()
 

@@ -19,6 +19,7 @@ class FakeBatch:
         self.mode = params.mode
         self.cols = []
         self.events = []
+        self.divides = []
         FakeBatch.instances.append(self)
 
     def add(self, bodies, params):
@@ -30,8 +31,20 @@ class FakeBatch:
     def remove(self, q):
         del self.cols[q]
 
+    @property
+    def sizes(self):
+        return [c[0].n for c in self.cols]
+
+    def divide(self, seeds, step, length_noise=0.0, angle_noise=0.01):
+        # no cell reaches twice its rest length here; record the call
+        self.divides.append((list(seeds), step))
+        return self.sizes
+
     def step(self, i):
-        time.sleep(0.002)  # a device step takes time
+        # a device step takes time; a colony whose first length is exactly
+        # 100 is fresh and unrelaxed: its first step is slow
+        slow = any(c[0].length[0] == 100.0 for c in self.cols)
+        time.sleep(0.3 if slow else 0.002)
         for c in self.cols:
             c[0].length = c[0].length + 1.0
             c[1] += 1
@@ -65,7 +78,7 @@ def make_requests(spec):
 
 def test_every_request_runs_exactly_its_steps():
     reqs = make_requests([(100, 0, 3, 0.0), (200, 1, 5, 0.0), (150, 0, 2, 0.0), (80, 1, 4, 0.0)])
-    srv = serving.ColonyServer(10**12, bytes_per_cell=1000)
+    srv = serving.ColonyServer(10**12, bytes_per_cell=1000, warm_steps=0, divide=False)
     st = srv.run(reqs)
     assert st["completed"] == 4
     for r in reqs:
@@ -81,7 +94,8 @@ def test_every_request_runs_exactly_its_steps():
 def test_size_buckets_get_their_own_batches():
     reqs = make_requests([(100, 0, 2, 0.0), (5000, 0, 2, 0.0), (25000, 0, 2, 0.0),
                           (150, 0, 2, 0.0)])
-    srv = serving.ColonyServer(10**12, bytes_per_cell=1000, bucket_edges=(4000, 20000))
+    srv = serving.ColonyServer(10**12, bytes_per_cell=1000, bucket_edges=(4000, 20000),
+                               warm_steps=0, divide=False)
     st = srv.run(reqs)
     assert st["completed"] == 4
     assert len(FakeBatch.instances) == 3
@@ -91,7 +105,7 @@ def test_size_buckets_get_their_own_batches():
 def test_admission_respects_budget_and_order():
     # budget for 300 cells at 1000 B/cell: the requests must take turns
     reqs = make_requests([(200, 0, 3, 0.0), (100, 0, 5, 0.0), (250, 0, 2, 0.0), (50, 0, 1, 0.0)])
-    srv = serving.ColonyServer(300 * 1000, bytes_per_cell=1000)
+    srv = serving.ColonyServer(300 * 1000, bytes_per_cell=1000, divide=False)
     st = srv.run(reqs)
     assert st["completed"] == 4
     assert st["peak_reserved_bytes"] <= 300 * 1000
@@ -106,7 +120,7 @@ def test_admission_respects_budget_and_order():
 
 def test_late_arrival_joins_a_running_batch():
     reqs = make_requests([(100, 0, 40, 0.0), (100, 0, 3, 0.02)])
-    srv = serving.ColonyServer(10**12, bytes_per_cell=1000)
+    srv = serving.ColonyServer(10**12, bytes_per_cell=1000, warm_steps=0, divide=False)
     srv.run(reqs)
     adds = FakeBatch.instances[0].events  # both colonies fall in the same bucket
     # the second colony was added while the first had already stepped
@@ -116,7 +130,72 @@ def test_late_arrival_joins_a_running_batch():
 
 def test_oversized_request_runs_alone():
     reqs = make_requests([(500, 0, 2, 0.0), (100, 0, 2, 0.0)])
-    srv = serving.ColonyServer(100 * 1000, bytes_per_cell=1000)
+    srv = serving.ColonyServer(100 * 1000, bytes_per_cell=1000, divide=False)
     st = srv.run(reqs)
     assert st["completed"] == 2
     assert srv.results[1].admitted >= srv.results[0].finished
+
+
+def test_warm_up_moves_keep_state_and_step_count():
+    """Colonies run warm_steps steps in the warm-up batch, then move to the
+    steady batch of their (model, bucket) with their state; each still runs
+    exactly its steps (requests shorter than the warm-up never move)."""
+    spec = [(100 + 10 * i, i % 2, 1 + i, 0.001 * i) for i in range(7)]
+    reqs = make_requests(spec)
+    srv = serving.ColonyServer(10**12, bytes_per_cell=1000, warm_steps=3)
+    st = srv.run(reqs)
+    assert st["completed"] == len(reqs)
+    for r in reqs:
+        res = srv.results[r.rid]
+        assert len(res.step_ms) == r.steps
+        want = r.bodies.length.copy()
+        for _ in range(r.steps):  # the stand-in adds 1 per step
+            want = want + 1.0
+        assert np.array_equal(res.state.length, want)
+    # warm-up batches saw every colony, steady batches only those with > 3 steps
+    added = {}
+    for b in FakeBatch.instances:
+        added.setdefault(b.mode, []).append(len(b.events))
+    assert sum(sum(v) for v in added.values()) == len(reqs) + sum(r.steps > 3 for r in reqs)
+
+
+def slow_pair(warm_steps):
+    a = make_requests([(100, 0, 60, 0.0)])[0]
+    b = make_requests([(777, 0, 5, 0.0)])[0]
+    b.rid, b.arrival = 1, 0.03
+    b.bodies.length[0] = 100.0  # fresh: slow first step
+    srv = serving.ColonyServer(10**12, bytes_per_cell=1000, warm_steps=warm_steps)
+    srv.run([a, b])
+    return srv.results
+
+
+def test_warm_up_batch_shields_steady_colonies():
+    """A fresh colony's slow first step runs in the warm-up batch: the steady
+    colony beside it in the same (model, bucket) keeps its step time; without
+    the warm-up stage it waits for the slow step."""
+    r = slow_pair(2)
+    assert r[1].step_ms[0] >= 300
+    assert max(r[0].step_ms) < 100
+    r0 = slow_pair(0)
+    assert max(r0[0].step_ms) >= 300
+
+
+def test_division_before_each_step_seeded_per_colony():
+    """Every batch step divides first, each colony with the seed of its own
+    request id and step count (so a colony's run is the same alone and in
+    any mix); admission reserves memory for the cells division can add."""
+    reqs = make_requests([(100, 0, 6, 0.0), (120, 0, 4, 0.004), (90, 1, 5, 0.0)])
+    srv = serving.ColonyServer(10**12, bytes_per_cell=1000, warm_steps=2)
+    st = srv.run(reqs)
+    assert st["completed"] == 3
+    seen = {}
+    for b in FakeBatch.instances:
+        for seeds, step in b.divides:
+            assert step == 0
+            for sd in seeds:
+                rid, k = sd >> 32, sd & 0xFFFFFFFF
+                seen.setdefault(rid, []).append(k)
+    for r in reqs:
+        assert sorted(seen[r.rid]) == list(range(r.steps))
+        assert serving.max_cells(r) > r.bodies.n
+    assert st["peak_reserved_bytes"] >= 1000 * max(serving.max_cells(r) for r in reqs)
