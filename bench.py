@@ -90,26 +90,36 @@ def run_single(colonies, precision, steps, warmup):
             "tuning": tuning}
 
 
-def run_batched(colonies, precision, steps, warmup, timing=True, restore=True, report=0):
+def run_batched(colonies, precision, steps, warmup, timing=True, restore=True, report=0,
+                divide=False):
     """All colonies in one batched context; one latency per batch step
-    (a progress line every `report` steps when report > 0)."""
+    (a progress line every `report` steps when report > 0).  divide: every
+    step is the time-loop body, device division (seeded per colony and step)
+    then the step."""
     from pcc.engine import ColonyBatch
+    from pcc.serving import division_seed
     cb = ColonyBatch(params_for(precision), device=0)
     cb.set_state(colonies, [params_for(precision)] * len(colonies))
     if restore:
         cb.snapshot()
+
+    def one(i):
+        if divide:
+            cb.divide([division_seed(q, i) for q in range(len(colonies))], 0, 0.0, 0.01)
+        return cb.step(i)
     for i in range(warmup):
         if restore:
             cb.restore()
-        cb.step(i)
-    lat, results = [], []
+        one(i)
+    lat, results, cell_steps = [], [], 0
     t0 = time.perf_counter()
     for i in range(steps):
         a = time.perf_counter()
         if restore:
             cb.restore()
-        results.append(cb.step(i))
+        results.append(one(warmup + i))
         lat.append(time.perf_counter() - a)
+        cell_steps += cb.n
         if report and (i + 1) % report == 0:
             print(f"[bench] step {i + 1}/{steps}: {lat[-1] * 1e3:.1f} ms", file=sys.stderr,
                   flush=True)
@@ -126,9 +136,10 @@ def run_batched(colonies, precision, steps, warmup, timing=True, restore=True, r
         cb.step(0)
         tim = cb.timing()
         cb.set_timing(False)
+    cells_end = list(cb.sizes)
     cb.close()
     return {"total_s": total, "lat": lat, "results": results, "timing": tim, "phases": phases,
-            "tuning": tuning}
+            "tuning": tuning, "cell_steps": cell_steps, "cells_end": cells_end}
 
 
 PMC_SUMMARY = os.path.join(REPO, "profiles", "bbpgd_mixed_300k_colony_pmc.txt")
@@ -224,13 +235,14 @@ def main_ensemble(args):
     print(f"[bench] ensemble: {len(cols)} colonies, {ncell} cells, sizes "
           f"{min(c.n for c in cols)}..{max(c.n for c in cols)}", file=sys.stderr, flush=True)
     steps = args.steps if args.steps_given else 50
-    r = run_batched(cols, PREC_MIXED, steps, args.warmup, timing=True, restore=False, report=5)
+    r = run_batched(cols, PREC_MIXED, steps, args.warmup, timing=True, restore=False, report=5,
+                    divide=True)
     p50, p99 = latency_stats(r["lat"])
     its = [sum(x["bbpgd_iterations"] for x in step) for step in r["results"]]
     rf = roofline(r["timing"])
     line = {
         "metric": "cell-steps/s, batched ensemble of independent colonies; p50/p99 step latency",
-        "value": ncell * steps / r["total_s"],
+        "value": r["cell_steps"] / r["total_s"],
         "unit": "cell-steps/s",
         "n_gpus": 1,
         "steps": steps,
@@ -244,9 +256,12 @@ def main_ensemble(args):
         "config": {"workload": (f"{len(cols)} independent hard-model colony packings (disc "
                                 f"nematic domains, relaxed by the warm-up step), sizes "
                                 f"log-uniform in [500, 50000] ({ncell} cells), mixed precision, "
-                                f"{steps} consecutive steps in one batched context with captured "
-                                f"BBPGD graphs and per-colony convergence"),
-                   "n_cells": ncell, "colonies": len(cols)},
+                                f"{steps} consecutive time-loop steps (device division seeded "
+                                f"per colony and step, N(0, 0.01 rad) turns, then the step) in "
+                                f"one batched context with captured BBPGD graphs and "
+                                f"per-colony convergence"),
+                   "n_cells": ncell, "n_cells_end": int(sum(r["cells_end"])),
+                   "colonies": len(cols)},
         "p50_step_ms": p50,
         "p99_step_ms": p99,
         "bbpgd_iterations_per_step": {"min": int(min(its)), "max": int(max(its))},
