@@ -596,6 +596,10 @@ pcc_status pcc_create(const pcc_params* params, int device, pcc_ctx** out) {
   ctx->use_graphs = !(graph_env && graph_env[0] == '0');
   const char* tune_env = getenv("PCC_AUTOTUNE");
   ctx->autotune = !(tune_env && tune_env[0] == '0');
+  const char* persist_env = getenv("PCC_PERSIST");
+  ctx->use_persist = !(persist_env && persist_env[0] == '0');
+  if (const char* v = getenv("PCC_PERSIST_MAX_C")) ctx->persist_max_c = atoll(v);
+  if (const char* v = getenv("PCC_PERSIST_MAX_N")) ctx->persist_max_n = atoll(v);
   // PCC_BODY_VARIANT=v (0..PCC_BODY_VARIANTS-1) fixes the body-kernel variant
   if (const char* v = getenv("PCC_BODY_VARIANT")) {
     const int x = atoi(v);
@@ -1078,7 +1082,7 @@ pcc_status pcc_device_bytes(pcc_ctx* ctx, int64_t* bytes) {
   add(ctx->bstart); add(ctx->bcursor); add(ctx->dcount); add(ctx->rI_); add(ctx->rJ_);
   add(ctx->rrec); add(ctx->ridx); add(ctx->ridx2); add(ctx->rkey); add(ctx->rkey2);
   add(ctx->sort_tmp); add_set(ctx->S); add_set(ctx->S2); add(ctx->rowptr); add(ctx->inccur);
-  add(ctx->inc); add(ctx->increc); add(ctx->increc32); add(ctx->incidx); add(ctx->gold_e); add(ctx->fold);
+  add(ctx->inc); add(ctx->increc); add(ctx->increc32); add(ctx->incidx); add(ctx->gold_e); add(ctx->fold); add(ctx->plist);
   add(ctx->sord); add(ctx->spos); add(ctx->skey); add(ctx->skey2); add(ctx->sval); add(ctx->sI);
   add(ctx->sJ); add(ctx->sphi); add(ctx->prow); add(ctx->prow32); add(ctx->flag);
   add(ctx->iter); add(ctx->gprev); add(ctx->ubuf); add(ctx->mob); add(ctx->partials);

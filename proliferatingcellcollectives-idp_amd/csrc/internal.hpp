@@ -197,6 +197,13 @@ struct pcc_ctx {
   bool planar = false;        // current step uses the 2D monolayer kernels
   bool allow_planar = true;   // PCC_PLANAR=0 in the environment forces the 3D kernels
   bool use_graphs = true;     // BBPGD chunks as captured HIP graphs (PCC_GRAPH=0: plain launches)
+  // one-launch BBPGD when every active colony has at most persist_max_c
+  // constraints and persist_max_n bodies (PCC_PERSIST=0: off); measured
+  // faster than the multi-kernel path up to ~2000 constraints, slower from
+  // ~7000 (one workgroup per colony)
+  bool use_persist = true;
+  int64_t persist_max_c = 4096;
+  int64_t persist_max_n = 1024;
   // autotuner (PCC_AUTOTUNE=0: off): body-kernel staging size
   bool autotune = true;
   bool body_tuned = false;
@@ -235,6 +242,7 @@ struct pcc_ctx {
   pcc::DArr<int32_t> incidx;               // planar: solver position per incidence
   pcc::DArr<double> gold_e;                // gamma_old per incidence
   pcc::DArr<double> fold;                  // mixed: D^T gamma_old per body (6, planar 3)
+  pcc::DArr<int32_t> plist;                // colonies of a persistent BBPGD launch
   pcc::DArr<int32_t> sord, spos;           // solver order and its inverse
   pcc::DArr<int32_t> skey, skey2, sval;    // sort scratch
   pcc::DArr<int32_t> sI, sJ;               // solver order

@@ -897,6 +897,250 @@ __global__ void __launch_bounds__(256) k_finish(ConstrArgs a) {
   if (k + 1 == D.maxit) st->failed = 1;
 }
 
+// Persistent BBPGD for colonies small enough for one workgroup: a block of
+// kPersistThreads threads runs one colony's whole solve (BBPGD.cpp:32-127)
+// in a single launch, with no launch and no host round trip per iteration.
+// It evaluates exactly the sums of k_body_coop, k_constr and k_finish, in
+// their order: every body sum in ascending incidence order; the BB dots and
+// the residual of each logical block of 256 lanes through a 64-lane
+// butterfly and ((w0 + w1) + w2) + w3 (one group of four waves per logical
+// block); the G block partials strided over 256 threads and reduced the same
+// way.  So it is bit-identical to the multi-kernel path.  The loop ends at
+// convergence or at max_bbpgd_iterations.
+constexpr int kPersistThreads = 1024;
+constexpr int kPersistGroups = kPersistThreads / PCC_BLOCK;
+
+template <class CT, bool PLANAR>
+__global__ void __launch_bounds__(kPersistThreads)
+    k_bbpgd_persist(BodyArgs ba, ConstrArgs ca, const int32_t* __restrict__ plist) {
+  constexpr bool FOLD = sizeof(CT) == 4;
+  constexpr int UW = PLANAR ? 4 : 8;
+  __shared__ double red[kPersistGroups][4][4];  // group, quantity, wave
+  __shared__ double s_alpha;
+  __shared__ int64_t s_iter;
+  __shared__ int32_t s_done;
+  const int32_t q = plist[blockIdx.x];
+  SolverState* st = ca.st + q;
+  const InstDev& D = ca.idev[q];
+  const int tid = threadIdx.x;
+  const int grp = tid / PCC_BLOCK, lt = tid % PCC_BLOCK;
+  const int wv = lt / PCC_WAVE, lane = lt % PCC_WAVE;
+  if (tid == 0) {
+    s_alpha = st->alpha;
+    s_iter = st->iter;
+    s_done = (st->skip | st->converged | st->failed) ? 1 : 0;
+  }
+  __syncthreads();
+  const int64_t bend = D.boff + D.N, cend = D.coff + D.C;
+  const int64_t T = ca.Gtot, o = D.lb, G = D.G, K = D.K;
+  const double dt = D.dt;
+  while (!s_done) {
+    const int64_t k = s_iter;
+    const double malpha = -s_alpha;
+    // ---- body apply (k_body_coop)
+    for (int64_t i = D.boff + tid; i < bend; i += kPersistThreads) {
+      double F[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      double sig = 0.0;
+      const int32_t e1 = ba.rowptr[i + 1];
+      for (int32_t e = ba.rowptr[i]; e < e1; ++e) {
+        double co[7];
+        int64_t c;
+        if (PLANAR) {
+          double p[4];
+          load_prec<CT>(ba.rec, e, p);
+          co[0] = p[0];
+          co[1] = p[1];
+          co[5] = p[2];
+          co[6] = p[3];
+          c = ba.rc[e];
+        } else {
+          load_rec<CT>(ba.rec, e, co, c);
+        }
+        const double2 yg = reinterpret_cast<const double2*>(ba.it)[c];
+        const double gc = (k == 0) ? yg.x : proj_step(yg.x, malpha, yg.y);
+        const double dg = FOLD ? gc : gc - ba.gold[e];
+        if (PLANAR) {
+          F[0] += dg * co[0];
+          F[1] += dg * co[1];
+          F[5] += dg * co[5];
+        } else {
+#pragma unroll
+          for (int m = 0; m < 6; ++m) F[m] += dg * co[m];
+        }
+        sig += gc * co[6];
+      }
+      if (FOLD) unfold(ba.fold, i, F, PLANAR);
+      const double mt = ba.mob[2 * i], mr = ba.mob[2 * i + 1];
+      double u[6];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) u[m] = (0.0 + mt * F[m]) + 0.0;
+#pragma unroll
+      for (int m = 3; m < 6; ++m) u[m] = (0.0 + mr * F[m]) + 0.0;
+      const double imp = det_exp(sig * D.mlam);
+      const double ld = (ba.len[i] * imp) * D.itau;
+      if (PLANAR) {
+        double2* w = reinterpret_cast<double2*>(ba.ubuf + 4 * i);
+        w[0] = make_double2(u[0], u[1]);
+        w[1] = make_double2(u[5], ld - ba.ldot_prev[i]);
+      } else {
+        double2* w = reinterpret_cast<double2*>(ba.ubuf + 8 * i);
+        w[0] = make_double2(u[0], u[1]);
+        w[1] = make_double2(u[2], u[3]);
+        w[2] = make_double2(u[4], u[5]);
+        w[3] = make_double2(ld - ba.ldot_prev[i], 0.0);
+      }
+    }
+    __syncthreads();
+    // ---- constraint apply (k_constr): logical block b is taken by group b % 4
+    const bool save_prev = k > 0 && k % D.check == 0;
+    for (int64_t b0 = 0; b0 < G; b0 += kPersistGroups) {
+      const int64_t b = b0 + grp;
+      double v0 = 0.0, v1 = 0.0, v2 = 0.0, vr = 0.0;
+      if (b < G) {
+        for (int64_t j = 0; j < K; ++j) {
+          const int64_t c = D.coff + (b * K + j) * PCC_BLOCK + lt;
+          if (c >= cend) break;
+          const int32_t I = ca.I[c], J = ca.J[c];
+          const double2 yg = reinterpret_cast<const double2*>(ca.it)[c];
+          const double phi = ca.phi[c];
+          const Row<CT, PLANAR> row = load_row<CT, PLANAR>(ca.coef, c);
+          const double2* UI = reinterpret_cast<const double2*>(ca.ubuf + UW * (int64_t)I);
+          const double2* UJ = reinterpret_cast<const double2*>(ca.ubuf + UW * (int64_t)J);
+          double2 ui[UW / 2], uj[UW / 2];
+#pragma unroll
+          for (int m = 0; m < UW / 2; ++m) {
+            ui[m] = UI[m];
+            uj[m] = UJ[m];
+          }
+          const double gc = (k == 0) ? yg.x : proj_step(yg.x, malpha, yg.y);
+          double pd = 0.0;
+          double pg = 0.0;
+          const CT* co = row.v;
+          if (PLANAR) {
+            pd += (double)co[0] * ui[0].x;
+            pd += (double)co[1] * ui[0].y;
+            pd += (double)co[2] * ui[1].x;
+            pd += (-(double)co[0]) * uj[0].x;
+            pd += (-(double)co[1]) * uj[0].y;
+            pd += (double)co[3] * uj[1].x;
+            pg += (double)co[4] * ui[1].y;
+            pg += (double)co[5] * uj[1].y;
+          } else {
+            pd += (double)co[0] * ui[0].x;
+            pd += (double)co[1] * ui[0].y;
+            pd += (double)co[2] * ui[1].x;
+            pd += (double)co[3] * ui[1].y;
+            pd += (double)co[4] * ui[2].x;
+            pd += (double)co[5] * ui[2].y;
+            pd += (-(double)co[0]) * uj[0].x;
+            pd += (-(double)co[1]) * uj[0].y;
+            pd += (-(double)co[2]) * uj[1].x;
+            pd += (double)co[6] * uj[1].y;
+            pd += (double)co[7] * uj[2].x;
+            pd += (double)co[8] * uj[2].y;
+            pg += (double)co[9] * ui[3].x;
+            pg += (double)co[10] * uj[3].x;
+          }
+          const double g = phi + (dt * pd + (-dt) * pg);
+          if (save_prev) ca.gprev[c] = yg.x;
+          reinterpret_cast<double2*>(ca.it)[c] = make_double2(gc, g);
+          if (k > 0) {
+            const double dgam = gc - yg.x;
+            const double dphi = g - yg.y;
+            v0 += dgam * dgam;
+            v1 += dgam * dphi;
+            v2 += dphi * dphi;
+          }
+          const double pj = gc > 0 ? g : (g < 0.0 ? g : 0.0);
+          const double ar = fabs(pj);
+          vr = vr < ar ? ar : vr;
+        }
+      }
+      const double w0 = wave_sum(v0), w1 = wave_sum(v1), w2 = wave_sum(v2);
+      const double wm = wave_max(vr);
+      if (lane == 0) {
+        red[grp][0][wv] = w0;
+        red[grp][1][wv] = w1;
+        red[grp][2][wv] = w2;
+        red[grp][3][wv] = wm;
+      }
+      __syncthreads();
+      if (lt == 0 && b < G) {
+        const double(*r)[4] = red[grp];
+        ca.partials[o + b] = ((r[0][0] + r[0][1]) + r[0][2]) + r[0][3];
+        ca.partials[T + o + b] = ((r[1][0] + r[1][1]) + r[1][2]) + r[1][3];
+        ca.partials[2 * T + o + b] = ((r[2][0] + r[2][1]) + r[2][2]) + r[2][3];
+        double mx = r[3][0];
+        for (int m = 1; m < 4; ++m) mx = mx < r[3][m] ? r[3][m] : mx;
+        ca.partials[3 * T + o + b] = mx;
+      }
+      __syncthreads();
+    }
+    // ---- finish (k_finish), on the first group
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, am = 0.0;
+    if (grp == 0) {
+      for (int64_t i = lt; i < G; i += PCC_BLOCK) {
+        a0 += ca.partials[o + i];
+        a1 += ca.partials[T + o + i];
+        a2 += ca.partials[2 * T + o + i];
+        const double m = ca.partials[3 * T + o + i];
+        am = am < m ? m : am;
+      }
+    }
+    const double f0 = wave_sum(a0), f1 = wave_sum(a1), f2 = wave_sum(a2);
+    const double fm = wave_max(am);
+    if (grp == 0 && lane == 0) {
+      red[0][0][wv] = f0;
+      red[0][1][wv] = f1;
+      red[0][2][wv] = f2;
+      red[0][3][wv] = fm;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      const double(*r)[4] = red[0];
+      const double S0 = ((r[0][0] + r[0][1]) + r[0][2]) + r[0][3];
+      const double S1 = ((r[1][0] + r[1][1]) + r[1][2]) + r[1][3];
+      const double S2 = ((r[2][0] + r[2][1]) + r[2][2]) + r[2][3];
+      double R = r[3][0];
+      for (int m = 1; m < 4; ++m) R = R < r[3][m] ? r[3][m] : R;
+      if (k == 0) {  // initial gradient: res = residual(g, Gamma), alpha = 1/res
+        st->res = R;
+        st->alpha = 1.0 / R;
+        st->iter = 1;
+        if (1 == D.maxit) st->failed = 1;
+      } else {
+        bool stop = false;
+        if (k % D.check == 0) {
+          st->res = R;
+          if (R <= D.tol) {
+            st->converged = 1;
+            stop = true;
+          }
+        }
+        if (!stop) {
+          double num, den;
+          if (k % 2 == 0) {  // BB1: (dg.dg)/(dg.dphi)
+            num = S0;
+            den = S1;
+          } else {  // BB2: (dphi.dg)/(dphi.dphi)
+            den = S2;
+            num = S1;
+          }
+          const double eps10 = 10 * 2.220446049250313080847e-16;
+          if (fabs(den) < eps10) den += eps10;
+          st->alpha = num / den;
+          st->iter = k + 1;
+          if (k + 1 == D.maxit) st->failed = 1;
+        }
+      }
+      s_alpha = st->alpha;
+      s_iter = st->iter;
+      s_done = (st->converged | st->failed) ? 1 : 0;
+    }
+    __syncthreads();
+  }
+}
+
 // dC = G(q) U (Physics.cpp:220-265), Euler step and renormalisation
 // (Particle.cpp:37-57,110-120), write-back of ldot/impedance/stress
 // (ParticleManager.cpp:89-130).
@@ -1282,6 +1526,31 @@ static void ensure_state_host(pcc_ctx* ctx, int64_t slots) {
   ctx->state_host_cap = slots;
 }
 
+// Final solver state of every active colony (BBPGDResult, BBPGD.h:13-16);
+// with `account`, the algorithmic bytes of the launches in which each
+// colony's blocks worked: iterations 0..iter (the converging one included),
+// or 0..iter-1 on failure.
+static void collect_solve_results(pcc_ctx* ctx, bool account) {
+  const int64_t B = (int64_t)ctx->inst.size();
+  const bool mixed = ctx->params.precision == PCC_PREC_MIXED;
+  hipStream_t s = ctx->stream;
+  std::vector<SolverState> fin(B);
+  PCC_HIP(hipMemcpyAsync(fin.data(), ctx->state.p, sizeof(SolverState) * B, hipMemcpyDeviceToHost, s));
+  PCC_HIP(hipStreamSynchronize(s));
+  for (int64_t q = 0; q < B; ++q) {
+    Instance& I = ctx->inst[q];
+    if (!(I.flags & kInstActive)) continue;
+    I.solve_its = fin[q].iter;
+    I.solve_res = fin[q].res;
+    I.solve_failed = fin[q].failed != 0;
+    if (!account) continue;
+    const double worked = (double)(fin[q].iter + (I.solve_failed ? 0 : 1));
+    add_timing_bytes(ctx, PCC_K_BODY_APPLY, worked * body_bytes(I.N, I.C, mixed, ctx->planar));
+    add_timing_bytes(ctx, PCC_K_CONSTR_APPLY,
+                     worked * constr_bytes(I.N, I.C, mixed, ctx->planar));
+  }
+}
+
 // Runs BBPGD for every active colony at once on the current phase operators
 // (BBPGD.cpp:32-127).  Each colony has its own step size, iteration count and
 // convergence flag on the device; a converged colony's blocks exit at once
@@ -1316,6 +1585,40 @@ void bbpgd_solve(pcc_ctx* ctx) {
   BodyArgs ba = body_args(ctx);
   ConstrArgs ca = constr_args(ctx);
   const unsigned gb = (unsigned)ba.gp, gc = (unsigned)ca.gp;
+  // Persistent path: when every active colony fits one workgroup, one launch
+  // runs all their solves (a block per colony) to the end.
+  {
+    std::vector<int32_t> plist;
+    bool persist = ctx->use_persist && !ctx->trace_on && !ctx->timing.enabled;
+    for (int64_t q = 0; q < B && persist; ++q) {
+      const Instance& I = ctx->inst[q];
+      if (!(I.flags & kInstActive)) continue;
+      if (I.C > ctx->persist_max_c || I.N > ctx->persist_max_n)
+        persist = false;
+      else
+        plist.push_back((int32_t)q);
+    }
+    if (persist && !plist.empty()) {
+      const int64_t np = (int64_t)plist.size();
+      ctx->plist.ensure(np);
+      PCC_HIP(hipMemcpyAsync(ctx->plist.p, plist.data(), sizeof(int32_t) * np,
+                             hipMemcpyHostToDevice, s));
+      const unsigned g = (unsigned)np;
+      if (planar) {
+        if (mixed)
+          k_bbpgd_persist<float, true><<<g, kPersistThreads, 0, s>>>(ba, ca, ctx->plist.p);
+        else
+          k_bbpgd_persist<double, true><<<g, kPersistThreads, 0, s>>>(ba, ca, ctx->plist.p);
+      } else {
+        if (mixed)
+          k_bbpgd_persist<float, false><<<g, kPersistThreads, 0, s>>>(ba, ca, ctx->plist.p);
+        else
+          k_bbpgd_persist<double, false><<<g, kPersistThreads, 0, s>>>(ba, ca, ctx->plist.p);
+      }
+      collect_solve_results(ctx, false);
+      return;
+    }
+  }
   auto body = [&](int variant) { launch_body(variant, planar, mixed, gb, s, ba); };
   auto one_iteration = [&]() {
     {
@@ -1442,21 +1745,7 @@ void bbpgd_solve(pcc_ctx* ctx) {
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
   }
-  std::vector<SolverState> fin(B);
-  PCC_HIP(hipMemcpyAsync(fin.data(), ctx->state.p, sizeof(SolverState) * B, hipMemcpyDeviceToHost, s));
-  PCC_HIP(hipStreamSynchronize(s));
-  for (int64_t q = 0; q < B; ++q) {
-    Instance& I = ctx->inst[q];
-    if (!(I.flags & kInstActive)) continue;
-    I.solve_its = fin[q].iter;
-    I.solve_res = fin[q].res;
-    I.solve_failed = fin[q].failed != 0;
-    // algorithmic bytes of the launches in which this colony's blocks worked:
-    // iterations 0..iter (the converging one included), or 0..iter-1 on failure
-    const double worked = (double)(fin[q].iter + (I.solve_failed ? 0 : 1));
-    add_timing_bytes(ctx, PCC_K_BODY_APPLY, worked * body_bytes(I.N, I.C, mixed, planar));
-    add_timing_bytes(ctx, PCC_K_CONSTR_APPLY, worked * constr_bytes(I.N, I.C, mixed, planar));
-  }
+  collect_solve_results(ctx, true);
 }
 
 // Workspace of the converged iterate, move, write-back, constraint update for
