@@ -901,21 +901,34 @@ __global__ void __launch_bounds__(256) k_finish(ConstrArgs a) {
 // kPersistThreads threads runs one colony's whole solve (BBPGD.cpp:32-127)
 // in a single launch, with no launch and no host round trip per iteration.
 // It evaluates exactly the sums of k_body_coop, k_constr and k_finish, in
-// their order: every body sum in ascending incidence order; the BB dots and
-// the residual of each logical block of 256 lanes through a 64-lane
-// butterfly and ((w0 + w1) + w2) + w3 (one group of four waves per logical
-// block); the G block partials strided over 256 threads and reduced the same
-// way.  So it is bit-identical to the multi-kernel path.  The loop ends at
-// convergence or at max_bbpgd_iterations.
+// their order:
+//   * body apply as k_body_coop, over groups of kPersistThreads bodies: the
+//     group's incidences staged through LDS in chunks (all loads of a chunk
+//     in flight), each thread adding its own body's products in ascending
+//     incidence order;
+//   * constraint apply as k_constr: the lanes of each logical block of 256
+//     (taken by one group of four waves) reduced by a 64-lane butterfly per
+//     wave, kept in LDS, and combined as ((w0 + w1) + w2) + w3;
+//   * the block partials strided over 256 threads and reduced as k_finish
+//     does.
+// So it is bit-identical to the multi-kernel path.  The loop ends at
+// convergence or at max_bbpgd_iterations.  Requires G <= kPersistMaxTiles.
 constexpr int kPersistThreads = 1024;
 constexpr int kPersistGroups = kPersistThreads / PCC_BLOCK;
+constexpr int kPersistMaxTiles = 32;
+constexpr int kPersistChunk = 1024;  // incidences staged per pass
 
 template <class CT, bool PLANAR>
 __global__ void __launch_bounds__(kPersistThreads)
     k_bbpgd_persist(BodyArgs ba, ConstrArgs ca, const int32_t* __restrict__ plist) {
   constexpr bool FOLD = sizeof(CT) == 4;
   constexpr int UW = PLANAR ? 4 : 8;
-  __shared__ double red[kPersistGroups][4][4];  // group, quantity, wave
+  constexpr int NC = PLANAR ? 4 : 7;  // products per incidence
+  constexpr int CHUNK = kPersistChunk;
+  constexpr int PER = CHUNK / kPersistThreads;
+  __shared__ double prod[NC][CHUNK];
+  __shared__ double red[kPersistMaxTiles][4][4];  // logical block, quantity, wave
+  __shared__ double fin[4][4];                    // quantity, wave
   __shared__ double s_alpha;
   __shared__ int64_t s_iter;
   __shared__ int32_t s_done;
@@ -932,177 +945,214 @@ __global__ void __launch_bounds__(kPersistThreads)
   }
   __syncthreads();
   const int64_t bend = D.boff + D.N, cend = D.coff + D.C;
-  const int64_t T = ca.Gtot, o = D.lb, G = D.G, K = D.K;
+  const int64_t G = D.G, K = D.K;
   const double dt = D.dt;
   while (!s_done) {
     const int64_t k = s_iter;
     const double malpha = -s_alpha;
-    // ---- body apply (k_body_coop)
-    for (int64_t i = D.boff + tid; i < bend; i += kPersistThreads) {
+    // ---- body apply (k_body_coop), a group of kPersistThreads bodies at a time
+    for (int64_t g0 = D.boff; g0 < bend; g0 += kPersistThreads) {
+      const int64_t g1 = g0 + kPersistThreads < bend ? g0 + kPersistThreads : bend;
+      const int64_t i = g0 + tid;
+      const bool active = i < g1;
+      const int32_t E0 = ba.rowptr[g0], E1 = ba.rowptr[g1];
+      int32_t e_my = active ? ba.rowptr[i] : 0;
+      const int32_t e_end = active ? ba.rowptr[i + 1] : 0;
       double F[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
       double sig = 0.0;
-      const int32_t e1 = ba.rowptr[i + 1];
-      for (int32_t e = ba.rowptr[i]; e < e1; ++e) {
-        double co[7];
-        int64_t c;
-        if (PLANAR) {
-          double p[4];
-          load_prec<CT>(ba.rec, e, p);
-          co[0] = p[0];
-          co[1] = p[1];
-          co[5] = p[2];
-          co[6] = p[3];
-          c = ba.rc[e];
-        } else {
-          load_rec<CT>(ba.rec, e, co, c);
-        }
-        const double2 yg = reinterpret_cast<const double2*>(ba.it)[c];
-        const double gc = (k == 0) ? yg.x : proj_step(yg.x, malpha, yg.y);
-        const double dg = FOLD ? gc : gc - ba.gold[e];
-        if (PLANAR) {
-          F[0] += dg * co[0];
-          F[1] += dg * co[1];
-          F[5] += dg * co[5];
-        } else {
+      for (int32_t base = E0; base < E1; base += CHUNK) {
+        const int32_t lim = base + CHUNK < E1 ? base + CHUNK : E1;
+        double co[PER][7];
+        int64_t cc[PER];
+        bool ok[PER];
 #pragma unroll
-          for (int m = 0; m < 6; ++m) F[m] += dg * co[m];
+        for (int j = 0; j < PER; ++j) {
+          const int32_t e = base + tid + j * kPersistThreads;
+          ok[j] = e < lim;
+          cc[j] = 0;
+          if (ok[j]) {
+            if (PLANAR) {
+              double p[4];
+              load_prec<CT>(ba.rec, e, p);
+              co[j][0] = p[0];
+              co[j][1] = p[1];
+              co[j][5] = p[2];
+              co[j][6] = p[3];
+              cc[j] = ba.rc[e];
+            } else {
+              load_rec<CT>(ba.rec, e, co[j], cc[j]);
+            }
+          }
         }
-        sig += gc * co[6];
+        double2 yg[PER];
+        double gold[PER];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+          if (ok[j]) {
+            yg[j] = reinterpret_cast<const double2*>(ba.it)[cc[j]];
+            if (!FOLD) gold[j] = ba.gold[base + tid + j * kPersistThreads];
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+          if (!ok[j]) continue;
+          const double gc = (k == 0) ? yg[j].x : proj_step(yg[j].x, malpha, yg[j].y);
+          const double dg = FOLD ? gc : gc - gold[j];
+          const int qq = tid + j * kPersistThreads;
+          if (PLANAR) {
+            prod[0][qq] = dg * co[j][0];
+            prod[1][qq] = dg * co[j][1];
+            prod[2][qq] = dg * co[j][5];
+            prod[3][qq] = gc * co[j][6];
+          } else {
+#pragma unroll
+            for (int m = 0; m < 6; ++m) prod[m][qq] = dg * co[j][m];
+            prod[6][qq] = gc * co[j][6];
+          }
+        }
+        __syncthreads();
+        for (; e_my < e_end && e_my < lim; ++e_my) {
+          const int qq = e_my - base;
+          if (PLANAR) {
+            F[0] += prod[0][qq];
+            F[1] += prod[1][qq];
+            F[5] += prod[2][qq];
+            sig += prod[3][qq];
+          } else {
+#pragma unroll
+            for (int m = 0; m < 6; ++m) F[m] += prod[m][qq];
+            sig += prod[6][qq];
+          }
+        }
+        __syncthreads();
       }
-      if (FOLD) unfold(ba.fold, i, F, PLANAR);
-      const double mt = ba.mob[2 * i], mr = ba.mob[2 * i + 1];
-      double u[6];
+      if (active) {
+        if (FOLD) unfold(ba.fold, i, F, PLANAR);
+        const double mt = ba.mob[2 * i], mr = ba.mob[2 * i + 1];
+        double u[6];
 #pragma unroll
-      for (int m = 0; m < 3; ++m) u[m] = (0.0 + mt * F[m]) + 0.0;
+        for (int m = 0; m < 3; ++m) u[m] = (0.0 + mt * F[m]) + 0.0;
 #pragma unroll
-      for (int m = 3; m < 6; ++m) u[m] = (0.0 + mr * F[m]) + 0.0;
-      const double imp = det_exp(sig * D.mlam);
-      const double ld = (ba.len[i] * imp) * D.itau;
-      if (PLANAR) {
-        double2* w = reinterpret_cast<double2*>(ba.ubuf + 4 * i);
-        w[0] = make_double2(u[0], u[1]);
-        w[1] = make_double2(u[5], ld - ba.ldot_prev[i]);
-      } else {
-        double2* w = reinterpret_cast<double2*>(ba.ubuf + 8 * i);
-        w[0] = make_double2(u[0], u[1]);
-        w[1] = make_double2(u[2], u[3]);
-        w[2] = make_double2(u[4], u[5]);
-        w[3] = make_double2(ld - ba.ldot_prev[i], 0.0);
+        for (int m = 3; m < 6; ++m) u[m] = (0.0 + mr * F[m]) + 0.0;
+        const double imp = det_exp(sig * D.mlam);
+        const double ld = (ba.len[i] * imp) * D.itau;
+        if (PLANAR) {
+          double2* w = reinterpret_cast<double2*>(ba.ubuf + 4 * i);
+          w[0] = make_double2(u[0], u[1]);
+          w[1] = make_double2(u[5], ld - ba.ldot_prev[i]);
+        } else {
+          double2* w = reinterpret_cast<double2*>(ba.ubuf + 8 * i);
+          w[0] = make_double2(u[0], u[1]);
+          w[1] = make_double2(u[2], u[3]);
+          w[2] = make_double2(u[4], u[5]);
+          w[3] = make_double2(ld - ba.ldot_prev[i], 0.0);
+        }
       }
     }
     __syncthreads();
     // ---- constraint apply (k_constr): logical block b is taken by group b % 4
     const bool save_prev = k > 0 && k % D.check == 0;
-    for (int64_t b0 = 0; b0 < G; b0 += kPersistGroups) {
-      const int64_t b = b0 + grp;
+    for (int64_t b = grp; b < G; b += kPersistGroups) {
       double v0 = 0.0, v1 = 0.0, v2 = 0.0, vr = 0.0;
-      if (b < G) {
-        for (int64_t j = 0; j < K; ++j) {
-          const int64_t c = D.coff + (b * K + j) * PCC_BLOCK + lt;
-          if (c >= cend) break;
-          const int32_t I = ca.I[c], J = ca.J[c];
-          const double2 yg = reinterpret_cast<const double2*>(ca.it)[c];
-          const double phi = ca.phi[c];
-          const Row<CT, PLANAR> row = load_row<CT, PLANAR>(ca.coef, c);
-          const double2* UI = reinterpret_cast<const double2*>(ca.ubuf + UW * (int64_t)I);
-          const double2* UJ = reinterpret_cast<const double2*>(ca.ubuf + UW * (int64_t)J);
-          double2 ui[UW / 2], uj[UW / 2];
+      for (int64_t j = 0; j < K; ++j) {
+        const int64_t c = D.coff + (b * K + j) * PCC_BLOCK + lt;
+        if (c >= cend) break;
+        const int32_t I = ca.I[c], J = ca.J[c];
+        const double2 yg = reinterpret_cast<const double2*>(ca.it)[c];
+        const double phi = ca.phi[c];
+        const Row<CT, PLANAR> row = load_row<CT, PLANAR>(ca.coef, c);
+        const double2* UI = reinterpret_cast<const double2*>(ca.ubuf + UW * (int64_t)I);
+        const double2* UJ = reinterpret_cast<const double2*>(ca.ubuf + UW * (int64_t)J);
+        double2 ui[UW / 2], uj[UW / 2];
 #pragma unroll
-          for (int m = 0; m < UW / 2; ++m) {
-            ui[m] = UI[m];
-            uj[m] = UJ[m];
-          }
-          const double gc = (k == 0) ? yg.x : proj_step(yg.x, malpha, yg.y);
-          double pd = 0.0;
-          double pg = 0.0;
-          const CT* co = row.v;
-          if (PLANAR) {
-            pd += (double)co[0] * ui[0].x;
-            pd += (double)co[1] * ui[0].y;
-            pd += (double)co[2] * ui[1].x;
-            pd += (-(double)co[0]) * uj[0].x;
-            pd += (-(double)co[1]) * uj[0].y;
-            pd += (double)co[3] * uj[1].x;
-            pg += (double)co[4] * ui[1].y;
-            pg += (double)co[5] * uj[1].y;
-          } else {
-            pd += (double)co[0] * ui[0].x;
-            pd += (double)co[1] * ui[0].y;
-            pd += (double)co[2] * ui[1].x;
-            pd += (double)co[3] * ui[1].y;
-            pd += (double)co[4] * ui[2].x;
-            pd += (double)co[5] * ui[2].y;
-            pd += (-(double)co[0]) * uj[0].x;
-            pd += (-(double)co[1]) * uj[0].y;
-            pd += (-(double)co[2]) * uj[1].x;
-            pd += (double)co[6] * uj[1].y;
-            pd += (double)co[7] * uj[2].x;
-            pd += (double)co[8] * uj[2].y;
-            pg += (double)co[9] * ui[3].x;
-            pg += (double)co[10] * uj[3].x;
-          }
-          const double g = phi + (dt * pd + (-dt) * pg);
-          if (save_prev) ca.gprev[c] = yg.x;
-          reinterpret_cast<double2*>(ca.it)[c] = make_double2(gc, g);
-          if (k > 0) {
-            const double dgam = gc - yg.x;
-            const double dphi = g - yg.y;
-            v0 += dgam * dgam;
-            v1 += dgam * dphi;
-            v2 += dphi * dphi;
-          }
-          const double pj = gc > 0 ? g : (g < 0.0 ? g : 0.0);
-          const double ar = fabs(pj);
-          vr = vr < ar ? ar : vr;
+        for (int m = 0; m < UW / 2; ++m) {
+          ui[m] = UI[m];
+          uj[m] = UJ[m];
         }
+        const double gc = (k == 0) ? yg.x : proj_step(yg.x, malpha, yg.y);
+        double pd = 0.0;
+        double pg = 0.0;
+        const CT* co = row.v;
+        if (PLANAR) {
+          pd += (double)co[0] * ui[0].x;
+          pd += (double)co[1] * ui[0].y;
+          pd += (double)co[2] * ui[1].x;
+          pd += (-(double)co[0]) * uj[0].x;
+          pd += (-(double)co[1]) * uj[0].y;
+          pd += (double)co[3] * uj[1].x;
+          pg += (double)co[4] * ui[1].y;
+          pg += (double)co[5] * uj[1].y;
+        } else {
+          pd += (double)co[0] * ui[0].x;
+          pd += (double)co[1] * ui[0].y;
+          pd += (double)co[2] * ui[1].x;
+          pd += (double)co[3] * ui[1].y;
+          pd += (double)co[4] * ui[2].x;
+          pd += (double)co[5] * ui[2].y;
+          pd += (-(double)co[0]) * uj[0].x;
+          pd += (-(double)co[1]) * uj[0].y;
+          pd += (-(double)co[2]) * uj[1].x;
+          pd += (double)co[6] * uj[1].y;
+          pd += (double)co[7] * uj[2].x;
+          pd += (double)co[8] * uj[2].y;
+          pg += (double)co[9] * ui[3].x;
+          pg += (double)co[10] * uj[3].x;
+        }
+        const double g = phi + (dt * pd + (-dt) * pg);
+        if (save_prev) ca.gprev[c] = yg.x;
+        reinterpret_cast<double2*>(ca.it)[c] = make_double2(gc, g);
+        if (k > 0) {
+          const double dgam = gc - yg.x;
+          const double dphi = g - yg.y;
+          v0 += dgam * dgam;
+          v1 += dgam * dphi;
+          v2 += dphi * dphi;
+        }
+        const double pj = gc > 0 ? g : (g < 0.0 ? g : 0.0);
+        const double ar = fabs(pj);
+        vr = vr < ar ? ar : vr;
       }
       const double w0 = wave_sum(v0), w1 = wave_sum(v1), w2 = wave_sum(v2);
       const double wm = wave_max(vr);
       if (lane == 0) {
-        red[grp][0][wv] = w0;
-        red[grp][1][wv] = w1;
-        red[grp][2][wv] = w2;
-        red[grp][3][wv] = wm;
+        red[b][0][wv] = w0;
+        red[b][1][wv] = w1;
+        red[b][2][wv] = w2;
+        red[b][3][wv] = wm;
       }
-      __syncthreads();
-      if (lt == 0 && b < G) {
-        const double(*r)[4] = red[grp];
-        ca.partials[o + b] = ((r[0][0] + r[0][1]) + r[0][2]) + r[0][3];
-        ca.partials[T + o + b] = ((r[1][0] + r[1][1]) + r[1][2]) + r[1][3];
-        ca.partials[2 * T + o + b] = ((r[2][0] + r[2][1]) + r[2][2]) + r[2][3];
-        double mx = r[3][0];
-        for (int m = 1; m < 4; ++m) mx = mx < r[3][m] ? r[3][m] : mx;
-        ca.partials[3 * T + o + b] = mx;
-      }
-      __syncthreads();
     }
-    // ---- finish (k_finish), on the first group
+    __syncthreads();
+    // ---- finish (k_finish): block partials, then the strided reduction
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, am = 0.0;
     if (grp == 0) {
       for (int64_t i = lt; i < G; i += PCC_BLOCK) {
-        a0 += ca.partials[o + i];
-        a1 += ca.partials[T + o + i];
-        a2 += ca.partials[2 * T + o + i];
-        const double m = ca.partials[3 * T + o + i];
-        am = am < m ? m : am;
+        const double(*r)[4] = red[i];
+        const double p0 = ((r[0][0] + r[0][1]) + r[0][2]) + r[0][3];
+        const double p1 = ((r[1][0] + r[1][1]) + r[1][2]) + r[1][3];
+        const double p2 = ((r[2][0] + r[2][1]) + r[2][2]) + r[2][3];
+        double pm = r[3][0];
+        for (int m = 1; m < 4; ++m) pm = pm < r[3][m] ? r[3][m] : pm;
+        a0 += p0;
+        a1 += p1;
+        a2 += p2;
+        am = am < pm ? pm : am;
       }
     }
     const double f0 = wave_sum(a0), f1 = wave_sum(a1), f2 = wave_sum(a2);
     const double fm = wave_max(am);
     if (grp == 0 && lane == 0) {
-      red[0][0][wv] = f0;
-      red[0][1][wv] = f1;
-      red[0][2][wv] = f2;
-      red[0][3][wv] = fm;
+      fin[0][wv] = f0;
+      fin[1][wv] = f1;
+      fin[2][wv] = f2;
+      fin[3][wv] = fm;
     }
     __syncthreads();
     if (tid == 0) {
-      const double(*r)[4] = red[0];
-      const double S0 = ((r[0][0] + r[0][1]) + r[0][2]) + r[0][3];
-      const double S1 = ((r[1][0] + r[1][1]) + r[1][2]) + r[1][3];
-      const double S2 = ((r[2][0] + r[2][1]) + r[2][2]) + r[2][3];
-      double R = r[3][0];
-      for (int m = 1; m < 4; ++m) R = R < r[3][m] ? r[3][m] : R;
+      const double S0 = ((fin[0][0] + fin[0][1]) + fin[0][2]) + fin[0][3];
+      const double S1 = ((fin[1][0] + fin[1][1]) + fin[1][2]) + fin[1][3];
+      const double S2 = ((fin[2][0] + fin[2][1]) + fin[2][2]) + fin[2][3];
+      double R = fin[3][0];
+      for (int m = 1; m < 4; ++m) R = R < fin[3][m] ? fin[3][m] : R;
       if (k == 0) {  // initial gradient: res = residual(g, Gamma), alpha = 1/res
         st->res = R;
         st->alpha = 1.0 / R;
@@ -1593,7 +1643,7 @@ void bbpgd_solve(pcc_ctx* ctx) {
     for (int64_t q = 0; q < B && persist; ++q) {
       const Instance& I = ctx->inst[q];
       if (!(I.flags & kInstActive)) continue;
-      if (I.C > ctx->persist_max_c || I.N > ctx->persist_max_n)
+      if (I.C > ctx->persist_max_c || I.N > ctx->persist_max_n || I.G > kPersistMaxTiles)
         persist = false;
       else
         plist.push_back((int32_t)q);
