@@ -21,9 +21,13 @@ the timed region starts.
 
 `--workload ensemble`: 128 independent colony packings (disc-shaped nematic
 domains, standing in for grown colonies) with log-uniform sizes in
-[500, 50000] (~1.5M cells) stepped 50 times in mixed precision by one batched
-context; the state carries on from step to step.  The untimed warm-up step
-relaxes the packings' initial overlaps.
+[500, 50000] (~1.5M cells) stepped 50 times (device division, then the step)
+in mixed precision by one batched context; the state carries on from step to
+step.  The untimed warm-up step relaxes the packings' initial overlaps.
+
+`--workload soft`: the soft Hertz model on one 100,000-cell colony packing,
+E* = 1e4, r_eff = 0.125, dt = 1e-3/30, 100 time-loop steps, mixed and FP64,
+with the CPU restatement's soft step timed beside it.
 
 `--workload serving`: colony-growth requests with Poisson arrivals (20/s for
 10 s), 1k-100k cells each, hard or soft 50/50, growth rate U[0.5,1.5] ln2,
@@ -297,6 +301,128 @@ def serving_requests(rate, duration, seed, steps, lo=1_000, hi=100_000):
     return reqs
 
 
+def soft_params(precision):
+    """Config-3 soft model: F = (4/3) E* sqrt(r_eff) delta^1.5 with E* = 1e4 and
+    r_eff = R1 R2/(R1 + R2) = 0.125 for d = 0.5 (kcc = 4/3 E*), dt 30x below
+    the hard model's, growth at ln2."""
+    from pcc.abi_types import MODE_SOFT
+    return reference_params(mode=MODE_SOFT, dt=1e-3 / 30, tolerance=1e-6, tau=1 / math.log(2),
+                            lam=1e-2, kcc=4.0 / 3.0 * 1e4, precision=precision)
+
+
+def run_soft(bodies, precision, steps, warmup):
+    """`steps` time-loop steps (device division, then the soft step) of one
+    colony; per-kernel timing on one extra step."""
+    from pcc.engine import ParticleManager
+    from pcc.serving import division_seed
+    pm = ParticleManager(soft_params(precision), device=0)
+    pm.set_state(bodies)
+    for i in range(warmup):
+        pm.divide(division_seed(0, i), 0, 0.0, 0.01)
+        pm.step(i)
+    lat, cell_steps = [], 0
+    t0 = time.perf_counter()
+    for i in range(warmup, warmup + steps):
+        a = time.perf_counter()
+        pm.divide(division_seed(0, i), 0, 0.0, 0.01)
+        r = pm.step(i)
+        lat.append(time.perf_counter() - a)
+        cell_steps += pm.n
+    total = time.perf_counter() - t0
+    pm.set_timing(True)
+    pm.step(warmup + steps)
+    tim = pm.timing()
+    pm.set_timing(False)
+    n_end = pm.n
+    pm.close()
+    return {"total_s": total, "lat": lat, "cell_steps": cell_steps, "timing": tim,
+            "n_constraints": r["n_constraints"], "max_overlap": r["max_overlap"],
+            "n_end": n_end}
+
+
+def kernel_table(tim):
+    out = {}
+    for name, t in tim.items():
+        if not t["launches"]:
+            continue
+        avg = t["total_ms"] / t["launches"] / 1e3
+        row = {"launches": t["launches"], "avg_us": avg * 1e6, "total_ms": t["total_ms"]}
+        if t["bytes_per_launch"] > 0:
+            row["algorithmic_bytes_per_launch"] = t["bytes_per_launch"]
+            row["gbs"] = t["bytes_per_launch"] / avg / 1e9
+            row["frac"] = row["gbs"] / HBM_PEAK_GBS
+        out[name] = row
+    return out
+
+
+def cpu_soft_baseline(bodies, reps=3):
+    """The FP64 oracle (Release flags + OpenMP) on this host's cores: `reps`
+    soft steps of the same colony, timed."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+    p = soft_params(PREC_FP64)
+    b = bodies
+    oracle.step(p, b, 0, fast=True)  # warm
+    t = time.perf_counter()
+    for i in range(reps):
+        b = oracle.step(p, b, i, fast=True)[2]
+    dt = (time.perf_counter() - t) / reps
+    return {"value": bodies.n / dt, "unit": "cell-steps/s", "cores": oracle.threads(fast=True),
+            "kind": "port",
+            "sample": (f"FP64 CPU restatement (-O3 -ffast-math, OpenMP, "
+                       f"{oracle.threads(fast=True)} threads): {reps} soft steps of the same "
+                       f"{bodies.n}-cell colony, {dt * 1e3:.1f} ms per step (no division)")}
+
+
+def main_soft(args):
+    from pcc.engine import copy_bandwidth
+    n = args.n if args.n != 300_000 else 100_000
+    steps = args.steps if args.steps_given else 100
+    b = G.dense_colony(n, seed=args.seed)
+    print(f"[bench] soft: n={b.n}, {steps} steps", file=sys.stderr, flush=True)
+    copy_gbs = copy_bandwidth(1 << 30, 10)
+    rm = run_soft(b, PREC_MIXED, steps, args.warmup)
+    r64 = run_soft(b, PREC_FP64, steps, args.warmup)
+    p50, p99 = latency_stats(rm["lat"])
+    km = kernel_table(rm["timing"])
+    h = km.get("hertz", {})
+    line = {
+        "metric": "cell-steps/s, soft Hertz model; p50/p99 step latency",
+        "value": rm["cell_steps"] / rm["total_s"],
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": rm["total_s"] / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": (f"soft Hertz model, one {b.n}-cell colony packing (nematic "
+                                f"domains), F = (4/3) E* sqrt(r_eff) delta^1.5 with E* = 1e4, "
+                                f"r_eff = 0.125, dt = 1e-3/30, {steps} time-loop steps (device "
+                                f"division, then the step), Hertz sums FP32, integration FP64"),
+                   "n_cells": b.n, "n_cells_end": rm["n_end"],
+                   "n_contacts": rm["n_constraints"]},
+        "p50_step_ms": p50,
+        "p99_step_ms": p99,
+        "roofline": {"bound": "hbm", "kernel": "hertz", "achieved": h.get("gbs"),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": h.get("frac"),
+                     "traffic": None, "avg_launch_us": h.get("avg_us"),
+                     "algorithmic_bytes_per_launch": h.get("algorithmic_bytes_per_launch"),
+                     "achievable_copy_gbs": copy_gbs},
+        "kernels": km,
+        "fp64": {"cell_steps_per_s": r64["cell_steps"] / r64["total_s"],
+                 "ms_per_step": r64["total_s"] / steps * 1e3,
+                 "kernels": kernel_table(r64["timing"])},
+        "max_overlap": rm["max_overlap"],
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_soft_baseline(b)
+    print(json.dumps(line))
+
+
 def main_serving(args):
     from pcc.serving import ColonyServer
     reqs = serving_requests(args.rate, args.duration, args.seed, args.serve_steps)
@@ -379,7 +505,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--instances", type=int, default=2)
     ap.add_argument("--colonies", type=int, default=128)
-    ap.add_argument("--workload", choices=["colony300k", "ensemble", "serving"],
+    ap.add_argument("--workload", choices=["colony300k", "ensemble", "serving", "soft"],
                     default="colony300k")
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--duration", type=float, default=10.0)
@@ -401,6 +527,8 @@ def main():
         return main_ensemble(args)
     if args.workload == "serving":
         return main_serving(args)
+    if args.workload == "soft":
+        return main_soft(args)
     from pcc.engine import copy_bandwidth
     copy_gbs = copy_bandwidth(1 << 30, 10)
     cols = [G.dense_colony(args.n, seed=args.seed + q) for q in range(args.instances)]

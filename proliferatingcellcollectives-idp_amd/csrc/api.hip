@@ -259,6 +259,7 @@ void begin_step(pcc_ctx* ctx) {
 void hard_step(pcc_ctx* ctx) {
   begin_step(ctx);
   ctx->planar = ctx->allow_planar && bodies_planar(ctx);
+  ctx->detect_flat = ctx->planar;  // z = 0 stays exact through a planar step
   std::vector<double> mo;
   for (int32_t k = 0;; ++k) {
     bool any = false;
@@ -320,6 +321,7 @@ void hard_step(pcc_ctx* ctx) {
 // solveSoftPotential (Soft.cpp:21-114) for every colony of the context.
 void soft_step(pcc_ctx* ctx) {
   begin_step(ctx);
+  ctx->detect_flat = ctx->allow_planar && bodies_planar(ctx);
   for (auto& I : ctx->inst) I.cutoff = 0.0;  // Soft.cpp:35
   upload_instances(ctx);
   detect_batch(ctx);
@@ -740,6 +742,7 @@ pcc_status pcc_detect(pcc_ctx* ctx, double cutoff, pcc_constraints* out, int64_t
     Instance& I = ctx->inst[0];
     I.flags = kInstActive;
     I.cutoff = cutoff;
+    ctx->detect_flat = ctx->allow_planar && bodies_planar(ctx);
     upload_instances(ctx);
     detect_batch(ctx);
     if (count) *count = ctx->batch_count;
@@ -1079,7 +1082,7 @@ pcc_status pcc_device_bytes(pcc_ctx* ctx, int64_t* bytes) {
   };
   add(ctx->body); add(ctx->snap); add(ctx->F); add(ctx->U); add(ctx->sig); add(ctx->imp);
   add(ctx->ldot_new); add(ctx->nc); add(ctx->cell); add(ctx->bucket); add(ctx->perm);
-  add(ctx->bstart); add(ctx->bcursor); add(ctx->dcount); add(ctx->rI_); add(ctx->rJ_);
+  add(ctx->bstart); add(ctx->bcursor); add(ctx->dcount); add(ctx->dslot); add(ctx->dslot_off); add(ctx->rI_); add(ctx->rJ_);
   add(ctx->rrec); add(ctx->ridx); add(ctx->ridx2); add(ctx->rkey); add(ctx->rkey2);
   add(ctx->sort_tmp); add_set(ctx->S); add_set(ctx->S2); add(ctx->rowptr); add(ctx->inccur);
   add(ctx->inc); add(ctx->increc); add(ctx->increc32); add(ctx->incidx); add(ctx->gold_e); add(ctx->fold); add(ctx->plist);

@@ -11,10 +11,13 @@
 //  3. k_scatter    bodies into buckets, then k_bucket_sort orders each bucket
 //                  by body index: a stable counting sort, identical to the
 //                  CPU restatement's
-//  4. k_detect     one thread per body I visits the 27 neighbour cells and
-//                  tests every J > I of the same or an adjacent cell with the
-//                  reference's filters and exact FP64 segment-segment DCP;
-//                  pass 0 counts, pass 1 writes at the scanned offset
+//  4. k_detect     one thread per (body I, neighbour cell) tests every J > I
+//                  of that cell with the reference's filters and exact FP64
+//                  segment-segment DCP; 27 neighbour cells, or the 9 of the
+//                  body's own z-layer when every body has z = 0 (the other
+//                  cells are then empty); pass 0 counts, pass 1 writes at the
+//                  scanned offset, so each body's records keep the cell order
+//                  dz, dy, dx
 //  5. k_segsort + radix sort: order = (contact x, I, J); with several colonies
 //                  in one context a second stable radix sort by colony makes
 //                  the order (colony, contact x, I, J)
@@ -85,14 +88,18 @@ __global__ void k_seg_sort(int64_t nseg, const int32_t* __restrict__ start,
 // Raw contact record layout (16 doubles): phi, n[3], rI[3], rJ[3], p[3], sI, sJ, gamma0
 enum { R_PHI = 0, R_N = 1, R_RI = 4, R_RJ = 7, R_P = 10, R_SI = 13, R_SJ = 14, R_G = 15, R_LEN = 16 };
 
+template <int S>  // neighbour cells per body: 9 (one z-layer) or 27
 __global__ void __launch_bounds__(256)
 k_detect(int pass, int64_t N, BodyArrays b, const int32_t* __restrict__ cell,
          const int32_t* __restrict__ bstart, const int32_t* __restrict__ perm, uint32_t mask,
          const int32_t* __restrict__ binst, const InstDev* __restrict__ idev,
          int32_t* __restrict__ cnt_off, int32_t* __restrict__ outI,
          int32_t* __restrict__ outJ, double* __restrict__ rec) {
-  int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (I >= N) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N * S) return;
+  const int64_t I = t / S;
+  const int sl = (int)(t - I * S);
+  const int dx = sl % 3 - 1, dy = (sl / 3) % 3 - 1, dz = S == 27 ? sl / 9 - 1 : 0;
   const int32_t inst = binst[I];
   if (!(idev[inst].flags & kInstActive)) return;  // count stays 0
   const double cutoff = idev[inst].cutoff, warm = idev[inst].warm;
@@ -110,11 +117,9 @@ k_detect(int pass, int64_t N, BodyArrays b, const int32_t* __restrict__ cell,
     P1[k] = xI[k] + hv;
   }
   const int32_t cx = cell[3 * I], cy = cell[3 * I + 1], cz = cell[3 * I + 2];
-  int32_t w = pass ? cnt_off[I] : 0;
+  int32_t w = pass ? cnt_off[t] : 0;
   int32_t c = 0;
-  for (int dz = -1; dz <= 1; ++dz)
-    for (int dy = -1; dy <= 1; ++dy)
-      for (int dx = -1; dx <= 1; ++dx) {
+      {
         const int32_t nx = cx + dx, ny = cy + dy, nz = cz + dz;
         const uint32_t bk = (cell_hash(nx, ny, nz) ^ salt) & mask;
         const int32_t e1 = bstart[bk + 1];
@@ -172,7 +177,14 @@ k_detect(int pass, int64_t N, BodyArrays b, const int32_t* __restrict__ cell,
           ++c;
         }
       }
-  if (!pass) cnt_off[I] = c;
+  if (!pass) cnt_off[t] = c;
+}
+
+// Per-body record offsets from the per-(body, neighbour cell) offsets.
+__global__ void k_body_offsets(int64_t N, int S, const int32_t* __restrict__ slot_off,
+                               int32_t* __restrict__ off) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= N) off[i] = slot_off[i * S];
 }
 
 // Per body segment of the batch: order its records by J (insertion sort of
@@ -359,23 +371,34 @@ void detect_batch(pcc_ctx* ctx) {
   build_grid(ctx);
   BodyArrays b = body_arrays(ctx->body.p, N);
   const uint32_t mask = (uint32_t)(ctx->table_size - 1);
+  // one slot per (body, neighbour cell); 9 neighbour cells when every body
+  // lies in the z = 0 layer
+  const int S = ctx->detect_flat ? 9 : 27;
+  const int64_t NS = N * S;
+  ctx->dslot.ensure(NS + 1);
+  ctx->dslot_off.ensure(NS + 1);
   ctx->dcount.ensure(N + 1);
-  PCC_HIP(hipMemsetAsync(ctx->dcount.p, 0, sizeof(int32_t) * (size_t)(N + 1), s));
-  if (N > 0)
-    k_detect<<<nblk(N), 256, 0, s>>>(0, N, b, ctx->cell.p, ctx->bstart.p, ctx->perm.p, mask,
-                                     ctx->binst.p, ctx->idev.p, ctx->dcount.p, nullptr, nullptr,
-                                     nullptr);
-  // offsets (in place into a second array)
-  ctx->ridx2.ensure(N + 1);
-  excl_scan<int32_t>(ctx, ctx->dcount.p, ctx->ridx2.p, N + 1);
+  PCC_HIP(hipMemsetAsync(ctx->dslot.p, 0, sizeof(int32_t) * (size_t)(NS + 1), s));
+  auto detect = [&](int pass, int32_t* cnt_off) {
+    if (S == 9)
+      k_detect<9><<<nblk(NS), 256, 0, s>>>(pass, N, b, ctx->cell.p, ctx->bstart.p, ctx->perm.p,
+                                           mask, ctx->binst.p, ctx->idev.p, cnt_off, ctx->rI_.p,
+                                           ctx->rJ_.p, ctx->rrec.p);
+    else
+      k_detect<27><<<nblk(NS), 256, 0, s>>>(pass, N, b, ctx->cell.p, ctx->bstart.p, ctx->perm.p,
+                                            mask, ctx->binst.p, ctx->idev.p, cnt_off, ctx->rI_.p,
+                                            ctx->rJ_.p, ctx->rrec.p);
+  };
+  if (N > 0) detect(0, ctx->dslot.p);
+  // per-slot offsets, then per-body offsets into dcount
+  excl_scan<int32_t>(ctx, ctx->dslot.p, ctx->dslot_off.p, NS + 1);
   int32_t total = 0;
-  PCC_HIP(hipMemcpyAsync(&total, ctx->ridx2.p + N, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  PCC_HIP(hipMemcpyAsync(&total, ctx->dslot_off.p + NS, sizeof(int32_t), hipMemcpyDeviceToHost,
+                         s));
   PCC_HIP(hipStreamSynchronize(s));
   const int64_t m = total;
   ctx->batch_count = m;
-  // keep offsets in dcount
-  PCC_HIP(hipMemcpyAsync(ctx->dcount.p, ctx->ridx2.p, sizeof(int32_t) * (size_t)(N + 1),
-                         hipMemcpyDeviceToDevice, s));
+  k_body_offsets<<<nblk(N + 1), 256, 0, s>>>(N, S, ctx->dslot_off.p, ctx->dcount.p);
   ctx->rI_.ensure(m + 1);
   ctx->rJ_.ensure(m + 1);
   ctx->rrec.ensure(R_LEN * (m + 1));
@@ -395,9 +418,7 @@ void detect_batch(pcc_ctx* ctx) {
     ctx->inst[q].m = bounds[q + 1] - bounds[q];
   }
   if (m == 0) return;
-  k_detect<<<nblk(N), 256, 0, s>>>(1, N, b, ctx->cell.p, ctx->bstart.p, ctx->perm.p, mask,
-                                   ctx->binst.p, ctx->idev.p, ctx->dcount.p, ctx->rI_.p,
-                                   ctx->rJ_.p, ctx->rrec.p);
+  detect(1, ctx->dslot_off.p);
   k_segsort_by_J<<<nblk(N), 256, 0, s>>>(N, ctx->dcount.p, ctx->rJ_.p, ctx->ridx.p);
   k_make_keys<<<nblk(m), 256, 0, s>>>(m, ctx->ridx.p, ctx->rrec.p, ctx->rkey.p);
   size_t bytes = 0;

@@ -195,6 +195,7 @@ struct pcc_ctx {
   int64_t N = 0;
   bool has_state = false;
   bool planar = false;        // current step uses the 2D monolayer kernels
+  bool detect_flat = false;   // every body has z = 0: detection visits 9 neighbour cells
   bool allow_planar = true;   // PCC_PLANAR=0 in the environment forces the 3D kernels
   bool use_graphs = true;     // BBPGD chunks as captured HIP graphs (PCC_GRAPH=0: plain launches)
   // one-launch BBPGD when every active colony has at most persist_max_c
@@ -226,7 +227,8 @@ struct pcc_ctx {
   int64_t table_size = 0;
 
   // detection batch
-  pcc::DArr<int32_t> dcount;   // N+1 (per-body counts, then offsets)
+  pcc::DArr<int32_t> dcount;   // N+1 (per-body record offsets)
+  pcc::DArr<int32_t> dslot, dslot_off;  // per (body, neighbour cell) counts and offsets
   pcc::DArr<int32_t> rI_, rJ_; // raw pair ids
   pcc::DArr<double> rrec;      // raw geometry: 16 doubles per contact
   pcc::DArr<int32_t> ridx, ridx2;

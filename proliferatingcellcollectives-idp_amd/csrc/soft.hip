@@ -185,6 +185,18 @@ __global__ void k_soft_body(int64_t N, int mixed, BodyArrays b, const int32_t* _
 
 }  // namespace
 
+// Algorithmic bytes per launch.  k_hertz, per contact: I, J, phi, n, rI, rJ,
+// sI, sJ, the two diameters and the colony index read (124 B); 15 values of
+// the pair record written (FP32 in mixed precision).  k_soft_body: per
+// incidence the index and the 8 record values it reads (6 force/torque
+// entries, the stress term, the active flag); per body rowptr and colony
+// index, 9 state doubles read (length, l0, position, quaternion) and 23
+// written (position, quaternion, ldot, length, F, U, stress, impedance).
+double hertz_bytes(int64_t C, bool mixed) { return (double)C * (124.0 + 15.0 * (mixed ? 4 : 8)); }
+double soft_body_bytes(int64_t N, int64_t C, bool mixed) {
+  return 2.0 * C * (4.0 + 8.0 * (mixed ? 4 : 8)) + (double)N * (8.0 + 9 * 8 + 23 * 8);
+}
+
 void soft_forces_and_integrate(pcc_ctx* ctx) {
   const int64_t N = ctx->N, C = ctx->S.C;
   const bool mixed = ctx->params.precision == PCC_PREC_MIXED;
@@ -193,14 +205,14 @@ void soft_forces_and_integrate(pcc_ctx* ctx) {
   ctx->hertz.ensure(16 * (C + 1));
   ctx->hertz32.ensure(16 * (C + 1));
   {
-    KTimer t(ctx, PCC_K_HERTZ, (double)C * (mixed ? 4 : 8) * 24.0);
+    KTimer t(ctx, PCC_K_HERTZ, hertz_bytes(C, mixed));
     ConstraintSet& S = ctx->S;
     if (C > 0)
       k_hertz<<<nblk(C), 256, 0, s>>>(C, mixed ? 1 : 0, S.I.p, S.J.p, S.phi.p, S.n.p, S.rI.p,
                                       S.rJ.p, S.sI.p, S.sJ.p, b.diam, S.inst.p, ctx->idev.p,
                                       ctx->hertz.p, ctx->hertz32.p);
   }
-  KTimer t(ctx, PCC_K_INTEGRATE, 0.0);
+  KTimer t(ctx, PCC_K_INTEGRATE, soft_body_bytes(N, C, mixed));
   if (N > 0)
     k_soft_body<<<nblk(N), 256, 0, s>>>(N, mixed ? 1 : 0, b, ctx->rowptr.p, ctx->inc.p,
                                         ctx->hertz.p, ctx->hertz32.p, ctx->binst.p, ctx->idev.p,
