@@ -32,7 +32,9 @@ with the CPU restatement's soft step timed beside it.
 `--workload serving`: colony-growth requests with Poisson arrivals (20/s for
 10 s), 1k-100k cells each, hard or soft 50/50, growth rate U[0.5,1.5] ln2,
 200 steps each, served by the continuous-batching loop (pcc.serving) under an
-HBM budget; p50/p99 are over every request's per-step latencies.
+HBM budget; p50/p99 are over every request's per-step latencies.  The hard
+colonies are relaxed by one untimed device step before the run, as growing
+colonies would be (--no-relax submits the raw packings).
 
 Prints ONE JSON line.  --gpus N > 1 exits: this build runs on one GPU.
 """
@@ -279,7 +281,8 @@ def main_ensemble(args):
 
 def serving_requests(rate, duration, seed, steps, lo=1_000, hi=100_000):
     """Poisson arrivals at `rate`/s over `duration` s; colony sizes
-    log-uniform in [lo, hi]; hard or soft 50/50; growth rate U[0.5,1.5] ln2."""
+    log-uniform in [lo, hi]; hard or soft 50/50; growth rate U[0.5,1.5] ln2.
+    Colonies are disc packings of nematic domains (the ensemble's generator)."""
     from pcc.abi_types import MODE_SOFT
     from pcc.serving import Request
     rng = np.random.default_rng(seed)
@@ -296,7 +299,7 @@ def serving_requests(rate, duration, seed, steps, lo=1_000, hi=100_000):
                                  kcc=4.0 / 3.0 * 1e4, precision=PREC_MIXED)
         else:
             p = params_for(PREC_MIXED).replace(tau=1 / rate_g)
-        reqs.append(Request(len(reqs), G.dense_colony(n, seed=seed * 100_000 + len(reqs)), p,
+        reqs.append(Request(len(reqs), G.packed_colony(n, seed=seed * 100_000 + len(reqs)), p,
                             steps, arrival=t))
     return reqs
 
@@ -423,9 +426,33 @@ def main_soft(args):
     print(json.dumps(line))
 
 
+def relax_requests(reqs, group=16):
+    """One untimed step of every hard request's initial colony on the device
+    (batches of `group` colonies of similar size, each with its own
+    parameters), standing in for colonies that have been growing: the
+    synthetic packings start with 1e-3..1e-2 overlaps whose first step costs
+    tens of thousands of BBPGD iterations.  Returns the seconds it took."""
+    from pcc.abi_types import MODE_HARD
+    from pcc.engine import ColonyBatch
+    hard = sorted((r for r in reqs if r.params.mode == MODE_HARD), key=lambda r: r.bodies.n)
+    t = time.perf_counter()
+    for k in range(0, len(hard), group):
+        part = hard[k:k + group]
+        cb = ColonyBatch(part[0].params, device=0)
+        cb.set_state([r.bodies for r in part], [r.params for r in part])
+        cb.step(0)
+        for q, r in enumerate(part):
+            r.bodies = cb.get_state(q)
+        cb.close()
+        print(f"[bench] relaxed hard requests {k + len(part)}/{len(hard)}", file=sys.stderr,
+              flush=True)
+    return time.perf_counter() - t
+
+
 def main_serving(args):
     from pcc.serving import ColonyServer
     reqs = serving_requests(args.rate, args.duration, args.seed, args.serve_steps)
+    relax_s = relax_requests(reqs) if args.relax else 0.0
     ncell = sum(r.bodies.n for r in reqs)
     print(f"[bench] serving: {len(reqs)} requests over {args.duration:.0f} s, {ncell} cells, "
           f"{sum(r.params.mode == 1 for r in reqs)} soft", file=sys.stderr, flush=True)
@@ -468,11 +495,15 @@ def main_serving(args):
         "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": (f"Poisson arrivals at {args.rate:g} requests/s for "
-                                f"{args.duration:g} s, colonies of 1k-100k cells (log-uniform), "
+                                f"{args.duration:g} s, colony packings of 1k-100k cells "
+                                f"(log-uniform), "
                                 f"hard or soft 50/50, growth rate U[0.5,1.5] ln2, "
                                 f"{args.serve_steps} steps each (divide, then step), HBM "
                                 f"budget {args.hbm_budget_gb:g} GiB, mixed precision"),
                    "requests": len(reqs), "n_cells": ncell, "warm_steps": args.warm_steps,
+                   "initial_states": ("hard colonies relaxed by one untimed device step "
+                                      f"({relax_s:.1f} s before the run)" if args.relax
+                                      else "unrelaxed packings"),
                    "division": "device, seeded per request and step, exact halves, "
                                "N(0, 0.01 rad) turns"},
         "p50_step_ms": st["p50_step_ms"],
@@ -501,6 +532,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--warm-steps", type=int, default=2,
                     help="serving: steps a new colony runs in the warm-up batch")
+    ap.add_argument("--no-relax", dest="relax", action="store_false",
+                    help="serving: submit the hard colonies unrelaxed")
     ap.add_argument("--n", type=int, default=300_000)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--instances", type=int, default=2)
