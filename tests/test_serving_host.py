@@ -119,7 +119,9 @@ def test_admission_respects_budget_and_order():
 
 
 def test_late_arrival_joins_a_running_batch():
-    reqs = make_requests([(100, 0, 40, 0.0), (100, 0, 3, 0.02)])
+    # the first colony stays resident for at least 500 x 2 ms; the second
+    # arrives well after its first step, also on a loaded machine
+    reqs = make_requests([(100, 0, 500, 0.0), (100, 0, 3, 0.25)])
     srv = serving.ColonyServer(10**12, bytes_per_cell=1000, warm_steps=0, divide=False)
     srv.run(reqs)
     adds = FakeBatch.instances[0].events  # both colonies fall in the same bucket
