@@ -263,13 +263,15 @@ pcc_status pcc_batch_divide(pcc_ctx* ctx, const uint64_t* seeds, int32_t step,
  * fastest for the context (PCC_AUTOTUNE=0 in the environment keeps the
  * default, variant 1).  Variants v = 0..5: staging size (incidences per LDS
  * pass) 512/1024/2048 planar or 256/512/1024 3D for v % 3, threads per block
- * 256 (v < 3) or 128 (two bodies per thread); the sums do not depend on
- * either.  *body_chunk, *body_threads: the variant in use; *tuned: 1 once
- * tuned; variant_us[PCC_BODY_VARIANTS] (may be NULL): the measured launch
- * times.  Any output pointer may be NULL.  No reference counterpart. */
-#define PCC_BODY_VARIANTS 6
-pcc_status pcc_get_tuning(pcc_ctx* ctx, int32_t* body_chunk, int32_t* body_threads,
-                          int32_t* tuned, double* variant_us);
+ * 256 (v < 3) or 128 (two bodies per thread); v = 6..8: 256 threads per
+ * block, staging 256/512/1024, the next pass's incidence records loaded while
+ * the current pass is summed.  The sums depend on none of these.  *variant,
+ * *body_chunk, *body_threads: the variant in use; *tuned: 1 once tuned;
+ * variant_us[PCC_BODY_VARIANTS] (may be NULL): the measured launch times.
+ * Any output pointer may be NULL.  No reference counterpart. */
+#define PCC_BODY_VARIANTS 9
+pcc_status pcc_get_tuning(pcc_ctx* ctx, int32_t* variant, int32_t* body_chunk,
+                          int32_t* body_threads, int32_t* tuned, double* variant_us);
 
 /* CFL time-step adaptation between steps (Domain::adaptDt, Domain.cpp:69-118),
  * on the device: the median over the colony's cells of |v| + ldot after the

@@ -1027,9 +1027,10 @@ pcc_status pcc_get_timing(pcc_ctx* ctx, int kernel, int64_t* launches, double* t
   });
 }
 
-pcc_status pcc_get_tuning(pcc_ctx* ctx, int32_t* body_chunk, int32_t* body_threads,
-                          int32_t* tuned, double* variant_us) {
+pcc_status pcc_get_tuning(pcc_ctx* ctx, int32_t* variant, int32_t* body_chunk,
+                          int32_t* body_threads, int32_t* tuned, double* variant_us) {
   if (!ctx) return PCC_E_INVALID_ARGUMENT;
+  if (variant) *variant = ctx->body_variant;
   if (body_chunk) *body_chunk = body_chunk_of(ctx->body_variant, ctx->planar);
   if (body_threads) *body_threads = body_threads_of(ctx->body_variant);
   if (tuned) *tuned = ctx->body_tuned ? 1 : 0;

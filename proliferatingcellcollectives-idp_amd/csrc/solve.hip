@@ -306,6 +306,90 @@ __device__ __forceinline__ void load_prec<float>(const void* base, int64_t e, do
   co[3] = a.w;
 }
 
+// One incidence record in the form it is loaded (the values of load_rec /
+// load_prec, converted only where they are used), so that a kernel can keep
+// the next records in flight while it finishes the current ones.
+template <class CT, bool PLANAR>
+struct IncRaw;
+
+template <>
+struct IncRaw<float, true> {
+  float4 r;
+  int32_t c;
+  __device__ __forceinline__ void load(const void* rec, const int32_t* rc, int64_t e) {
+    r = reinterpret_cast<const float4*>(rec)[e];
+    c = rc[e];
+  }
+  __device__ __forceinline__ int64_t index() const { return c; }
+  __device__ __forceinline__ void coef(double co[7]) const {
+    co[0] = r.x;
+    co[1] = r.y;
+    co[5] = r.z;
+    co[6] = r.w;
+  }
+};
+
+template <>
+struct IncRaw<double, true> {
+  double2 r0, r1;
+  int32_t c;
+  __device__ __forceinline__ void load(const void* rec, const int32_t* rc, int64_t e) {
+    const double2* p = reinterpret_cast<const double2*>(rec) + 2 * e;
+    r0 = p[0];
+    r1 = p[1];
+    c = rc[e];
+  }
+  __device__ __forceinline__ int64_t index() const { return c; }
+  __device__ __forceinline__ void coef(double co[7]) const {
+    co[0] = r0.x;
+    co[1] = r0.y;
+    co[5] = r1.x;
+    co[6] = r1.y;
+  }
+};
+
+template <>
+struct IncRaw<float, false> {
+  float4 r0, r1;
+  __device__ __forceinline__ void load(const void* rec, const int32_t*, int64_t e) {
+    const float4* p = reinterpret_cast<const float4*>(rec) + 2 * e;
+    r0 = p[0];
+    r1 = p[1];
+  }
+  __device__ __forceinline__ int64_t index() const { return (int64_t)__float_as_int(r1.w); }
+  __device__ __forceinline__ void coef(double co[7]) const {
+    co[0] = r0.x;
+    co[1] = r0.y;
+    co[2] = r0.z;
+    co[3] = r0.w;
+    co[4] = r1.x;
+    co[5] = r1.y;
+    co[6] = r1.z;
+  }
+};
+
+template <>
+struct IncRaw<double, false> {
+  double2 r0, r1, r2, r3;
+  __device__ __forceinline__ void load(const void* rec, const int32_t*, int64_t e) {
+    const double2* p = reinterpret_cast<const double2*>(rec) + 4 * e;
+    r0 = p[0];
+    r1 = p[1];
+    r2 = p[2];
+    r3 = p[3];
+  }
+  __device__ __forceinline__ int64_t index() const { return (int64_t)__double_as_longlong(r3.y); }
+  __device__ __forceinline__ void coef(double co[7]) const {
+    co[0] = r0.x;
+    co[1] = r0.y;
+    co[2] = r1.x;
+    co[3] = r1.y;
+    co[4] = r2.x;
+    co[5] = r2.y;
+    co[6] = r3.x;
+  }
+};
+
 // Mixed precision: F = D^T (gamma - gamma_old) is evaluated as D^T gamma
 // minus the per-body sum D^T gamma_old, which is fixed within a phase, so the
 // iteration reads 3 or 6 doubles per body instead of gamma_old per incidence.
@@ -432,12 +516,16 @@ __global__ void __launch_bounds__(256) k_body_final(BodyArgs a, FinalOut fo) {
 // above.  Blocks never straddle two colonies (each colony starts a new
 // logical block).  CHUNK (incidences staged per pass) and TPB (threads per
 // block: 256, or 128 so that a 600k-body grid fits the device in one round)
-// do not change any sum and are picked by the autotuner.
-template <class CT, bool PLANAR, int CHUNK, int TPB>
+// do not change any sum and are picked by the autotuner.  With PIPE, each
+// lane issues the next chunk's incidence records as soon as the current
+// chunk's iterate gathers are in flight, so that their latency overlaps the
+// products, the LDS sums and the two barriers of the current chunk.
+template <class CT, bool PLANAR, int CHUNK, int TPB, bool PIPE>
 __global__ void __launch_bounds__(TPB) k_body_coop(BodyArgs a) {
   constexpr int NC = PLANAR ? 4 : 7;  // products per incidence
   constexpr bool FOLD = sizeof(CT) == 4;  // mixed: gamma_old enters as the per-body sum
   constexpr int R = PCC_BLOCK / TPB;      // bodies per thread
+  static_assert(CHUNK % TPB == 0 && CHUNK >= TPB && PCC_BLOCK % TPB == 0, "staging shape");
   __shared__ double prod[NC][CHUNK];
   const int64_t l = xcd_block(blockIdx.x, a.gp);
   if (l >= a.nbb) return;
@@ -464,56 +552,59 @@ __global__ void __launch_bounds__(TPB) k_body_coop(BodyArgs a) {
     sig[r] = 0.0;
   }
   constexpr int PER = CHUNK / TPB;  // incidences per lane and chunk
-  for (int32_t base = E0; base < E1; base += CHUNK) {
-    const int32_t lim = base + CHUNK < E1 ? base + CHUNK : E1;
-    // Phase 1 in three sweeps so that every lane has all of its loads in
-    // flight at once: records, then the iterate-record gathers, then products.
-    double co[PER][7];
-    int64_t cc[PER];
-    bool ok[PER];
+  // Phase 1 in three sweeps so that every lane has all of its loads in flight
+  // at once: records, then the iterate-record gathers, then products.  Every
+  // lane loads and writes all PER slots of a chunk, also past the end of the
+  // range (loads clamped to the block's last incidence; the sums never read
+  // those slots), and scheduling barriers keep the sweeps apart: with no
+  // loads under branches and no loads interleaved, the compiler waits for
+  // each load exactly, so the gathers of a sweep overlap and, with PIPE, the
+  // next chunk's records stay in flight through the products and the sums.
+  IncRaw<CT, PLANAR> raw[PER];
+  bool ok[PER];
+  auto fetch = [&](int32_t from, int32_t lim) {  // requires E0 < E1
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-      const int32_t e = base + (int32_t)threadIdx.x + j * TPB;
+      const int32_t e = from + (int32_t)threadIdx.x + j * TPB;
       ok[j] = e < lim;
-      cc[j] = 0;
-      if (ok[j]) {
-        if (PLANAR) {
-          double p[4];
-          load_prec<CT>(a.rec, e, p);
-          co[j][0] = p[0];
-          co[j][1] = p[1];
-          co[j][5] = p[2];
-          co[j][6] = p[3];
-          cc[j] = a.rc[e];
-        } else {
-          load_rec<CT>(a.rec, e, co[j], cc[j]);
-        }
-      }
+      raw[j].load(a.rec, a.rc, ok[j] ? e : E1 - 1);
     }
+  };
+  if (PIPE && E0 < E1) fetch(E0, E0 + CHUNK < E1 ? E0 + CHUNK : E1);
+  for (int32_t base = E0; base < E1; base += CHUNK) {
+    const int32_t lim = base + CHUNK < E1 ? base + CHUNK : E1;
+    if (!PIPE) fetch(base, lim);
+    __builtin_amdgcn_sched_barrier(0);
     double2 yg[PER];
     double gold[PER];
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-      if (ok[j]) {
-        yg[j] = reinterpret_cast<const double2*>(a.it)[cc[j]];
-        if (!FOLD) gold[j] = a.gold[base + (int32_t)threadIdx.x + j * TPB];
-      }
+      yg[j] = reinterpret_cast<const double2*>(a.it)[raw[j].index()];
+      if (!FOLD) gold[j] = a.gold[ok[j] ? base + (int32_t)threadIdx.x + j * TPB : E1 - 1];
     }
+    IncRaw<CT, PLANAR> cur[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) cur[j] = raw[j];
+    __builtin_amdgcn_sched_barrier(0);
+    // the next chunk's records (past E1 on the last chunk: unused)
+    if (PIPE) fetch(lim, lim + CHUNK < E1 ? lim + CHUNK : E1);
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-      if (!ok[j]) continue;
+      double co[7];
+      cur[j].coef(co);
       const double gc = (k == 0) ? yg[j].x : proj_step(yg[j].x, malpha, yg[j].y);
       const double dg = FOLD ? gc : gc - gold[j];
       const int q = (int)threadIdx.x + j * TPB;
       if (PLANAR) {
-        prod[0][q] = dg * co[j][0];
-        prod[1][q] = dg * co[j][1];
-        prod[2][q] = dg * co[j][5];
-        prod[3][q] = gc * co[j][6];
+        prod[0][q] = dg * co[0];
+        prod[1][q] = dg * co[1];
+        prod[2][q] = dg * co[5];
+        prod[3][q] = gc * co[6];
       } else {
 #pragma unroll
-        for (int m = 0; m < 6; ++m) prod[m][q] = dg * co[j][m];
-        prod[6][q] = gc * co[j][6];
+        for (int m = 0; m < 6; ++m) prod[m][q] = dg * co[m];
+        prod[6][q] = gc * co[6];
       }
     }
     __syncthreads();
@@ -1534,16 +1625,21 @@ double constr_bytes(int64_t N, int64_t C, bool mixed, bool planar) {
 
 // Body-kernel variants the autotuner chooses from: staging size (incidences
 // per pass: planar 512 / 1024 / 2048, 3D 256 / 512 / 1024) times threads per
-// block (variants 0-2: 256, one body each; 3-5: 128, two bodies each).
+// block (variants 0-2: 256, one body each; 3-5: 128, two bodies each), and
+// variants 6-8: 256 threads with the next chunk's records prefetched, staging
+// 256 / 512 / 1024.
 template <class CT, bool PL, int C0>
 static void launch_body_t(int variant, unsigned grid, hipStream_t s, const BodyArgs& ba) {
   switch (variant) {
-    case 0: k_body_coop<CT, PL, C0, 256><<<grid, 256, 0, s>>>(ba); break;
-    case 1: k_body_coop<CT, PL, 2 * C0, 256><<<grid, 256, 0, s>>>(ba); break;
-    case 2: k_body_coop<CT, PL, 4 * C0, 256><<<grid, 256, 0, s>>>(ba); break;
-    case 3: k_body_coop<CT, PL, C0, 128><<<grid, 128, 0, s>>>(ba); break;
-    case 4: k_body_coop<CT, PL, 2 * C0, 128><<<grid, 128, 0, s>>>(ba); break;
-    default: k_body_coop<CT, PL, 4 * C0, 128><<<grid, 128, 0, s>>>(ba); break;
+    case 0: k_body_coop<CT, PL, C0, 256, false><<<grid, 256, 0, s>>>(ba); break;
+    case 1: k_body_coop<CT, PL, 2 * C0, 256, false><<<grid, 256, 0, s>>>(ba); break;
+    case 2: k_body_coop<CT, PL, 4 * C0, 256, false><<<grid, 256, 0, s>>>(ba); break;
+    case 3: k_body_coop<CT, PL, C0, 128, false><<<grid, 128, 0, s>>>(ba); break;
+    case 4: k_body_coop<CT, PL, 2 * C0, 128, false><<<grid, 128, 0, s>>>(ba); break;
+    case 5: k_body_coop<CT, PL, 4 * C0, 128, false><<<grid, 128, 0, s>>>(ba); break;
+    case 6: k_body_coop<CT, PL, 256, 256, true><<<grid, 256, 0, s>>>(ba); break;
+    case 7: k_body_coop<CT, PL, 512, 256, true><<<grid, 256, 0, s>>>(ba); break;
+    default: k_body_coop<CT, PL, 1024, 256, true><<<grid, 256, 0, s>>>(ba); break;
   }
 }
 
@@ -1562,8 +1658,10 @@ static void launch_body(int variant, bool planar, bool mixed, unsigned grid, hip
   }
 }
 
-int body_chunk_of(int variant, bool planar) { return (planar ? 512 : 256) << (variant % 3); }
-int body_threads_of(int variant) { return variant < 3 ? 256 : 128; }
+int body_chunk_of(int variant, bool planar) {
+  return variant >= 6 ? 256 << (variant - 6) : (planar ? 512 : 256) << (variant % 3);
+}
+int body_threads_of(int variant) { return variant < 3 || variant >= 6 ? 256 : 128; }
 
 static void ensure_state_host(pcc_ctx* ctx, int64_t slots) {
   if (ctx->state_host_cap >= slots) return;

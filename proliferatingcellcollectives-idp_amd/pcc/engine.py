@@ -21,7 +21,7 @@ from . import abi_types as T
 _PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(_PKG_DIR, "libpcc.so")
 _lib = None
-BODY_VARIANTS = 6  # PCC_BODY_VARIANTS
+BODY_VARIANTS = 9  # PCC_BODY_VARIANTS
 
 
 def lib():
@@ -72,7 +72,8 @@ def lib():
                                                 C.POINTER(i64)]
         L.pcc_device_bytes.argtypes = [vp, C.POINTER(i64)]
         L.pcc_get_tuning.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
-                                     C.POINTER(C.c_int32), C.POINTER(C.c_double)]
+                                     C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                     C.POINTER(C.c_double)]
         dpt = C.POINTER(C.c_double)
         L.pcc_adapt_dt.argtypes = [vp, C.c_double, C.c_double, C.c_double, dpt, dpt]
         L.pcc_batch_adapt_dt.argtypes = [vp, C.c_double, C.c_double, dpt, dpt]
@@ -187,13 +188,16 @@ class ParticleManager:
         return float(dt.value), float(med.value)
 
     def tuning(self) -> dict:
-        """Autotuner state: the body-kernel variant in use (staging size,
-        threads per block), whether it was tuned, and the measured time of
-        each of the BODY_VARIANTS candidates (us)."""
-        ch, th, tu = C.c_int32(0), C.c_int32(0), C.c_int32(0)
+        """Autotuner state: the body-kernel variant in use (its index,
+        staging size, threads per block, whether it prefetches the next
+        pass), whether it was tuned, and the measured time of each of the
+        BODY_VARIANTS candidates (us)."""
+        va, ch, th, tu = C.c_int32(0), C.c_int32(0), C.c_int32(0), C.c_int32(0)
         us = (C.c_double * BODY_VARIANTS)()
-        self._check(lib().pcc_get_tuning(self._ctx, C.byref(ch), C.byref(th), C.byref(tu), us))
-        return {"body_chunk": int(ch.value), "body_threads": int(th.value),
+        self._check(lib().pcc_get_tuning(self._ctx, C.byref(va), C.byref(ch), C.byref(th),
+                                         C.byref(tu), us))
+        return {"variant": int(va.value), "body_chunk": int(ch.value),
+                "body_threads": int(th.value), "prefetch": int(va.value) >= 6,
                 "tuned": bool(tu.value), "variant_us": [float(x) for x in us]}
 
     def set_trace(self, enable: bool = True, capacity: int = 100000):

@@ -213,8 +213,10 @@ def test_autotuned_body_kernel_gives_the_same_bits(monkeypatch):
 def test_every_body_kernel_variant_gives_the_same_bits(precision, monkeypatch):
     """Each staging size and threads-per-block variant (forced with
     PCC_BODY_VARIANT) sums every body in the same order: planar and 3D steps
-    equal the default variant's bit for bit."""
+    equal the default variant's bit for bit.  PCC_PERSIST=0 keeps the small
+    3D case on the multi-kernel path, which the variants belong to."""
     from pcc.engine import BODY_VARIANTS, ParticleManager
+    monkeypatch.setenv("PCC_PERSIST", "0")
     for name, b, p in (
             ("planar", G.dense_colony(6000, seed=161), hard(precision=precision)),
             # a hard 3D packing, capped: failing solves must agree bit for bit too
@@ -229,7 +231,9 @@ def test_every_body_kernel_variant_gives_the_same_bits(precision, monkeypatch):
             t = pm.tuning()
             runs.append((r, pm.get_state(), pm.constraints()))
             pm.close()
+            assert t["variant"] == v
             print(f"  [{name} variant {v}] chunk={t['body_chunk']} threads={t['body_threads']} "
+                  f"prefetch={t['prefetch']} "
                   f"its={r['bbpgd_iterations']}", flush=True)
         for r, st, cons in runs[1:]:
             assert r == runs[0][0]
