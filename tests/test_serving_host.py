@@ -162,9 +162,11 @@ def test_warm_up_moves_keep_state_and_step_count():
 
 
 def slow_pair(warm_steps):
-    a = make_requests([(100, 0, 60, 0.0)])[0]
+    # the steady colony has left the warm-up batch long before the fresh one
+    # arrives (also on a loaded machine) and is still running when it does
+    a = make_requests([(100, 0, 500, 0.0)])[0]
     b = make_requests([(777, 0, 5, 0.0)])[0]
-    b.rid, b.arrival = 1, 0.03
+    b.rid, b.arrival = 1, 0.25
     b.bodies.length[0] = 100.0  # fresh: slow first step
     srv = serving.ColonyServer(10**12, bytes_per_cell=1000, warm_steps=warm_steps)
     srv.run([a, b])
