@@ -103,6 +103,9 @@ typedef struct pcc_bodies {
   double* impedance;  /* n (out) */
   double* stress;     /* n (out) */
   int32_t* num_constraints; /* n (out) */
+  int32_t* age;       /* n: generation (ParticleData::age, ParticleData.h:16): 1 for a new cell,
+                       * +1 for both halves at each division (Particle.cpp:26,144,148).  May be
+                       * NULL: pcc_set_state then starts every cell at 1. */
 } pcc_bodies;
 
 /* SoA image of std::vector<Constraint> (code/cpp/src/dynamics/Constraint.h:6-49).

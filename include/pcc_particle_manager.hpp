@@ -34,6 +34,7 @@ struct ParticleRecord {
   std::array<double, 3> force{}, torque{}, velocityLinear{}, velocityAngular{};
   double impedance = 0.0, stress = 0.0;
   int num_constraints = 0;
+  int age = 1;  // generation: 1 for a new cell, +1 for both halves at each division
 };
 
 // Constraint (code/cpp/src/dynamics/Constraint.h:6-49), host copy.
@@ -87,7 +88,7 @@ class ParticleManager {
  private:
   pcc_ctx* ctx_ = nullptr;
   std::vector<double> pos_, quat_, len_, l0_, diam_, ldot_, f_, t_, v_, w_, imp_, str_;
-  std::vector<int32_t> nc_;
+  std::vector<int32_t> nc_, age_;
 
   pcc_bodies view(size_t n) {
     pos_.resize(3 * n);
@@ -103,6 +104,7 @@ class ParticleManager {
     imp_.resize(n);
     str_.resize(n);
     nc_.resize(n);
+    age_.resize(n);
     pcc_bodies b{};
     b.n = (int64_t)n;
     b.pos = pos_.data();
@@ -118,6 +120,7 @@ class ParticleManager {
     b.impedance = imp_.data();
     b.stress = str_.data();
     b.num_constraints = nc_.data();
+    b.age = age_.data();
     return b;
   }
 
@@ -133,6 +136,7 @@ class ParticleManager {
       l0_[i] = p.l0;
       diam_[i] = p.diameter;
       ldot_[i] = p.ldot;
+      age_[i] = p.age;
     }
     pcc_status s = pcc_set_state(ctx_, &b);
     if (s != PCC_OK) throw std::runtime_error(pcc_last_error(ctx_));
@@ -158,6 +162,7 @@ class ParticleManager {
       p.impedance = imp_[i];
       p.stress = str_[i];
       p.num_constraints = nc_[i];
+      p.age = age_[i];
     }
   }
 

@@ -1377,10 +1377,11 @@ static int xkey_cmp(const void* a, const void* b) {
  * Domain.cpp:492-508; ties keep the staging order).  l1 = U[1-a, 1+a] L/2,
  * l2 = L - l1, centres x -/+ (l/2) t; each child then turns in-plane by
  * angle_noise * N(0,1).  Staging order: every parent's left child in its
- * place, right children appended in parent order. */
+ * place, right children appended in parent order.  age (may be NULL): the
+ * parent's age grows by one and the new cell copies it (Particle.cpp:144,148). */
 int64_t oracle_divide(int64_t n, int64_t cap, double* pos, double* quat, double* length,
-                      double* l0, double* diameter, double* ldot, uint64_t seed, int32_t step,
-                      double length_noise, double angle_noise) {
+                      double* l0, double* diameter, double* ldot, double* age, uint64_t seed,
+                      int32_t step, double length_noise, double angle_noise) {
   int64_t d = 0;
   for (int64_t i = 0; i < n; ++i) d += length[i] >= 2 * l0[i];
   const int64_t m = n + d;
@@ -1414,6 +1415,10 @@ int64_t oracle_divide(int64_t n, int64_t cap, double* pos, double* quat, double*
     l0[j] = l0[i];
     diameter[j] = diameter[i];
     ldot[j] = ldot[i];
+    if (age) {
+      age[i] = age[i] + 1;
+      age[j] = age[i];
+    }
   }
   /* gids by ascending x */
   xkey_t* ord = (xkey_t*)malloc(sizeof(xkey_t) * (size_t)(m > 0 ? m : 1));
@@ -1435,6 +1440,7 @@ int64_t oracle_divide(int64_t n, int64_t cap, double* pos, double* quat, double*
   PERMUTE(l0, 1);
   PERMUTE(diameter, 1);
   PERMUTE(ldot, 1);
+  if (age) PERMUTE(age, 1);
 #undef PERMUTE
   free(tmp);
   free(ord);

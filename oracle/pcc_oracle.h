@@ -42,10 +42,11 @@ double oracle_exp(double x);
 
 /* Division of every cell with length >= 2 l0 and gid reassignment by x
  * (Particle.cpp:122-150, Domain.cpp:492-508), seeded; arrays hold `cap`
- * bodies.  Returns the new body count, -1 if it exceeds cap. */
+ * bodies; age (generation, may be NULL) is carried as a double.  Returns the
+ * new body count, -1 if it exceeds cap. */
 int64_t oracle_divide(int64_t n, int64_t cap, double* pos, double* quat, double* length,
-                      double* l0, double* diameter, double* ldot, uint64_t seed, int32_t step,
-                      double length_noise, double angle_noise);
+                      double* l0, double* diameter, double* ldot, double* age, uint64_t seed,
+                      int32_t step, double length_noise, double angle_noise);
 
 /* Broad phase.  cell = floor((x - origin)/h) per axis (SpatialGrid.cpp:29-36);
  * bucket = hash(cell) & (table_size-1); stable counting sort of bodies by

@@ -35,7 +35,7 @@ std::recursive_mutex& device_mutex() {
 BodyArrays body_arrays(double* base, int64_t N) {
   BodyArrays b;
   double** f[kBodyFields] = {&b.px, &b.py, &b.pz, &b.qw, &b.qx, &b.qy,
-                             &b.qz, &b.len, &b.l0, &b.diam, &b.ldot};
+                             &b.qz, &b.len, &b.l0, &b.diam, &b.ldot, &b.age};
   for (int k = 0; k < kBodyFields; ++k) *f[k] = base + (int64_t)k * N;
   return b;
 }
@@ -396,6 +396,7 @@ void pack_bodies(const pcc_bodies* b, int64_t Ntot, int64_t boff, double* h) {
     h[(size_t)8 * Ntot + j] = b->l0[i];
     h[(size_t)9 * Ntot + j] = b->diameter[i];
     h[(size_t)10 * Ntot + j] = b->ldot[i];
+    h[(size_t)11 * Ntot + j] = b->age ? (double)b->age[i] : 1.0;
   }
 }
 
@@ -478,6 +479,7 @@ pcc_status get_colony_state(pcc_ctx* ctx, int64_t q, pcc_bodies* b) {
       if (b->l0) b->l0[i] = h[(size_t)8 * N + i];
       if (b->diameter) b->diameter[i] = h[(size_t)9 * N + i];
       if (b->ldot) b->ldot[i] = h[(size_t)10 * N + i];
+      if (b->age) b->age[i] = (int32_t)h[(size_t)11 * N + i];
       for (int k = 0; k < 3; ++k) {
         if (b->force) b->force[3 * i + k] = F[6 * i + k];
         if (b->torque) b->torque[3 * i + k] = F[6 * i + 3 + k];

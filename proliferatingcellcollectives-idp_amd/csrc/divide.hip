@@ -11,7 +11,8 @@
 //                   becomes its left child), then its right children in
 //                   parent order; l1 = U[1-a, 1+a] L/2, l2 = L - l1, centres
 //                   x -/+ (l/2) t, each child turned in-plane by an angle drawn
-//                   from N(0, angle_noise) (Irwin-Hall, twelve uniforms)
+//                   from N(0, angle_noise) (Irwin-Hall, twelve uniforms), both
+//                   children one generation older than the parent
 //  4. radix sorts   stable by x, then stable by colony: gids by ascending x
 //  5. k_div_gather  body fields into the new order
 // The draws are counter-based on (seed of the colony, step, index of the parent
@@ -134,8 +135,8 @@ __global__ void k_div_write(int64_t N, int64_t M, BodyArrays b, const int32_t* _
   const int64_t j = c.noff + local;
   double x[3] = {b.px[i], b.py[i], b.pz[i]};
   double qq[4] = {b.qw[i], b.qx[i], b.qy[i], b.qz[i]};
-  const double L = b.len[i], l0 = b.l0[i], dm = b.diam[i], ld = b.ldot[i];
-  auto put = [&](int64_t s, const double p[3], const double qv[4], double len) {
+  const double L = b.len[i], l0 = b.l0[i], dm = b.diam[i], ld = b.ldot[i], age = b.age[i];
+  auto put = [&](int64_t s, const double p[3], const double qv[4], double len, double ag) {
     stage[0 * M + s] = p[0];
     stage[1 * M + s] = p[1];
     stage[2 * M + s] = p[2];
@@ -147,12 +148,13 @@ __global__ void k_div_write(int64_t N, int64_t M, BodyArrays b, const int32_t* _
     stage[8 * M + s] = l0;
     stage[9 * M + s] = dm;
     stage[10 * M + s] = ld;
+    stage[11 * M + s] = ag;
     key[s] = order_key(p[0]);
     val[s] = (int32_t)s;
     sinst[s] = q;
   };
   if (!(L >= 2 * l0)) {
-    put(j, x, qq, L);
+    put(j, x, qq, L, age);
     return;
   }
   const double half = L / 2;
@@ -172,9 +174,10 @@ __global__ void k_div_write(int64_t N, int64_t M, BodyArrays b, const int32_t* _
     pr[k] = x[k] + (l2 / 2) * t[k];
     pl[k] = x[k] - (l1 / 2) * t[k];
   }
-  put(j, pl, ql, l1);
+  // Particle::divide: the parent's age grows by one and the new cell copies it
+  put(j, pl, ql, l1, age + 1.0);
   const int64_t jr = c.noff + c.N + (off[i] - c.rank0);
-  put(jr, pr, qr, l2);
+  put(jr, pr, qr, l2, age + 1.0);
 }
 
 __global__ void k_div_colony_keys(int64_t M, const int32_t* __restrict__ perm,

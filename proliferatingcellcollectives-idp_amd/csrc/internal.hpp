@@ -92,8 +92,9 @@ struct DArr {
 // Body state, FP64 SoA, gid order.
 struct BodyArrays {
   double *px, *py, *pz, *qw, *qx, *qy, *qz, *len, *l0, *diam, *ldot;
+  double* age;  // generation, an integer held exactly
 };
-static constexpr int kBodyFields = 11;
+static constexpr int kBodyFields = 12;
 static constexpr int kBodyVariants = PCC_BODY_VARIANTS;
 int body_chunk_of(int variant, bool planar);
 int body_threads_of(int variant);

@@ -112,6 +112,7 @@ class BodiesC(C.Structure):
         ("impedance", _dp),
         ("stress", _dp),
         ("num_constraints", _ip),
+        ("age", _ip),
     ]
 
 
@@ -172,11 +173,12 @@ def _ptr(a, ct):
 
 
 class Bodies:
-    """Host-side body state in gid order (numpy, FP64)."""
+    """Host-side body state in gid order (numpy, FP64; age, the generation of
+    ParticleData.h:16, int32: 1 for a new cell)."""
 
-    FIELDS_IN = ("pos", "quat", "length", "l0", "diameter", "ldot")
+    FIELDS_IN = ("pos", "quat", "length", "l0", "diameter", "ldot", "age")
 
-    def __init__(self, pos, quat, length, l0=None, diameter=None, ldot=None):
+    def __init__(self, pos, quat, length, l0=None, diameter=None, ldot=None, age=None):
         n = len(length)
         self.pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(n, 3)
         self.quat = np.ascontiguousarray(quat, dtype=np.float64).reshape(n, 4)
@@ -192,6 +194,8 @@ class Bodies:
         self.impedance = np.zeros(n)
         self.stress = np.zeros(n)
         self.num_constraints = np.zeros(n, dtype=np.int32)
+        self.age = np.ascontiguousarray(np.ones(n) if age is None else age,
+                                        dtype=np.int32).reshape(n)
 
     @property
     def n(self):
@@ -199,7 +203,7 @@ class Bodies:
 
     def copy(self):
         b = Bodies(self.pos.copy(), self.quat.copy(), self.length.copy(), self.l0.copy(),
-                   self.diameter.copy(), self.ldot.copy())
+                   self.diameter.copy(), self.ldot.copy(), self.age.copy())
         for f in ("force", "torque", "velocity", "omega", "impedance", "stress",
                   "num_constraints"):
             setattr(b, f, getattr(self, f).copy())
@@ -212,12 +216,13 @@ class Bodies:
                   "velocity", "omega", "impedance", "stress"):
             setattr(s, f, _ptr(getattr(self, f), C.c_double))
         s.num_constraints = _ptr(self.num_constraints, C.c_int32)
+        s.age = _ptr(self.age, C.c_int32)
         return s
 
     def subset(self, idx):
         idx = np.asarray(idx)
         return Bodies(self.pos[idx], self.quat[idx], self.length[idx], self.l0[idx],
-                      self.diameter[idx], self.ldot[idx])
+                      self.diameter[idx], self.ldot[idx], self.age[idx])
 
 
 class Constraints:

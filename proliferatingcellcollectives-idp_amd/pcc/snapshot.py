@@ -68,7 +68,10 @@ CHECKPOINT_FIELDS = (
     ("impedance", "<f8", 1),
     ("stress", "<f8", 1),
     ("num_constraints", "<i4", 1),
+    ("age", "<i4", 1),
 )
+# sections a checkpoint may lack (written before they existed): their default
+OPTIONAL_FIELDS = {"age": 1}
 
 
 def params_to_dict(p: T.Params) -> dict:
@@ -172,6 +175,8 @@ def read_checkpoint_header(path: str) -> dict:
         raise ValueError(f"{path}: colony sizes do not add up to n_bodies")
     for name, dt, comp in CHECKPOINT_FIELDS:
         s = header["fields"].get(name)
+        if s is None and name in OPTIONAL_FIELDS:
+            continue
         if s is None:
             raise ValueError(f"{path}: missing section {name}")
         if s["dtype"] != dt or s["shape"] != ([n, comp] if comp > 1 else [n]):
@@ -188,7 +193,10 @@ def load_checkpoint(path: str, mmap: bool = False) -> Checkpoint:
     n = h["n_bodies"]
     arrays = {}
     for name, dt, comp in CHECKPOINT_FIELDS:
-        s = h["fields"][name]
+        s = h["fields"].get(name)
+        if s is None:  # an optional section this file predates
+            arrays[name] = np.full(n, OPTIONAL_FIELDS[name], dtype=dt)
+            continue
         shape = tuple(s["shape"])
         if mmap:
             arrays[name] = np.memmap(path, dtype=dt, mode="r", offset=s["offset"], shape=shape)
@@ -201,7 +209,8 @@ def load_checkpoint(path: str, mmap: bool = False) -> Checkpoint:
     for sz in h["colony_sizes"]:
         sl = slice(o, o + sz)
         b = T.Bodies(arrays["pos"][sl], arrays["quat"][sl], arrays["length"][sl],
-                     arrays["l0"][sl], arrays["diameter"][sl], arrays["ldot"][sl])
+                     arrays["l0"][sl], arrays["diameter"][sl], arrays["ldot"][sl],
+                     arrays["age"][sl])
         for f_ in ("force", "torque", "velocity", "omega", "impedance", "stress",
                    "num_constraints"):
             setattr(b, f_, np.ascontiguousarray(arrays[f_][sl]))
@@ -486,8 +495,8 @@ def direction_vectors(quat: np.ndarray) -> np.ndarray:
 
 class ParticleLogger:
     """ParticleLogger::log (ParticleLogger.cpp:20-77): one VTU point per cell
-    with the reference's fourteen arrays.  `age` (steps since the cell's
-    birth) is the caller's, the device does not track it; zeros if None."""
+    with the reference's fourteen arrays.  `age` is the generation
+    (Particle::getAge, ParticleLogger.cpp:51): bodies.age unless given."""
 
     def __init__(self, output_directory: str, base_filename: str = "particles",
                  preserve_existing: bool = False, step: int = 0, precision: str = "reference",
@@ -512,8 +521,7 @@ class ParticleLogger:
         L.add_point_data("ldot", bodies.ldot)
         L.add_point_data("stress", bodies.stress)
         L.add_point_data("number_of_constraints", bodies.num_constraints.astype(np.int32))
-        L.add_point_data("age", np.zeros(n, np.int32) if age is None
-                         else np.asarray(age, dtype=np.int32))
+        L.add_point_data("age", np.asarray(bodies.age if age is None else age, dtype=np.int32))
         L.add_point_data("forces", bodies.force, 3)
         L.add_point_data("velocity_linear", bodies.velocity, 3)
         L.add_point_data("velocity_angular", bodies.omega, 3)

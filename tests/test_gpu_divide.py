@@ -13,7 +13,7 @@ from pcc.abi_types import reference_params
 
 pytestmark = pytest.mark.gpu
 LN2 = math.log(2.0)
-FIELDS = ("pos", "quat", "length", "l0", "diameter", "ldot")
+FIELDS = ("pos", "quat", "length", "l0", "diameter", "ldot", "age")
 
 
 def same_bodies(a, b):
@@ -23,8 +23,11 @@ def same_bodies(a, b):
 
 
 def with_dividers(b, every=3):
+    """Every `every`-th cell at its division length; generations 1..5 so
+    that both the carried and the incremented ages are checked."""
     b = b.copy()
     b.length[::every] = 2 * b.l0[::every]
+    b.age = (1 + np.arange(b.n) % 5).astype(np.int32)
     return b
 
 

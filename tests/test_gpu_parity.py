@@ -19,7 +19,7 @@ from pcc.abi_types import (MODE_SOFT, PCC_E_BBPGD_NOT_CONVERGED, PCC_E_INVALID_M
 pytestmark = pytest.mark.gpu
 LN2 = math.log(2.0)
 STATE_FIELDS = ("pos", "quat", "length", "l0", "diameter", "ldot", "force", "torque",
-                "velocity", "omega", "impedance", "stress", "num_constraints")
+                "velocity", "omega", "impedance", "stress", "num_constraints", "age")
 
 
 def pm_for(params, bodies):

@@ -6,7 +6,10 @@ Domain.cpp:492-508), pinned by closed forms and invariants:
 * each child's in-plane turn equals the rotation by the drawn angle (checked
   against numpy's cos/sin), and the angles have mean 0, sd angle_noise;
 * cells below 2 l0 are untouched; the output is sorted by x;
-* draws depend on (seed, step, parent) only.
+* draws depend on (seed, step, parent) only;
+* age (the generation, Particle.cpp:26,144,148): k rounds of division of a
+  cell of age 1 give 2^k cells of age k + 1; over any sequence of divisions
+  sum 2^-(age-1) is conserved; a cell that does not divide keeps its age.
 """
 import math
 
@@ -106,3 +109,30 @@ def test_draws_depend_on_seed_and_step_only():
     a3 = oracle.divide(b.copy(), seed=3, step=2, length_noise=0.02, angle_noise=0.01)
     assert np.array_equal(a1.pos, a2.pos) and np.array_equal(a1.quat, a2.quat)
     assert not np.array_equal(a1.length, a3.length)
+
+
+def test_synchronous_generations_have_closed_form_ages():
+    b = rods([0.2, 0.9], [2.0, 2.0], x=[0.0, 50.0])
+    assert list(b.age) == [1, 1]
+    for k in range(1, 5):
+        b = oracle.divide(b, seed=3, step=k, length_noise=0.0, angle_noise=0.0)
+        assert b.n == 2 * 2**k
+        assert np.all(b.age == k + 1), k
+        b.length[:] = 2.0  # every cell at its division length again
+
+
+def test_age_weight_is_conserved_and_non_dividers_keep_their_age():
+    rng = np.random.default_rng(4)
+    n = 300
+    b = rods(rng.uniform(0, np.pi, n), rng.uniform(1.0, 2.4, n))
+    b.age = rng.integers(1, 7, n).astype(np.int32)
+    weight = np.sum(2.0 ** -(b.age - 1.0))
+    for k in range(6):
+        divides = b.length >= 2 * b.l0
+        keep = sorted(zip(b.pos[~divides, 0], b.age[~divides]))
+        b = oracle.divide(b, seed=11, step=k, length_noise=0.02, angle_noise=0.01)
+        assert np.sum(2.0 ** -(b.age - 1.0)) == weight, k
+        # cells that did not divide are where they were, with their age
+        by_x = dict(zip(b.pos[:, 0], b.age))
+        assert all(by_x[x] == a for x, a in keep)
+        b.length = b.length * rng.uniform(1.0, 2.2, b.n)

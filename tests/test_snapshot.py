@@ -40,6 +40,7 @@ def random_bodies(n, seed):
     b.impedance = rng.random(n)
     b.stress = rng.normal(size=n) * 1e-7
     b.num_constraints = rng.integers(0, 9, n).astype(np.int32)
+    b.age = rng.integers(1, 40, n).astype(np.int32)
     return b
 
 
@@ -85,6 +86,21 @@ def test_checkpoint_sections_are_the_abi_arrays(tmp_path):
     raw = np.fromfile(path, dtype="<f8", count=50 * 4, offset=s["offset"]).reshape(50, 4)
     assert np.array_equal(raw, np.concatenate([cols[0].quat, cols[1].quat]))
     assert S.load_checkpoint(path).params is None
+
+
+def test_checkpoint_without_age_section_loads_with_age_one(tmp_path, monkeypatch):
+    """A checkpoint written before the age section existed still loads; its
+    cells are of age 1, as after the reference's restart."""
+    cols = [random_bodies(25, 3)]
+    path = str(tmp_path / "old.pccsnap")
+    monkeypatch.setattr(S, "CHECKPOINT_FIELDS",
+                        tuple(f for f in S.CHECKPOINT_FIELDS if f[0] != "age"))
+    S.save_checkpoint(path, cols)
+    monkeypatch.undo()
+    assert "age" not in S.read_checkpoint_header(path)["fields"]
+    got = S.load_checkpoint(path).colonies[0]
+    assert got.age.dtype == np.int32 and np.all(got.age == 1)
+    same(got, cols[0], tuple(f for f, _, _ in S.CHECKPOINT_FIELDS if f != "age"))
 
 
 def test_checkpoint_rejects_damaged_files(tmp_path):
@@ -161,6 +177,8 @@ def test_vtu_layout_and_names_follow_the_reference(tmp_path):
     assert got == PARTICLE_ARRAYS
     for a, (_, k, _) in zip(arrays, PARTICLE_ARRAYS):
         assert len(a.text.split()) == 50 * k
+    ages = next(a for a in arrays if a.attrib["Name"] == "age")
+    assert [int(v) for v in ages.text.split()] == b.age.tolist()
     proot = ET.parse(tmp_path / "particles_0000017.pvtu").getroot()
     pp = [(a.attrib["Name"], int(a.attrib["NumberOfComponents"]), a.attrib["type"])
           for a in proot.findall("PUnstructuredGrid/PPointData/PDataArray")]
@@ -197,8 +215,10 @@ def test_lossless_round_trip_and_reference_loader_defaults(tmp_path):
     got = st.bodies
     for f in ("pos", "quat", "length", "force", "velocity", "omega"):
         assert np.array_equal(getattr(got, f), getattr(b, f)), f
-    # VTKStateLoader.cpp:131-136 and :139-160: l0 = 1, d = 0.5, no torque
+    # VTKStateLoader.cpp:131-136 and :139-160: l0 = 1, d = 0.5, no torque, and
+    # the Particle constructor's age 1 (Particle.cpp:26)
     assert np.all(got.l0 == 1.0) and np.all(got.diameter == 0.5) and np.all(got.ldot == 0.0)
+    assert np.all(got.age == 1) and np.any(b.age != 1)
     assert np.all(got.torque == 0.0)
     assert np.array_equal(st.gids, np.arange(500))
 
