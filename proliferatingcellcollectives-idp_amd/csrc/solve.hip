@@ -905,23 +905,60 @@ __global__ void __launch_bounds__(256) k_energy_body(BodyArgs a, double* __restr
 // PCC_BLOCK threads per colony.
 __global__ void __launch_bounds__(256) k_finish(ConstrArgs a) {
   __shared__ double lds[4];
+  __shared__ double l4[4][4];
   SolverState* st = a.st + blockIdx.x;
   if (st->skip | st->converged | st->failed) return;
   const InstDev& D = a.idev[blockIdx.x];
   const int64_t k = st->iter;
   const int64_t G = D.G, o = D.lb, T = a.Gtot;
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, am = 0.0;
-  for (int64_t i = threadIdx.x; i < G; i += PCC_BLOCK) {
-    a0 += a.partials[o + i];
-    a1 += a.partials[T + o + i];
-    a2 += a.partials[2 * T + o + i];
-    const double m = a.partials[3 * T + o + i];
-    am = am < m ? m : am;
+  // Thread t adds partials t, t + 256, ... (G <= kBbGridMax, so at most KP of
+  // them) in ascending order.  All of its loads are issued first, at an
+  // in-range index for the slots past G, and every sum is formed and then
+  // selected (a slot past G leaves it unchanged), so that no load waits
+  // behind a branch and they all overlap.
+  constexpr int KP = kBbGridMax / PCC_BLOCK;
+  double p0[KP], p1[KP], p2[KP], pm[KP];
+#pragma unroll
+  for (int j = 0; j < KP; ++j) {
+    const int64_t i = threadIdx.x + j * PCC_BLOCK;
+    const int64_t c = o + (i < G ? i : 0);
+    p0[j] = a.partials[c];
+    p1[j] = a.partials[T + c];
+    p2[j] = a.partials[2 * T + c];
+    pm[j] = a.partials[3 * T + c];
   }
-  const double S0 = block_sum(a0, lds);
-  const double S1 = block_sum(a1, lds);
-  const double S2 = block_sum(a2, lds);
-  const double R = block_max(am, lds);
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, am = 0.0;
+#pragma unroll
+  for (int j = 0; j < KP; ++j) {
+    const bool in = threadIdx.x + j * PCC_BLOCK < G;
+    const double s0 = a0 + p0[j], s1 = a1 + p1[j], s2 = a2 + p2[j];
+    const double sm = am < pm[j] ? pm[j] : am;
+    a0 = in ? s0 : a0;
+    a1 = in ? s1 : a1;
+    a2 = in ? s2 : a2;
+    am = in ? sm : am;
+  }
+  // block_sum x 3 and block_max with one LDS exchange: the same wave
+  // butterflies and the same ((w0 + w1) + w2) + w3 / max chain over waves
+  {
+    const double w0 = wave_sum(a0), w1 = wave_sum(a1), w2 = wave_sum(a2), wm = wave_max(am);
+    if ((threadIdx.x & (PCC_WAVE - 1)) == 0) {
+      const int w = threadIdx.x >> 6;
+      l4[0][w] = w0;
+      l4[1][w] = w1;
+      l4[2][w] = w2;
+      l4[3][w] = wm;
+    }
+    __syncthreads();
+  }
+  double S0 = 0.0, S1 = 0.0, S2 = 0.0, R = 0.0;
+  if (threadIdx.x == 0) {
+    S0 = ((l4[0][0] + l4[0][1]) + l4[0][2]) + l4[0][3];
+    S1 = ((l4[1][0] + l4[1][1]) + l4[1][2]) + l4[1][3];
+    S2 = ((l4[2][0] + l4[2][1]) + l4[2][2]) + l4[2][3];
+    R = l4[3][0];
+    for (int i = 1; i < 4; ++i) R = R < l4[3][i] ? l4[3][i] : R;
+  }
   // BBPGD trace record of this iterate (BBPGD.cpp:57-60,76-80): energy terms
   // from the gamma.Phi block partials and the colony's body-block partials,
   // each combined in the fixed strided order above
