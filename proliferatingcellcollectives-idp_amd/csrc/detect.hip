@@ -15,9 +15,12 @@
 //                  of that cell with the reference's filters and exact FP64
 //                  segment-segment DCP; 27 neighbour cells, or the 9 of the
 //                  body's own z-layer when every body has z = 0 (the other
-//                  cells are then empty); pass 0 counts, pass 1 writes at the
-//                  scanned offset, so each body's records keep the cell order
-//                  dz, dy, dx
+//                  cells are then empty); pass 0 counts and keeps, per thread,
+//                  a bit mask of the accepted positions of its bucket; pass 1
+//                  re-evaluates only those (buckets of at most 64 bodies;
+//                  longer ones are walked again) and writes at the scanned
+//                  offset, so each body's records keep the cell order dz, dy,
+//                  dx and the bucket order
 //  5. k_segsort + radix sort: order = (contact x, I, J); with several colonies
 //                  in one context a second stable radix sort by colony makes
 //                  the order (colony, contact x, I, J)
@@ -93,8 +96,8 @@ __global__ void __launch_bounds__(256)
 k_detect(int pass, int64_t N, BodyArrays b, const int32_t* __restrict__ cell,
          const int32_t* __restrict__ bstart, const int32_t* __restrict__ perm, uint32_t mask,
          const int32_t* __restrict__ binst, const InstDev* __restrict__ idev,
-         int32_t* __restrict__ cnt_off, int32_t* __restrict__ outI,
-         int32_t* __restrict__ outJ, double* __restrict__ rec) {
+         int32_t* __restrict__ cnt_off, uint64_t* __restrict__ acc,
+         int32_t* __restrict__ outI, int32_t* __restrict__ outJ, double* __restrict__ rec) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= N * S) return;
   const int64_t I = t / S;
@@ -117,23 +120,24 @@ k_detect(int pass, int64_t N, BodyArrays b, const int32_t* __restrict__ cell,
     P1[k] = xI[k] + hv;
   }
   const int32_t cx = cell[3 * I], cy = cell[3 * I + 1], cz = cell[3 * I + 2];
-  int32_t w = pass ? cnt_off[t] : 0;
+  const int32_t w = pass ? cnt_off[t] : 0;
   int32_t c = 0;
-      {
-        const int32_t nx = cx + dx, ny = cy + dy, nz = cz + dz;
-        const uint32_t bk = (cell_hash(nx, ny, nz) ^ salt) & mask;
-        const int32_t e1 = bstart[bk + 1];
-        for (int32_t e = bstart[bk]; e < e1; ++e) {
+  const int32_t nx = cx + dx, ny = cy + dy, nz = cz + dz;
+  const uint32_t bk = (cell_hash(nx, ny, nz) ^ salt) & mask;
+  const int32_t e0 = bstart[bk], e1 = bstart[bk + 1];
+  // The reference's filters and the record of candidate perm[e]; true if it
+  // is a contact (then, in pass 1, written at slot w + c).
+  auto candidate = [&](int32_t e) -> bool {
           const int32_t J = perm[e];
-          if (J <= I) continue;
+          if (J <= I) return false;
           if (cell[3 * (int64_t)J] != nx || cell[3 * (int64_t)J + 1] != ny ||
               cell[3 * (int64_t)J + 2] != nz || binst[J] != inst)
-            continue;
+            return false;
           const double xJ[3] = {b.px[J], b.py[J], b.pz[J]};
           const double lenJ = b.len[J], dJ = b.diam[J];
           double dd[3] = {xI[0] - xJ[0], xI[1] - xJ[1], xI[2] - xJ[2]};
           double rough = sqrt(dot3(dd, dd));
-          if (rough > (lenI + lenJ) / 2.0 + cutoff) continue;
+          if (rough > (lenI + lenJ) / 2.0 + cutoff) return false;
           double tJ[3];
           direction(b.qw[J], b.qx[J], b.qy[J], b.qz[J], tJ);
           const double halfJ = lenJ / 2 - dJ / 2;
@@ -148,7 +152,7 @@ k_detect(int pass, int64_t N, BodyArrays b, const int32_t* __restrict__ cell,
           dcp(P0, P1, Q0, Q1, o);
           const double sum_radii = (dI + dJ) / 2.0;
           const double phi = o.dist - sum_radii;
-          if (phi > cutoff) continue;
+          if (phi > cutoff) return false;
           if (pass) {
             const int64_t slot = (int64_t)w + c;
             outI[slot] = (int32_t)I;
@@ -174,10 +178,24 @@ k_detect(int pass, int64_t N, BodyArrays b, const int32_t* __restrict__ cell,
             r[R_SJ] = 0.5 * fabs(dot3(mn, tJ));
             r[R_G] = phi >= 0.0 ? 0.0 : -phi * warm;
           }
-          ++c;
-        }
+          return true;
+  };
+  if (pass && e1 - e0 <= 64) {
+    // the contacts pass 0 found, in ascending bucket position
+    for (uint64_t m = acc[t]; m; m &= m - 1)
+      if (candidate(e0 + (int32_t)__builtin_ctzll(m))) ++c;
+  } else {
+    uint64_t bits = 0;
+    for (int32_t e = e0; e < e1; ++e)
+      if (candidate(e)) {
+        if (e - e0 < 64) bits |= 1ull << (e - e0);
+        ++c;
       }
-  if (!pass) cnt_off[t] = c;
+    if (!pass) {
+      cnt_off[t] = c;
+      acc[t] = bits;
+    }
+  }
 }
 
 // Per-body record offsets from the per-(body, neighbour cell) offsets.
@@ -377,17 +395,18 @@ void detect_batch(pcc_ctx* ctx) {
   const int64_t NS = N * S;
   ctx->dslot.ensure(NS + 1);
   ctx->dslot_off.ensure(NS + 1);
+  ctx->dacc.ensure(NS + 1);
   ctx->dcount.ensure(N + 1);
   PCC_HIP(hipMemsetAsync(ctx->dslot.p, 0, sizeof(int32_t) * (size_t)(NS + 1), s));
   auto detect = [&](int pass, int32_t* cnt_off) {
     if (S == 9)
       k_detect<9><<<nblk(NS), 256, 0, s>>>(pass, N, b, ctx->cell.p, ctx->bstart.p, ctx->perm.p,
-                                           mask, ctx->binst.p, ctx->idev.p, cnt_off, ctx->rI_.p,
-                                           ctx->rJ_.p, ctx->rrec.p);
+                                           mask, ctx->binst.p, ctx->idev.p, cnt_off, ctx->dacc.p,
+                                           ctx->rI_.p, ctx->rJ_.p, ctx->rrec.p);
     else
       k_detect<27><<<nblk(NS), 256, 0, s>>>(pass, N, b, ctx->cell.p, ctx->bstart.p, ctx->perm.p,
-                                            mask, ctx->binst.p, ctx->idev.p, cnt_off, ctx->rI_.p,
-                                            ctx->rJ_.p, ctx->rrec.p);
+                                            mask, ctx->binst.p, ctx->idev.p, cnt_off, ctx->dacc.p,
+                                            ctx->rI_.p, ctx->rJ_.p, ctx->rrec.p);
   };
   if (N > 0) detect(0, ctx->dslot.p);
   // per-slot offsets, then per-body offsets into dcount

@@ -230,6 +230,7 @@ struct pcc_ctx {
   // detection batch
   pcc::DArr<int32_t> dcount;   // N+1 (per-body record offsets)
   pcc::DArr<int32_t> dslot, dslot_off;  // per (body, neighbour cell) counts and offsets
+  pcc::DArr<uint64_t> dacc;  // per (body, neighbour cell): accepted bucket positions (pass 0)
   pcc::DArr<int32_t> rI_, rJ_; // raw pair ids
   pcc::DArr<double> rrec;      // raw geometry: 16 doubles per contact
   pcc::DArr<int32_t> ridx, ridx2;
