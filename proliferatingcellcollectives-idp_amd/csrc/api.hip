@@ -1085,7 +1085,7 @@ pcc_status pcc_device_bytes(pcc_ctx* ctx, int64_t* bytes) {
   };
   add(ctx->body); add(ctx->snap); add(ctx->F); add(ctx->U); add(ctx->sig); add(ctx->imp);
   add(ctx->ldot_new); add(ctx->nc); add(ctx->cell); add(ctx->bucket); add(ctx->perm);
-  add(ctx->bstart); add(ctx->bcursor); add(ctx->dcount); add(ctx->dslot); add(ctx->dslot_off); add(ctx->dacc); add(ctx->rI_); add(ctx->rJ_);
+  add(ctx->bstart); add(ctx->bcursor); add(ctx->dcount); add(ctx->dslot); add(ctx->dslot_off); add(ctx->dacc); add(ctx->dgeo); add(ctx->rI_); add(ctx->rJ_);
   add(ctx->rrec); add(ctx->ridx); add(ctx->ridx2); add(ctx->rkey); add(ctx->rkey2);
   add(ctx->sort_tmp); add_set(ctx->S); add_set(ctx->S2); add(ctx->rowptr); add(ctx->inccur);
   add(ctx->inc); add(ctx->increc); add(ctx->increc32); add(ctx->incidx); add(ctx->gold_e); add(ctx->fold); add(ctx->plist);

@@ -41,20 +41,28 @@ namespace pcc {
 
 namespace {
 
+// Per body: its cell and colony (cell4), and the detection geometry record
+// geo = (x, y, z, t_x | t_y, t_z, length, diameter) with t = rotate(q, e_x), so
+// that the detection kernel reads a candidate from two cache lines.
 __global__ void k_cells(int64_t N, BodyArrays b, const int32_t* __restrict__ binst,
                         const InstDev* __restrict__ idev, uint32_t mask,
-                        int32_t* __restrict__ cell, uint32_t* __restrict__ bucket,
-                        int32_t* __restrict__ count) {
+                        int4* __restrict__ cell4, double2* __restrict__ geo,
+                        uint32_t* __restrict__ bucket, int32_t* __restrict__ count) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const int32_t inst = binst[i];
   const double origin = idev[inst].origin, h = idev[inst].h;
-  int32_t cx = cell_coord(b.px[i], origin, h);
-  int32_t cy = cell_coord(b.py[i], origin, h);
-  int32_t cz = cell_coord(b.pz[i], origin, h);
-  cell[3 * i] = cx;
-  cell[3 * i + 1] = cy;
-  cell[3 * i + 2] = cz;
+  const double x = b.px[i], y = b.py[i], z = b.pz[i];
+  int32_t cx = cell_coord(x, origin, h);
+  int32_t cy = cell_coord(y, origin, h);
+  int32_t cz = cell_coord(z, origin, h);
+  cell4[i] = make_int4(cx, cy, cz, inst);
+  double t[3];
+  direction(b.qw[i], b.qx[i], b.qy[i], b.qz[i], t);
+  geo[4 * i] = make_double2(x, y);
+  geo[4 * i + 1] = make_double2(z, t[0]);
+  geo[4 * i + 2] = make_double2(t[1], t[2]);
+  geo[4 * i + 3] = make_double2(b.len[i], b.diam[i]);
   uint32_t bk = (cell_hash(cx, cy, cz) ^ inst_mix(inst)) & mask;
   bucket[i] = bk;
   atomicAdd(&count[bk], 1);
@@ -93,24 +101,25 @@ enum { R_PHI = 0, R_N = 1, R_RI = 4, R_RJ = 7, R_P = 10, R_SI = 13, R_SJ = 14, R
 
 template <int S>  // neighbour cells per body: 9 (one z-layer) or 27
 __global__ void __launch_bounds__(256)
-k_detect(int pass, int64_t N, BodyArrays b, const int32_t* __restrict__ cell,
+k_detect(int pass, int64_t N, const int4* __restrict__ cell4, const double2* __restrict__ geo,
          const int32_t* __restrict__ bstart, const int32_t* __restrict__ perm, uint32_t mask,
-         const int32_t* __restrict__ binst, const InstDev* __restrict__ idev,
-         int32_t* __restrict__ cnt_off, uint64_t* __restrict__ acc,
-         int32_t* __restrict__ outI, int32_t* __restrict__ outJ, double* __restrict__ rec) {
+         const InstDev* __restrict__ idev, int32_t* __restrict__ cnt_off,
+         uint64_t* __restrict__ acc, int32_t* __restrict__ outI, int32_t* __restrict__ outJ,
+         double* __restrict__ rec) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= N * S) return;
   const int64_t I = t / S;
   const int sl = (int)(t - I * S);
   const int dx = sl % 3 - 1, dy = (sl / 3) % 3 - 1, dz = S == 27 ? sl / 9 - 1 : 0;
-  const int32_t inst = binst[I];
+  const int4 cI = cell4[I];
+  const int32_t inst = cI.w;
   if (!(idev[inst].flags & kInstActive)) return;  // count stays 0
   const double cutoff = idev[inst].cutoff, warm = idev[inst].warm;
   const uint32_t salt = inst_mix(inst);
-  const double xI[3] = {b.px[I], b.py[I], b.pz[I]};
-  const double lenI = b.len[I], dI = b.diam[I];
-  double tI[3];
-  direction(b.qw[I], b.qx[I], b.qy[I], b.qz[I], tI);
+  const double2 gI0 = geo[4 * I], gI1 = geo[4 * I + 1], gI2 = geo[4 * I + 2], gI3 = geo[4 * I + 3];
+  const double xI[3] = {gI0.x, gI0.y, gI1.x};
+  const double tI[3] = {gI1.y, gI2.x, gI2.y};
+  const double lenI = gI3.x, dI = gI3.y;
   const double halfI = lenI / 2 - dI / 2;
   double P0[3], P1[3];
 #pragma unroll
@@ -119,7 +128,7 @@ k_detect(int pass, int64_t N, BodyArrays b, const int32_t* __restrict__ cell,
     P0[k] = xI[k] - hv;
     P1[k] = xI[k] + hv;
   }
-  const int32_t cx = cell[3 * I], cy = cell[3 * I + 1], cz = cell[3 * I + 2];
+  const int32_t cx = cI.x, cy = cI.y, cz = cI.z;
   const int32_t w = pass ? cnt_off[t] : 0;
   int32_t c = 0;
   const int32_t nx = cx + dx, ny = cy + dy, nz = cz + dz;
@@ -130,16 +139,16 @@ k_detect(int pass, int64_t N, BodyArrays b, const int32_t* __restrict__ cell,
   auto candidate = [&](int32_t e) -> bool {
           const int32_t J = perm[e];
           if (J <= I) return false;
-          if (cell[3 * (int64_t)J] != nx || cell[3 * (int64_t)J + 1] != ny ||
-              cell[3 * (int64_t)J + 2] != nz || binst[J] != inst)
-            return false;
-          const double xJ[3] = {b.px[J], b.py[J], b.pz[J]};
-          const double lenJ = b.len[J], dJ = b.diam[J];
+          const int4 cJ = cell4[J];
+          if (cJ.x != nx || cJ.y != ny || cJ.z != nz || cJ.w != inst) return false;
+          const double2 g0 = geo[4 * (int64_t)J], g1 = geo[4 * (int64_t)J + 1];
+          const double2 g2 = geo[4 * (int64_t)J + 2], g3 = geo[4 * (int64_t)J + 3];
+          const double xJ[3] = {g0.x, g0.y, g1.x};
+          const double lenJ = g3.x, dJ = g3.y;
           double dd[3] = {xI[0] - xJ[0], xI[1] - xJ[1], xI[2] - xJ[2]};
           double rough = sqrt(dot3(dd, dd));
           if (rough > (lenI + lenJ) / 2.0 + cutoff) return false;
-          double tJ[3];
-          direction(b.qw[J], b.qx[J], b.qy[J], b.qz[J], tJ);
+          const double tJ[3] = {g1.y, g2.x, g2.y};
           const double halfJ = lenJ / 2 - dJ / 2;
           double Q0[3], Q1[3];
 #pragma unroll
@@ -359,7 +368,8 @@ void build_grid(pcc_ctx* ctx) {
   const int64_t N = ctx->N;
   const int64_t T = table_size_for(N);
   ctx->table_size = T;
-  ctx->cell.ensure(3 * N + 3);
+  ctx->cell.ensure(4 * N + 4);
+  ctx->dgeo.ensure(8 * N + 8);
   ctx->bucket.ensure(N + 1);
   ctx->perm.ensure(N + 1);
   ctx->bstart.ensure(T + 1);
@@ -369,7 +379,9 @@ void build_grid(pcc_ctx* ctx) {
   PCC_HIP(hipMemsetAsync(ctx->bcursor.p, 0, sizeof(int32_t) * (size_t)(T + 1), s));
   if (N > 0)
     k_cells<<<nblk(N), 256, 0, s>>>(N, b, ctx->binst.p, ctx->idev.p, (uint32_t)(T - 1),
-                                    ctx->cell.p, ctx->bucket.p, ctx->bcursor.p);
+                                    reinterpret_cast<int4*>(ctx->cell.p),
+                                    reinterpret_cast<double2*>(ctx->dgeo.p), ctx->bucket.p,
+                                    ctx->bcursor.p);
   excl_scan<int32_t>(ctx, ctx->bcursor.p, ctx->bstart.p, T + 1);
   PCC_HIP(hipMemsetAsync(ctx->bcursor.p, 0, sizeof(int32_t) * (size_t)(T + 1), s));
   if (N > 0) {
@@ -387,7 +399,6 @@ void detect_batch(pcc_ctx* ctx) {
   hipStream_t s = ctx->stream;
   KTimer tm(ctx, PCC_K_DETECT, 0.0);
   build_grid(ctx);
-  BodyArrays b = body_arrays(ctx->body.p, N);
   const uint32_t mask = (uint32_t)(ctx->table_size - 1);
   // one slot per (body, neighbour cell); 9 neighbour cells when every body
   // lies in the z = 0 layer
@@ -398,15 +409,17 @@ void detect_batch(pcc_ctx* ctx) {
   ctx->dacc.ensure(NS + 1);
   ctx->dcount.ensure(N + 1);
   PCC_HIP(hipMemsetAsync(ctx->dslot.p, 0, sizeof(int32_t) * (size_t)(NS + 1), s));
+  const int4* c4 = reinterpret_cast<const int4*>(ctx->cell.p);
+  const double2* geo = reinterpret_cast<const double2*>(ctx->dgeo.p);
   auto detect = [&](int pass, int32_t* cnt_off) {
     if (S == 9)
-      k_detect<9><<<nblk(NS), 256, 0, s>>>(pass, N, b, ctx->cell.p, ctx->bstart.p, ctx->perm.p,
-                                           mask, ctx->binst.p, ctx->idev.p, cnt_off, ctx->dacc.p,
-                                           ctx->rI_.p, ctx->rJ_.p, ctx->rrec.p);
+      k_detect<9><<<nblk(NS), 256, 0, s>>>(pass, N, c4, geo, ctx->bstart.p, ctx->perm.p, mask,
+                                           ctx->idev.p, cnt_off, ctx->dacc.p, ctx->rI_.p,
+                                           ctx->rJ_.p, ctx->rrec.p);
     else
-      k_detect<27><<<nblk(NS), 256, 0, s>>>(pass, N, b, ctx->cell.p, ctx->bstart.p, ctx->perm.p,
-                                            mask, ctx->binst.p, ctx->idev.p, cnt_off, ctx->dacc.p,
-                                            ctx->rI_.p, ctx->rJ_.p, ctx->rrec.p);
+      k_detect<27><<<nblk(NS), 256, 0, s>>>(pass, N, c4, geo, ctx->bstart.p, ctx->perm.p, mask,
+                                            ctx->idev.p, cnt_off, ctx->dacc.p, ctx->rI_.p,
+                                            ctx->rJ_.p, ctx->rrec.p);
   };
   if (N > 0) detect(0, ctx->dslot.p);
   // per-slot offsets, then per-body offsets into dcount

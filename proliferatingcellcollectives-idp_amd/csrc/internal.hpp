@@ -220,7 +220,7 @@ struct pcc_ctx {
   pcc::DArr<int32_t> nc;                        // N
 
   // grid
-  pcc::DArr<int32_t> cell;     // 3N
+  pcc::DArr<int32_t> cell;     // 4N: cell x, y, z and colony per body
   pcc::DArr<uint32_t> bucket;  // N
   pcc::DArr<int32_t> perm;     // N
   pcc::DArr<int32_t> bstart;   // T+1
@@ -231,6 +231,7 @@ struct pcc_ctx {
   pcc::DArr<int32_t> dcount;   // N+1 (per-body record offsets)
   pcc::DArr<int32_t> dslot, dslot_off;  // per (body, neighbour cell) counts and offsets
   pcc::DArr<uint64_t> dacc;  // per (body, neighbour cell): accepted bucket positions (pass 0)
+  pcc::DArr<double> dgeo;    // per body: x, y, z, t (3), length, diameter (detection)
   pcc::DArr<int32_t> rI_, rJ_; // raw pair ids
   pcc::DArr<double> rrec;      // raw geometry: 16 doubles per contact
   pcc::DArr<int32_t> ridx, ridx2;
