@@ -275,6 +275,7 @@ def main_ensemble(args):
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": rf["gbs"] / HBM_PEAK_GBS,
                      "traffic": None, "avg_launch_us": rf["avg_us"],
                      "algorithmic_bytes_per_launch": rf["bytes"]},
+        "tuning": r["tuning"],
     }
     print(json.dumps(line))
 
