@@ -65,6 +65,30 @@ def test_detect_bitexact(name, make, cutoff):
     pm.close()
 
 
+def test_detect_bitexact_with_crowded_cells():
+    """Grid cells of size 12 hold up to ~200 bodies of a dense packing: the
+    writing pass walks buckets of more than 64 bodies again and uses the
+    counting pass's mask of accepted positions for the others.  The list is
+    bit-exact against the oracle on the same grid, and equal to the list of
+    the default grid (the contacts do not depend on the cell size)."""
+    b = G.dense_colony(3000, seed=17)
+    h = 12.0
+    g = oracle.grid(b, h=h)
+    counts = np.bincount(g["bucket"])
+    assert counts.max() > 64 and ((counts > 0) & (counts <= 64)).any()
+    print(f"[detect crowded] n={b.n} h={h} largest bucket {counts.max()}", flush=True)
+    pm = pm_for(reference_params(linked_cell_size=h), b)
+    got, bucket, perm = pm.detect(0.05)
+    want = oracle.detect(b, 0.05, h=h)
+    assert np.array_equal(bucket, g["bucket"]) and np.array_equal(perm, g["perm"])
+    default = oracle.detect(b, 0.05)
+    assert len(got["gidI"]) == len(want["gidI"]) > b.n
+    for k in want:
+        assert np.array_equal(got[k], want[k]), k
+        assert np.array_equal(got[k], default[k]), k
+    pm.close()
+
+
 def assert_same_state(a, b, rtol=0.0):
     for f in STATE_FIELDS:
         x, y = getattr(a, f), getattr(b, f)
