@@ -55,6 +55,19 @@ def test_grid_detection_equals_brute_force_3d():
         assert np.array_equal(a[k], c[k]), k
 
 
+@pytest.mark.parametrize("h", [2.2, 5.0, 12.0])
+def test_grid_detection_does_not_depend_on_cell_size(h):
+    """Any cell size at least the largest contact reach gives the same list
+    (records and order) as brute force: crowded cells (h = 12 puts ~200
+    bodies in a cell) only add candidates that the filters reject."""
+    b = G.dense_colony(800, seed=21)
+    a = oracle.detect(b, 0.05, h=h)
+    c = oracle.detect(b, 0.05, brute=True)
+    assert len(a["gidI"]) > b.n
+    for k in a:
+        assert np.array_equal(a[k], c[k]), (h, k)
+
+
 def test_records_closed_forms_and_order():
     b = G.random_disc(1000, seed=4)
     r = oracle.detect(b, 0.05)
