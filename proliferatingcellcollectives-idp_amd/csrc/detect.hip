@@ -422,15 +422,20 @@ void detect_batch(pcc_ctx* ctx) {
                                             ctx->rJ_.p, ctx->rrec.p);
   };
   if (N > 0) detect(0, ctx->dslot.p);
-  // per-slot offsets, then per-body offsets into dcount
+  // per-slot offsets, then per-body offsets into dcount, then the colony
+  // boundaries of the batch (bodies are stored colony after colony, and a
+  // stable sort by colony keeps them); bounds[B] is the batch's record count,
+  // so one read-back gives both
   excl_scan<int32_t>(ctx, ctx->dslot.p, ctx->dslot_off.p, NS + 1);
-  int32_t total = 0;
-  PCC_HIP(hipMemcpyAsync(&total, ctx->dslot_off.p + NS, sizeof(int32_t), hipMemcpyDeviceToHost,
-                         s));
-  PCC_HIP(hipStreamSynchronize(s));
-  const int64_t m = total;
-  ctx->batch_count = m;
   k_body_offsets<<<nblk(N + 1), 256, 0, s>>>(N, S, ctx->dslot_off.p, ctx->dcount.p);
+  ctx->bbounds.ensure(B + 1);
+  k_batch_bounds<<<nblk(B + 1), 256, 0, s>>>(B, N, ctx->idev.p, ctx->dcount.p, ctx->bbounds.p);
+  std::vector<int32_t> bounds(B + 1);
+  PCC_HIP(hipMemcpyAsync(bounds.data(), ctx->bbounds.p, sizeof(int32_t) * (B + 1),
+                         hipMemcpyDeviceToHost, s));
+  PCC_HIP(hipStreamSynchronize(s));
+  const int64_t m = bounds[B];
+  ctx->batch_count = m;
   ctx->rI_.ensure(m + 1);
   ctx->rJ_.ensure(m + 1);
   ctx->rrec.ensure(R_LEN * (m + 1));
@@ -438,13 +443,6 @@ void detect_batch(pcc_ctx* ctx) {
   ctx->ridx2.ensure(m + 1);
   ctx->rkey.ensure(m + 1);
   ctx->rkey2.ensure(m + 1);
-  // colony boundaries of the batch (a stable sort by colony keeps them)
-  ctx->bbounds.ensure(B + 1);
-  k_batch_bounds<<<nblk(B + 1), 256, 0, s>>>(B, N, ctx->idev.p, ctx->dcount.p, ctx->bbounds.p);
-  std::vector<int32_t> bounds(B + 1);
-  PCC_HIP(hipMemcpyAsync(bounds.data(), ctx->bbounds.p, sizeof(int32_t) * (B + 1),
-                         hipMemcpyDeviceToHost, s));
-  PCC_HIP(hipStreamSynchronize(s));
   for (int64_t q = 0; q < B; ++q) {
     ctx->inst[q].bstart = bounds[q];
     ctx->inst[q].m = bounds[q + 1] - bounds[q];
