@@ -36,7 +36,8 @@ HBM budget; p50/p99 are over every request's per-step latencies.  The hard
 colonies are relaxed by one untimed device step before the run, as growing
 colonies would be (--no-relax submits the raw packings).
 
-Prints ONE JSON line.  --gpus N > 1 exits: this build runs on one GPU.
+Prints ONE JSON line.  --gpus N > 1 exits: the library drives one GPU per
+process and the benchmark measures one GPU.
 """
 import argparse
 import json
@@ -554,7 +555,7 @@ def main():
     if args.steps is None:
         args.steps = 3
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported",
+        print(f"bench.py: the benchmark measures one GPU; --gpus {args.gpus} is not supported",
               file=sys.stderr)
         sys.exit(2)
     if args.workload == "ensemble":
