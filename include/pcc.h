@@ -46,7 +46,7 @@ typedef enum pcc_status {
   PCC_E_INVALID_ARGUMENT = 4,
   PCC_E_DEVICE = 5,              /* a HIP runtime call failed */
   PCC_E_OUT_OF_MEMORY = 6,
-  PCC_E_UNSUPPORTED = 7,         /* e.g. temperature > 0 (Brownian input) */
+  PCC_E_UNSUPPORTED = 7,         /* a request the library does not serve */
   PCC_E_NO_STATE = 8             /* pcc_step before pcc_set_state */
 } pcc_status;
 
@@ -72,7 +72,7 @@ typedef struct pcc_params {
   double xi;         /* drag coefficient (-xi) */
   double tau;        /* growth time constant (-tau) */
   double lambda;     /* stress sensitivity (-lambda) */
-  double temperature;/* Brownian temperature; only 0 is supported */
+  double temperature;/* Brownian temperature (-temperature): see brownian_seed */
   double kcc;        /* soft-model collision constant (-kcc) */
   double bbpgd_tol;      /* BBPGD projected-gradient residual bound */
   double relcp_tol;      /* ReLCP stops once max overlap <= relcp_tol (k > 0) */
@@ -83,6 +83,24 @@ typedef struct pcc_params {
   double linked_cell_size;          /* -linked_cell_size (default 2.2*l0) */
   double grid_origin;               /* -1 on every axis (CollisionDetector.cpp:24) */
   double warm_start_factor;         /* gamma0 = -factor*phi for phi < 0 (500, CollisionDetector.cpp:308) */
+  /* Brownian input (calculate_external_velocities, Physics.cpp:59-81, and
+   * Particle::calculateBrownianVelocity, Particle.cpp:174-190).  With
+   * temperature T > 0, ReLCP phase 0 of a hard step and every soft step add
+   * to each body's angular velocity
+   *   omega_z = ((T u1) u_z) iota iota iota,
+   *   omega_x = ((T u1) u_x) iota^3, omega_y = ((T u1) u_y) iota^3 unless
+   *   monolayer,
+   * where iota is the body's impedance from its last step (0 for a body that
+   * has not been stepped, as Particle.cpp:25,34 initialise it) and the u are
+   * uniform on [-1, 1), drawn in the reference's order (u1, then u_x, u_y
+   * unless monolayer, then u_z).  The reference draws them from one unseeded
+   * global std::mt19937 (Physics.cpp:19); here they come from a counter-based
+   * generator keyed by (brownian_seed, the step index `iter` of the step call,
+   * the body's index within its colony, draw number), so a run is
+   * reproducible and a colony's draws do not depend on its batch mates. */
+  int32_t monolayer;                /* -monolayer (default true, CLIParser.h:183) */
+  int32_t reserved0;
+  uint64_t brownian_seed;
 } pcc_params;
 
 /* Host-side body state, gid order.  Vectors are interleaved per body
@@ -100,7 +118,10 @@ typedef struct pcc_bodies {
   double* torque;     /* 3n (out) */
   double* velocity;   /* 3n (out) */
   double* omega;      /* 3n (out) */
-  double* impedance;  /* n (out) */
+  double* impedance;  /* n: exp(-lambda stress) of the last step.  Read by
+                       * pcc_set_state as the impedance the Brownian input of
+                       * the next step uses (NULL: 0, a new body's value) and
+                       * written by pcc_get_state. */
   double* stress;     /* n (out) */
   int32_t* num_constraints; /* n (out) */
   int32_t* age;       /* n: generation (ParticleData::age, ParticleData.h:16): 1 for a new cell,
@@ -159,7 +180,8 @@ typedef struct pcc_bbpgd_record {
 typedef struct pcc_ctx pcc_ctx;
 
 /* Reference defaults (code/cpp/src/util/CLIParser.h:167-194,232) with the
- * tolerance coupling of Q8, for mode hard, FP64, l0 = 1; temperature 0. */
+ * tolerance coupling of Q8, for mode hard, FP64, l0 = 1: temperature 1e-28,
+ * monolayer on, brownian_seed 0. */
 void pcc_default_params(pcc_params* p);
 
 pcc_status pcc_create(const pcc_params* params, int device, pcc_ctx** out);
@@ -264,12 +286,16 @@ pcc_status pcc_batch_divide(pcc_ctx* ctx, const uint64_t* seeds, int32_t step,
 /* Autotuner state.  The first BBPGD solve with at least 1e5 constraints
  * times every variant of the body kernel on its own operators and keeps the
  * fastest for the context (PCC_AUTOTUNE=0 in the environment keeps the
- * default, variant 1).  Variants v = 0..5: staging size (incidences per LDS
+ * default, variant 1).  The choice is also kept for the process, per (device
+ * architecture, planar, precision, power-of-two bucket of the bodies, bucket
+ * of 4 C / N): a later context whose first such solve falls in a known class
+ * takes it without timing.  Variants v = 0..5: staging size (incidences per LDS
  * pass) 512/1024/2048 planar or 256/512/1024 3D for v % 3, threads per block
  * 256 (v < 3) or 128 (two bodies per thread); v = 6..8: 256 threads per
  * block, staging 256/512/1024, the next pass's incidence records loaded while
  * the current pass is summed.  The sums depend on none of these.  *variant,
- * *body_chunk, *body_threads: the variant in use; *tuned: 1 once tuned;
+ * *body_chunk, *body_threads: the variant in use; *tuned: 1 once timed in
+ * this context, 2 when taken from the process's memory, 0 before;
  * variant_us[PCC_BODY_VARIANTS] (may be NULL): the measured launch times.
  * Any output pointer may be NULL.  No reference counterpart. */
 #define PCC_BODY_VARIANTS 9
@@ -281,12 +307,13 @@ pcc_status pcc_get_tuning(pcc_ctx* ctx, int32_t* variant, int32_t* body_chunk,
  * last step (mean of the two middle values for an even count), then
  * dt' = clamp(dt (1 - alpha) + alpha cfl_factor tolerance / median,
  * 0.9 dt, 1.1 dt); dt is kept when the median is 0.  The reference uses
- * cfl_factor 0.5 (-cfl_factor), alpha 0.01 and its single `tolerance`.
- * pcc_adapt_dt updates the context's dt; pcc_batch_adapt_dt every colony's,
- * with the colony's relcp_tol as the tolerance.  *dt / dts[q] receive the new
- * step, *median / medians[q] the median speed (either may be NULL). */
-pcc_status pcc_adapt_dt(pcc_ctx* ctx, double cfl_factor, double tolerance, double alpha,
-                        double* dt, double* median);
+ * cfl_factor 0.5 (-cfl_factor), alpha 0.01 and its single `tolerance`
+ * (Domain.cpp:106), which is the colony's relcp_tol here in both entry
+ * points.  pcc_adapt_dt updates the context's dt; pcc_batch_adapt_dt every
+ * colony's.  *dt / dts[q] receive the new step, *median / medians[q] the
+ * median speed (either may be NULL). */
+pcc_status pcc_adapt_dt(pcc_ctx* ctx, double cfl_factor, double alpha, double* dt,
+                        double* median);
 pcc_status pcc_batch_adapt_dt(pcc_ctx* ctx, double cfl_factor, double alpha, double* dts,
                               double* medians);
 

@@ -34,7 +34,7 @@ def lib(fast: bool = False):
         L.oracle_normalize.argtypes = [dp, dp]
         L.oracle_exp.argtypes = [C.c_double]
         L.oracle_exp.restype = C.c_double
-        L.oracle_divide.argtypes = [C.c_int64, C.c_int64, dp, dp, dp, dp, dp, dp, dp,
+        L.oracle_divide.argtypes = [C.c_int64, C.c_int64, dp, dp, dp, dp, dp, dp, dp, dp,
                                     C.c_uint64, C.c_int32, C.c_double, C.c_double]
         L.oracle_divide.restype = C.c_int64
         L.oracle_block_sum.argtypes = [dp, C.c_int64]
@@ -99,15 +99,17 @@ def divide(bodies: Bodies, seed: int, step: int, length_noise: float = 0.0,
     cap = 2 * n + 1
     pos = np.zeros((cap, 3))
     quat = np.zeros((cap, 4))
-    ln, l0, dm, ld, ag = (np.zeros(cap) for _ in range(5))
+    ln, l0, dm, ld, ag, im = (np.zeros(cap) for _ in range(6))
     pos[:n], quat[:n], ln[:n] = bodies.pos, bodies.quat, bodies.length
     l0[:n], dm[:n], ld[:n], ag[:n] = bodies.l0, bodies.diameter, bodies.ldot, bodies.age
+    im[:n] = bodies.impedance
     m = lib().oracle_divide(n, cap, _d(pos), _d(quat), _d(ln), _d(l0), _d(dm), _d(ld), _d(ag),
-                            C.c_uint64(seed & 0xFFFFFFFFFFFFFFFF), int(step),
+                            _d(im), C.c_uint64(seed & 0xFFFFFFFFFFFFFFFF), int(step),
                             float(length_noise), float(angle_noise))
     assert m >= 0
     return Bodies(pos[:m].copy(), quat[:m].copy(), ln[:m].copy(), l0[:m].copy(),
-                  dm[:m].copy(), ld[:m].copy(), age=ag[:m].astype(np.int32))
+                  dm[:m].copy(), ld[:m].copy(), age=ag[:m].astype(np.int32),
+                  impedance=im[:m].copy())
 
 
 def block_sum(v):

@@ -603,6 +603,54 @@ static void bb_sums(const double* gi, const double* gp, const double* ri, const 
 }
 
 /* ------------------------------------------------------------------------ */
+/* Brownian input                                                            */
+/* ------------------------------------------------------------------------ */
+/* splitmix64 finaliser (Steele, Lea, Flood 2014) */
+static inline uint64_t mix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+/* Draw k of body i at step iter, uniform on [-1, 1) (the
+ * std::uniform_real_distribution(-1, 1) of Particle.cpp:177), from a
+ * counter-based generator in place of the reference's unseeded global
+ * std::mt19937 (Physics.cpp:19). */
+static inline double brownian_uniform(uint64_t seed, int32_t iter, int64_t i, int k) {
+  const uint64_t key = mix64(mix64(seed ^ 0xB7E151628AED2A6Bull) ^ (uint64_t)(uint32_t)iter);
+  const uint64_t z = mix64(key ^ ((uint64_t)i * 4u + (uint64_t)k));
+  return (double)(z >> 11) * 0x1.0p-52 - 1.0;
+}
+
+/* calculate_external_velocities (Physics.cpp:59-81) with
+ * Particle::calculateBrownianVelocity (Particle.cpp:174-190): U_ext zeroed,
+ * then per body the angular part added, each product in the reference's
+ * left-to-right order and the draws in its order (the monolayer ternaries
+ * skip the x/y draws).  imp: each body's impedance from its last step (NULL:
+ * 0, a new Particle's value, Particle.cpp:25).  Returns NULL when T <= 0. */
+static double* brownian_uext(const pcc_params* p, const pcc_bodies* b, int32_t iter) {
+  if (!(p->temperature > 0)) return NULL;
+  const int64_t N = b->n;
+  double* w = (double*)malloc(sizeof(double) * (size_t)(3 * N > 0 ? 3 * N : 1));
+  for (int64_t i = 0; i < N; ++i) {
+    const double imp = b->impedance ? b->impedance[i] : 0.0;
+    const double rot = p->temperature * brownian_uniform(p->brownian_seed, iter, i, 0);
+    double v[3];
+    if (p->monolayer) {
+      v[0] = 0.0;
+      v[1] = 0.0;
+      v[2] = (((rot * brownian_uniform(p->brownian_seed, iter, i, 1)) * imp) * imp) * imp;
+    } else {
+      for (int k = 0; k < 3; ++k)
+        v[k] = (((rot * brownian_uniform(p->brownian_seed, iter, i, 1 + k)) * imp) * imp) * imp;
+    }
+    for (int k = 0; k < 3; ++k) w[3 * i + k] = 0.0 + v[k]; /* VecSetValues ADD_VALUES on 0 */
+  }
+  return w;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Hard model                                                                */
 /* ------------------------------------------------------------------------ */
 typedef struct cset_t { /* all_constraints_set (Hard.cpp:101) as SoA */
@@ -655,6 +703,7 @@ typedef struct phase_t {
   double* imp;      /* N */
   double* ldot;     /* N */
   double* dldot;    /* N */
+  const double* uext; /* 3N Brownian angular velocity of phase 0 (U_ext), or NULL */
 } phase_t;
 
 static void phase_free(phase_t* ph) {
@@ -788,14 +837,15 @@ static void gradient(phase_t* ph, const double* gam, double* g) {
       for (int k = 0; k < 6; ++k) F[k] += a * Dv[k];
       sig += gam[c] * (side ? ph->sJ[c] : ph->sI[c]);
     }
-    /* U_c = M F (:18), U_total = U_c + U_ext with U_ext = 0 (:21) */
+    /* U_c = M F (:18), U_total = 1.0 U_ext + U_c (:21); U_ext is zero
+     * except the Brownian angular part on phase 0 */
     for (int k = 0; k < 3; ++k) {
       ph->F[6 * i + k] = F[k];
       ph->U[6 * i + k] = (0.0 + ph->mt[i] * F[k]) + 0.0;
     }
     for (int k = 3; k < 6; ++k) {
       ph->F[6 * i + k] = F[k];
-      ph->U[6 * i + k] = (0.0 + ph->mr[i] * F[k]) + 0.0;
+      ph->U[6 * i + k] = (0.0 + ph->mr[i] * F[k]) + (ph->uext ? ph->uext[3 * i + k - 3] : 0.0);
     }
     /* impedance = exp(-lambda sigma), ldot = (l .* impedance) * (1/tau) (:34-52) */
     ph->sig[i] = sig;
@@ -1046,9 +1096,10 @@ static void write_set(pcc_constraints* out, const cset_t* S) {
 }
 
 /* solveHardModel (Hard.cpp:91-193). */
-static pcc_status hard_step(const pcc_params* p, pcc_bodies* b, pcc_step_result* result,
-                            pcc_constraints* cons, int64_t* phase_iters, int64_t max_phase,
-                            char* err, int64_t err_len) {
+static pcc_status hard_step(const pcc_params* p, pcc_bodies* b, int32_t iter,
+                            pcc_step_result* result, pcc_constraints* cons,
+                            int64_t* phase_iters, int64_t max_phase, char* err,
+                            int64_t err_len) {
   const int64_t N = b->n;
   const double dt = p->dt;
   body_view bv = {b->pos, b->quat, b->length, b->diameter};
@@ -1087,6 +1138,8 @@ static pcc_status hard_step(const pcc_params* p, pcc_bodies* b, pcc_step_result*
     phase_t ph;
     memset(&ph, 0, sizeof(ph));
     phase_build(&ph, &S, b, p); /* :136 */
+    double* uext = k == 0 ? brownian_uext(p, b, iter) : NULL; /* :140-144 */
+    ph.uext = uext;
     double* Gamma = ALLOC(double, S.C);
     double* g_last = ALLOC(double, S.C);
     for (int64_t c = 0; c < S.C; ++c) Gamma[c] = S.r[c].gamma;
@@ -1098,6 +1151,7 @@ static pcc_status hard_step(const pcc_params* p, pcc_bodies* b, pcc_step_result*
       set_err(err, err_len, "BBPGD did not converge");
       free(Gamma);
       free(g_last);
+      free(uext);
       phase_free(&ph);
       total_it += its;
       break;
@@ -1125,6 +1179,7 @@ static pcc_status hard_step(const pcc_params* p, pcc_bodies* b, pcc_step_result*
     }
     free(Gamma);
     free(g_last);
+    free(uext);
     phase_free(&ph);
     if (k < ORACLE_MAX_PHASES) {
       g_phase_time[k] = now_s() - t_phase;
@@ -1156,8 +1211,8 @@ static pcc_status hard_step(const pcc_params* p, pcc_bodies* b, pcc_step_result*
 }
 
 /* solveSoftPotential (Soft.cpp:21-114). */
-static pcc_status soft_step(const pcc_params* p, pcc_bodies* b, pcc_step_result* result,
-                            pcc_constraints* cons) {
+static pcc_status soft_step(const pcc_params* p, pcc_bodies* b, int32_t iter,
+                            pcc_step_result* result, pcc_constraints* cons) {
   const int64_t N = b->n;
   const double dt = p->dt;
   body_view bv = {b->pos, b->quat, b->length, b->diameter};
@@ -1209,6 +1264,7 @@ static pcc_status soft_step(const pcc_params* p, pcc_bodies* b, pcc_step_result*
   }
   const double mlam = -p->lambda;
   const double itau = 1 / p->tau;
+  double* uext = brownian_uext(p, b, iter); /* :94, before the impedance is updated */
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < N; ++i) {
     double l = b->length[i];
@@ -1216,7 +1272,7 @@ static pcc_status soft_step(const pcc_params* p, pcc_bodies* b, pcc_step_result*
     double mr = 12.0 / (l * l * l * p->xi);
     double U[6];
     for (int k = 0; k < 3; ++k) U[k] = (0.0 + mt * F[6 * i + k]) + 0.0; /* calculate_forces :8-19 */
-    for (int k = 3; k < 6; ++k) U[k] = (0.0 + mr * F[6 * i + k]) + 0.0;
+    for (int k = 3; k < 6; ++k) U[k] = (0.0 + mr * F[6 * i + k]) + (uext ? uext[3 * i + k - 3] : 0.0);
     integrate_body(b, i, U, dt); /* :99 */
     for (int k = 0; k < 3; ++k) {
       if (b->force) b->force[3 * i + k] = F[6 * i + k];
@@ -1242,6 +1298,7 @@ static pcc_status soft_step(const pcc_params* p, pcc_bodies* b, pcc_step_result*
   free(F);
   free(sig);
   free(recs);
+  free(uext);
   return PCC_OK;
 }
 
@@ -1271,15 +1328,14 @@ int64_t oracle_phase0_energy(const pcc_params* p, const pcc_bodies* b, const dou
 pcc_status oracle_step(const pcc_params* p, pcc_bodies* b, int32_t iter, pcc_step_result* result,
                        pcc_constraints* cons, int64_t* phase_iters, int64_t max_phase,
                        char* err, int64_t err_len) {
-  (void)iter;
   memset(result, 0, sizeof(*result));
-  if (p->temperature != 0.0) {
-    set_err(err, err_len, "temperature > 0 (Brownian input) is not supported");
-    return PCC_E_UNSUPPORTED;
+  if (!(p->temperature >= 0.0)) {
+    set_err(err, err_len, "invalid temperature");
+    return PCC_E_INVALID_ARGUMENT;
   }
   if (p->mode == PCC_MODE_HARD)
-    return hard_step(p, b, result, cons, phase_iters, max_phase, err, err_len);
-  if (p->mode == PCC_MODE_SOFT) return soft_step(p, b, result, cons);
+    return hard_step(p, b, iter, result, cons, phase_iters, max_phase, err, err_len);
+  if (p->mode == PCC_MODE_SOFT) return soft_step(p, b, iter, result, cons);
   set_err(err, err_len, "Invalid mode");
   return PCC_E_INVALID_MODE;
 }
@@ -1380,8 +1436,9 @@ static int xkey_cmp(const void* a, const void* b) {
  * place, right children appended in parent order.  age (may be NULL): the
  * parent's age grows by one and the new cell copies it (Particle.cpp:144,148). */
 int64_t oracle_divide(int64_t n, int64_t cap, double* pos, double* quat, double* length,
-                      double* l0, double* diameter, double* ldot, double* age, uint64_t seed,
-                      int32_t step, double length_noise, double angle_noise) {
+                      double* l0, double* diameter, double* ldot, double* age,
+                      double* impedance, uint64_t seed, int32_t step, double length_noise,
+                      double angle_noise) {
   int64_t d = 0;
   for (int64_t i = 0; i < n; ++i) d += length[i] >= 2 * l0[i];
   const int64_t m = n + d;
@@ -1419,6 +1476,7 @@ int64_t oracle_divide(int64_t n, int64_t cap, double* pos, double* quat, double*
       age[i] = age[i] + 1;
       age[j] = age[i];
     }
+    if (impedance) impedance[j] = 0.0; /* a new Particle's (Particle.cpp:25) */
   }
   /* gids by ascending x */
   xkey_t* ord = (xkey_t*)malloc(sizeof(xkey_t) * (size_t)(m > 0 ? m : 1));
@@ -1441,6 +1499,7 @@ int64_t oracle_divide(int64_t n, int64_t cap, double* pos, double* quat, double*
   PERMUTE(diameter, 1);
   PERMUTE(ldot, 1);
   if (age) PERMUTE(age, 1);
+  if (impedance) PERMUTE(impedance, 1);
 #undef PERMUTE
   free(tmp);
   free(ord);

@@ -17,7 +17,9 @@
  *     (256-wide blocks, 64-lane butterflies) instead of one sequential loop;
  *   - exp() (HardModelGradient.cpp:40, VecExp) is a fixed Cody-Waite +
  *     degree-13 Taylor evaluation so CPU and GPU round identically;
- *   - Brownian input (temperature > 0) is not supported.
+ *   - the Brownian draws (temperature > 0) come from a counter-based
+ *     generator keyed by (brownian_seed, step index, body index, draw) in
+ *     place of the reference's unseeded global std::mt19937 (Physics.cpp:19).
  */
 #ifndef PCC_ORACLE_H_
 #define PCC_ORACLE_H_
@@ -42,11 +44,13 @@ double oracle_exp(double x);
 
 /* Division of every cell with length >= 2 l0 and gid reassignment by x
  * (Particle.cpp:122-150, Domain.cpp:492-508), seeded; arrays hold `cap`
- * bodies; age (generation, may be NULL) is carried as a double.  Returns the
- * new body count, -1 if it exceeds cap. */
+ * bodies; age (generation, may be NULL) is carried as a double; impedance
+ * (may be NULL): the parent keeps its value, the new cell starts at 0
+ * (Particle.cpp:25).  Returns the new body count, -1 if it exceeds cap. */
 int64_t oracle_divide(int64_t n, int64_t cap, double* pos, double* quat, double* length,
-                      double* l0, double* diameter, double* ldot, double* age, uint64_t seed,
-                      int32_t step, double length_noise, double angle_noise);
+                      double* l0, double* diameter, double* ldot, double* age,
+                      double* impedance, uint64_t seed, int32_t step, double length_noise,
+                      double angle_noise);
 
 /* Broad phase.  cell = floor((x - origin)/h) per axis (SpatialGrid.cpp:29-36);
  * bucket = hash(cell) & (table_size-1); stable counting sort of bodies by
@@ -69,7 +73,9 @@ int64_t oracle_detect(const pcc_bodies* b, double cutoff, double h, double origi
                       pcc_constraints* out, int32_t* num_constraints);
 
 /* One timestep: ParticleManager::step (ParticleManager.cpp:150-166).
- * Bodies are updated in place (all (out) fields written).  The constraint set
+ * Bodies are updated in place (all (out) fields written); b->impedance (may
+ * be NULL: 0) is also read, as the impedance of the Brownian input, and iter
+ * keys the Brownian draws.  The constraint set
  * of the step is written to `cons` (may be NULL) up to its capacity;
  * result->n_constraints is the full count.  Optional diagnostics: if non-NULL,
  * phase_iters[k] receives the BBPGD iterations of phase k (up to

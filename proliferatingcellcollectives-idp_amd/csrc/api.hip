@@ -32,10 +32,49 @@ std::recursive_mutex& device_mutex() {
   return m;
 }
 
+hipStream_t& alloc_stream() {
+  static thread_local hipStream_t s = nullptr;
+  return s;
+}
+
+void* dev_alloc(size_t bytes, bool* pooled) {
+  void* q = nullptr;
+  const hipStream_t s = alloc_stream();
+  const hipError_t e = s ? hipMallocAsync(&q, bytes, s) : hipMalloc(&q, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw StatusError{PCC_E_OUT_OF_MEMORY, "device allocation failed"};
+  }
+  *pooled = s != nullptr;
+  return q;
+}
+
+void dev_free(void* p, bool pooled) {
+  const hipStream_t s = alloc_stream();
+  if (pooled && s)
+    (void)hipFreeAsync(p, s);
+  else
+    (void)hipFree(p);
+}
+
+// Keeps freed pool memory in the process, so that a buffer freed on one
+// context's stream serves the next allocation without a trip to the driver.
+static void keep_pool_memory(int device) {
+  static std::once_flag once[64];
+  if (device < 0 || device >= 64) return;
+  std::call_once(once[device], [device]() {
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
+      uint64_t thr = UINT64_MAX;
+      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+    }
+  });
+}
+
 BodyArrays body_arrays(double* base, int64_t N) {
   BodyArrays b;
-  double** f[kBodyFields] = {&b.px, &b.py, &b.pz, &b.qw, &b.qx, &b.qy,
-                             &b.qz, &b.len, &b.l0, &b.diam, &b.ldot, &b.age};
+  double** f[kBodyFields] = {&b.px, &b.py, &b.pz,   &b.qw,   &b.qx,  &b.qy, &b.qz,
+                             &b.len, &b.l0, &b.diam, &b.ldot, &b.age, &b.imp};
   for (int k = 0; k < kBodyFields; ++k) *f[k] = base + (int64_t)k * N;
   return b;
 }
@@ -132,6 +171,10 @@ void upload_instances(pcc_ctx* ctx) {
     d.h = p.linked_cell_size;
     d.lam = p.lambda;
     d.tau = p.tau;
+    d.temp = p.temperature;
+    d.bseed = p.brownian_seed;
+    d.mono = p.monolayer ? 1 : 0;
+    d.pad1 = 0;
     d.phase = I.k;
     d.flags = I.flags;
     d.check = ctx->trace_on ? 1 : 10;
@@ -156,6 +199,7 @@ pcc_status fail(pcc_ctx* ctx, pcc_status s, const std::string& msg) {
 
 template <class F>
 pcc_status guarded(pcc_ctx* ctx, F&& f) {
+  StreamScope scope(ctx ? ctx->stream : nullptr);
   try {
     return f();
   } catch (const HipError& e) {
@@ -176,9 +220,9 @@ pcc_status check_params(const pcc_params* p, std::string* why) {
     *why = "invalid precision";
     return PCC_E_INVALID_ARGUMENT;
   }
-  if (p->temperature != 0.0) {
-    *why = "temperature > 0 (Brownian input) is not supported";
-    return PCC_E_UNSUPPORTED;
+  if (!(p->temperature >= 0.0) || !std::isfinite(p->temperature)) {
+    *why = "invalid temperature";
+    return PCC_E_INVALID_ARGUMENT;
   }
   if (!(p->linked_cell_size > 0) || !(p->dt > 0) || !(p->xi > 0) || !(p->tau != 0) ||
       p->max_bbpgd_iterations < 1 || p->max_recursive_iterations < 1) {
@@ -233,6 +277,7 @@ void fill_result(const Instance& I, pcc_step_result* r) {
 
 void begin_step(pcc_ctx* ctx) {
   reset_step(ctx);
+  ctx->brownian = false;
   const int64_t B = (int64_t)ctx->inst.size();
   ctx->trace_B = 0;
   if (ctx->trace_on && B > 0) {
@@ -258,7 +303,9 @@ void begin_step(pcc_ctx* ctx) {
 // solveHardModel (Hard.cpp:91-193) for every colony of the context.
 void hard_step(pcc_ctx* ctx) {
   begin_step(ctx);
-  ctx->planar = ctx->allow_planar && bodies_planar(ctx);
+  // planar kernels while every body lies in z = 0 with an in-plane
+  // orientation and no Brownian input turns it out of the plane
+  ctx->planar = ctx->allow_planar && !brownian_offplane(ctx) && bodies_planar(ctx);
   ctx->detect_flat = ctx->planar;  // z = 0 stays exact through a planar step
   std::vector<double> mo;
   for (int32_t k = 0;; ++k) {
@@ -285,6 +332,10 @@ void hard_step(pcc_ctx* ctx) {
     }
     if (!any) break;
     build_incidence(ctx, true);  // createMatrices, Hard.cpp:136
+    if (k == 0) {
+      upload_instances(ctx);
+      brownian_inputs(ctx);  // U_ext of phase 0 (Hard.cpp:140-144)
+    }
     bbpgd_solve(ctx);            // Hard.cpp:153
     for (auto& I : ctx->inst) {
       if (!I.active) continue;
@@ -303,6 +354,7 @@ void hard_step(pcc_ctx* ctx) {
     }
     upload_instances(ctx);
     integrate_hard(ctx);  // Hard.cpp:160-170
+    ctx->brownian = false;  // U_ext = 0 from phase 1 on
     for (auto& I : ctx->inst)
       if (I.active) I.k = k + 1;
   }
@@ -333,7 +385,9 @@ void soft_step(pcc_ctx* ctx) {
     ctx->inst[q].res = mo[q];
   }
   build_incidence(ctx, false);
+  brownian_inputs(ctx);  // U_ext of the step (Soft.cpp:94)
   soft_forces_and_integrate(ctx);
+  ctx->brownian = false;
   PCC_HIP(hipStreamSynchronize(ctx->stream));
 }
 
@@ -368,7 +422,6 @@ void ensure_body_arrays(pcc_ctx* ctx, int64_t N) {
   ctx->F.ensure(6 * N + 6);
   ctx->U.ensure(6 * N + 6);
   ctx->sig.ensure(N + 1);
-  ctx->imp.ensure(N + 1);
   ctx->ldot_new.ensure(N + 1);
   ctx->nc.ensure(N + 1);
   ctx->binst.ensure(N + 1);
@@ -397,6 +450,7 @@ void pack_bodies(const pcc_bodies* b, int64_t Ntot, int64_t boff, double* h) {
     h[(size_t)9 * Ntot + j] = b->diameter[i];
     h[(size_t)10 * Ntot + j] = b->ldot[i];
     h[(size_t)11 * Ntot + j] = b->age ? (double)b->age[i] : 1.0;
+    h[(size_t)12 * Ntot + j] = b->impedance ? b->impedance[i] : 0.0;  // Particle.cpp:25
   }
 }
 
@@ -438,7 +492,6 @@ pcc_status set_colonies(pcc_ctx* ctx, int64_t B, const pcc_bodies* bodies,
     PCC_HIP(hipMemsetAsync(ctx->F.p, 0, sizeof(double) * (6 * Ntot + 6), s));
     PCC_HIP(hipMemsetAsync(ctx->U.p, 0, sizeof(double) * (6 * Ntot + 6), s));
     PCC_HIP(hipMemsetAsync(ctx->sig.p, 0, sizeof(double) * (Ntot + 1), s));
-    PCC_HIP(hipMemsetAsync(ctx->imp.p, 0, sizeof(double) * (Ntot + 1), s));
     PCC_HIP(hipMemsetAsync(ctx->nc.p, 0, sizeof(int32_t) * (Ntot + 1), s));
     upload_binst(ctx);
     upload_instances(ctx);
@@ -456,7 +509,7 @@ pcc_status get_colony_state(pcc_ctx* ctx, int64_t q, pcc_bodies* b) {
   return guarded(ctx, [&]() {
     PCC_HIP(hipSetDevice(ctx->device));
     const int64_t N = I.N, Nt = ctx->N, o = I.boff;
-    std::vector<double> h((size_t)kBodyFields * N), F(6 * N), U(6 * N), sg(N), im(N);
+    std::vector<double> h((size_t)kBodyFields * N), F(6 * N), U(6 * N), sg(N);
     std::vector<int32_t> nc(N);
     hipStream_t s = ctx->stream;
     if (N > 0) {
@@ -466,7 +519,6 @@ pcc_status get_colony_state(pcc_ctx* ctx, int64_t q, pcc_bodies* b) {
       PCC_HIP(hipMemcpyAsync(F.data(), ctx->F.p + 6 * o, sizeof(double) * 6 * N, hipMemcpyDeviceToHost, s));
       PCC_HIP(hipMemcpyAsync(U.data(), ctx->U.p + 6 * o, sizeof(double) * 6 * N, hipMemcpyDeviceToHost, s));
       PCC_HIP(hipMemcpyAsync(sg.data(), ctx->sig.p + o, sizeof(double) * N, hipMemcpyDeviceToHost, s));
-      PCC_HIP(hipMemcpyAsync(im.data(), ctx->imp.p + o, sizeof(double) * N, hipMemcpyDeviceToHost, s));
       PCC_HIP(hipMemcpyAsync(nc.data(), ctx->nc.p + o, sizeof(int32_t) * N, hipMemcpyDeviceToHost, s));
     }
     PCC_HIP(hipStreamSynchronize(s));
@@ -486,7 +538,7 @@ pcc_status get_colony_state(pcc_ctx* ctx, int64_t q, pcc_bodies* b) {
         if (b->velocity) b->velocity[3 * i + k] = U[6 * i + k];
         if (b->omega) b->omega[3 * i + k] = U[6 * i + 3 + k];
       }
-      if (b->impedance) b->impedance[i] = im[i];
+      if (b->impedance) b->impedance[i] = h[(size_t)12 * N + i];
       if (b->stress) b->stress[i] = sg[i];
       if (b->num_constraints) b->num_constraints[i] = nc[i];
     }
@@ -516,8 +568,7 @@ void move_rows(pcc_ctx* ctx, DArr<T>& a, int64_t oldN, int64_t newN, int width, 
                                a.p + r * oldN * per + m.src * per, sizeof(T) * m.count * per,
                                hipMemcpyDeviceToDevice, s));
   PCC_HIP(hipStreamSynchronize(s));
-  std::swap(a.p, fresh.p);
-  std::swap(a.cap, fresh.cap);
+  a.swap(fresh);
 }
 
 void repack(pcc_ctx* ctx, int64_t newN, const std::vector<Move>& mv) {
@@ -526,7 +577,6 @@ void repack(pcc_ctx* ctx, int64_t newN, const std::vector<Move>& mv) {
   move_rows(ctx, ctx->F, oldN, newN, 6, false, mv);
   move_rows(ctx, ctx->U, oldN, newN, 6, false, mv);
   move_rows(ctx, ctx->sig, oldN, newN, 1, false, mv);
-  move_rows(ctx, ctx->imp, oldN, newN, 1, false, mv);
   move_rows(ctx, ctx->ldot_new, oldN, newN, 1, false, mv);
   move_rows(ctx, ctx->nc, oldN, newN, 1, false, mv);
   ctx->binst.release();
@@ -557,7 +607,7 @@ void pcc_default_params(pcc_params* p) {
   p->xi = 1.0;
   p->tau = 1.0;
   p->lambda = 1e-2;
-  p->temperature = 0.0;
+  p->temperature = 1e-28;  // CLIParser.h:181
   p->kcc = 20000.0;
   p->bbpgd_tol = 1e-3;
   p->relcp_tol = 1e-3;
@@ -568,6 +618,8 @@ void pcc_default_params(pcc_params* p) {
   p->linked_cell_size = 2.2;
   p->grid_origin = -1.0;
   p->warm_start_factor = 500.0;
+  p->monolayer = 1;  // CLIParser.h:183
+  p->brownian_seed = 0;
 }
 
 const char* pcc_status_string(pcc_status s) {
@@ -614,6 +666,7 @@ pcc_status pcc_create(const pcc_params* params, int device, pcc_ctx** out) {
   }
   pcc_status s = guarded(ctx, [&]() {
     PCC_HIP(hipSetDevice(device));
+    keep_pool_memory(device);
     {
       DeviceLock lock(device_mutex());
       PCC_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
@@ -621,6 +674,7 @@ pcc_status pcc_create(const pcc_params* params, int device, pcc_ctx** out) {
                             hipHostMallocDefault));
     }
     ctx->state_host_cap = 4;
+    StreamScope scope(ctx->stream);
     ctx->state.ensure(1);
     return PCC_OK;
   });
@@ -635,7 +689,8 @@ pcc_status pcc_create(const pcc_params* params, int device, pcc_ctx** out) {
 void pcc_destroy(pcc_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
-  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  const hipStream_t s = ctx->stream;
+  if (s) (void)hipStreamSynchronize(s);
   {
     DeviceLock lock(device_mutex());
     for (auto& p : ctx->timing.pending) {
@@ -644,9 +699,16 @@ void pcc_destroy(pcc_ctx* ctx) {
     }
     for (auto e : ctx->timing.free_events) (void)hipEventDestroy(e);
     if (ctx->state_host) (void)hipHostFree(ctx->state_host);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   }
-  delete ctx;  // DArr members free their device memory
+  {
+    StreamScope scope(s);  // pooled buffers go back to the pool in the stream's order
+    delete ctx;            // DArr members free their device memory
+  }
+  if (s) {
+    DeviceLock lock(device_mutex());
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
+  }
 }
 
 pcc_status pcc_set_params(pcc_ctx* ctx, const pcc_params* params) {
@@ -700,7 +762,6 @@ pcc_status pcc_restore(pcc_ctx* ctx) {
 }
 
 pcc_status pcc_step(pcc_ctx* ctx, int32_t iter, pcc_step_result* out) {
-  (void)iter;  // the reference uses it only to gate BBPGD tracing (Hard.cpp:109)
   if (!ctx || !out) return PCC_E_INVALID_ARGUMENT;
   memset(out, 0, sizeof(*out));
   if (!ctx->has_state) return fail(ctx, PCC_E_NO_STATE, "no state");
@@ -708,6 +769,7 @@ pcc_status pcc_step(pcc_ctx* ctx, int32_t iter, pcc_step_result* out) {
   pcc_status ps = check_params(&ctx->params, &why);
   if (ps != PCC_OK) return fail(ctx, ps, why);
   if (ctx->inst.size() != 1) return fail(ctx, PCC_E_INVALID_ARGUMENT, "batched context: use pcc_batch_step");
+  ctx->step_iter = iter;  // keys the Brownian draws
   pcc_status s = guarded(ctx, [&]() {
     PCC_HIP(hipSetDevice(ctx->device));
     return run_step(ctx);
@@ -802,12 +864,12 @@ pcc_status pcc_batch_size(pcc_ctx* ctx, int64_t* n_colonies, int64_t* n_bodies) 
 
 pcc_status pcc_batch_step(pcc_ctx* ctx, int32_t iter, pcc_step_result* results,
                           pcc_status* statuses) {
-  (void)iter;
   if (!ctx || !results) return PCC_E_INVALID_ARGUMENT;
   if (!ctx->has_state) return fail(ctx, PCC_E_NO_STATE, "no state");
   std::string why;
   pcc_status ps = check_params(&ctx->params, &why);
   if (ps != PCC_OK) return fail(ctx, ps, why);
+  ctx->step_iter = iter;  // keys the Brownian draws
   pcc_status s = guarded(ctx, [&]() {
     PCC_HIP(hipSetDevice(ctx->device));
     return run_step(ctx);
@@ -860,7 +922,6 @@ static pcc_status divide_impl(pcc_ctx* ctx, const uint64_t* seeds, int32_t step,
       PCC_HIP(hipMemsetAsync(ctx->F.p, 0, sizeof(double) * (6 * N + 6), s));
       PCC_HIP(hipMemsetAsync(ctx->U.p, 0, sizeof(double) * (6 * N + 6), s));
       PCC_HIP(hipMemsetAsync(ctx->sig.p, 0, sizeof(double) * (N + 1), s));
-      PCC_HIP(hipMemsetAsync(ctx->imp.p, 0, sizeof(double) * (N + 1), s));
       PCC_HIP(hipMemsetAsync(ctx->ldot_new.p, 0, sizeof(double) * (N + 1), s));
       PCC_HIP(hipMemsetAsync(ctx->nc.p, 0, sizeof(int32_t) * (N + 1), s));
       upload_binst(ctx);
@@ -889,19 +950,21 @@ pcc_status pcc_batch_divide(pcc_ctx* ctx, const uint64_t* seeds, int32_t step,
   return st;
 }
 
-pcc_status pcc_adapt_dt(pcc_ctx* ctx, double cfl_factor, double tolerance, double alpha,
-                        double* dt, double* median) {
+pcc_status pcc_adapt_dt(pcc_ctx* ctx, double cfl_factor, double alpha, double* dt,
+                        double* median) {
   if (!ctx) return PCC_E_INVALID_ARGUMENT;
   if (!ctx->has_state) return fail(ctx, PCC_E_NO_STATE, "no state");
   if (ctx->inst.size() != 1)
     return fail(ctx, PCC_E_INVALID_ARGUMENT, "batched context: use pcc_batch_adapt_dt");
-  if (!(alpha >= 0.0 && alpha <= 1.0) || !(cfl_factor > 0.0) || !(tolerance > 0.0))
+  if (!(alpha >= 0.0 && alpha <= 1.0) || !(cfl_factor > 0.0))
     return fail(ctx, PCC_E_INVALID_ARGUMENT, "invalid time-step adaptation parameter");
   return guarded(ctx, [&]() {
     PCC_HIP(hipSetDevice(ctx->device));
     std::vector<double> m;
     colony_median_speeds(ctx, m);
-    const double nd = adapted_dt(ctx->params.dt, m[0], cfl_factor, tolerance, alpha);
+    // the colony's relcp_tol: the reference's one tolerance (Domain.cpp:106)
+    const double nd =
+        adapted_dt(ctx->params.dt, m[0], cfl_factor, ctx->inst[0].p.relcp_tol, alpha);
     ctx->params.dt = nd;
     ctx->inst[0].p.dt = nd;
     if (dt) *dt = nd;
@@ -1035,7 +1098,7 @@ pcc_status pcc_get_tuning(pcc_ctx* ctx, int32_t* variant, int32_t* body_chunk,
   if (variant) *variant = ctx->body_variant;
   if (body_chunk) *body_chunk = body_chunk_of(ctx->body_variant, ctx->planar);
   if (body_threads) *body_threads = body_threads_of(ctx->body_variant);
-  if (tuned) *tuned = ctx->body_tuned ? 1 : 0;
+  if (tuned) *tuned = ctx->body_tuned ? (ctx->body_from_cache ? 2 : 1) : 0;
   if (variant_us)
     for (int v = 0; v < kBodyVariants; ++v) variant_us[v] = ctx->body_variant_ms[v] * 1e3;
   return PCC_OK;
@@ -1083,7 +1146,10 @@ pcc_status pcc_device_bytes(pcc_ctx* ctx, int64_t* bytes) {
     add(S.n); add(S.rI); add(S.rJ); add(S.p); add(S.sI); add(S.sJ); add(S.coef64);
     add(S.coef32);
   };
-  add(ctx->body); add(ctx->snap); add(ctx->F); add(ctx->U); add(ctx->sig); add(ctx->imp);
+  add(ctx->body); add(ctx->snap); add(ctx->F); add(ctx->U); add(ctx->sig); add(ctx->uext);
+  add(ctx->dv_flag); add(ctx->dv_off); add(ctx->dv_bounds); add(ctx->dv_val); add(ctx->dv_perm);
+  add(ctx->dv_perm2); add(ctx->dv_sinst); add(ctx->dv_stage); add(ctx->dv_body); add(ctx->dv_key);
+  add(ctx->dv_key2); add(ctx->dv_ckey); add(ctx->dv_ckey2); add(ctx->dv_col);
   add(ctx->ldot_new); add(ctx->nc); add(ctx->cell); add(ctx->bucket); add(ctx->perm);
   add(ctx->bstart); add(ctx->bcursor); add(ctx->dcount); add(ctx->dslot); add(ctx->dslot_off); add(ctx->dacc); add(ctx->dgeo); add(ctx->rI_); add(ctx->rJ_);
   add(ctx->rrec); add(ctx->ridx); add(ctx->ridx2); add(ctx->rkey); add(ctx->rkey2);

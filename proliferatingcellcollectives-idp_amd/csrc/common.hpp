@@ -308,6 +308,41 @@ __device__ __forceinline__ double block_max(double v, double* lds4) {
   return r;
 }
 
+// splitmix64 finaliser (Steele, Lea, Flood 2014)
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+// Brownian draw k of body i (index within its colony) at step `iter`:
+// uniform on [-1, 1) with 53 random bits, exact in double arithmetic
+// (std::uniform_real_distribution(-1, 1), Particle.cpp:177).  The stream
+// constant keeps these draws apart from the division draws of the same seed.
+__device__ __forceinline__ double brownian_uniform(uint64_t seed, int32_t iter, int64_t i, int k) {
+  const uint64_t key = mix64(mix64(seed ^ 0xB7E151628AED2A6Bull) ^ (uint64_t)(uint32_t)iter);
+  const uint64_t z = mix64(key ^ ((uint64_t)i * 4u + (uint64_t)k));
+  return (double)(z >> 11) * 0x1.0p-52 - 1.0;
+}
+
+// Particle::calculateBrownianVelocity (Particle.cpp:174-190) added into a
+// zeroed U_ext (Physics.cpp:60,75): angular velocity (wx, wy, wz) of a body
+// with impedance imp; every product in the reference's left-to-right order.
+__device__ __forceinline__ void brownian_omega(double T, int mono, uint64_t seed, int32_t iter,
+                                               int64_t i, double imp, double w[3]) {
+  const double rot = T * brownian_uniform(seed, iter, i, 0);
+  if (mono) {
+    w[0] = 0.0 + 0.0;
+    w[1] = 0.0 + 0.0;
+    w[2] = 0.0 + (((rot * brownian_uniform(seed, iter, i, 1)) * imp) * imp) * imp;
+  } else {
+    w[0] = 0.0 + (((rot * brownian_uniform(seed, iter, i, 1)) * imp) * imp) * imp;
+    w[1] = 0.0 + (((rot * brownian_uniform(seed, iter, i, 2)) * imp) * imp) * imp;
+    w[2] = 0.0 + (((rot * brownian_uniform(seed, iter, i, 3)) * imp) * imp) * imp;
+  }
+}
+
 // Order-preserving map of a double to uint64 for radix sorting (-0.0 is
 // folded onto +0.0 first, matching operator< semantics).
 __device__ __forceinline__ uint64_t order_key(double x) {

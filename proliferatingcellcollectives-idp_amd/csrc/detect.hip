@@ -458,12 +458,7 @@ void detect_batch(pcc_ctx* ctx) {
   PCC_HIP(rocprim::radix_sort_pairs(ctx->sort_tmp.p, bytes, ctx->rkey.p, ctx->rkey2.p, ctx->ridx.p,
                                     ctx->ridx2.p, (size_t)m, 0, 64, s));
   // sorted order now in ridx2; swap so ridx holds it
-  int32_t* t = ctx->ridx.p;
-  ctx->ridx.p = ctx->ridx2.p;
-  ctx->ridx2.p = t;
-  int64_t tc = ctx->ridx.cap;
-  ctx->ridx.cap = ctx->ridx2.cap;
-  ctx->ridx2.cap = tc;
+  ctx->ridx.swap(ctx->ridx2);
   if (B > 1) {
     // (colony, contact x, I, J): stable LSD sort by colony on top
     uint32_t* k0 = reinterpret_cast<uint32_t*>(ctx->rkey.p);
@@ -477,8 +472,7 @@ void detect_batch(pcc_ctx* ctx) {
     ctx->sort_tmp.ensure((int64_t)bytes + 16);
     PCC_HIP(rocprim::radix_sort_pairs(ctx->sort_tmp.p, bytes, k0, k1, ctx->ridx.p, ctx->ridx2.p,
                                       (size_t)m, 0, bits, s));
-    std::swap(ctx->ridx.p, ctx->ridx2.p);
-    std::swap(ctx->ridx.cap, ctx->ridx2.cap);
+    ctx->ridx.swap(ctx->ridx2);
   }
 }
 
@@ -507,8 +501,7 @@ static void ensure_set_capacity(ConstraintSet& S, int64_t need, hipStream_t s) {
 
 template <class T>
 static void swap_arr(DArr<T>& a, DArr<T>& b) {
-  std::swap(a.p, b.p);
-  std::swap(a.cap, b.cap);
+  a.swap(b);
 }
 
 // Appends the sorted batch; with several colonies the set is then regrouped

@@ -136,7 +136,9 @@ __global__ void k_div_write(int64_t N, int64_t M, BodyArrays b, const int32_t* _
   double x[3] = {b.px[i], b.py[i], b.pz[i]};
   double qq[4] = {b.qw[i], b.qx[i], b.qy[i], b.qz[i]};
   const double L = b.len[i], l0 = b.l0[i], dm = b.diam[i], ld = b.ldot[i], age = b.age[i];
-  auto put = [&](int64_t s, const double p[3], const double qv[4], double len, double ag) {
+  const double imp = b.imp[i];
+  auto put = [&](int64_t s, const double p[3], const double qv[4], double len, double ag,
+                 double im) {
     stage[0 * M + s] = p[0];
     stage[1 * M + s] = p[1];
     stage[2 * M + s] = p[2];
@@ -149,12 +151,13 @@ __global__ void k_div_write(int64_t N, int64_t M, BodyArrays b, const int32_t* _
     stage[9 * M + s] = dm;
     stage[10 * M + s] = ld;
     stage[11 * M + s] = ag;
+    stage[12 * M + s] = im;
     key[s] = order_key(p[0]);
     val[s] = (int32_t)s;
     sinst[s] = q;
   };
   if (!(L >= 2 * l0)) {
-    put(j, x, qq, L, age);
+    put(j, x, qq, L, age, imp);
     return;
   }
   const double half = L / 2;
@@ -174,10 +177,12 @@ __global__ void k_div_write(int64_t N, int64_t M, BodyArrays b, const int32_t* _
     pr[k] = x[k] + (l2 / 2) * t[k];
     pl[k] = x[k] - (l1 / 2) * t[k];
   }
-  // Particle::divide: the parent's age grows by one and the new cell copies it
-  put(j, pl, ql, l1, age + 1.0);
+  // Particle::divide: the parent's age grows by one and the new cell copies
+  // it; the parent keeps its impedance, the new cell starts at 0 as a newly
+  // constructed Particle does (Particle.cpp:25)
+  put(j, pl, ql, l1, age + 1.0, imp);
   const int64_t jr = c.noff + c.N + (off[i] - c.rank0);
-  put(jr, pr, qr, l2, age + 1.0);
+  put(jr, pr, qr, l2, age + 1.0, 0.0);
 }
 
 __global__ void k_div_colony_keys(int64_t M, const int32_t* __restrict__ perm,
@@ -208,7 +213,9 @@ void divide_colonies(pcc_ctx* ctx, const uint64_t* seeds, int32_t step, double l
   hipStream_t s = ctx->stream;
   if (N == 0) return;
   BodyArrays b = body_arrays(ctx->body.p, N);
-  DArr<int32_t> flag, off, bounds;
+  DArr<int32_t>& flag = ctx->dv_flag;
+  DArr<int32_t>& off = ctx->dv_off;
+  DArr<int32_t>& bounds = ctx->dv_bounds;
   flag.ensure(N + 1);
   off.ensure(N + 1);
   bounds.ensure(B + 1);
@@ -229,13 +236,18 @@ void divide_colonies(pcc_ctx* ctx, const uint64_t* seeds, int32_t step, double l
     hc[q] = {I.boff, I.N, noff, (int64_t)hb[q], seeds ? seeds[q] : 0};
     noff += I.N + (hb[q + 1] - hb[q]);
   }
-  DArr<DivColony> dcol;
-  dcol.ensure(B);
-  PCC_HIP(hipMemcpyAsync(dcol.p, hc.data(), sizeof(DivColony) * B, hipMemcpyHostToDevice, s));
-  DArr<double> stage;
-  DArr<uint64_t> key, key2;
-  DArr<int32_t> val, perm, perm2, sinst;
-  DArr<uint32_t> ckey, ckey2;
+  ctx->dv_col.ensure((int64_t)sizeof(DivColony) * B);
+  DivColony* dcol = reinterpret_cast<DivColony*>(ctx->dv_col.p);
+  PCC_HIP(hipMemcpyAsync(dcol, hc.data(), sizeof(DivColony) * B, hipMemcpyHostToDevice, s));
+  DArr<double>& stage = ctx->dv_stage;
+  DArr<uint64_t>& key = ctx->dv_key;
+  DArr<uint64_t>& key2 = ctx->dv_key2;
+  DArr<int32_t>& val = ctx->dv_val;
+  DArr<int32_t>& perm = ctx->dv_perm;
+  DArr<int32_t>& perm2 = ctx->dv_perm2;
+  DArr<int32_t>& sinst = ctx->dv_sinst;
+  DArr<uint32_t>& ckey = ctx->dv_ckey;
+  DArr<uint32_t>& ckey2 = ctx->dv_ckey2;
   stage.ensure(kBodyFields * M + 1);
   key.ensure(M + 1);
   key2.ensure(M + 1);
@@ -245,7 +257,7 @@ void divide_colonies(pcc_ctx* ctx, const uint64_t* seeds, int32_t step, double l
   sinst.ensure(M + 1);
   ckey.ensure(M + 1);
   ckey2.ensure(M + 1);
-  k_div_write<<<nblk(N), 256, 0, s>>>(N, M, b, ctx->binst.p, dcol.p, off.p, step, length_noise,
+  k_div_write<<<nblk(N), 256, 0, s>>>(N, M, b, ctx->binst.p, dcol, off.p, step, length_noise,
                                       angle_noise, stage.p, key.p, val.p, sinst.p);
   // gids by ascending x within each colony: stable sort by x, then by colony
   size_t bytes = 0;
@@ -267,12 +279,11 @@ void divide_colonies(pcc_ctx* ctx, const uint64_t* seeds, int32_t step, double l
                                       (size_t)M, 0, bits, s));
     order = perm2.p;
   }
-  DArr<double> body;
+  DArr<double>& body = ctx->dv_body;
   body.ensure(kBodyFields * M + 1);
   k_div_gather<<<nblk(M), 256, 0, s>>>(M, stage.p, order, body.p);
   PCC_HIP(hipStreamSynchronize(s));
-  std::swap(ctx->body.p, body.p);
-  std::swap(ctx->body.cap, body.cap);
+  ctx->body.swap(body);
   // new colony ranges; per-body outputs of the last step no longer apply
   for (int64_t q = 0; q < B; ++q) {
     Instance& I = ctx->inst[q];

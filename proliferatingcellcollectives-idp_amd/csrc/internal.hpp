@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <mutex>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -37,12 +38,30 @@ struct StatusError {
   std::string msg;
 };
 
+// Device buffers are allocated from the device's stream-ordered pool on the
+// stream of the context whose API call runs on this thread (set by
+// StreamScope): a buffer that grows is freed in that stream's order, after the
+// work already queued on it, so growing never waits for other contexts'
+// streams (a plain hipFree synchronises the whole device).  Outside an API
+// call (no stream set) the plain calls are used.
+hipStream_t& alloc_stream();
+struct StreamScope {
+  hipStream_t prev;
+  explicit StreamScope(hipStream_t s) : prev(alloc_stream()) { alloc_stream() = s; }
+  ~StreamScope() { alloc_stream() = prev; }
+};
+// *pooled receives whether the buffer came from the stream-ordered pool;
+// dev_free must be given the same flag.
+void* dev_alloc(size_t bytes, bool* pooled);
+void dev_free(void* p, bool pooled);
+
 // Growable device array (never shrinks; contents are not preserved on growth
-// unless grow_keep is used).
+// unless grow_keep is used).  Capacity doubles, so regrowth is rare.
 template <class T>
 struct DArr {
   T* p = nullptr;
   int64_t cap = 0;
+  bool pooled = false;
   DArr() = default;
   DArr(const DArr&) = delete;
   DArr& operator=(const DArr&) = delete;
@@ -52,40 +71,42 @@ struct DArr {
     int64_t nc = cap > 0 ? cap : 1024;
     while (nc < n) nc *= 2;
     DeviceLock lock(device_mutex());
-    T* q = nullptr;
-    hipError_t e = hipMalloc(&q, sizeof(T) * (size_t)nc);
-    if (e != hipSuccess) throw StatusError{PCC_E_OUT_OF_MEMORY, "hipMalloc failed"};
-    if (p) {
-      // queued work may still read the old buffer
-      PCC_HIP(hipDeviceSynchronize());
-      (void)hipFree(p);
-    }
+    bool pl = false;
+    T* q = static_cast<T*>(dev_alloc(sizeof(T) * (size_t)nc, &pl));
+    if (p) dev_free(p, pooled);  // ordered after the queued work that may read it
     p = q;
     cap = nc;
+    pooled = pl;
   }
+  // Grows keeping the first `keep` elements; s must be the stream the array
+  // is used on.
   void grow_keep(int64_t n, int64_t keep, hipStream_t s) {
     if (n <= cap) return;
     int64_t nc = cap > 0 ? cap : 1024;
     while (nc < n) nc *= 2;
     DeviceLock lock(device_mutex());
-    T* q = nullptr;
-    hipError_t e = hipMalloc(&q, sizeof(T) * (size_t)nc);
-    if (e != hipSuccess) throw StatusError{PCC_E_OUT_OF_MEMORY, "hipMalloc failed"};
+    bool pl = false;
+    T* q = static_cast<T*>(dev_alloc(sizeof(T) * (size_t)nc, &pl));
     if (p && keep > 0) PCC_HIP(hipMemcpyAsync(q, p, sizeof(T) * (size_t)keep, hipMemcpyDeviceToDevice, s));
-    if (p) {
-      PCC_HIP(hipStreamSynchronize(s));
-      (void)hipFree(p);
-    }
+    if (p) dev_free(p, pooled);
     p = q;
     cap = nc;
+    pooled = pl;
   }
   void release() {
     if (p) {
       DeviceLock lock(device_mutex());
-      (void)hipFree(p);
+      dev_free(p, pooled);
     }
     p = nullptr;
     cap = 0;
+    pooled = false;
+  }
+  // Exchanges buffers with another array (ping-pong without copies).
+  void swap(DArr& o) {
+    std::swap(p, o.p);
+    std::swap(cap, o.cap);
+    std::swap(pooled, o.pooled);
   }
 };
 
@@ -93,8 +114,9 @@ struct DArr {
 struct BodyArrays {
   double *px, *py, *pz, *qw, *qx, *qy, *qz, *len, *l0, *diam, *ldot;
   double* age;  // generation, an integer held exactly
+  double* imp;  // impedance of the last step (input of the Brownian velocity)
 };
-static constexpr int kBodyFields = 12;
+static constexpr int kBodyFields = 13;
 static constexpr int kBodyVariants = PCC_BODY_VARIANTS;
 int body_chunk_of(int variant, bool planar);
 int body_threads_of(int variant);
@@ -131,6 +153,10 @@ struct InstDev {
   int64_t maxit;
   double dt, xi, itau, mlam, kcc, tol, cutoff, warm, origin, h;
   double lam, tau;   // for the energy's growth term dt / (lambda tau)
+  double temp;       // Brownian temperature
+  uint64_t bseed;    // Brownian draws: seed
+  int32_t mono;      // Brownian draws: monolayer (omega_z only)
+  int32_t pad1;
   int64_t phase;     // ReLCP phase of the current solve
   int32_t flags;
   int32_t check;     // residual checked every `check` iterations (10; 1 while tracing)
@@ -209,6 +235,7 @@ struct pcc_ctx {
   // autotuner (PCC_AUTOTUNE=0: off): body-kernel staging size
   bool autotune = true;
   bool body_tuned = false;
+  bool body_from_cache = false;  // the variant came from the process-wide tuning memory
   int body_variant = 1;
   float body_variant_ms[PCC_BODY_VARIANTS] = {};
 
@@ -216,7 +243,10 @@ struct pcc_ctx {
   pcc::DArr<double> body;      // kBodyFields * N
   pcc::DArr<double> snap;      // kBodyFields * N
   bool has_snapshot = false;
-  pcc::DArr<double> F, U, sig, imp, ldot_new;  // 6N, 6N, N, N, N
+  pcc::DArr<double> F, U, sig, ldot_new;  // 6N, 6N, N, N
+  pcc::DArr<double> uext;  // 3N: Brownian angular velocity of the step (T > 0)
+  bool brownian = false;   // uext holds this step's draws
+  int32_t step_iter = 0;   // step index of the running step call (Brownian key)
   pcc::DArr<int32_t> nc;                        // N
 
   // grid
@@ -231,6 +261,12 @@ struct pcc_ctx {
   pcc::DArr<int32_t> dcount;   // N+1 (per-body record offsets)
   pcc::DArr<int32_t> dslot, dslot_off;  // per (body, neighbour cell) counts and offsets
   pcc::DArr<uint64_t> dacc;  // per (body, neighbour cell): accepted bucket positions (pass 0)
+  // division scratch (kept between calls)
+  pcc::DArr<int32_t> dv_flag, dv_off, dv_bounds, dv_val, dv_perm, dv_perm2, dv_sinst;
+  pcc::DArr<double> dv_stage, dv_body;
+  pcc::DArr<uint64_t> dv_key, dv_key2;
+  pcc::DArr<uint32_t> dv_ckey, dv_ckey2;
+  pcc::DArr<uint8_t> dv_col;
   pcc::DArr<double> dgeo;    // per body: x, y, z, t (3), length, diameter (detection)
   pcc::DArr<int32_t> rI_, rJ_; // raw pair ids
   pcc::DArr<double> rrec;      // raw geometry: 16 doubles per contact
@@ -302,6 +338,8 @@ bool bodies_planar(pcc_ctx* ctx);
 void bbpgd_solve(pcc_ctx* ctx);
 void integrate_hard(pcc_ctx* ctx);
 void grow(pcc_ctx* ctx);
+void brownian_inputs(pcc_ctx* ctx);
+bool brownian_offplane(const pcc_ctx* ctx);
 void soft_forces_and_integrate(pcc_ctx* ctx);
 void reset_step(pcc_ctx* ctx);
 void divide_colonies(pcc_ctx* ctx, const uint64_t* seeds, int32_t step, double length_noise,

@@ -93,9 +93,9 @@ __global__ void k_hertz(int64_t C, int mixed, const int32_t* __restrict__ I,
 __global__ void k_soft_body(int64_t N, int mixed, BodyArrays b, const int32_t* __restrict__ rowptr,
                             const int32_t* __restrict__ inc, const double* __restrict__ rec,
                             const float* __restrict__ rec32, const int32_t* __restrict__ binst,
-                            const InstDev* __restrict__ idev, double* __restrict__ F_out,
-                            double* __restrict__ U_out, double* __restrict__ sig_out,
-                            double* __restrict__ imp_out) {
+                            const InstDev* __restrict__ idev, const double* __restrict__ uext,
+                            double* __restrict__ F_out, double* __restrict__ U_out,
+                            double* __restrict__ sig_out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const InstDev& D = idev[binst[i]];
@@ -135,11 +135,17 @@ __global__ void k_soft_body(int64_t N, int mixed, BodyArrays b, const int32_t* _
   const double l = b.len[i];
   const double mt = 1.0 / (l * xi);
   const double mr = 12.0 / (l * l * l * xi);
+  double ue[3] = {0.0, 0.0, 0.0};  // U_ext: Brownian angular velocity (T > 0)
+  if (uext) {
+    ue[0] = uext[3 * i];
+    ue[1] = uext[3 * i + 1];
+    ue[2] = uext[3 * i + 2];
+  }
   double u[6];
 #pragma unroll
   for (int k = 0; k < 3; ++k) u[k] = (0.0 + mt * F[k]) + 0.0;
 #pragma unroll
-  for (int k = 3; k < 6; ++k) u[k] = (0.0 + mr * F[k]) + 0.0;
+  for (int k = 3; k < 6; ++k) u[k] = (0.0 + mr * F[k]) + ue[k - 3];
   double q[4] = {b.qw[i], b.qx[i], b.qy[i], b.qz[i]};
   const double s = q[0], wx = q[1], wy = q[2], wz = q[3];
   const double xq[4][3] = {{-0.5 * wx, -0.5 * wy, -0.5 * wz},
@@ -176,7 +182,7 @@ __global__ void k_soft_body(int64_t N, int mixed, BodyArrays b, const int32_t* _
   const double imp = det_exp(sig * mlam);
   const double ld = (l * imp) * itau;
   sig_out[i] = sig;
-  imp_out[i] = imp;
+  b.imp[i] = imp;
   b.ldot[i] = ld;
   double ln = l + dt * ld;
   if (ln > 2 * b.l0[i]) ln = 2 * b.l0[i];
@@ -216,7 +222,8 @@ void soft_forces_and_integrate(pcc_ctx* ctx) {
   if (N > 0)
     k_soft_body<<<nblk(N), 256, 0, s>>>(N, mixed ? 1 : 0, b, ctx->rowptr.p, ctx->inc.p,
                                         ctx->hertz.p, ctx->hertz32.p, ctx->binst.p, ctx->idev.p,
-                                        ctx->F.p, ctx->U.p, ctx->sig.p, ctx->imp.p);
+                                        ctx->brownian ? ctx->uext.p : nullptr, ctx->F.p,
+                                        ctx->U.p, ctx->sig.p);
 }
 
 }  // namespace pcc

@@ -32,6 +32,10 @@
 // contiguous range of bodies / constraints and keeps its gathers in its own L2.
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
 #include <vector>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
@@ -213,6 +217,23 @@ __global__ void k_mobility(int64_t N, const double* __restrict__ len,
   mob[2 * i + 1] = 12.0 / (l * l * l * xi);  // Physics.cpp:204
 }
 
+// Brownian angular velocities of the step (calculate_external_velocities,
+// Physics.cpp:59-81): draws for the bodies of colonies with T > 0, zeros for
+// the others.
+__global__ void k_brownian(int64_t N, const double* __restrict__ imp,
+                           const int32_t* __restrict__ binst, const InstDev* __restrict__ idev,
+                           int32_t iter, double* __restrict__ uext) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const InstDev& D = idev[binst[i]];
+  double w[3] = {0.0, 0.0, 0.0};
+  if (D.temp > 0.0 && (D.flags & kInstActive))
+    brownian_omega(D.temp, D.mono, D.bseed, iter, i - D.boff, imp[i], w);
+  uext[3 * i] = w[0];
+  uext[3 * i + 1] = w[1];
+  uext[3 * i + 2] = w[2];
+}
+
 // 1 if every body is in the z = 0 plane with an in-plane orientation.
 __global__ void k_planar_check(int64_t N, BodyArrays b, int32_t* __restrict__ nonplanar) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -249,7 +270,25 @@ struct BodyArgs {
   const double* ldot_prev;
   const SolverState* st;  // one per colony
   double* ubuf;  // 3D: 8 per body (U[6], dldot, pad); 2D: 4 (ux, uy, wz, dldot)
+  const double* uext;  // Brownian angular velocity (3 per body) of phase 0, or null
 };
+
+// U_total = U_ext + M F (HardModelGradient.cpp:15-21, VecWAXPY with
+// U_ext = 0 except the Brownian angular part on phase 0).
+__device__ __forceinline__ void body_velocity(const double* __restrict__ uext, int64_t i,
+                                              double mt, double mr, const double F[6],
+                                              double u[6]) {
+  double ue[3] = {0.0, 0.0, 0.0};
+  if (uext) {
+    ue[0] = uext[3 * i];
+    ue[1] = uext[3 * i + 1];
+    ue[2] = uext[3 * i + 2];
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) u[q] = (0.0 + mt * F[q]) + 0.0;
+#pragma unroll
+  for (int q = 3; q < 6; ++q) u[q] = (0.0 + mr * F[q]) + ue[q - 3];
+}
 
 template <class CT>
 __device__ __forceinline__ void load_rec(const void* base, int64_t e, double co[7], int64_t& c);
@@ -490,10 +529,7 @@ __global__ void __launch_bounds__(256) k_body_final(BodyArgs a, FinalOut fo) {
   if (FOLD) unfold(a.fold, i, F, PLANAR);
   const double mt = a.mob[2 * i], mr = a.mob[2 * i + 1];
   double u[6];
-#pragma unroll
-  for (int q = 0; q < 3; ++q) u[q] = (0.0 + mt * F[q]) + 0.0;
-#pragma unroll
-  for (int q = 3; q < 6; ++q) u[q] = (0.0 + mr * F[q]) + 0.0;
+  body_velocity(a.uext, i, mt, mr, F, u);
   const double imp = det_exp(sig * D.mlam);
   const double ld = (a.len[i] * imp) * D.itau;
 #pragma unroll
@@ -633,10 +669,7 @@ __global__ void __launch_bounds__(TPB) k_body_coop(BodyArgs a) {
     if (FOLD) unfold(a.fold, i, F[r], PLANAR);
     const double mt = a.mob[2 * i], mr = a.mob[2 * i + 1];
     double u[6];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) u[q] = (0.0 + mt * F[r][q]) + 0.0;
-#pragma unroll
-    for (int q = 3; q < 6; ++q) u[q] = (0.0 + mr * F[r][q]) + 0.0;
+    body_velocity(a.uext, i, mt, mr, F[r], u);
     const double imp = det_exp(sig[r] * D.mlam);
     const double ld = (a.len[i] * imp) * D.itau;
     if (PLANAR) {
@@ -1158,10 +1191,7 @@ __global__ void __launch_bounds__(kPersistThreads)
         if (FOLD) unfold(ba.fold, i, F, PLANAR);
         const double mt = ba.mob[2 * i], mr = ba.mob[2 * i + 1];
         double u[6];
-#pragma unroll
-        for (int m = 0; m < 3; ++m) u[m] = (0.0 + mt * F[m]) + 0.0;
-#pragma unroll
-        for (int m = 3; m < 6; ++m) u[m] = (0.0 + mr * F[m]) + 0.0;
+        body_velocity(ba.uext, i, mt, mr, F, u);
         const double imp = det_exp(sig * D.mlam);
         const double ld = (ba.len[i] * imp) * D.itau;
         if (PLANAR) {
@@ -1605,6 +1635,7 @@ static BodyArgs body_args(pcc_ctx* ctx) {
   a.ldot_prev = b.ldot;
   a.st = ctx->state.p;
   a.ubuf = ctx->ubuf.p;
+  a.uext = ctx->brownian ? ctx->uext.p : nullptr;
   return a;
 }
 
@@ -1699,6 +1730,46 @@ int body_chunk_of(int variant, bool planar) {
   return variant >= 6 ? 256 << (variant - 6) : (planar ? 512 : 256) << (variant % 3);
 }
 int body_threads_of(int variant) { return variant < 3 || variant >= 6 ? 256 : 128; }
+
+// Process-wide memory of the autotuner: the body-kernel variant chosen for a
+// (device architecture, planar, precision, N bucket, C/N bucket) class of
+// solves, so a new context (serving creates one per batch) reuses it instead
+// of timing the nine variants inside its first step.  N buckets by powers of
+// two of the bodies of the solve, C/N by halves of the mean incidences per
+// body.
+using TuneKey = std::tuple<std::string, int, int, int, int>;
+static std::mutex& tune_mutex() {
+  static std::mutex m;
+  return m;
+}
+struct TuneEntry {
+  int variant;
+  float ms[PCC_BODY_VARIANTS];  // the timings it was chosen from
+};
+static std::map<TuneKey, TuneEntry>& tune_cache() {
+  static std::map<TuneKey, TuneEntry> m;
+  return m;
+}
+static TuneKey tune_key(pcc_ctx* ctx, int64_t N, int64_t C) {
+  static std::mutex arch_m;
+  static std::map<int, std::string> arch;
+  std::string a;
+  {
+    std::lock_guard<std::mutex> g(arch_m);
+    auto it = arch.find(ctx->device);
+    if (it == arch.end()) {
+      hipDeviceProp_t prop;
+      std::string name = "?";
+      if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) name = prop.gcnArchName;
+      it = arch.emplace(ctx->device, name).first;
+    }
+    a = it->second;
+  }
+  int nb = 0;
+  while (((int64_t)1 << (nb + 1)) <= std::max<int64_t>(N, 1)) ++nb;
+  const int cn = (int)std::min<int64_t>(4 * C / std::max<int64_t>(N, 1), 64);
+  return TuneKey(a, ctx->planar ? 1 : 0, ctx->params.precision, nb, cn);
+}
 
 static void ensure_state_host(pcc_ctx* ctx, int64_t slots) {
   if (ctx->state_host_cap >= slots) return;
@@ -1841,6 +1912,20 @@ void bbpgd_solve(pcc_ctx* ctx) {
     k_finish<<<(unsigned)B, 256, 0, s>>>(ca);
   };
   one_iteration();  // k = 0
+  if (ctx->autotune && !ctx->body_tuned && C >= kTuneMinConstraints) {
+    int64_t na = 0;
+    for (const auto& I : ctx->inst)
+      if (I.flags & kInstActive) na += I.N;
+    const TuneKey key = tune_key(ctx, na, C);
+    std::lock_guard<std::mutex> g(tune_mutex());
+    auto it = tune_cache().find(key);
+    if (it != tune_cache().end()) {
+      ctx->body_variant = it->second.variant;
+      memcpy(ctx->body_variant_ms, it->second.ms, sizeof(ctx->body_variant_ms));
+      ctx->body_tuned = true;
+      ctx->body_from_cache = true;
+    }
+  }
   if (ctx->autotune && !ctx->body_tuned && C >= kTuneMinConstraints && !ctx->timing.enabled) {
     // Autotuner: time every staging size of the body kernel on this solve's
     // operators.  The kernel only rewrites the velocity records of the
@@ -1866,6 +1951,17 @@ void bbpgd_solve(pcc_ctx* ctx) {
         best = ms;
         ctx->body_variant = v;
       }
+    }
+    {
+      int64_t na = 0;
+      for (const auto& I : ctx->inst)
+        if (I.flags & kInstActive) na += I.N;
+      const TuneKey key = tune_key(ctx, na, C);
+      TuneEntry en;
+      en.variant = ctx->body_variant;
+      memcpy(en.ms, ctx->body_variant_ms, sizeof(en.ms));
+      std::lock_guard<std::mutex> g(tune_mutex());
+      tune_cache()[key] = en;
     }
     {
       DeviceLock lock(device_mutex());
@@ -1944,7 +2040,7 @@ void integrate_hard(pcc_ctx* ctx) {
   if (N > 0) {
     check_fold(ctx);
     BodyArgs ba = body_args(ctx);
-    const FinalOut fo = {ctx->F.p, ctx->U.p, ctx->sig.p, ctx->imp.p, ctx->ldot_new.p};
+    const FinalOut fo = {ctx->F.p, ctx->U.p, ctx->sig.p, b.imp, ctx->ldot_new.p};
     const unsigned g = nblk(N);
     if (ctx->planar) {
       if (mixed)
@@ -1964,6 +2060,31 @@ void integrate_hard(pcc_ctx* ctx) {
     k_update_constraints<<<nblk(C), 256, 0, s>>>(C, ctx->S.inst.p, ctx->idev.p, ctx->spos.p,
                                                  (const double2*)ctx->iter.p, ctx->gprev.p,
                                                  ctx->S.gamma.p, ctx->S.phi.p);
+}
+
+// Brownian inputs of the current phase-0 solve or soft step: ctx->uext holds
+// every body's draws (zeros outside colonies with T > 0) and ctx->brownian
+// is set when some active colony has T > 0.  The instance records must be
+// current (flags, colony ranges).
+void brownian_inputs(pcc_ctx* ctx) {
+  ctx->brownian = false;
+  bool any = false;
+  for (const auto& I : ctx->inst)
+    if (I.p.temperature > 0.0 && (I.flags & kInstActive)) any = true;
+  const int64_t N = ctx->N;
+  if (!any || N == 0) return;
+  ctx->uext.ensure(3 * N + 3);
+  BodyArrays b = body_arrays(ctx->body.p, N);
+  k_brownian<<<nblk(N), 256, 0, ctx->stream>>>(N, b.imp, ctx->binst.p, ctx->idev.p,
+                                                ctx->step_iter, ctx->uext.p);
+  ctx->brownian = true;
+}
+
+// True if some colony's Brownian input turns bodies out of the plane.
+bool brownian_offplane(const pcc_ctx* ctx) {
+  for (const auto& I : ctx->inst)
+    if (I.p.temperature > 0.0 && !I.p.monolayer) return true;
+  return false;
 }
 
 // Growth of the colonies flagged kInstGrow.

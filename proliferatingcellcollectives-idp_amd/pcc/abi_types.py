@@ -56,6 +56,9 @@ class Params(C.Structure):
         ("linked_cell_size", C.c_double),
         ("grid_origin", C.c_double),
         ("warm_start_factor", C.c_double),
+        ("monolayer", C.c_int32),
+        ("reserved0", C.c_int32),
+        ("brownian_seed", C.c_uint64),
     ]
 
     def copy(self):
@@ -72,7 +75,9 @@ class Params(C.Structure):
 
 def reference_params(**kw) -> Params:
     """Reference defaults (code/cpp/src/util/CLIParser.h:167-194,232) with the
-    single-tolerance coupling (Hard.cpp:121,132,153); temperature 0."""
+    single-tolerance coupling (Hard.cpp:121,132,153): temperature 1e-28 and
+    monolayer on (CLIParser.h:181,183), Brownian seed 0.  Workloads with
+    rotational diffusion off pass temperature=0."""
     tol = kw.pop("tolerance", 1e-3)
     p = Params(
         mode=MODE_HARD,
@@ -81,7 +86,7 @@ def reference_params(**kw) -> Params:
         xi=1.0,
         tau=1.0,
         lam=1e-2,
-        temperature=0.0,
+        temperature=1e-28,
         kcc=20000.0,
         bbpgd_tol=tol,
         relcp_tol=tol,
@@ -92,6 +97,9 @@ def reference_params(**kw) -> Params:
         linked_cell_size=2.2,
         grid_origin=-1.0,
         warm_start_factor=500.0,
+        monolayer=1,
+        reserved0=0,
+        brownian_seed=0,
     )
     return p.replace(**kw) if kw else p
 
@@ -174,11 +182,13 @@ def _ptr(a, ct):
 
 class Bodies:
     """Host-side body state in gid order (numpy, FP64; age, the generation of
-    ParticleData.h:16, int32: 1 for a new cell)."""
+    ParticleData.h:16, int32: 1 for a new cell; impedance, that of the last
+    step, 0 for a new cell, is the input of the Brownian velocity)."""
 
-    FIELDS_IN = ("pos", "quat", "length", "l0", "diameter", "ldot", "age")
+    FIELDS_IN = ("pos", "quat", "length", "l0", "diameter", "ldot", "age", "impedance")
 
-    def __init__(self, pos, quat, length, l0=None, diameter=None, ldot=None, age=None):
+    def __init__(self, pos, quat, length, l0=None, diameter=None, ldot=None, age=None,
+                 impedance=None):
         n = len(length)
         self.pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(n, 3)
         self.quat = np.ascontiguousarray(quat, dtype=np.float64).reshape(n, 4)
@@ -191,7 +201,8 @@ class Bodies:
         self.torque = np.zeros((n, 3))
         self.velocity = np.zeros((n, 3))
         self.omega = np.zeros((n, 3))
-        self.impedance = np.zeros(n)
+        self.impedance = np.ascontiguousarray(
+            np.zeros(n) if impedance is None else impedance, dtype=np.float64).reshape(n)
         self.stress = np.zeros(n)
         self.num_constraints = np.zeros(n, dtype=np.int32)
         self.age = np.ascontiguousarray(np.ones(n) if age is None else age,
@@ -222,7 +233,7 @@ class Bodies:
     def subset(self, idx):
         idx = np.asarray(idx)
         return Bodies(self.pos[idx], self.quat[idx], self.length[idx], self.l0[idx],
-                      self.diameter[idx], self.ldot[idx], self.age[idx])
+                      self.diameter[idx], self.ldot[idx], self.age[idx], self.impedance[idx])
 
 
 class Constraints:

@@ -75,7 +75,7 @@ def lib():
                                      C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                      C.POINTER(C.c_double)]
         dpt = C.POINTER(C.c_double)
-        L.pcc_adapt_dt.argtypes = [vp, C.c_double, C.c_double, C.c_double, dpt, dpt]
+        L.pcc_adapt_dt.argtypes = [vp, C.c_double, C.c_double, dpt, dpt]
         L.pcc_batch_adapt_dt.argtypes = [vp, C.c_double, C.c_double, dpt, dpt]
         L.pcc_divide.argtypes = [vp, C.c_uint64, C.c_int32, C.c_double, C.c_double,
                                  C.POINTER(i64)]
@@ -177,12 +177,11 @@ class ParticleManager:
         self._check(lib().pcc_get_state(self._ctx, C.byref(s)))
         return b
 
-    def adapt_dt(self, cfl_factor: float = 0.5, tolerance: float = None,
-                 alpha: float = 0.01):
-        """Domain::adaptDt on the device; returns (new dt, median speed)."""
-        tol = self.params.relcp_tol if tolerance is None else tolerance
+    def adapt_dt(self, cfl_factor: float = 0.5, alpha: float = 0.01):
+        """Domain::adaptDt on the device, with relcp_tol as the reference's
+        tolerance; returns (new dt, median speed)."""
         dt, med = C.c_double(0), C.c_double(0)
-        self._check(lib().pcc_adapt_dt(self._ctx, float(cfl_factor), float(tol), float(alpha),
+        self._check(lib().pcc_adapt_dt(self._ctx, float(cfl_factor), float(alpha),
                                        C.byref(dt), C.byref(med)))
         self.params.dt = float(dt.value)
         return float(dt.value), float(med.value)
@@ -198,7 +197,8 @@ class ParticleManager:
                                          C.byref(tu), us))
         return {"variant": int(va.value), "body_chunk": int(ch.value),
                 "body_threads": int(th.value), "prefetch": int(va.value) >= 6,
-                "tuned": bool(tu.value), "variant_us": [float(x) for x in us]}
+                "tuned": bool(tu.value), "from_cache": int(tu.value) == 2,
+                "variant_us": [float(x) for x in us]}
 
     def set_trace(self, enable: bool = True, capacity: int = 100000):
         """BBPGD trace of the following steps (the reference's bbpgd_logger):

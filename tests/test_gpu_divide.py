@@ -113,7 +113,7 @@ def test_adapt_dt_matches_reference_rule(n, tol):
     pm.set_state(b)
     pm.step(0)
     want_dt, want_med = expected_dt(pm.get_state(), p.dt, 0.5, tol, 0.01)
-    dt, med = pm.adapt_dt(0.5, tol, 0.01)
+    dt, med = pm.adapt_dt(0.5, 0.01)
     print(f"[adapt_dt] n={n} median={med:.6g} dt={dt:.6g}", flush=True)
     assert med == want_med and dt == want_dt
     pm.step(1)  # the next step runs with the new dt
@@ -132,4 +132,25 @@ def test_batch_adapt_dt_per_colony():
     for q in range(3):
         want_dt, want_med = expected_dt(cb.get_state(q), ps[q].dt, 0.5, ps[q].relcp_tol, 0.01)
         assert meds[q] == want_med and dts[q] == want_dt
+    cb.close()
+
+
+def test_adapt_dt_one_colony_batch_equals_single():
+    """Both entry points take the tolerance from the colony's relcp_tol: a
+    batch of one colony and the single-colony context adapt to the same dt."""
+    from pcc.engine import ColonyBatch, ParticleManager
+    p = reference_params(dt=1e-3, tolerance=3e-4, tau=1 / LN2, cutoff_phase0=0.05,
+                         cutoff_phaseK=0.05)
+    b = G.dense_colony(900, seed=185)
+    pm = ParticleManager(p)
+    pm.set_state(b)
+    pm.step(0)
+    cb = ColonyBatch(p)
+    cb.set_state([b], [p])
+    cb.step(0)
+    dt1, med1 = pm.adapt_dt(0.5, 0.01)
+    dts, meds = cb.adapt_dt(0.5, 0.01)
+    print(f"[adapt_dt] single {dt1!r} batch {dts[0]!r}", flush=True)
+    assert dt1 == dts[0] and med1 == meds[0]
+    pm.close()
     cb.close()

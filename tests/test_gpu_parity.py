@@ -148,8 +148,8 @@ def test_hard_step_fp64_bitexact(name, make, params):
 
 
 def test_growth_sequence_fp64_bitexact():
-    """Config-1 style growth from one cell: 400 steps with division between
-    steps, state carried on the device; compared every step."""
+    """Config-1 style growth of a 40-cell colony: 60 steps with host division
+    between steps, the state uploaded before each step; compared every step."""
     p = reference_params(dt=1e-3, tolerance=1e-6, tau=1 / LN2)
     b = G.packed_colony(40, seed=8)
     from pcc.engine import ParticleManager

@@ -19,8 +19,8 @@ from scipy.optimize import minimize
 
 import oracle
 from pcc import generators as G
-from pcc.abi_types import (MODE_SOFT, PCC_E_BBPGD_NOT_CONVERGED, PCC_E_INVALID_MODE,
-                           PCC_E_RELCP_MAX_ITERS, PCC_E_UNSUPPORTED, reference_params)
+from pcc.abi_types import (MODE_SOFT, PCC_E_BBPGD_NOT_CONVERGED, PCC_E_INVALID_ARGUMENT,
+                           PCC_E_INVALID_MODE, PCC_E_RELCP_MAX_ITERS, reference_params)
 
 LN2 = math.log(2.0)
 
@@ -174,8 +174,8 @@ def test_failure_modes():
     assert nb.length[0] > b.length[0]  # grown before the throw (Hard.cpp:185-190)
     st, *_, err = oracle.step(hard_params(mode=7), b)
     assert st == PCC_E_INVALID_MODE and err == "Invalid mode"
-    st, *_ = oracle.step(hard_params(temperature=1e-28), b)
-    assert st == PCC_E_UNSUPPORTED
+    st, *_, err = oracle.step(hard_params(temperature=-1.0), b)
+    assert st == PCC_E_INVALID_ARGUMENT and err == "invalid temperature"
 
 
 def test_relcp_grown_colony_converges_in_few_phases():

@@ -179,7 +179,9 @@ def test_warm_up_batch_shields_steady_colonies():
     the warm-up stage it waits for the slow step."""
     r = slow_pair(2)
     assert r[1].step_ms[0] >= 300
-    assert max(r[0].step_ms) < 100
+    # a 2 ms stand-in step; the margin absorbs a loaded host, and stays below
+    # the 300 ms step the steady colony would have waited for
+    assert max(r[0].step_ms) < 250
     r0 = slow_pair(0)
     assert max(r0[0].step_ms) >= 300
 
