@@ -95,9 +95,13 @@ typedef struct pcc_params {
    * uniform on [-1, 1), drawn in the reference's order (u1, then u_x, u_y
    * unless monolayer, then u_z).  The reference draws them from one unseeded
    * global std::mt19937 (Physics.cpp:19); here they come from a counter-based
-   * generator keyed by (brownian_seed, the step index `iter` of the step call,
-   * the body's index within its colony, draw number), so a run is
-   * reproducible and a colony's draws do not depend on its batch mates. */
+   * generator keyed by (brownian_seed, the colony's step count, the body's
+   * index within its colony, draw number), so a run is reproducible and a
+   * colony's draws are the same alone and in any batch.  The step count is
+   * the number of step calls since the colony was set (pcc_set_state,
+   * pcc_batch_set_state) or added (pcc_batch_add); pcc_snapshot and
+   * pcc_restore keep it with the state, pcc_batch_set_step_count sets it
+   * (to resume a checkpoint). */
   int32_t monolayer;                /* -monolayer (default true, CLIParser.h:183) */
   int32_t reserved0;
   uint64_t brownian_seed;
@@ -263,6 +267,12 @@ pcc_status pcc_batch_get_constraints(pcc_ctx* ctx, int64_t colony, pcc_constrain
                                      int64_t* count);
 pcc_status pcc_batch_get_phase_stats(pcc_ctx* ctx, int64_t colony, int64_t* bbpgd_iterations,
                                      int64_t* n_constraints, int64_t max, int64_t* count);
+/* Colony `colony`'s step count (0 for a single-colony context), which keys
+ * its Brownian draws (see pcc_params.brownian_seed): get it to checkpoint a
+ * run, set it to resume one.  No reference counterpart (the reference's
+ * generator is one global stream). */
+pcc_status pcc_batch_get_step_count(pcc_ctx* ctx, int64_t colony, int64_t* count);
+pcc_status pcc_batch_set_step_count(pcc_ctx* ctx, int64_t colony, int64_t count);
 
 /* Cell division and gid reassignment between steps, on the device
  * (ParticleManager::divideParticles / Particle::divide, ParticleManager.cpp:139-148,

@@ -613,11 +613,11 @@ static inline uint64_t mix64(uint64_t x) {
   return x ^ (x >> 31);
 }
 
-/* Draw k of body i at step iter, uniform on [-1, 1) (the
+/* Draw k of body i at the colony's step count iter, uniform on [-1, 1) (the
  * std::uniform_real_distribution(-1, 1) of Particle.cpp:177), from a
  * counter-based generator in place of the reference's unseeded global
  * std::mt19937 (Physics.cpp:19). */
-static inline double brownian_uniform(uint64_t seed, int32_t iter, int64_t i, int k) {
+static inline double brownian_uniform(uint64_t seed, int64_t iter, int64_t i, int k) {
   const uint64_t key = mix64(mix64(seed ^ 0xB7E151628AED2A6Bull) ^ (uint64_t)(uint32_t)iter);
   const uint64_t z = mix64(key ^ ((uint64_t)i * 4u + (uint64_t)k));
   return (double)(z >> 11) * 0x1.0p-52 - 1.0;

@@ -75,7 +75,8 @@ int64_t oracle_detect(const pcc_bodies* b, double cutoff, double h, double origi
 /* One timestep: ParticleManager::step (ParticleManager.cpp:150-166).
  * Bodies are updated in place (all (out) fields written); b->impedance (may
  * be NULL: 0) is also read, as the impedance of the Brownian input, and iter
- * keys the Brownian draws.  The constraint set
+ * keys the Brownian draws: it is the colony's step count (step calls since
+ * its state was set), which is what the library uses.  The constraint set
  * of the step is written to `cons` (may be NULL) up to its capacity;
  * result->n_constraints is the full count.  Optional diagnostics: if non-NULL,
  * phase_iters[k] receives the BBPGD iterations of phase k (up to

@@ -25,7 +25,6 @@
 namespace pcc {
 
 void batch_to_host(pcc_ctx* ctx, pcc_constraints* out, int32_t iteration);
-void build_grid(pcc_ctx* ctx);
 
 std::recursive_mutex& device_mutex() {
   static std::recursive_mutex m;
@@ -173,6 +172,7 @@ void upload_instances(pcc_ctx* ctx) {
     d.tau = p.tau;
     d.temp = p.temperature;
     d.bseed = p.brownian_seed;
+    d.bstep = I.nstep;
     d.mono = p.monolayer ? 1 : 0;
     d.pad1 = 0;
     d.phase = I.k;
@@ -744,6 +744,8 @@ pcc_status pcc_snapshot(pcc_ctx* ctx) {
     PCC_HIP(hipMemcpyAsync(ctx->snap.p, ctx->body.p, sizeof(double) * kBodyFields * ctx->N,
                            hipMemcpyDeviceToDevice, ctx->stream));
     PCC_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->snap_nstep.clear();
+    for (const auto& I : ctx->inst) ctx->snap_nstep.push_back(I.nstep);
     ctx->has_snapshot = true;
     return PCC_OK;
   });
@@ -757,6 +759,8 @@ pcc_status pcc_restore(pcc_ctx* ctx) {
     PCC_HIP(hipMemcpyAsync(ctx->body.p, ctx->snap.p, sizeof(double) * kBodyFields * ctx->N,
                            hipMemcpyDeviceToDevice, ctx->stream));
     PCC_HIP(hipStreamSynchronize(ctx->stream));
+    for (size_t q = 0; q < ctx->inst.size() && q < ctx->snap_nstep.size(); ++q)
+      ctx->inst[q].nstep = ctx->snap_nstep[q];
     return PCC_OK;
   });
 }
@@ -769,7 +773,7 @@ pcc_status pcc_step(pcc_ctx* ctx, int32_t iter, pcc_step_result* out) {
   pcc_status ps = check_params(&ctx->params, &why);
   if (ps != PCC_OK) return fail(ctx, ps, why);
   if (ctx->inst.size() != 1) return fail(ctx, PCC_E_INVALID_ARGUMENT, "batched context: use pcc_batch_step");
-  ctx->step_iter = iter;  // keys the Brownian draws
+  (void)iter;  // the reference uses it only to gate BBPGD tracing (Hard.cpp:109)
   pcc_status s = guarded(ctx, [&]() {
     PCC_HIP(hipSetDevice(ctx->device));
     return run_step(ctx);
@@ -779,6 +783,7 @@ pcc_status pcc_step(pcc_ctx* ctx, int32_t iter, pcc_step_result* out) {
     return PCC_OK;
   });
   if (s != PCC_OK) return s;
+  ++ctx->inst[0].nstep;
   const Instance& I = ctx->inst[0];
   fill_result(I, out);
   if (I.status != PCC_OK) return fail(ctx, I.status, I.error);
@@ -869,7 +874,7 @@ pcc_status pcc_batch_step(pcc_ctx* ctx, int32_t iter, pcc_step_result* results,
   std::string why;
   pcc_status ps = check_params(&ctx->params, &why);
   if (ps != PCC_OK) return fail(ctx, ps, why);
-  ctx->step_iter = iter;  // keys the Brownian draws
+  (void)iter;
   pcc_status s = guarded(ctx, [&]() {
     PCC_HIP(hipSetDevice(ctx->device));
     return run_step(ctx);
@@ -879,6 +884,7 @@ pcc_status pcc_batch_step(pcc_ctx* ctx, int32_t iter, pcc_step_result* results,
     return PCC_OK;
   });
   if (s != PCC_OK) return s;
+  for (auto& I : ctx->inst) ++I.nstep;
   for (size_t q = 0; q < ctx->inst.size(); ++q) {
     fill_result(ctx->inst[q], &results[q]);
     if (statuses) statuses[q] = ctx->inst[q].status;
@@ -991,6 +997,22 @@ pcc_status pcc_batch_adapt_dt(pcc_ctx* ctx, double cfl_factor, double alpha, dou
     }
     return PCC_OK;
   });
+}
+
+pcc_status pcc_batch_get_step_count(pcc_ctx* ctx, int64_t colony, int64_t* count) {
+  if (!ctx || !count) return PCC_E_INVALID_ARGUMENT;
+  if (colony < 0 || colony >= (int64_t)ctx->inst.size())
+    return fail(ctx, PCC_E_INVALID_ARGUMENT, "no such colony");
+  *count = ctx->inst[colony].nstep;
+  return PCC_OK;
+}
+
+pcc_status pcc_batch_set_step_count(pcc_ctx* ctx, int64_t colony, int64_t count) {
+  if (!ctx || count < 0) return PCC_E_INVALID_ARGUMENT;
+  if (colony < 0 || colony >= (int64_t)ctx->inst.size())
+    return fail(ctx, PCC_E_INVALID_ARGUMENT, "no such colony");
+  ctx->inst[colony].nstep = count;
+  return PCC_OK;
 }
 
 pcc_status pcc_batch_set_params(pcc_ctx* ctx, int64_t colony, const pcc_params* params) {

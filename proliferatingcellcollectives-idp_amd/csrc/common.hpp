@@ -316,11 +316,12 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
   return x ^ (x >> 31);
 }
 
-// Brownian draw k of body i (index within its colony) at step `iter`:
+// Brownian draw k of body i (index within its colony) at the colony's step
+// count `iter`:
 // uniform on [-1, 1) with 53 random bits, exact in double arithmetic
 // (std::uniform_real_distribution(-1, 1), Particle.cpp:177).  The stream
 // constant keeps these draws apart from the division draws of the same seed.
-__device__ __forceinline__ double brownian_uniform(uint64_t seed, int32_t iter, int64_t i, int k) {
+__device__ __forceinline__ double brownian_uniform(uint64_t seed, int64_t iter, int64_t i, int k) {
   const uint64_t key = mix64(mix64(seed ^ 0xB7E151628AED2A6Bull) ^ (uint64_t)(uint32_t)iter);
   const uint64_t z = mix64(key ^ ((uint64_t)i * 4u + (uint64_t)k));
   return (double)(z >> 11) * 0x1.0p-52 - 1.0;
@@ -329,7 +330,7 @@ __device__ __forceinline__ double brownian_uniform(uint64_t seed, int32_t iter, 
 // Particle::calculateBrownianVelocity (Particle.cpp:174-190) added into a
 // zeroed U_ext (Physics.cpp:60,75): angular velocity (wx, wy, wz) of a body
 // with impedance imp; every product in the reference's left-to-right order.
-__device__ __forceinline__ void brownian_omega(double T, int mono, uint64_t seed, int32_t iter,
+__device__ __forceinline__ void brownian_omega(double T, int mono, uint64_t seed, int64_t iter,
                                                int64_t i, double imp, double w[3]) {
   const double rot = T * brownian_uniform(seed, iter, i, 0);
   if (mono) {

@@ -363,10 +363,136 @@ int64_t table_size_for(int64_t n) {
   return t;
 }
 
+namespace {
+
+__global__ void k_fill_i32(int64_t n, int32_t v, int32_t* __restrict__ a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// rocPRIM calls on caller-provided temporary storage: the required size is
+// queried first and must fit.
+void scan_on(DetectBufs& d, const int32_t* in, int32_t* out, int64_t n) {
+  size_t bytes = 0;
+  PCC_HIP(rocprim::exclusive_scan(nullptr, bytes, in, out, int32_t(0), (size_t)n,
+                                  rocprim::plus<int32_t>(), d.s));
+  if (bytes > d.tmp_bytes) throw StatusError{PCC_E_DEVICE, "internal: scan storage too small"};
+  PCC_HIP(rocprim::exclusive_scan(d.tmp, bytes, in, out, int32_t(0), (size_t)n,
+                                  rocprim::plus<int32_t>(), d.s));
+}
+
+template <class K>
+void sort_pairs_on(DetectBufs& d, K* k0, K* k1, int32_t* v0, int32_t* v1, int64_t m, int bits) {
+  size_t bytes = 0;
+  PCC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, k0, k1, v0, v1, (size_t)m, 0, bits, d.s));
+  if (bytes > d.tmp_bytes) throw StatusError{PCC_E_DEVICE, "internal: sort storage too small"};
+  PCC_HIP(rocprim::radix_sort_pairs(d.tmp, bytes, k0, k1, v0, v1, (size_t)m, 0, bits, d.s));
+}
+
+int colony_bits(int64_t B) {
+  int bits = 1;
+  while (((int64_t)1 << bits) < B) ++bits;
+  return bits;
+}
+
+}  // namespace
+
+size_t detect_scan_bytes(int64_t N, int S, int64_t T) {
+  size_t a = 0, b = 0;
+  int32_t* p = nullptr;
+  PCC_HIP(rocprim::exclusive_scan(nullptr, a, p, p, int32_t(0), (size_t)(T + 1),
+                                  rocprim::plus<int32_t>(), (hipStream_t)0));
+  PCC_HIP(rocprim::exclusive_scan(nullptr, b, p, p, int32_t(0), (size_t)(N * S + 1),
+                                  rocprim::plus<int32_t>(), (hipStream_t)0));
+  return std::max(a, b);
+}
+
+size_t detect_sort_bytes(int64_t m, int64_t B) {
+  size_t a = 0, b = 0;
+  uint64_t* k = nullptr;
+  uint32_t* k32 = nullptr;
+  int32_t* v = nullptr;
+  PCC_HIP(rocprim::radix_sort_pairs(nullptr, a, k, k, v, v, (size_t)m, 0, 64, (hipStream_t)0));
+  if (B > 1)
+    PCC_HIP(rocprim::radix_sort_pairs(nullptr, b, k32, k32, v, v, (size_t)m, 0, colony_bits(B),
+                                      (hipStream_t)0));
+  return std::max(a, b);
+}
+
+// Stage 1 of a detection batch: the hashed cell list (a stable counting sort
+// of salted cell hashes), the counting pass over every (body, neighbour cell)
+// slot, and the offsets.  bounds_host[b] (B + 1 entries) receives the first
+// record of colony b in the batch; bounds_host[B] is the record count.
+void detect_count(DetectBufs& d, int32_t* bounds_host) {
+  const int64_t N = d.N, T = d.T, NS = N * d.S;
+  const hipStream_t s = d.s;
+  const uint32_t mask = (uint32_t)(T - 1);
+  k_fill_i32<<<nblk(T + 1), 256, 0, s>>>(T + 1, 0, d.bcursor);
+  if (N > 0)
+    k_cells<<<nblk(N), 256, 0, s>>>(N, d.body, d.binst, d.idev, mask, d.cell4, d.geo, d.bucket,
+                                    d.bcursor);
+  scan_on(d, d.bcursor, d.bstart, T + 1);
+  k_fill_i32<<<nblk(T + 1), 256, 0, s>>>(T + 1, 0, d.bcursor);
+  if (N > 0) {
+    k_scatter<<<nblk(N), 256, 0, s>>>(N, d.bucket, d.bstart, d.bcursor, d.perm);
+    k_seg_sort<<<nblk(T), 256, 0, s>>>(T, d.bstart, d.perm);
+  }
+  k_fill_i32<<<nblk(NS + 1), 256, 0, s>>>(NS + 1, 0, d.dslot);
+  if (N > 0) {
+    if (d.S == 9)
+      k_detect<9><<<nblk(NS), 256, 0, s>>>(0, N, d.cell4, d.geo, d.bstart, d.perm, mask, d.idev,
+                                           d.dslot, d.dacc, nullptr, nullptr, nullptr);
+    else
+      k_detect<27><<<nblk(NS), 256, 0, s>>>(0, N, d.cell4, d.geo, d.bstart, d.perm, mask, d.idev,
+                                            d.dslot, d.dacc, nullptr, nullptr, nullptr);
+  }
+  // per-slot offsets, then per-body offsets into dcount, then the colony
+  // boundaries of the batch (bodies are stored colony after colony, and a
+  // stable sort by colony keeps them); bounds[B] is the batch's record count,
+  // so one read-back gives both
+  scan_on(d, d.dslot, d.dslot_off, NS + 1);
+  k_body_offsets<<<nblk(N + 1), 256, 0, s>>>(N, d.S, d.dslot_off, d.dcount);
+  k_batch_bounds<<<nblk(d.B + 1), 256, 0, s>>>(d.B, N, d.idev, d.dcount, d.bbounds);
+  PCC_HIP(hipMemcpyAsync(bounds_host, d.bbounds, sizeof(int32_t) * (d.B + 1),
+                         hipMemcpyDeviceToHost, s));
+  PCC_HIP(hipStreamSynchronize(s));
+}
+
+// Stage 2: the writing pass of the m records found by detect_count, then
+// their order (colony, contact x, I, J); on return d.ridx lists them in that
+// order (d.ridx and d.ridx2 may have been exchanged).
+void detect_write(DetectBufs& d, int64_t m) {
+  if (m == 0) return;
+  const int64_t N = d.N, NS = N * d.S;
+  const hipStream_t s = d.s;
+  const uint32_t mask = (uint32_t)(d.T - 1);
+  if (d.S == 9)
+    k_detect<9><<<nblk(NS), 256, 0, s>>>(1, N, d.cell4, d.geo, d.bstart, d.perm, mask, d.idev,
+                                         d.dslot_off, d.dacc, d.rI, d.rJ, d.rec);
+  else
+    k_detect<27><<<nblk(NS), 256, 0, s>>>(1, N, d.cell4, d.geo, d.bstart, d.perm, mask, d.idev,
+                                          d.dslot_off, d.dacc, d.rI, d.rJ, d.rec);
+  k_segsort_by_J<<<nblk(N), 256, 0, s>>>(N, d.dcount, d.rJ, d.ridx);
+  k_make_keys<<<nblk(m), 256, 0, s>>>(m, d.ridx, d.rec, d.rkey);
+  sort_pairs_on<uint64_t>(d, d.rkey, d.rkey2, d.ridx, d.ridx2, m, 64);
+  std::swap(d.ridx, d.ridx2);  // sorted order now in ridx
+  if (d.B > 1) {
+    // (colony, contact x, I, J): stable LSD sort by colony on top
+    uint32_t* k0 = reinterpret_cast<uint32_t*>(d.rkey);
+    uint32_t* k1 = reinterpret_cast<uint32_t*>(d.rkey2);
+    k_inst_keys<<<nblk(m), 256, 0, s>>>(m, d.ridx, d.rI, d.binst, k0);
+    sort_pairs_on<uint32_t>(d, k0, k1, d.ridx, d.ridx2, m, colony_bits(d.B));
+    std::swap(d.ridx, d.ridx2);
+  }
+}
+
 // Builds the hashed cell list of the current body state (stable counting sort).
-void build_grid(pcc_ctx* ctx) {
+static DetectBufs ctx_detect_bufs(pcc_ctx* ctx) {
   const int64_t N = ctx->N;
   const int64_t T = table_size_for(N);
+  const int S = ctx->detect_flat ? 9 : 27;
+  const int64_t NS = N * S;
+  const int64_t B = (int64_t)ctx->inst.size();
   ctx->table_size = T;
   ctx->cell.ensure(4 * N + 4);
   ctx->dgeo.ensure(8 * N + 8);
@@ -374,66 +500,46 @@ void build_grid(pcc_ctx* ctx) {
   ctx->perm.ensure(N + 1);
   ctx->bstart.ensure(T + 1);
   ctx->bcursor.ensure(T + 1);
-  hipStream_t s = ctx->stream;
-  BodyArrays b = body_arrays(ctx->body.p, N);
-  PCC_HIP(hipMemsetAsync(ctx->bcursor.p, 0, sizeof(int32_t) * (size_t)(T + 1), s));
-  if (N > 0)
-    k_cells<<<nblk(N), 256, 0, s>>>(N, b, ctx->binst.p, ctx->idev.p, (uint32_t)(T - 1),
-                                    reinterpret_cast<int4*>(ctx->cell.p),
-                                    reinterpret_cast<double2*>(ctx->dgeo.p), ctx->bucket.p,
-                                    ctx->bcursor.p);
-  excl_scan<int32_t>(ctx, ctx->bcursor.p, ctx->bstart.p, T + 1);
-  PCC_HIP(hipMemsetAsync(ctx->bcursor.p, 0, sizeof(int32_t) * (size_t)(T + 1), s));
-  if (N > 0) {
-    k_scatter<<<nblk(N), 256, 0, s>>>(N, ctx->bucket.p, ctx->bstart.p, ctx->bcursor.p, ctx->perm.p);
-    k_seg_sort<<<nblk(T), 256, 0, s>>>(T, ctx->bstart.p, ctx->perm.p);
-  }
+  ctx->dslot.ensure(NS + 1);
+  ctx->dslot_off.ensure(NS + 1);
+  ctx->dacc.ensure(NS + 1);
+  ctx->dcount.ensure(N + 1);
+  ctx->bbounds.ensure(B + 1);
+  ctx->sort_tmp.ensure((int64_t)detect_scan_bytes(N, S, T) + 16);
+  DetectBufs d;
+  d.N = N;
+  d.body = body_arrays(ctx->body.p, N);
+  d.binst = ctx->binst.p;
+  d.idev = ctx->idev.p;
+  d.B = B;
+  d.S = S;
+  d.T = T;
+  d.cell4 = reinterpret_cast<int4*>(ctx->cell.p);
+  d.geo = reinterpret_cast<double2*>(ctx->dgeo.p);
+  d.bucket = ctx->bucket.p;
+  d.perm = ctx->perm.p;
+  d.bstart = ctx->bstart.p;
+  d.bcursor = ctx->bcursor.p;
+  d.dslot = ctx->dslot.p;
+  d.dslot_off = ctx->dslot_off.p;
+  d.dacc = ctx->dacc.p;
+  d.dcount = ctx->dcount.p;
+  d.bbounds = ctx->bbounds.p;
+  d.tmp = ctx->sort_tmp.p;
+  d.tmp_bytes = (size_t)ctx->sort_tmp.cap;
+  d.s = ctx->stream;
+  return d;
 }
 
 // One detection batch over the active colonies, each at the cutoff of its
 // instance record; on return ctx->ridx[0..batch_count) lists the raw records
 // in (colony, contact x, I, J) order and every colony's m / bstart is set.
 void detect_batch(pcc_ctx* ctx) {
-  const int64_t N = ctx->N;
   const int64_t B = (int64_t)ctx->inst.size();
-  hipStream_t s = ctx->stream;
   KTimer tm(ctx, PCC_K_DETECT, 0.0);
-  build_grid(ctx);
-  const uint32_t mask = (uint32_t)(ctx->table_size - 1);
-  // one slot per (body, neighbour cell); 9 neighbour cells when every body
-  // lies in the z = 0 layer
-  const int S = ctx->detect_flat ? 9 : 27;
-  const int64_t NS = N * S;
-  ctx->dslot.ensure(NS + 1);
-  ctx->dslot_off.ensure(NS + 1);
-  ctx->dacc.ensure(NS + 1);
-  ctx->dcount.ensure(N + 1);
-  PCC_HIP(hipMemsetAsync(ctx->dslot.p, 0, sizeof(int32_t) * (size_t)(NS + 1), s));
-  const int4* c4 = reinterpret_cast<const int4*>(ctx->cell.p);
-  const double2* geo = reinterpret_cast<const double2*>(ctx->dgeo.p);
-  auto detect = [&](int pass, int32_t* cnt_off) {
-    if (S == 9)
-      k_detect<9><<<nblk(NS), 256, 0, s>>>(pass, N, c4, geo, ctx->bstart.p, ctx->perm.p, mask,
-                                           ctx->idev.p, cnt_off, ctx->dacc.p, ctx->rI_.p,
-                                           ctx->rJ_.p, ctx->rrec.p);
-    else
-      k_detect<27><<<nblk(NS), 256, 0, s>>>(pass, N, c4, geo, ctx->bstart.p, ctx->perm.p, mask,
-                                            ctx->idev.p, cnt_off, ctx->dacc.p, ctx->rI_.p,
-                                            ctx->rJ_.p, ctx->rrec.p);
-  };
-  if (N > 0) detect(0, ctx->dslot.p);
-  // per-slot offsets, then per-body offsets into dcount, then the colony
-  // boundaries of the batch (bodies are stored colony after colony, and a
-  // stable sort by colony keeps them); bounds[B] is the batch's record count,
-  // so one read-back gives both
-  excl_scan<int32_t>(ctx, ctx->dslot.p, ctx->dslot_off.p, NS + 1);
-  k_body_offsets<<<nblk(N + 1), 256, 0, s>>>(N, S, ctx->dslot_off.p, ctx->dcount.p);
-  ctx->bbounds.ensure(B + 1);
-  k_batch_bounds<<<nblk(B + 1), 256, 0, s>>>(B, N, ctx->idev.p, ctx->dcount.p, ctx->bbounds.p);
+  DetectBufs d = ctx_detect_bufs(ctx);
   std::vector<int32_t> bounds(B + 1);
-  PCC_HIP(hipMemcpyAsync(bounds.data(), ctx->bbounds.p, sizeof(int32_t) * (B + 1),
-                         hipMemcpyDeviceToHost, s));
-  PCC_HIP(hipStreamSynchronize(s));
+  detect_count(d, bounds.data());
   const int64_t m = bounds[B];
   ctx->batch_count = m;
   ctx->rI_.ensure(m + 1);
@@ -443,37 +549,23 @@ void detect_batch(pcc_ctx* ctx) {
   ctx->ridx2.ensure(m + 1);
   ctx->rkey.ensure(m + 1);
   ctx->rkey2.ensure(m + 1);
+  ctx->sort_tmp.ensure((int64_t)detect_sort_bytes(m, B) + 16);
   for (int64_t q = 0; q < B; ++q) {
     ctx->inst[q].bstart = bounds[q];
     ctx->inst[q].m = bounds[q + 1] - bounds[q];
   }
   if (m == 0) return;
-  detect(1, ctx->dslot_off.p);
-  k_segsort_by_J<<<nblk(N), 256, 0, s>>>(N, ctx->dcount.p, ctx->rJ_.p, ctx->ridx.p);
-  k_make_keys<<<nblk(m), 256, 0, s>>>(m, ctx->ridx.p, ctx->rrec.p, ctx->rkey.p);
-  size_t bytes = 0;
-  PCC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, ctx->rkey.p, ctx->rkey2.p, ctx->ridx.p,
-                                    ctx->ridx2.p, (size_t)m, 0, 64, s));
-  ctx->sort_tmp.ensure((int64_t)bytes + 16);
-  PCC_HIP(rocprim::radix_sort_pairs(ctx->sort_tmp.p, bytes, ctx->rkey.p, ctx->rkey2.p, ctx->ridx.p,
-                                    ctx->ridx2.p, (size_t)m, 0, 64, s));
-  // sorted order now in ridx2; swap so ridx holds it
-  ctx->ridx.swap(ctx->ridx2);
-  if (B > 1) {
-    // (colony, contact x, I, J): stable LSD sort by colony on top
-    uint32_t* k0 = reinterpret_cast<uint32_t*>(ctx->rkey.p);
-    uint32_t* k1 = reinterpret_cast<uint32_t*>(ctx->rkey2.p);
-    k_inst_keys<<<nblk(m), 256, 0, s>>>(m, ctx->ridx.p, ctx->rI_.p, ctx->binst.p, k0);
-    int bits = 1;
-    while (((int64_t)1 << bits) < B) ++bits;
-    bytes = 0;
-    PCC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, k0, k1, ctx->ridx.p, ctx->ridx2.p,
-                                      (size_t)m, 0, bits, s));
-    ctx->sort_tmp.ensure((int64_t)bytes + 16);
-    PCC_HIP(rocprim::radix_sort_pairs(ctx->sort_tmp.p, bytes, k0, k1, ctx->ridx.p, ctx->ridx2.p,
-                                      (size_t)m, 0, bits, s));
-    ctx->ridx.swap(ctx->ridx2);
-  }
+  d.rI = ctx->rI_.p;
+  d.rJ = ctx->rJ_.p;
+  d.rec = ctx->rrec.p;
+  d.ridx = ctx->ridx.p;
+  d.ridx2 = ctx->ridx2.p;
+  d.rkey = ctx->rkey.p;
+  d.rkey2 = ctx->rkey2.p;
+  d.tmp = ctx->sort_tmp.p;
+  d.tmp_bytes = (size_t)ctx->sort_tmp.cap;
+  detect_write(d, m);
+  if (d.ridx != ctx->ridx.p) ctx->ridx.swap(ctx->ridx2);
 }
 
 static void ensure_set_capacity(ConstraintSet& S, int64_t need, hipStream_t s) {

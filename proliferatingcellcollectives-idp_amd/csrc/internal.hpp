@@ -155,6 +155,7 @@ struct InstDev {
   double lam, tau;   // for the energy's growth term dt / (lambda tau)
   double temp;       // Brownian temperature
   uint64_t bseed;    // Brownian draws: seed
+  int64_t bstep;     // Brownian draws: the colony's step count
   int32_t mono;      // Brownian draws: monolayer (omega_z only)
   int32_t pad1;
   int64_t phase;     // ReLCP phase of the current solve
@@ -198,6 +199,7 @@ struct Instance {
   double cutoff = 0.0;       // contact cutoff of the current detection
   int64_t G = 1, K = 0, lb = 0, bb = 0, nbb = 0;
   int32_t flags = 0;
+  int64_t nstep = 0;         // steps taken since the colony was set or added (Brownian key)
   // per-step outcome (SolverSolution, Hard.cpp:192)
   bool active = false;
   int32_t k = 0;
@@ -246,7 +248,7 @@ struct pcc_ctx {
   pcc::DArr<double> F, U, sig, ldot_new;  // 6N, 6N, N, N
   pcc::DArr<double> uext;  // 3N: Brownian angular velocity of the step (T > 0)
   bool brownian = false;   // uext holds this step's draws
-  int32_t step_iter = 0;   // step index of the running step call (Brownian key)
+  std::vector<int64_t> snap_nstep;  // step counts at pcc_snapshot
   pcc::DArr<int32_t> nc;                        // N
 
   // grid
@@ -329,6 +331,39 @@ struct pcc_ctx {
 };
 
 namespace pcc {
+
+// Device buffers of one detection batch as raw pointers (the context's
+// arrays, or a caller-independent static scratch).  Sizes: N bodies, S
+// neighbour cells per body (9 when every body has z = 0, else 27), hash table
+// of T (a power of two >= 2N) buckets, m records.
+struct DetectBufs {
+  int64_t N = 0;
+  BodyArrays body{};                 // body state (position, orientation, length, diameter read)
+  const int32_t* binst = nullptr;    // colony of every body
+  const InstDev* idev = nullptr;     // colony records (grid, cutoff, warm start, flags)
+  int64_t B = 1;
+  int S = 27;
+  int64_t T = 1024;
+  int4* cell4 = nullptr;             // N
+  double2* geo = nullptr;            // 4N
+  uint32_t* bucket = nullptr;        // N
+  int32_t *perm = nullptr, *bstart = nullptr, *bcursor = nullptr;  // N, T+1, T+1
+  int32_t *dslot = nullptr, *dslot_off = nullptr;                  // N*S+1 each
+  uint64_t* dacc = nullptr;          // N*S+1
+  int32_t *dcount = nullptr, *bbounds = nullptr;                   // N+1, B+1
+  int32_t *rI = nullptr, *rJ = nullptr;                            // m
+  double* rec = nullptr;             // 16 m
+  int32_t *ridx = nullptr, *ridx2 = nullptr;                       // m
+  uint64_t *rkey = nullptr, *rkey2 = nullptr;                      // m
+  void* tmp = nullptr;               // rocPRIM temporary storage
+  size_t tmp_bytes = 0;
+  hipStream_t s = nullptr;
+};
+int64_t table_size_for(int64_t n);
+size_t detect_scan_bytes(int64_t N, int S, int64_t T);
+size_t detect_sort_bytes(int64_t m, int64_t B);
+void detect_count(DetectBufs& d, int32_t* bounds_host);
+void detect_write(DetectBufs& d, int64_t m);
 
 // launchers (kernels in detect.hip / solve.hip / soft.hip)
 void detect_batch(pcc_ctx* ctx);  // active colonies at their cutoffs; fills ctx->ridx sorted

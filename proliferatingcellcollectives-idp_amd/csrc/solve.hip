@@ -222,13 +222,13 @@ __global__ void k_mobility(int64_t N, const double* __restrict__ len,
 // the others.
 __global__ void k_brownian(int64_t N, const double* __restrict__ imp,
                            const int32_t* __restrict__ binst, const InstDev* __restrict__ idev,
-                           int32_t iter, double* __restrict__ uext) {
+                           double* __restrict__ uext) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const InstDev& D = idev[binst[i]];
   double w[3] = {0.0, 0.0, 0.0};
   if (D.temp > 0.0 && (D.flags & kInstActive))
-    brownian_omega(D.temp, D.mono, D.bseed, iter, i - D.boff, imp[i], w);
+    brownian_omega(D.temp, D.mono, D.bseed, D.bstep, i - D.boff, imp[i], w);
   uext[3 * i] = w[0];
   uext[3 * i + 1] = w[1];
   uext[3 * i + 2] = w[2];
@@ -2076,7 +2076,7 @@ void brownian_inputs(pcc_ctx* ctx) {
   ctx->uext.ensure(3 * N + 3);
   BodyArrays b = body_arrays(ctx->body.p, N);
   k_brownian<<<nblk(N), 256, 0, ctx->stream>>>(N, b.imp, ctx->binst.p, ctx->idev.p,
-                                                ctx->step_iter, ctx->uext.p);
+                                                ctx->uext.p);
   ctx->brownian = true;
 }
 

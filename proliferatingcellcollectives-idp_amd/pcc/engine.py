@@ -71,6 +71,8 @@ def lib():
         L.pcc_batch_get_phase_stats.argtypes = [vp, i64, C.POINTER(i64), C.POINTER(i64), i64,
                                                 C.POINTER(i64)]
         L.pcc_device_bytes.argtypes = [vp, C.POINTER(i64)]
+        L.pcc_batch_get_step_count.argtypes = [vp, i64, C.POINTER(i64)]
+        L.pcc_batch_set_step_count.argtypes = [vp, i64, i64]
         L.pcc_get_tuning.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                      C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                      C.POINTER(C.c_double)]
@@ -93,7 +95,7 @@ def lib():
                   "pcc_batch_get_phase_stats", "pcc_device_bytes",
                   "pcc_measure_copy_bandwidth", "pcc_divide", "pcc_batch_divide",
                   "pcc_get_tuning", "pcc_adapt_dt", "pcc_batch_adapt_dt", "pcc_set_trace",
-                  "pcc_get_trace"):
+                  "pcc_get_trace", "pcc_batch_get_step_count", "pcc_batch_set_step_count"):
             getattr(L, f).restype = C.c_int
         _lib = L
     return _lib
@@ -107,7 +109,8 @@ EXPORTED = ("pcc_default_params", "pcc_create", "pcc_destroy", "pcc_set_params",
             "pcc_batch_size", "pcc_batch_step", "pcc_batch_get_state",
             "pcc_batch_get_constraints", "pcc_batch_get_phase_stats", "pcc_device_bytes",
             "pcc_measure_copy_bandwidth", "pcc_divide", "pcc_batch_divide", "pcc_get_tuning",
-            "pcc_adapt_dt", "pcc_batch_adapt_dt", "pcc_set_trace", "pcc_get_trace")
+            "pcc_adapt_dt", "pcc_batch_adapt_dt", "pcc_set_trace", "pcc_get_trace",
+            "pcc_batch_get_step_count", "pcc_batch_set_step_count")
 
 
 def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 10, device: int = 0) -> float:
@@ -217,6 +220,16 @@ class ParticleManager:
         self._check(lib().pcc_get_trace(self._ctx, int(q), buf.ctypes.data_as(C.c_void_p), n,
                                         C.byref(cnt)))
         return buf[:n].copy(), int(cnt.value)
+
+    def step_count(self, q: int = 0) -> int:
+        """Steps colony q has taken since it was set or added (the key of its
+        Brownian draws)."""
+        n = C.c_int64(0)
+        self._check(lib().pcc_batch_get_step_count(self._ctx, int(q), C.byref(n)))
+        return int(n.value)
+
+    def set_step_count(self, count: int, q: int = 0):
+        self._check(lib().pcc_batch_set_step_count(self._ctx, int(q), int(count)))
 
     def device_bytes(self) -> int:
         """Device memory held by this context."""

@@ -99,6 +99,7 @@ class Checkpoint:
     step: int = 0
     simulation_time_s: float = 0.0
     user: dict = field(default_factory=dict)
+    step_counts: Optional[List[int]] = None  # each colony's step count (Brownian key)
 
 
 def _pad(n: int) -> int:
@@ -106,9 +107,11 @@ def _pad(n: int) -> int:
 
 
 def save_checkpoint(path: str, colonies, params=None, step: int = 0,
-                    simulation_time_s: float = 0.0, user: Optional[dict] = None) -> int:
+                    simulation_time_s: float = 0.0, user: Optional[dict] = None,
+                    step_counts=None) -> int:
     """Writes `colonies` (one Bodies or a list of them) with their `params`
-    (one Params per colony, or one for all, or None); returns the file size."""
+    (one Params per colony, or one for all, or None) and, if given, each
+    colony's step count; returns the file size."""
     if isinstance(colonies, T.Bodies):
         colonies = [colonies]
     colonies = list(colonies)
@@ -126,6 +129,7 @@ def save_checkpoint(path: str, colonies, params=None, step: int = 0,
         "step": int(step),
         "simulation_time_s": float(simulation_time_s),
         "params": None if params is None else [params_to_dict(p) for p in params],
+        "step_counts": None if step_counts is None else [int(x) for x in step_counts],
         "user": user or {},
         "fields": {},
     }
@@ -217,7 +221,8 @@ def load_checkpoint(path: str, mmap: bool = False) -> Checkpoint:
         colonies.append(b)
         o += sz
     params = None if h["params"] is None else [params_from_dict(d) for d in h["params"]]
-    return Checkpoint(colonies, params, h["step"], h["simulation_time_s"], h["user"])
+    return Checkpoint(colonies, params, h["step"], h["simulation_time_s"], h["user"],
+                      h.get("step_counts"))
 
 
 def checkpoint_context(pm, path: str, step: int = 0, simulation_time_s: float = 0.0,
@@ -227,8 +232,11 @@ def checkpoint_context(pm, path: str, step: int = 0, simulation_time_s: float = 
     from .engine import ColonyBatch
     if isinstance(pm, ColonyBatch):
         cols = [pm.get_state(q) for q in range(len(pm))]
-        return save_checkpoint(path, cols, list(pm.colony_params), step, simulation_time_s, user)
-    return save_checkpoint(path, [pm.get_state()], [pm.params], step, simulation_time_s, user)
+        counts = [pm.step_count(q) for q in range(len(pm))]
+        return save_checkpoint(path, cols, list(pm.colony_params), step, simulation_time_s, user,
+                               counts)
+    return save_checkpoint(path, [pm.get_state()], [pm.params], step, simulation_time_s, user,
+                           [pm.step_count()])
 
 
 def restore_context(path: str, device: int = 0, batch: Optional[bool] = None):
@@ -247,9 +255,13 @@ def restore_context(path: str, device: int = 0, batch: Optional[bool] = None):
             raise ValueError("a ParticleManager holds one colony")
         pm = ParticleManager(ck.params[0], device=device)
         pm.set_state(ck.colonies[0])
+        if ck.step_counts:
+            pm.set_step_count(ck.step_counts[0])
         return pm, ck
     cb = ColonyBatch(ck.params[0], device=device)
     cb.set_state(ck.colonies, ck.params)
+    for q, n in enumerate(ck.step_counts or []):
+        cb.set_step_count(n, q)
     return cb, ck
 
 

@@ -112,8 +112,54 @@ def test_soft_brownian_matches():
     pm.set_state(b)
     want = b
     for k in range(5):
-        st, res, want, *_ = oracle.step(p, want, k)
+        st, res, want, *_ = oracle.step(p, want, k)  # k: the colony's step count
         pm.step(k)
-        same(pm.get_state(), want, rtol=1e-13)
-        pm.set_state(want)  # restart from the oracle's state: no drift from pow()
+        same(pm.get_state(), want, rtol=1e-11)  # pow()'s last bit may differ
+    assert pm.step_count() == 5
+    pm.close()
+
+
+def test_step_count_keys_and_snapshot():
+    """The draws follow the colony's own step count: a colony added to a
+    batch that has already stepped gets the draws it gets alone; the device
+    snapshot keeps the count, and setting it resumes a run."""
+    from pcc.engine import ColonyBatch, ParticleManager
+    p = hard()
+    a, c = colony(400, 61), colony(350, 62)
+    alone = ParticleManager(p)
+    alone.set_state(c)
+    alone.step(0)
+    alone.step(1)
+    want = alone.get_state()
+    assert alone.step_count() == 2
+    alone.close()
+    cb = ColonyBatch(p)
+    cb.set_state([a], [p])
+    cb.step(0)
+    q = cb.add(c, p)
+    cb.step(7)  # the call's step index does not key the draws
+    cb.step(8)
+    same(cb.get_state(q), want)
+    assert cb.step_count(0) == 3 and cb.step_count(q) == 2
+    cb.close()
+    # snapshot / restore and set_step_count
+    pm = ParticleManager(p)
+    pm.set_state(c)
+    pm.step(0)
+    pm.snapshot()
+    pm.step(1)
+    first = pm.get_state()
+    pm.restore()
+    assert pm.step_count() == 1
+    pm.step(1)
+    same(pm.get_state(), first)
+    pm.set_state(c)
+    pm.step(0)
+    mid = pm.get_state()
+    pm.close()
+    pm = ParticleManager(p)
+    pm.set_state(mid)
+    pm.set_step_count(1)
+    pm.step(1)
+    same(pm.get_state(), want)
     pm.close()
