@@ -59,9 +59,12 @@ HBM_PEAK_GBS = 8000.0
 
 
 def params_for(precision):
+    """Hard model of the configured workloads: rotational diffusion off
+    (temperature 0), dt 1e-3, tolerances 1e-6, growth at ln 2."""
     return reference_params(dt=1e-3, tolerance=1e-6, tau=1 / math.log(2), lam=1e-2,
                             cutoff_phase0=0.05, cutoff_phaseK=0.05, relcp_tol=1e-6,
-                            precision=precision, max_bbpgd_iterations=2_000_000)
+                            precision=precision, max_bbpgd_iterations=2_000_000,
+                            temperature=0.0)
 
 
 def run_single(colonies, precision, steps, warmup):
@@ -231,52 +234,64 @@ def cpu_baseline(bodies, phases, sample_caps=(10, 110)):
 
 
 def ensemble_colonies(n_colonies, seed, lo=500, hi=50_000):
-    u = np.array(G.uniform(seed + 977, 0, n_colonies))
-    sizes = np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo))).astype(int)
-    return [G.packed_colony(int(n), seed=seed + 1000 + i) for i, n in enumerate(sizes)]
+    return G.ensemble(n_colonies, seed, lo, hi)
 
 
-def main_ensemble(args):
-    cols = ensemble_colonies(args.colonies, args.seed)
+def ensemble_summary(n_colonies, seed, steps, warmup):
+    """Config 4: the ensemble stepped `steps` times in one batched context."""
+    cols = ensemble_colonies(n_colonies, seed)
     ncell = sum(c.n for c in cols)
     print(f"[bench] ensemble: {len(cols)} colonies, {ncell} cells, sizes "
           f"{min(c.n for c in cols)}..{max(c.n for c in cols)}", file=sys.stderr, flush=True)
-    steps = args.steps if args.steps_given else 50
-    r = run_batched(cols, PREC_MIXED, steps, args.warmup, timing=True, restore=False, report=5,
+    r = run_batched(cols, PREC_MIXED, steps, warmup, timing=True, restore=False, report=10,
                     divide=True)
     p50, p99 = latency_stats(r["lat"])
     its = [sum(x["bbpgd_iterations"] for x in step) for step in r["results"]]
     rf = roofline(r["timing"])
-    line = {
-        "metric": "cell-steps/s, batched ensemble of independent colonies; p50/p99 step latency",
-        "value": r["cell_steps"] / r["total_s"],
-        "unit": "cell-steps/s",
-        "n_gpus": 1,
-        "steps": steps,
-        "warmup": args.warmup,
+    return {
+        "cell_steps_per_s": r["cell_steps"] / r["total_s"],
         "ms_per_step": r["total_s"] / steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32+f64",
-        "data": "synthetic",
-        "config": {"workload": (f"{len(cols)} independent hard-model colony packings (disc "
-                                f"nematic domains, relaxed by the warm-up step), sizes "
-                                f"log-uniform in [500, 50000] ({ncell} cells), mixed precision, "
-                                f"{steps} consecutive time-loop steps (device division seeded "
-                                f"per colony and step, N(0, 0.01 rad) turns, then the step) in "
-                                f"one batched context with captured BBPGD graphs and "
-                                f"per-colony convergence"),
-                   "n_cells": ncell, "n_cells_end": int(sum(r["cells_end"])),
-                   "colonies": len(cols)},
-        "p50_step_ms": p50,
-        "p99_step_ms": p99,
+        "p50_step_ms": p50, "p99_step_ms": p99, "steps": steps,
+        "colonies": len(cols), "n_cells": ncell, "n_cells_end": int(sum(r["cells_end"])),
         "bbpgd_iterations_per_step": {"min": int(min(its)), "max": int(max(its))},
         "roofline": {"bound": "hbm", "kernel": rf["kernel"], "achieved": rf["gbs"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": rf["gbs"] / HBM_PEAK_GBS,
                      "traffic": None, "avg_launch_us": rf["avg_us"],
                      "algorithmic_bytes_per_launch": rf["bytes"]},
         "tuning": r["tuning"],
+    }
+
+
+def main_ensemble(args):
+    steps = args.steps if args.steps_given else 50
+    e = ensemble_summary(args.colonies, args.seed, steps, args.warmup)
+    line = {
+        "metric": "cell-steps/s, batched ensemble of independent colonies; p50/p99 step latency",
+        "value": e["cell_steps_per_s"],
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": e["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": (f"{e['colonies']} independent hard-model colony packings (disc "
+                                f"nematic domains, relaxed by the warm-up step), sizes "
+                                f"log-uniform in [500, 50000] ({e['n_cells']} cells), mixed "
+                                f"precision, {steps} consecutive time-loop steps (device "
+                                f"division seeded per colony and step, N(0, 0.01 rad) turns, "
+                                f"then the step) in one batched context with captured BBPGD "
+                                f"graphs and per-colony convergence"),
+                   "n_cells": e["n_cells"], "n_cells_end": e["n_cells_end"],
+                   "colonies": e["colonies"]},
+        "p50_step_ms": e["p50_step_ms"],
+        "p99_step_ms": e["p99_step_ms"],
+        "bbpgd_iterations_per_step": e["bbpgd_iterations_per_step"],
+        "roofline": e["roofline"],
+        "tuning": e["tuning"],
     }
     print(json.dumps(line))
 
@@ -297,8 +312,7 @@ def serving_requests(rate, duration, seed, steps, lo=1_000, hi=100_000):
         rate_g = rng.uniform(0.5, 1.5) * math.log(2)
         soft = rng.uniform() < 0.5
         if soft:
-            p = reference_params(mode=MODE_SOFT, dt=1e-3 / 30, tolerance=1e-6, tau=1 / rate_g,
-                                 kcc=4.0 / 3.0 * 1e4, precision=PREC_MIXED)
+            p = soft_params(PREC_MIXED).replace(tau=1 / rate_g)
         else:
             p = params_for(PREC_MIXED).replace(tau=1 / rate_g)
         reqs.append(Request(len(reqs), G.packed_colony(n, seed=seed * 100_000 + len(reqs)), p,
@@ -312,7 +326,8 @@ def soft_params(precision):
     the hard model's, growth at ln2."""
     from pcc.abi_types import MODE_SOFT
     return reference_params(mode=MODE_SOFT, dt=1e-3 / 30, tolerance=1e-6, tau=1 / math.log(2),
-                            lam=1e-2, kcc=4.0 / 3.0 * 1e4, precision=precision)
+                            lam=1e-2, kcc=4.0 / 3.0 * 1e4, precision=precision,
+                            temperature=0.0)
 
 
 def run_soft(bodies, precision, steps, warmup):
@@ -379,52 +394,73 @@ def cpu_soft_baseline(bodies, reps=3):
                        f"{bodies.n}-cell colony, {dt * 1e3:.1f} ms per step (no division)")}
 
 
+def soft_summary(n, seed, steps, warmup, fp64=True, cpu=False):
+    """Config 3: `steps` soft time-loop steps of an n-cell colony packing."""
+    b = G.dense_colony(n, seed=seed)
+    print(f"[bench] soft: n={b.n}, {steps} steps", file=sys.stderr, flush=True)
+    rm = run_soft(b, PREC_MIXED, steps, warmup)
+    p50, p99 = latency_stats(rm["lat"])
+    km = kernel_table(rm["timing"])
+    h = km.get("hertz", {})
+    out = {
+        "cell_steps_per_s": rm["cell_steps"] / rm["total_s"],
+        "ms_per_step": rm["total_s"] / steps * 1e3,
+        "p50_step_ms": p50, "p99_step_ms": p99, "steps": steps,
+        "n_cells": b.n, "n_cells_end": rm["n_end"], "n_contacts": rm["n_constraints"],
+        "max_overlap": rm["max_overlap"],
+        "roofline": {"bound": "hbm", "kernel": "hertz", "achieved": h.get("gbs"),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": h.get("frac"),
+                     "traffic": None, "avg_launch_us": h.get("avg_us"),
+                     "algorithmic_bytes_per_launch": h.get("algorithmic_bytes_per_launch")},
+        "kernels": km,
+    }
+    if fp64:
+        r64 = run_soft(b, PREC_FP64, steps, warmup)
+        out["fp64"] = {"cell_steps_per_s": r64["cell_steps"] / r64["total_s"],
+                       "ms_per_step": r64["total_s"] / steps * 1e3,
+                       "kernels": kernel_table(r64["timing"])}
+    if cpu:
+        out["cpu_baseline"] = cpu_soft_baseline(b)
+    return out
+
+
 def main_soft(args):
     from pcc.engine import copy_bandwidth
     n = args.n if args.n != 300_000 else 100_000
     steps = args.steps if args.steps_given else 100
-    b = G.dense_colony(n, seed=args.seed)
-    print(f"[bench] soft: n={b.n}, {steps} steps", file=sys.stderr, flush=True)
     copy_gbs = copy_bandwidth(1 << 30, 10)
-    rm = run_soft(b, PREC_MIXED, steps, args.warmup)
-    r64 = run_soft(b, PREC_FP64, steps, args.warmup)
-    p50, p99 = latency_stats(rm["lat"])
-    km = kernel_table(rm["timing"])
-    h = km.get("hertz", {})
+    sm = soft_summary(n, args.seed, steps, args.warmup, fp64=True, cpu=not args.no_cpu)
+    rf = dict(sm["roofline"])
+    rf["achievable_copy_gbs"] = copy_gbs
     line = {
         "metric": "cell-steps/s, soft Hertz model; p50/p99 step latency",
-        "value": rm["cell_steps"] / rm["total_s"],
+        "value": sm["cell_steps_per_s"],
         "unit": "cell-steps/s",
         "n_gpus": 1,
         "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": rm["total_s"] / steps * 1e3,
+        "ms_per_step": sm["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": (f"soft Hertz model, one {b.n}-cell colony packing (nematic "
-                                f"domains), F = (4/3) E* sqrt(r_eff) delta^1.5 with E* = 1e4, "
-                                f"r_eff = 0.125, dt = 1e-3/30, {steps} time-loop steps (device "
-                                f"division, then the step), Hertz sums FP32, integration FP64"),
-                   "n_cells": b.n, "n_cells_end": rm["n_end"],
-                   "n_contacts": rm["n_constraints"]},
-        "p50_step_ms": p50,
-        "p99_step_ms": p99,
-        "roofline": {"bound": "hbm", "kernel": "hertz", "achieved": h.get("gbs"),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": h.get("frac"),
-                     "traffic": None, "avg_launch_us": h.get("avg_us"),
-                     "algorithmic_bytes_per_launch": h.get("algorithmic_bytes_per_launch"),
-                     "achievable_copy_gbs": copy_gbs},
-        "kernels": km,
-        "fp64": {"cell_steps_per_s": r64["cell_steps"] / r64["total_s"],
-                 "ms_per_step": r64["total_s"] / steps * 1e3,
-                 "kernels": kernel_table(r64["timing"])},
-        "max_overlap": rm["max_overlap"],
+        "config": {"workload": (f"soft Hertz model, one {sm['n_cells']}-cell colony packing "
+                                f"(nematic domains), F = (4/3) E* sqrt(r_eff) delta^1.5 with "
+                                f"E* = 1e4, r_eff = 0.125, dt = 1e-3/30, {steps} time-loop steps "
+                                f"(device division, then the step), Hertz sums FP32, "
+                                f"integration FP64"),
+                   "n_cells": sm["n_cells"], "n_cells_end": sm["n_cells_end"],
+                   "n_contacts": sm["n_contacts"]},
+        "p50_step_ms": sm["p50_step_ms"],
+        "p99_step_ms": sm["p99_step_ms"],
+        "roofline": rf,
+        "kernels": sm["kernels"],
+        "fp64": sm["fp64"],
+        "max_overlap": sm["max_overlap"],
     }
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_soft_baseline(b)
+    if "cpu_baseline" in sm:
+        line["cpu_baseline"] = sm["cpu_baseline"]
     print(json.dumps(line))
 
 
@@ -550,6 +586,8 @@ def main():
                     help="seconds of serving after the last arrival before the run stops")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["both", "fp64", "mixed"], default="both")
+    ap.add_argument("--no-extras", dest="extras", action="store_false",
+                    help="headline: skip the soft (100k) and ensemble (128 colonies) summaries")
     args = ap.parse_args()
     args.steps_given = args.steps is not None
     if args.steps is None:
@@ -669,6 +707,25 @@ def main():
     line["tuning"] = {k: v["tuning"] for k, v in out.items()}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(bodies, (out.get("fp64") or head)["phases"])
+    if args.extras:
+        # the other configured workloads, at their configured sizes and step
+        # counts, as summaries beside the headline
+        sm = soft_summary(100_000, args.seed, 100, 1, fp64=False)
+        line["soft_100k"] = {k: sm[k] for k in ("cell_steps_per_s", "ms_per_step",
+                                                "p50_step_ms", "p99_step_ms", "steps",
+                                                "n_cells", "n_contacts", "roofline")}
+        line["soft_100k"]["workload"] = ("soft Hertz model, 100,000-cell colony packing, "
+                                         "E* = 1e4, r_eff = 0.125, dt = 1e-3/30, 100 time-loop "
+                                         "steps (device division, then the step), mixed")
+        e = ensemble_summary(128, args.seed, 50, 1)
+        line["ensemble_128"] = {k: e[k] for k in ("cell_steps_per_s", "ms_per_step",
+                                                  "p50_step_ms", "p99_step_ms", "steps",
+                                                  "colonies", "n_cells", "n_cells_end",
+                                                  "roofline")}
+        line["ensemble_128"]["workload"] = ("128 hard-model colony packings, log-uniform "
+                                            "[500, 50000] cells, 50 time-loop steps (device "
+                                            "division, then the step), mixed, one batched "
+                                            "context with captured BBPGD graphs")
     print(json.dumps(line))
 
 

@@ -210,6 +210,14 @@ def dense_colony(n: int, seed: int = 0, diameter: float = 0.5, domain: float = 3
     return _sorted_bodies(pos, np.mod(theta[keep] + noise, math.pi), length[keep], diameter)
 
 
+def ensemble(n_colonies: int, seed: int = 0, lo: int = 500, hi: int = 50_000):
+    """Colony packings (packed_colony) with sizes log-uniform in [lo, hi],
+    colony i from seed + 1000 + i: the ensemble of independent colonies."""
+    u = np.array(uniform(seed + 977, 0, n_colonies))
+    sizes = np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo))).astype(int)
+    return [packed_colony(int(n), seed=seed + 1000 + i) for i, n in enumerate(sizes)]
+
+
 def area_fraction(bodies: Bodies) -> float:
     """Summed spherocylinder area over the area of the enclosing disc."""
     d = bodies.diameter

@@ -210,8 +210,11 @@ class ColonyServer:
         for m in new:
             if self.batches[key] is None:
                 self.batches[key] = ColonyBatch(m.req.params, device=self.device)
-            self.batches[key].add(m.state if m.state is not None else m.req.bodies,
-                                  m.req.params)
+            q = self.batches[key].add(m.state if m.state is not None else m.req.bodies,
+                                      m.req.params)
+            # a colony moved from the warm-up batch keeps its step count (the
+            # key of its Brownian draws), as a context running it alone does
+            self.batches[key].set_step_count(m.done, q)
             m.state = None
         if not ms:
             return False

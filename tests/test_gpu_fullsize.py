@@ -109,13 +109,17 @@ def test_fp64_and_mixed_single_phase_vs_oracle_dense_colony():
     rm = pm.step()
     cm = pm.constraints()
     sm = pm.get_state()
+    # phase 0's contacts (the set also holds the phase-1 detection batch,
+    # whose geometry follows each path's own positions after phase 0)
+    m0, g0 = wc["iteration"] == 0, cm["iteration"] == 0
     for k in ("gidI", "gidJ", "normal", "rPosI", "rPosJ", "contact_point", "gid"):
-        assert np.array_equal(cm[k], wc[k]), k
-    eg = float(np.linalg.norm(cm["gamma"] - wc["gamma"]) / np.linalg.norm(wc["gamma"]))
-    egmax = rel(cm["gamma"], wc["gamma"])
+        assert np.array_equal(cm[k][g0], wc[k][m0]), k
+    gm, gw = cm["gamma"][g0], wc["gamma"][m0]
+    eg = float(np.linalg.norm(gm - gw) / np.linalg.norm(gw))
+    egmax = rel(gm, gw)
     eu = rel(np.concatenate([sm.velocity, sm.omega], 1),
              np.concatenate([want.velocity, want.omega], 1))
-    print(f"[mixed vs oracle, 300k one phase] C={res['n_constraints']} its "
+    print(f"[mixed vs oracle, 300k one phase] C0={int(m0.sum())} its "
           f"oracle={res['bbpgd_iterations']} mixed={rm['bbpgd_iterations']} "
           f"gamma_err normwise={eg:.3e} max={egmax:.3e} U_err={eu:.3e}", flush=True)
     assert rm["residual"] <= p.bbpgd_tol

@@ -116,3 +116,52 @@ def test_concurrent_contexts_allocate_during_graph_capture():
             pm.step(k)
         assert_same_state(got, pm.get_state())
         pm.close()
+
+
+def test_config5_request_mix_under_a_binding_budget():
+    """Config 5's request mix over a short horizon: Poisson arrivals at 20/s,
+    colonies of 1k-100k cells (log-uniform, the largest hard and soft ones
+    near 100k), hard or soft 50/50, growth rate U[0.5, 1.5] ln 2, 3 steps
+    each.  The HBM budget holds about two of the largest colonies, so
+    requests wait for memory.  The budget is never exceeded, every request
+    completes, and the largest hard and the largest soft request equal their
+    alone runs bit for bit."""
+    import numpy as np
+    from pcc.engine import ParticleManager
+    from pcc.serving import ColonyServer, Request, division_seed
+    rng = np.random.default_rng(5)
+    sizes = [int(np.exp(rng.uniform(np.log(1e3), np.log(1e5)))) for _ in range(8)]
+    sizes[0], sizes[1] = 99_000, 97_000  # the largest hard and the largest soft
+    reqs, t = [], 0.0
+    for rid, n in enumerate(sizes):
+        t += rng.exponential(1 / 20.0)
+        mode = T.MODE_HARD if rid % 2 == 0 else T.MODE_SOFT
+        rate = rng.uniform(0.5, 1.5) * LN2
+        p = request_params(mode, rate).replace(temperature=0.0)
+        reqs.append(Request(rid, G.packed_colony(n, seed=300 + rid), p, 3, t))
+    bpc = 12_000
+    srv = ColonyServer(1, precision=T.PREC_MIXED, bytes_per_cell=bpc)
+    budget = int(2.2 * max(srv.reservation(r) for r in reqs))
+    srv = ColonyServer(budget, precision=T.PREC_MIXED, bytes_per_cell=bpc)
+    print(f"[config5] sizes {sizes}, budget {budget / 2**30:.2f} GiB", flush=True)
+    st = srv.run(reqs, time_limit=900)
+    srv.close()
+    print(f"[config5] {st}", flush=True)
+    assert st["completed"] == len(reqs)
+    assert st["peak_reserved_bytes"] <= budget
+    assert st["peak_device_bytes"] <= budget
+    waits = [srv.results[r.rid].admitted - r.arrival for r in reqs]
+    assert max(waits) > 0.05, "no request waited for memory"
+    for rid in (0, 1):
+        r = reqs[rid]
+        res = srv.results[rid]
+        assert res.statuses == [T.PCC_OK] * r.steps
+        pm = ParticleManager(r.params)
+        pm.set_state(r.bodies)
+        for k in range(r.steps):
+            pm.divide(division_seed(r.rid, k), 0, 0.0, 0.01)
+            pm.step(k)
+        print(f"[config5] request {rid} ({'hard' if rid == 0 else 'soft'}, {r.bodies.n} cells): "
+              f"served == alone", flush=True)
+        assert_same_state(res.state, pm.get_state())
+        pm.close()

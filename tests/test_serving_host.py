@@ -31,6 +31,10 @@ class FakeBatch:
     def remove(self, q):
         del self.cols[q]
 
+    def set_step_count(self, count, q):
+        assert self.cols[q][1] == 0
+        self.cols[q][1] = count
+
     @property
     def sizes(self):
         return [c[0].n for c in self.cols]
