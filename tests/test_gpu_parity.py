@@ -157,6 +157,7 @@ def test_growth_sequence_fp64_bitexact():
     for k in range(60):
         st, res, want_b, want_c, ph, err = oracle.step(p, b, k)
         pm.set_state(b)
+        pm.set_step_count(k)  # the Brownian key the oracle got (set_state resets it)
         r = pm.step(k)
         assert r["bbpgd_iterations"] == res["bbpgd_iterations"]
         got_b = pm.get_state()
@@ -181,6 +182,7 @@ def test_soft_step_fp64_matches():
         assert_same_constraints(pm.constraints(), want_c, rtol=1e-13)
         b = want_b
         pm.set_state(b)
+        pm.set_step_count(k + 1)
     pm.close()
 
 
