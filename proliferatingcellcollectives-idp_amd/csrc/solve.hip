@@ -72,6 +72,53 @@ __device__ __forceinline__ double proj_step(double yp, double malpha, double gp)
   return v < 0.0 ? 0.0 : v;     // VecPointwiseMax(gamma_i, gamma_i, 0)
 }
 
+// Iterate record of constraint c (solver order): the impulse gamma and the
+// gradient g of the last evaluated iterate.  FP64: one double2 (gamma, g) per
+// constraint in `it`.  Mixed: gamma in FP64 (it[c]) and g in FP32 (itg[c]);
+// g is rounded once where it is evaluated, and every later use (projection,
+// residual, BB dot products, the linearised gap) reads the rounded value.
+// Phi likewise: FP64, or FP32 in mixed precision.
+template <class CT>
+__device__ __forceinline__ double2 load_it(const double* __restrict__ it,
+                                           const float* __restrict__ itg, int64_t c) {
+  if constexpr (sizeof(CT) == 8)
+    return reinterpret_cast<const double2*>(it)[c];
+  else
+    return make_double2(it[c], (double)itg[c]);
+}
+template <class CT>
+__device__ __forceinline__ double load_gamma(const double* __restrict__ it, int64_t c) {
+  if constexpr (sizeof(CT) == 8)
+    return it[2 * c];
+  else
+    return it[c];
+}
+template <class CT>
+__device__ __forceinline__ void store_it(double* __restrict__ it, float* __restrict__ itg,
+                                         int64_t c, double gam, double g) {
+  if constexpr (sizeof(CT) == 8) {
+    reinterpret_cast<double2*>(it)[c] = make_double2(gam, g);
+  } else {
+    it[c] = gam;
+    itg[c] = (float)g;
+  }
+}
+template <class CT>
+__device__ __forceinline__ double load_phi(const void* __restrict__ phi, int64_t c) {
+  if constexpr (sizeof(CT) == 8)
+    return static_cast<const double*>(phi)[c];
+  else
+    return (double)static_cast<const float*>(phi)[c];
+}
+// g as stored: FP64, or rounded to FP32 in mixed precision.
+template <class CT>
+__device__ __forceinline__ double stored_g(double g) {
+  if constexpr (sizeof(CT) == 8)
+    return g;
+  else
+    return (double)(float)g;
+}
+
 // Logical block of hardware block h in a grid of gp blocks (gp % 8 == 0).
 __device__ __forceinline__ int64_t xcd_block(int64_t h, int64_t gp) {
   return (h % kXcd) * (gp / kXcd) + h / kXcd;
@@ -113,21 +160,24 @@ __global__ void k_invert(int64_t n, const int32_t* __restrict__ ord, int32_t* __
 // (the 2D monolayer operator: every body has z = 0 and q = (w, 0, 0, z), so
 // n_z, r_z, the x/y torque components and U_z, omega_x, omega_y are exact
 // zeros); 3D the packed row (n, rI x n, rJ x -n, sI, sJ, pad).
-template <bool PLANAR>
+template <class CT, bool PLANAR>
 __global__ void k_solver_rows(int64_t C, const int32_t* __restrict__ ord,
                               const int32_t* __restrict__ I, const int32_t* __restrict__ J,
                               const double* __restrict__ phi, const double* __restrict__ gam,
                               const double* __restrict__ coef, int32_t* __restrict__ sI,
-                              int32_t* __restrict__ sJ, double* __restrict__ sphi,
-                              double2* __restrict__ it, double* __restrict__ row,
-                              float* __restrict__ row32) {
+                              int32_t* __restrict__ sJ, void* __restrict__ sphi,
+                              double* __restrict__ it, float* __restrict__ itg,
+                              double* __restrict__ row, float* __restrict__ row32) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= C) return;
   const int64_t c = ord[p];
   sI[p] = I[c];
   sJ[p] = J[c];
-  sphi[p] = phi[c];
-  it[p] = make_double2(gam[c], 0.0);
+  if constexpr (sizeof(CT) == 8)
+    static_cast<double*>(sphi)[p] = phi[c];
+  else
+    static_cast<float*>(sphi)[p] = (float)phi[c];
+  store_it<CT>(it, itg, p, gam[c], 0.0);
   const double* r = coef + 12 * c;
   if (PLANAR) {
     const double v[6] = {r[0], r[1], r[5], r[8], r[9], r[10]};
@@ -264,7 +314,8 @@ struct BodyArgs {
   const int32_t* rc;   // 2D: constraint index per incidence
   const double* gold;  // gamma_old per incidence (FP64 kernels)
   const double* fold;  // mixed kernels: D^T gamma_old per body (3D: 6, planar: x, y, z-torque)
-  double* it;          // iterate records (gamma, g), solver order
+  double* it;          // iterate records (gamma, g), solver order (mixed: gamma)
+  float* itg;          // mixed: g, solver order
   const double* mob;
   const double* len;
   const double* ldot_prev;
@@ -514,7 +565,7 @@ __global__ void __launch_bounds__(256) k_body_final(BodyArgs a, FinalOut fo) {
     } else {
       load_rec<CT>(a.rec, e, co, c);
     }
-    const double gc = reinterpret_cast<const double2*>(a.it)[c].x;
+    const double gc = load_gamma<CT>(a.it, c);
     const double dg = FOLD ? gc : gc - a.gold[e];
     if (PLANAR) {
       F[0] += dg * co[0];
@@ -615,7 +666,7 @@ __global__ void __launch_bounds__(TPB) k_body_coop(BodyArgs a) {
     double gold[PER];
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-      yg[j] = reinterpret_cast<const double2*>(a.it)[raw[j].index()];
+      yg[j] = load_it<CT>(a.it, a.itg, raw[j].index());
       if (!FOLD) gold[j] = a.gold[ok[j] ? base + (int32_t)threadIdx.x + j * TPB : E1 - 1];
     }
     IncRaw<CT, PLANAR> cur[PER];
@@ -694,8 +745,9 @@ struct ConstrArgs {
   const int32_t* I;
   const int32_t* J;
   const void* coef;     // 3D: 12 CT per constraint (n, rIxn, rJx-n, sI, sJ, pad); 2D: 6 CT
-  const double* phi;    // Phi, solver order
+  const void* phi;      // Phi, solver order (mixed: FP32)
   double* it;
+  float* itg;           // mixed: g
   double* gprev;        // lagging iterate saved on iterations % 10 == 0
   const double* ubuf;
   SolverState* st;      // one per colony
@@ -785,8 +837,8 @@ __global__ void __launch_bounds__(256) k_constr(ConstrArgs a) {
   auto fetch = [&](int64_t c, TileIn<CT, PLANAR>& t) {
     t.I = a.I[c];
     t.J = a.J[c];
-    t.yg = reinterpret_cast<const double2*>(a.it)[c];
-    t.phi = a.phi[c];
+    t.yg = load_it<CT>(a.it, a.itg, c);
+    t.phi = load_phi<CT>(a.phi, c);
     t.row = load_row<CT, PLANAR>(a.coef, c);
   };
   double v0 = 0.0, v1 = 0.0, v2 = 0.0, vr = 0.0, ve = 0.0;
@@ -837,9 +889,9 @@ __global__ void __launch_bounds__(256) k_constr(ConstrArgs a) {
       pg += (double)co[9] * ui[3].x;
       pg += (double)co[10] * uj[3].x;
     }
-    const double g = cur.phi + (dt * pd + (-dt) * pg);
+    const double g = stored_g<CT>(cur.phi + (dt * pd + (-dt) * pg));
     if (save_prev) a.gprev[c] = yg.x;
-    reinterpret_cast<double2*>(a.it)[c] = make_double2(gc, g);
+    store_it<CT>(a.it, a.itg, c, gc, g);
     if (k > 0) {
       const double dgam = gc - yg.x;
       const double dphi = g - yg.y;
@@ -904,7 +956,7 @@ __global__ void __launch_bounds__(256) k_energy_body(BodyArgs a, double* __restr
       } else {
         load_rec<CT>(a.rec, e, co, c);
       }
-      const double gc = reinterpret_cast<const double2*>(a.it)[c].x;
+      const double gc = load_gamma<CT>(a.it, c);
       if (PLANAR) {
         F[0] += gc * co[0];
         F[1] += gc * co[1];
@@ -1150,7 +1202,7 @@ __global__ void __launch_bounds__(kPersistThreads)
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
           if (ok[j]) {
-            yg[j] = reinterpret_cast<const double2*>(ba.it)[cc[j]];
+            yg[j] = load_it<CT>(ba.it, ba.itg, cc[j]);
             if (!FOLD) gold[j] = ba.gold[base + tid + j * kPersistThreads];
           }
         }
@@ -1216,8 +1268,8 @@ __global__ void __launch_bounds__(kPersistThreads)
         const int64_t c = D.coff + (b * K + j) * PCC_BLOCK + lt;
         if (c >= cend) break;
         const int32_t I = ca.I[c], J = ca.J[c];
-        const double2 yg = reinterpret_cast<const double2*>(ca.it)[c];
-        const double phi = ca.phi[c];
+        const double2 yg = load_it<CT>(ca.it, ca.itg, c);
+        const double phi = load_phi<CT>(ca.phi, c);
         const Row<CT, PLANAR> row = load_row<CT, PLANAR>(ca.coef, c);
         const double2* UI = reinterpret_cast<const double2*>(ca.ubuf + UW * (int64_t)I);
         const double2* UJ = reinterpret_cast<const double2*>(ca.ubuf + UW * (int64_t)J);
@@ -1256,9 +1308,9 @@ __global__ void __launch_bounds__(kPersistThreads)
           pg += (double)co[9] * ui[3].x;
           pg += (double)co[10] * uj[3].x;
         }
-        const double g = phi + (dt * pd + (-dt) * pg);
+        const double g = stored_g<CT>(phi + (dt * pd + (-dt) * pg));
         if (save_prev) ca.gprev[c] = yg.x;
-        reinterpret_cast<double2*>(ca.it)[c] = make_double2(gc, g);
+        store_it<CT>(ca.it, ca.itg, c, gc, g);
         if (k > 0) {
           const double dgam = gc - yg.x;
           const double dphi = g - yg.y;
@@ -1394,10 +1446,11 @@ __global__ void k_integrate(int64_t N, BodyArrays b, const int32_t* __restrict__
 // updateConstraintsFromSolution (Hard.cpp:49-89): gamma <- lagging iterate
 // (saved at the converging iteration), phi <- linearised gap of the last
 // evaluated iterate; read at the constraint's solver position.
+template <class CT>
 __global__ void k_update_constraints(int64_t C, const int32_t* __restrict__ Sinst,
                                      const InstDev* __restrict__ idev,
                                      const int32_t* __restrict__ spos,
-                                     const double2* __restrict__ it,
+                                     const double* __restrict__ it, const float* __restrict__ itg,
                                      const double* __restrict__ gprev,
                                      double* __restrict__ Sgam, double* __restrict__ Sphi) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1405,7 +1458,7 @@ __global__ void k_update_constraints(int64_t C, const int32_t* __restrict__ Sins
   if (!(idev[Sinst[c]].flags & kInstInteg)) return;
   const int32_t p = spos[c];
   Sgam[c] = gprev[p];
-  Sphi[c] = it[p].y;
+  Sphi[c] = load_it<CT>(it, itg, p).y;
 }
 
 // Per colony, max over its constraints of max(0, -phi) (Hard.cpp:125-130).
@@ -1477,6 +1530,14 @@ bool bodies_planar(pcc_ctx* ctx) {
   return h == 0;
 }
 
+// Mixed precision: the FP32 g of the iterate records, stored after the C + 1
+// FP64 impulses in the same allocation (2 (C + 1) doubles hold both); null
+// in FP64, where (gamma, g) pairs fill the allocation.
+static float* iter_g(pcc_ctx* ctx) {
+  if (ctx->params.precision != PCC_PREC_MIXED) return nullptr;
+  return reinterpret_cast<float*>(ctx->iter.p + (ctx->S.C + 1));
+}
+
 // Body-incidence CSR over the whole constraint set, each body's incidences in
 // ascending constraint order, plus the per-incidence coefficient records (3D
 // or planar, FP64 and FP32) and, planar, the compact constraint rows.
@@ -1542,21 +1603,30 @@ void build_incidence(pcc_ctx* ctx, bool records) {
   ctx->sI.ensure(C + 1);
   ctx->sJ.ensure(C + 1);
   ctx->sphi.ensure(C + 1);
-  ctx->iter.ensure(2 * (C + 1));
+  ctx->iter.ensure(2 * (C + 1));  // (gamma, g) pairs; mixed: gamma, then the FP32 g
   ctx->gprev.ensure(C + 1);
   ctx->prow.ensure(W * (C + 1));
   ctx->prow32.ensure(W * (C + 1));
   ConstraintSet& S = ctx->S;
-  if (ctx->planar)
-    k_solver_rows<true><<<nblk(C), 256, 0, s>>>(C, ctx->sord.p, S.I.p, S.J.p, S.phi.p,
-                                                S.gamma.p, S.coef64.p, ctx->sI.p, ctx->sJ.p,
-                                                ctx->sphi.p, (double2*)ctx->iter.p, ctx->prow.p,
-                                                ctx->prow32.p);
-  else
-    k_solver_rows<false><<<nblk(C), 256, 0, s>>>(C, ctx->sord.p, S.I.p, S.J.p, S.phi.p,
-                                                 S.gamma.p, S.coef64.p, ctx->sI.p, ctx->sJ.p,
-                                                 ctx->sphi.p, (double2*)ctx->iter.p, ctx->prow.p,
-                                                 ctx->prow32.p);
+  const bool mixed = ctx->params.precision == PCC_PREC_MIXED;
+  float* itg = iter_g(ctx);
+#define PCC_SOLVER_ROWS(CT, PL)                                                              \
+  k_solver_rows<CT, PL><<<nblk(C), 256, 0, s>>>(C, ctx->sord.p, S.I.p, S.J.p, S.phi.p,      \
+                                                S.gamma.p, S.coef64.p, ctx->sI.p, ctx->sJ.p, \
+                                                ctx->sphi.p, ctx->iter.p, itg, ctx->prow.p,  \
+                                                ctx->prow32.p)
+  if (ctx->planar) {
+    if (mixed)
+      PCC_SOLVER_ROWS(float, true);
+    else
+      PCC_SOLVER_ROWS(double, true);
+  } else {
+    if (mixed)
+      PCC_SOLVER_ROWS(float, false);
+    else
+      PCC_SOLVER_ROWS(double, false);
+  }
+#undef PCC_SOLVER_ROWS
   const int RW = ctx->planar ? 4 : 8;
   ctx->increc.ensure(RW * (2 * C + 1));
   ctx->increc32.ensure(RW * (2 * C + 1));
@@ -1630,6 +1700,7 @@ static BodyArgs body_args(pcc_ctx* ctx) {
   a.gold = ctx->gold_e.p;
   a.fold = ctx->fold.p;
   a.it = ctx->iter.p;
+  a.itg = iter_g(ctx);
   a.mob = ctx->mob.p;
   a.len = b.len;
   a.ldot_prev = b.ldot;
@@ -1654,6 +1725,7 @@ static ConstrArgs constr_args(pcc_ctx* ctx) {
   a.coef = mixed ? (const void*)ctx->prow32.p : (const void*)ctx->prow.p;
   a.phi = ctx->sphi.p;
   a.it = ctx->iter.p;
+  a.itg = iter_g(ctx);
   a.gprev = ctx->gprev.p;
   a.ubuf = ctx->ubuf.p;
   a.st = ctx->state.p;
@@ -1673,22 +1745,26 @@ static ConstrArgs constr_args(pcc_ctx* ctx) {
 double body_bytes(int64_t N, int64_t C, bool mixed, bool planar) {
   const double sc = mixed ? 4 : 8;
   // per incidence: record (8 or 4 coefficient slots [+ 4-byte index]) +
-  // the iterate record's (gamma, g) + gamma_old (FP64 only)
+  // the iterate record's (gamma, g) (FP64 16 bytes; mixed 8 + 4) +
+  // gamma_old (FP64 only)
   // per body: rowptr, mobility (2), length, ldot_prev, velocity record out,
   // and in mixed precision D^T gamma_old (6 or 3 doubles)
   const double rec = planar ? 4 * sc + 4 : 8 * sc;
   const double ub = planar ? 32 : 64;
+  const double yg = mixed ? 12 : 16;
   const double gold_e = mixed ? 0 : 8;
   const double fold = mixed ? (planar ? 24 : 48) : 0;
-  return 2.0 * C * (rec + 16 + gold_e) + N * (4 + 16 + 16 + ub + fold);
+  return 2.0 * C * (rec + yg + gold_e) + N * (4 + 16 + 16 + ub + fold);
 }
 double constr_bytes(int64_t N, int64_t C, bool mixed, bool planar) {
   const double sc = mixed ? 4 : 8;
   // per constraint: I, J; operator row (12 or 6 slots); iterate record in
-  // (gamma, g, Phi) and out (gamma, g).  Per body: velocity record read once.
+  // and out ((gamma, g): FP64 16 bytes, mixed 8 + 4) and Phi (8, mixed 4).
+  // Per body: velocity record read once.
   const double row = planar ? 6 * sc : 12 * sc;
   const double ub = planar ? 32 : 64;
-  return C * (8 + row + 24 + 16) + N * ub;
+  const double yg = mixed ? 12 : 16;
+  return C * (8 + row + sc + 2 * yg) + N * ub;
 }
 
 // Body-kernel variants the autotuner chooses from: staging size (incidences
@@ -2056,10 +2132,19 @@ void integrate_hard(pcc_ctx* ctx) {
     k_integrate<<<g, 256, 0, s>>>(N, b, ctx->binst.p, ctx->idev.p, ctx->U.p, ctx->ldot_new.p,
                                   b.ldot);
   }
-  if (C > 0)
-    k_update_constraints<<<nblk(C), 256, 0, s>>>(C, ctx->S.inst.p, ctx->idev.p, ctx->spos.p,
-                                                 (const double2*)ctx->iter.p, ctx->gprev.p,
-                                                 ctx->S.gamma.p, ctx->S.phi.p);
+  if (C > 0) {
+    const float* itg = iter_g(ctx);
+    if (mixed)
+      k_update_constraints<float><<<nblk(C), 256, 0, s>>>(C, ctx->S.inst.p, ctx->idev.p,
+                                                          ctx->spos.p, ctx->iter.p, itg,
+                                                          ctx->gprev.p, ctx->S.gamma.p,
+                                                          ctx->S.phi.p);
+    else
+      k_update_constraints<double><<<nblk(C), 256, 0, s>>>(C, ctx->S.inst.p, ctx->idev.p,
+                                                           ctx->spos.p, ctx->iter.p, itg,
+                                                           ctx->gprev.p, ctx->S.gamma.p,
+                                                           ctx->S.phi.p);
+  }
 }
 
 // Brownian inputs of the current phase-0 solve or soft step: ctx->uext holds

@@ -122,8 +122,8 @@ def test_config5_request_mix_under_a_binding_budget():
     """Config 5's request mix over a short horizon: Poisson arrivals at 20/s,
     colonies of 1k-100k cells (log-uniform, the largest hard and soft ones
     near 100k), hard or soft 50/50, growth rate U[0.5, 1.5] ln 2, 3 steps
-    each.  The HBM budget holds about two of the largest colonies, so
-    requests wait for memory.  The budget is never exceeded, every request
+    each.  The HBM budget holds the largest colony's reservation but not the
+    two largest together, so requests wait for memory.  The budget is never exceeded, every request
     completes, and the largest hard and the largest soft request equal their
     alone runs bit for bit."""
     import numpy as np
@@ -141,7 +141,7 @@ def test_config5_request_mix_under_a_binding_budget():
         reqs.append(Request(rid, G.packed_colony(n, seed=300 + rid), p, 3, t))
     bpc = 12_000
     srv = ColonyServer(1, precision=T.PREC_MIXED, bytes_per_cell=bpc)
-    budget = int(2.2 * max(srv.reservation(r) for r in reqs))
+    budget = int(1.2 * max(srv.reservation(r) for r in reqs))
     srv = ColonyServer(budget, precision=T.PREC_MIXED, bytes_per_cell=bpc)
     print(f"[config5] sizes {sizes}, budget {budget / 2**30:.2f} GiB", flush=True)
     st = srv.run(reqs, time_limit=900)
