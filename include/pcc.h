@@ -191,6 +191,9 @@ void pcc_default_params(pcc_params* p);
 pcc_status pcc_create(const pcc_params* params, int device, pcc_ctx** out);
 void pcc_destroy(pcc_ctx* ctx);
 pcc_status pcc_set_params(pcc_ctx* ctx, const pcc_params* params);
+/* The message of the context's last failed call; with ctx == NULL, that of
+ * the calling thread's last failed call of an entry point without a context
+ * (pcc_detect_noalloc). */
 const char* pcc_last_error(const pcc_ctx* ctx);
 const char* pcc_status_string(pcc_status s);
 
@@ -220,6 +223,52 @@ pcc_status pcc_get_constraints(pcc_ctx* ctx, pcc_constraints* out, int64_t* coun
  * (bucket of every body and the stable body permutation by bucket). */
 pcc_status pcc_detect(pcc_ctx* ctx, double cutoff, pcc_constraints* out, int64_t* count,
                       uint32_t* bucket_out, int32_t* perm_out);
+
+/* ---------------------------------------------------------------------------
+ * Contact detection that allocates nothing.  No reference counterpart: it is
+ * for callers that cannot allocate device memory (set-up code around HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ *
+ * The same detection batch as pcc_detect (CollisionDetector::detectCollisions,
+ * CollisionDetector.cpp:84-111; records bit-identical, in (contact x, I, J)
+ * order, gid = position, iteration = 0) of ONE colony whose state is already
+ * in device memory, written into device arrays the caller owns.  There is no
+ * context: the library's scratch (grid, per-slot counts, raw records, sort
+ * storage) is one fixed array in device memory per GPU, reserved when the
+ * library's code for this entry point is first loaded on that GPU and held
+ * until the process ends (about 64 MiB).  It serves at most
+ * PCC_NOALLOC_MAX_BODIES bodies and PCC_NOALLOC_MAX_CONTACTS contacts; larger
+ * inputs are rejected with PCC_E_INVALID_ARGUMENT (a batch found to exceed
+ * the contact limit returns that status with *count set and nothing
+ * written).  Every call on one GPU shares that GPU's scratch and leaves its
+ * contents behind: calls that overlap on one GPU (from several threads, or on
+ * several streams) race and must be serialised by the caller.
+ *
+ *   device  the HIP device; it is made current for the call and the
+ *           previous current device restored on return
+ *   stream  a hipStream_t on that device (NULL: the null stream); the call
+ *           is ordered on it and synchronous at return (it reads the record
+ *           count back to the host once)
+ *   params  linked_cell_size, grid_origin and warm_start_factor are used
+ *   bodies  device pointers, gid order: pos (3n, interleaved), quat (4n),
+ *           length (n), diameter (n)
+ *   out     device pointers (pcc_constraints layout); up to out->capacity
+ *           records are written; NULL arrays are skipped
+ *   count   receives the number of contacts found
+ * Errors that have no context are reported by pcc_last_error(NULL) on the
+ * calling thread. */
+#define PCC_NOALLOC_MAX_BODIES 32768
+#define PCC_NOALLOC_MAX_CONTACTS 262144
+typedef struct pcc_device_bodies {
+  int64_t n;
+  const double* pos;
+  const double* quat;
+  const double* length;
+  const double* diameter;
+} pcc_device_bodies;
+pcc_status pcc_detect_noalloc(int device, void* stream, const pcc_params* params, double cutoff,
+                              const pcc_device_bodies* bodies, const pcc_constraints* out,
+                              int64_t* count);
 
 /* Per-ReLCP-phase diagnostics of the last step (the data the reference's
  * SimulationStep/BBPGD loggers print, SimulationLogger.h:7-30): BBPGD

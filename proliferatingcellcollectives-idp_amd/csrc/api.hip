@@ -31,6 +31,11 @@ std::recursive_mutex& device_mutex() {
   return m;
 }
 
+std::string& thread_error() {
+  static thread_local std::string e;
+  return e;
+}
+
 hipStream_t& alloc_stream() {
   static thread_local hipStream_t s = nullptr;
   return s;
@@ -721,7 +726,9 @@ pcc_status pcc_set_params(pcc_ctx* ctx, const pcc_params* params) {
   return PCC_OK;
 }
 
-const char* pcc_last_error(const pcc_ctx* ctx) { return ctx ? ctx->last_error.c_str() : ""; }
+const char* pcc_last_error(const pcc_ctx* ctx) {
+  return ctx ? ctx->last_error.c_str() : pcc::thread_error().c_str();
+}
 
 pcc_status pcc_set_state(pcc_ctx* ctx, const pcc_bodies* b) {
   if (!ctx || !b) return PCC_E_INVALID_ARGUMENT;

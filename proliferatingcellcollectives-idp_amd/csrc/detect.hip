@@ -337,7 +337,38 @@ __global__ void k_scatter_rows(int64_t total, const int64_t* __restrict__ dst,
 
 inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
+// Records [0, min(m, cap)) of the sorted batch into SoA arrays in device
+// memory (the pcc_constraints layout; a NULL array is skipped).
+__global__ void k_records_out(int64_t m, const int32_t* __restrict__ idx,
+                              const int32_t* __restrict__ rI_, const int32_t* __restrict__ rJ_,
+                              const double* __restrict__ rec, pcc_constraints o) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m || i >= o.capacity) return;
+  const int32_t src = idx[i];
+  const double* r = rec + R_LEN * (int64_t)src;
+  if (o.gidI) o.gidI[i] = rI_[src];
+  if (o.gidJ) o.gidJ[i] = rJ_[src];
+  if (o.signed_distance) o.signed_distance[i] = r[R_PHI];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    if (o.normal) o.normal[3 * i + k] = r[R_N + k];
+    if (o.rPosI) o.rPosI[3 * i + k] = r[R_RI + k];
+    if (o.rPosJ) o.rPosJ[3 * i + k] = r[R_RJ + k];
+    if (o.contact_point) o.contact_point[3 * i + k] = r[R_P + k];
+  }
+  if (o.stressI) o.stressI[i] = r[R_SI];
+  if (o.stressJ) o.stressJ[i] = r[R_SJ];
+  if (o.gamma) o.gamma[i] = r[R_G];
+  if (o.gid) o.gid[i] = (int32_t)i;
+  if (o.iteration) o.iteration[i] = 0;
+}
+
 }  // namespace
+
+void detect_records_out(const DetectBufs& d, int64_t m, const pcc_constraints& out_dev) {
+  if (m > 0 && out_dev.capacity > 0)
+    k_records_out<<<nblk(m), 256, 0, d.s>>>(m, d.ridx, d.rI, d.rJ, d.rec, out_dev);
+}
 
 template <class T>
 static void excl_scan(pcc_ctx* ctx, const T* in, T* out, int64_t n) {

@@ -44,6 +44,9 @@ struct StatusError {
 // work already queued on it, so growing never waits for other contexts'
 // streams (a plain hipFree synchronises the whole device).  Outside an API
 // call (no stream set) the plain calls are used.
+// Message of the calling thread's last failed context-free call.
+std::string& thread_error();
+
 hipStream_t& alloc_stream();
 struct StreamScope {
   hipStream_t prev;
@@ -364,6 +367,7 @@ size_t detect_scan_bytes(int64_t N, int S, int64_t T);
 size_t detect_sort_bytes(int64_t m, int64_t B);
 void detect_count(DetectBufs& d, int32_t* bounds_host);
 void detect_write(DetectBufs& d, int64_t m);
+void detect_records_out(const DetectBufs& d, int64_t m, const pcc_constraints& out_dev);
 
 // launchers (kernels in detect.hip / solve.hip / soft.hip)
 void detect_batch(pcc_ctx* ctx);  // active colonies at their cutoffs; fills ctx->ridx sorted
