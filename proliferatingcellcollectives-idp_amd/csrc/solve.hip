@@ -74,33 +74,44 @@ __device__ __forceinline__ double proj_step(double yp, double malpha, double gp)
 
 // Iterate record of constraint c (solver order): the impulse gamma and the
 // gradient g of the last evaluated iterate.  FP64: one double2 (gamma, g) per
-// constraint in `it`.  Mixed: gamma in FP64 (it[c]) and g in FP32 (itg[c]);
-// g is rounded once where it is evaluated, and every later use (projection,
-// residual, BB dot products, the linearised gap) reads the rounded value.
-// Phi likewise: FP64, or FP32 in mixed precision.
+// constraint in `it`.  Mixed: a packed 12-byte record, gamma in FP64 (as its
+// two 32-bit halves) and g in FP32, loaded and stored as three dwords, so that
+// a gather touches one record; g is rounded once where it is evaluated, and
+// every later use (projection, residual, BB dot products, the linearised gap)
+// reads the rounded value.  Phi likewise: FP64, or FP32 in mixed precision.
+struct Rec12 {
+  uint32_t lo, hi;  // gamma's bits
+  float g;
+};
+static_assert(sizeof(Rec12) == 12, "packed iterate record");
 template <class CT>
 __device__ __forceinline__ double2 load_it(const double* __restrict__ it,
                                            const float* __restrict__ itg, int64_t c) {
-  if constexpr (sizeof(CT) == 8)
+  (void)itg;
+  if constexpr (sizeof(CT) == 8) {
     return reinterpret_cast<const double2*>(it)[c];
-  else
-    return make_double2(it[c], (double)itg[c]);
+  } else {
+    const Rec12 r = reinterpret_cast<const Rec12*>(it)[c];
+    const double gam = __hiloint2double((int)r.hi, (int)r.lo);
+    return make_double2(gam, (double)r.g);
+  }
 }
 template <class CT>
 __device__ __forceinline__ double load_gamma(const double* __restrict__ it, int64_t c) {
-  if constexpr (sizeof(CT) == 8)
-    return it[2 * c];
-  else
-    return it[c];
+  return load_it<CT>(it, nullptr, c).x;
 }
 template <class CT>
 __device__ __forceinline__ void store_it(double* __restrict__ it, float* __restrict__ itg,
                                          int64_t c, double gam, double g) {
+  (void)itg;
   if constexpr (sizeof(CT) == 8) {
     reinterpret_cast<double2*>(it)[c] = make_double2(gam, g);
   } else {
-    it[c] = gam;
-    itg[c] = (float)g;
+    Rec12 r;
+    r.lo = (uint32_t)__double2loint(gam);
+    r.hi = (uint32_t)__double2hiint(gam);
+    r.g = (float)g;
+    reinterpret_cast<Rec12*>(it)[c] = r;
   }
 }
 template <class CT>
@@ -1530,13 +1541,9 @@ bool bodies_planar(pcc_ctx* ctx) {
   return h == 0;
 }
 
-// Mixed precision: the FP32 g of the iterate records, stored after the C + 1
-// FP64 impulses in the same allocation (2 (C + 1) doubles hold both); null
-// in FP64, where (gamma, g) pairs fill the allocation.
-static float* iter_g(pcc_ctx* ctx) {
-  if (ctx->params.precision != PCC_PREC_MIXED) return nullptr;
-  return reinterpret_cast<float*>(ctx->iter.p + (ctx->S.C + 1));
-}
+// The iterate records hold g in place (FP64 pairs or packed mixed records):
+// no separate array.
+static float* iter_g(pcc_ctx*) { return nullptr; }
 
 // Body-incidence CSR over the whole constraint set, each body's incidences in
 // ascending constraint order, plus the per-incidence coefficient records (3D
@@ -1603,7 +1610,7 @@ void build_incidence(pcc_ctx* ctx, bool records) {
   ctx->sI.ensure(C + 1);
   ctx->sJ.ensure(C + 1);
   ctx->sphi.ensure(C + 1);
-  ctx->iter.ensure(2 * (C + 1));  // (gamma, g) pairs; mixed: gamma, then the FP32 g
+  ctx->iter.ensure(2 * (C + 1));  // (gamma, g): 16-byte FP64 pairs or 12-byte mixed records
   ctx->gprev.ensure(C + 1);
   ctx->prow.ensure(W * (C + 1));
   ctx->prow32.ensure(W * (C + 1));
