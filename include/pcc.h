@@ -236,11 +236,14 @@ pcc_status pcc_detect(pcc_ctx* ctx, double cutoff, pcc_constraints* out, int64_t
  * context: the library's scratch (grid, per-slot counts, raw records, sort
  * storage) is one fixed array in device memory per GPU, reserved when the
  * library's code for this entry point is first loaded on that GPU and held
- * until the process ends (about 64 MiB).  It serves at most
- * PCC_NOALLOC_MAX_BODIES bodies and PCC_NOALLOC_MAX_CONTACTS contacts; larger
- * inputs are rejected with PCC_E_INVALID_ARGUMENT (a batch found to exceed
- * the contact limit returns that status with *count set and nothing
- * written).  Every call on one GPU shares that GPU's scratch and leaves its
+ * until the process ends (about 80 MiB).  It serves at most
+ * PCC_NOALLOC_MAX_BODIES bodies and PCC_NOALLOC_MAX_CONTACTS contacts, and
+ * at most PCC_NOALLOC_MAX_BLOCK_CANDIDATES candidate pairs (pairs whose
+ * centres pass the reference's bounding-sphere filter) among the neighbour
+ * cells of each group of 256 (body, neighbour cell) slots in grid order
+ * (dense 2D packings with the default grid give ~100); larger inputs are
+ * rejected with PCC_E_INVALID_ARGUMENT (a batch found to exceed the contact
+ * limit returns that status with *count set and nothing written).  Every call on one GPU shares that GPU's scratch and leaves its
  * contents behind: calls that overlap on one GPU (from several threads, or on
  * several streams) race and must be serialised by the caller.
  *
@@ -259,6 +262,7 @@ pcc_status pcc_detect(pcc_ctx* ctx, double cutoff, pcc_constraints* out, int64_t
  * calling thread. */
 #define PCC_NOALLOC_MAX_BODIES 32768
 #define PCC_NOALLOC_MAX_CONTACTS 262144
+#define PCC_NOALLOC_MAX_BLOCK_CANDIDATES 512
 typedef struct pcc_device_bodies {
   int64_t n;
   const double* pos;

@@ -41,7 +41,7 @@ def lib(fast: bool = False):
         L.oracle_block_sum.restype = C.c_double
         L.oracle_table_size.argtypes = [C.c_int64]
         L.oracle_table_size.restype = C.c_int64
-        L.oracle_grid.argtypes = [C.c_int64, dp, C.c_double, C.c_double,
+        L.oracle_grid.argtypes = [C.c_int64, dp, dp, C.c_double, C.c_double,
                                   C.POINTER(C.c_int32), C.POINTER(C.c_uint32),
                                   C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int64]
         L.oracle_grid.restype = C.c_int64
@@ -124,7 +124,7 @@ def grid(bodies: Bodies, h=2.2, origin=-1.0):
     bucket = np.zeros(max(n, 1), np.uint32)
     perm = np.zeros(max(n, 1), np.int32)
     start = np.zeros(T + 1, np.int32)
-    lib().oracle_grid(n, _d(bodies.pos), h, origin,
+    lib().oracle_grid(n, _d(bodies.pos), _d(bodies.quat), h, origin,
                       cell.ctypes.data_as(C.POINTER(C.c_int32)),
                       bucket.ctypes.data_as(C.POINTER(C.c_uint32)),
                       perm.ctypes.data_as(C.POINTER(C.c_int32)),

@@ -298,22 +298,39 @@ static inline uint32_t cell_hash(int32_t cx, int32_t cy, int32_t cz) {
   return h;
 }
 
+/* Table slot of a cell (before the mask): bodies in one z-layer with their
+ * axes in the plane (z = 0, quaternion x = y = 0) map cells row by row, 1024
+ * to a row; others use the hash.  The slots only place cells in the table:
+ * every walk compares cell coordinates, so the contacts do not depend on it. */
+static inline uint32_t cell_slot(int32_t cx, int32_t cy, int32_t cz, int flat) {
+  return flat ? (uint32_t)cx + ((uint32_t)cy << 10) : cell_hash(cx, cy, cz);
+}
+
+int oracle_grid_flat(int64_t n, const double* pos, const double* quat) {
+  for (int64_t i = 0; i < n; ++i) {
+    if (pos[3 * i + 2] != 0.0) return 0;
+    if (quat && (quat[4 * i + 1] != 0.0 || quat[4 * i + 2] != 0.0)) return 0;
+  }
+  return 1;
+}
+
 int64_t oracle_table_size(int64_t n) {
   int64_t t = 1024;
   while (t < 2 * n) t <<= 1;
   return t;
 }
 
-int64_t oracle_grid(int64_t n, const double* pos, double h, double origin, int32_t* cell,
-                    uint32_t* bucket, int32_t* perm, int32_t* bucket_start,
+int64_t oracle_grid(int64_t n, const double* pos, const double* quat, double h, double origin,
+                    int32_t* cell, uint32_t* bucket, int32_t* perm, int32_t* bucket_start,
                     int64_t bucket_start_cap) {
   int64_t T = oracle_table_size(n);
+  const int flat = oracle_grid_flat(n, pos, quat);
   uint32_t mask = (uint32_t)(T - 1);
   int32_t* cl = cell ? cell : (int32_t*)malloc(sizeof(int32_t) * 3 * (size_t)(n > 0 ? n : 1));
   uint32_t* bk = bucket ? bucket : (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
   for (int64_t i = 0; i < n; ++i) {
     for (int k = 0; k < 3; ++k) cl[3 * i + k] = cell_coord(pos[3 * i + k], origin, h);
-    bk[i] = cell_hash(cl[3 * i], cl[3 * i + 1], cl[3 * i + 2]) & mask;
+    bk[i] = cell_slot(cl[3 * i], cl[3 * i + 1], cl[3 * i + 2], flat) & mask;
   }
   if (perm && bucket_start && bucket_start_cap >= T + 1) {
     memset(bucket_start, 0, sizeof(int32_t) * (size_t)(T + 1));
@@ -413,7 +430,8 @@ static int64_t detect_records(const body_view* bv, int64_t n, double cutoff, dou
   uint32_t* bkt = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
   int32_t* perm = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
   int32_t* start = (int32_t*)malloc(sizeof(int32_t) * (size_t)(T + 1));
-  oracle_grid(n, bv->pos, h, origin, cell, bkt, perm, start, T + 1);
+  oracle_grid(n, bv->pos, bv->quat, h, origin, cell, bkt, perm, start, T + 1);
+  const int flat = oracle_grid_flat(n, bv->pos, bv->quat);
   uint32_t mask = (uint32_t)(T - 1);
 
   int64_t* cnt = (int64_t*)calloc((size_t)n + 1, sizeof(int64_t));
@@ -438,7 +456,7 @@ static int64_t detect_records(const body_view* bv, int64_t n, double cutoff, dou
           for (int dy = -1; dy <= 1; ++dy)
             for (int dx = -1; dx <= 1; ++dx) {
               int32_t nx = cx + dx, ny = cy + dy, nz = cz + dz;
-              uint32_t b = cell_hash(nx, ny, nz) & mask;
+              uint32_t b = cell_slot(nx, ny, nz, flat) & mask;
               for (int32_t e = start[b]; e < start[b + 1]; ++e) {
                 int32_t J = perm[e];
                 if (J <= I) continue;

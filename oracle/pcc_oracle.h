@@ -53,13 +53,18 @@ int64_t oracle_divide(int64_t n, int64_t cap, double* pos, double* quat, double*
                       double angle_noise);
 
 /* Broad phase.  cell = floor((x - origin)/h) per axis (SpatialGrid.cpp:29-36);
- * bucket = hash(cell) & (table_size-1); stable counting sort of bodies by
+ * bucket = slot(cell) & (table_size-1), the slot row-major (1024 cells to a
+ * row) when every body lies in z = 0 with its axis in the plane (quat NULL:
+ * positions decide alone), else a hash; stable counting sort of bodies by
  * bucket.  Returns table_size; fills cell (3n), bucket (n), perm (n) and
  * bucket_start (table_size+1) when non-NULL and large enough
  * (bucket_start_cap >= table_size+1). */
-int64_t oracle_grid(int64_t n, const double* pos, double h, double origin, int32_t* cell,
-                    uint32_t* bucket, int32_t* perm, int32_t* bucket_start,
+int64_t oracle_grid(int64_t n, const double* pos, const double* quat, double h, double origin,
+                    int32_t* cell, uint32_t* bucket, int32_t* perm, int32_t* bucket_start,
                     int64_t bucket_start_cap);
+/* 1 if every body lies in z = 0 with its axis in the plane (the grid's
+ * row-major slots). */
+int oracle_grid_flat(int64_t n, const double* pos, const double* quat);
 int64_t oracle_table_size(int64_t n);
 
 /* One detection batch (CollisionDetector::detectCollisions,

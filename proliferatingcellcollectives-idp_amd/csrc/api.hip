@@ -1180,7 +1180,7 @@ pcc_status pcc_device_bytes(pcc_ctx* ctx, int64_t* bytes) {
   add(ctx->dv_perm2); add(ctx->dv_sinst); add(ctx->dv_stage); add(ctx->dv_body); add(ctx->dv_key);
   add(ctx->dv_key2); add(ctx->dv_ckey); add(ctx->dv_ckey2); add(ctx->dv_col);
   add(ctx->ldot_new); add(ctx->nc); add(ctx->cell); add(ctx->bucket); add(ctx->perm);
-  add(ctx->bstart); add(ctx->bcursor); add(ctx->dcount); add(ctx->dslot); add(ctx->dslot_off); add(ctx->dacc); add(ctx->dgeo); add(ctx->rI_); add(ctx->rJ_);
+  add(ctx->bstart); add(ctx->bcursor); add(ctx->dcount); add(ctx->dslot); add(ctx->dslot_off); add(ctx->dcstart); add(ctx->dcand); add(ctx->daccept); add(ctx->dbcount); add(ctx->dgeo); add(ctx->scell); add(ctx->sgeo); add(ctx->sfrec); add(ctx->rI_); add(ctx->rJ_);
   add(ctx->rrec); add(ctx->ridx); add(ctx->ridx2); add(ctx->rkey); add(ctx->rkey2);
   add(ctx->sort_tmp); add_set(ctx->S); add_set(ctx->S2); add(ctx->rowptr); add(ctx->inccur);
   add(ctx->inc); add(ctx->increc); add(ctx->increc32); add(ctx->incidx); add(ctx->gold_e); add(ctx->fold); add(ctx->plist);

@@ -260,6 +260,27 @@ __host__ __device__ __forceinline__ uint32_t inst_mix(int32_t b) {
   return b == 0 ? 0u : cell_hash(b, 0x5bd1e995, 0x27d4eb2f);
 }
 
+// XCD-aware block mapping: the hardware dispatches block h to XCD h % 8, so
+// logical block (h % 8) * (G/8) + h / 8 makes each XCD sweep one contiguous
+// range and keeps its gathers in its own L2 (gp, the grid, is a multiple of 8).
+static constexpr int kXcd = 8;
+__device__ __forceinline__ int64_t xcd_block(int64_t h, int64_t gp) {
+  return (h % kXcd) * (gp / kXcd) + h / kXcd;
+}
+inline int64_t pad8(int64_t g) { return (g + kXcd - 1) / kXcd * kXcd; }
+
+// Table slot of a cell (before the mask).  A single z-layer (flat) maps cells
+// row by row, 1024 to a row, so that neighbouring cells sit in nearby slots
+// and a contiguous range of the table covers a band of the colony (the grid
+// walk then stays in one L2); other layouts use the hash.  Colonies are
+// offset by their salt.  Cells that share a slot are told apart by their
+// coordinates wherever the table is walked.
+__host__ __device__ __forceinline__ uint32_t cell_slot(int32_t cx, int32_t cy, int32_t cz,
+                                                       int32_t inst, bool flat) {
+  const uint32_t k = flat ? (uint32_t)cx + ((uint32_t)cy << 10) : cell_hash(cx, cy, cz);
+  return flat ? k + inst_mix(inst) : k ^ inst_mix(inst);
+}
+
 // ---------------------------------------------------------------------------
 // Deterministic reductions.  A 64-lane butterfly (xor 32,16,...,1) leaves the
 // same value in every lane; a 256-thread block combines its four waves as

@@ -8,19 +8,26 @@
 //  1. k_cells      cell = floor((x+1)/h) per axis, bucket = hash(cell) & (T-1),
 //                  histogram of buckets (integer atomics)
 //  2. scan         bucket offsets
-//  3. k_scatter    bodies into buckets, then k_bucket_sort orders each bucket
+//  3. k_scatter    bodies into buckets, then k_seg_sort orders each bucket
 //                  by body index: a stable counting sort, identical to the
-//                  CPU restatement's
-//  4. k_detect     one thread per (body I, neighbour cell) tests every J > I
-//                  of that cell with the reference's filters and exact FP64
-//                  segment-segment DCP; 27 neighbour cells, or the 9 of the
-//                  body's own z-layer when every body has z = 0 (the other
-//                  cells are then empty); pass 0 counts and keeps, per thread,
-//                  a bit mask of the accepted positions of its bucket; pass 1
-//                  re-evaluates only those (buckets of at most 64 bodies;
-//                  longer ones are walked again) and writes at the scanned
-//                  offset, so each body's records keep the cell order dz, dy,
-//                  dx and the bucket order
+//                  CPU restatement's; k_stage copies every body's cell and
+//                  detection record into bucket order, so that a bucket's
+//                  members are contiguous (one bucket walk reads consecutive
+//                  records, with no index indirection before the loads)
+//  4. k_filter     one thread per (body I, neighbour cell), threads in bucket
+//                  order, tests every J > I of that cell against the
+//                  reference's centre-distance filter (after a conservative
+//                  FP32 pre-filter) and emits the passing pairs as one
+//                  contiguous run of candidates; 27 neighbour cells, or the 9
+//                  of the body's own z-layer when every body has z = 0 (the
+//                  other cells are then empty)
+//     k_pair_test  one thread per candidate: the exact FP64 segment-segment
+//                  DCP on dense wavefronts (a thread per (body, cell) would
+//                  run it with a few lanes of each wavefront busy); counts
+//                  the contacts of each (body, neighbour cell) slot
+//     scan         slot offsets, so each body's records keep the cell order
+//                  dz, dy, dx and the bucket order
+//     k_pair_write recomputes each accepted pair and writes its record
 //  5. k_segsort + radix sort: order = (contact x, I, J); with several colonies
 //                  in one context a second stable radix sort by colony makes
 //                  the order (colony, contact x, I, J)
@@ -45,7 +52,7 @@ namespace {
 // geo = (x, y, z, t_x | t_y, t_z, length, diameter) with t = rotate(q, e_x), so
 // that the detection kernel reads a candidate from two cache lines.
 __global__ void k_cells(int64_t N, BodyArrays b, const int32_t* __restrict__ binst,
-                        const InstDev* __restrict__ idev, uint32_t mask,
+                        const InstDev* __restrict__ idev, uint32_t mask, bool flat,
                         int4* __restrict__ cell4, double2* __restrict__ geo,
                         uint32_t* __restrict__ bucket, int32_t* __restrict__ count) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -63,7 +70,7 @@ __global__ void k_cells(int64_t N, BodyArrays b, const int32_t* __restrict__ bin
   geo[4 * i + 1] = make_double2(z, t[0]);
   geo[4 * i + 2] = make_double2(t[1], t[2]);
   geo[4 * i + 3] = make_double2(b.len[i], b.diam[i]);
-  uint32_t bk = (cell_hash(cx, cy, cz) ^ inst_mix(inst)) & mask;
+  uint32_t bk = cell_slot(cx, cy, cz, inst, flat) & mask;
   bucket[i] = bk;
   atomicAdd(&count[bk], 1);
 }
@@ -76,6 +83,29 @@ __global__ void k_scatter(int64_t N, const uint32_t* __restrict__ bucket,
   uint32_t bk = bucket[i];
   int32_t pos = start[bk] + atomicAdd(&cursor[bk], 1);
   perm[pos] = (int32_t)i;
+}
+
+// Every body's cell and detection record in bucket order (perm[pos] is the
+// body at position pos), and its filter record: FP32 centre relative to its
+// cell's corner (origin + cell * h per axis) and FP32 length.
+__global__ void k_stage(int64_t N, const int32_t* __restrict__ perm,
+                        const int4* __restrict__ cell4, const double2* __restrict__ geo,
+                        const InstDev* __restrict__ idev, int4* __restrict__ scell,
+                        double2* __restrict__ sgeo, float4* __restrict__ frec) {
+  const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pos >= N) return;
+  const int64_t J = perm[pos];
+  const int4 c = cell4[J];
+  scell[pos] = c;
+  double2 g[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    g[q] = geo[4 * J + q];
+    sgeo[4 * pos + q] = g[q];
+  }
+  const double origin = idev[c.w].origin, h = idev[c.w].h;
+  frec[pos] = make_float4((float)(g[0].x - (origin + c.x * h)), (float)(g[0].y - (origin + c.y * h)),
+                          (float)(g[1].x - (origin + c.z * h)), (float)g[3].x);
 }
 
 // Orders the members of each segment ascending (insertion sort; segments are
@@ -99,111 +129,253 @@ __global__ void k_seg_sort(int64_t nseg, const int32_t* __restrict__ start,
 // Raw contact record layout (16 doubles): phi, n[3], rI[3], rJ[3], p[3], sI, sJ, gamma0
 enum { R_PHI = 0, R_N = 1, R_RI = 4, R_RJ = 7, R_P = 10, R_SI = 13, R_SJ = 14, R_G = 15, R_LEN = 16 };
 
-template <int S>  // neighbour cells per body: 9 (one z-layer) or 27
-__global__ void __launch_bounds__(256)
-k_detect(int pass, int64_t N, const int4* __restrict__ cell4, const double2* __restrict__ geo,
-         const int32_t* __restrict__ bstart, const int32_t* __restrict__ perm, uint32_t mask,
-         const InstDev* __restrict__ idev, int32_t* __restrict__ cnt_off,
-         uint64_t* __restrict__ acc, int32_t* __restrict__ outI, int32_t* __restrict__ outJ,
-         double* __restrict__ rec) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= N * S) return;
-  const int64_t I = t / S;
-  const int sl = (int)(t - I * S);
-  const int dx = sl % 3 - 1, dy = (sl / 3) % 3 - 1, dz = S == 27 ? sl / 9 - 1 : 0;
-  const int4 cI = cell4[I];
-  const int32_t inst = cI.w;
-  if (!(idev[inst].flags & kInstActive)) return;  // count stays 0
-  const double cutoff = idev[inst].cutoff, warm = idev[inst].warm;
-  const uint32_t salt = inst_mix(inst);
-  const double2 gI0 = geo[4 * I], gI1 = geo[4 * I + 1], gI2 = geo[4 * I + 2], gI3 = geo[4 * I + 3];
-  const double xI[3] = {gI0.x, gI0.y, gI1.x};
-  const double tI[3] = {gI1.y, gI2.x, gI2.y};
-  const double lenI = gI3.x, dI = gI3.y;
-  const double halfI = lenI / 2 - dI / 2;
-  double P0[3], P1[3];
+// A body's segment as the contact tests use it: centre x, direction t,
+// length, diameter and the end points P0 = x - t h, P1 = x + t h with
+// h = length/2 - diameter/2 (CollisionDetector.cpp:215-217), from its
+// detection record g[0..3].
+struct Seg {
+  double x[3], t[3], len, d, P0[3], P1[3];
+};
+
+__device__ __forceinline__ void seg_of(const double2* __restrict__ g, Seg& a) {
+  const double2 g0 = g[0], g1 = g[1], g2 = g[2], g3 = g[3];
+  a.x[0] = g0.x;
+  a.x[1] = g0.y;
+  a.x[2] = g1.x;
+  a.t[0] = g1.y;
+  a.t[1] = g2.x;
+  a.t[2] = g2.y;
+  a.len = g3.x;
+  a.d = g3.y;
+  const double half = a.len / 2 - a.d / 2;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    double hv = tI[k] * halfI;
-    P0[k] = xI[k] - hv;
-    P1[k] = xI[k] + hv;
+    const double hv = a.t[k] * half;
+    a.P0[k] = a.x[k] - hv;
+    a.P1[k] = a.x[k] + hv;
   }
-  const int32_t cx = cI.x, cy = cI.y, cz = cI.z;
-  const int32_t w = pass ? cnt_off[t] : 0;
-  int32_t c = 0;
-  const int32_t nx = cx + dx, ny = cy + dy, nz = cz + dz;
-  const uint32_t bk = (cell_hash(nx, ny, nz) ^ salt) & mask;
-  const int32_t e0 = bstart[bk], e1 = bstart[bk + 1];
-  // The reference's filters and the record of candidate perm[e]; true if it
-  // is a contact (then, in pass 1, written at slot w + c).
-  auto candidate = [&](int32_t e) -> bool {
-          const int32_t J = perm[e];
-          if (J <= I) return false;
-          const int4 cJ = cell4[J];
-          if (cJ.x != nx || cJ.y != ny || cJ.z != nz || cJ.w != inst) return false;
-          const double2 g0 = geo[4 * (int64_t)J], g1 = geo[4 * (int64_t)J + 1];
-          const double2 g2 = geo[4 * (int64_t)J + 2], g3 = geo[4 * (int64_t)J + 3];
-          const double xJ[3] = {g0.x, g0.y, g1.x};
-          const double lenJ = g3.x, dJ = g3.y;
-          double dd[3] = {xI[0] - xJ[0], xI[1] - xJ[1], xI[2] - xJ[2]};
-          double rough = sqrt(dot3(dd, dd));
-          if (rough > (lenI + lenJ) / 2.0 + cutoff) return false;
-          const double tJ[3] = {g1.y, g2.x, g2.y};
-          const double halfJ = lenJ / 2 - dJ / 2;
-          double Q0[3], Q1[3];
+}
+
+// Signed distance of the pair (closest points in o): segment-segment DCP
+// minus the sum of the radii (CollisionDetector.cpp:219-237).
+__device__ __forceinline__ double pair_phi(const Seg& a, const Seg& b, DcpOut& o) {
+  dcp(a.P0, a.P1, b.P0, b.P1, o);
+  const double sum_radii = (a.d + b.d) / 2.0;
+  return o.dist - sum_radii;
+}
+
+// The raw record of contact (I = a, J = b) (CollisionDetector.cpp:239-258, 306).
+__device__ __forceinline__ void pair_record(const Seg& a, const Seg& b, const DcpOut& o,
+                                            double phi, double warm, double* __restrict__ r) {
+  double p[3], d01[3], nrm[3];
 #pragma unroll
-          for (int k = 0; k < 3; ++k) {
-            double hv = tJ[k] * halfJ;
-            Q0[k] = xJ[k] - hv;
-            Q1[k] = xJ[k] + hv;
-          }
-          DcpOut o;
-          dcp(P0, P1, Q0, Q1, o);
-          const double sum_radii = (dI + dJ) / 2.0;
-          const double phi = o.dist - sum_radii;
-          if (phi > cutoff) return false;
-          if (pass) {
-            const int64_t slot = (int64_t)w + c;
-            outI[slot] = (int32_t)I;
-            outJ[slot] = J;
-            double* r = rec + R_LEN * slot;
-            double p[3], d01[3], nrm[3];
+  for (int k = 0; k < 3; ++k) {
+    p[k] = (o.c0[k] + o.c1[k]) * 0.5;
+    d01[k] = o.c0[k] - o.c1[k];
+  }
+  normalize3(d01, nrm);
+  const double mn[3] = {-nrm[0], -nrm[1], -nrm[2]};
+  r[R_PHI] = phi;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-              p[k] = (o.c0[k] + o.c1[k]) * 0.5;
-              d01[k] = o.c0[k] - o.c1[k];
-            }
-            normalize3(d01, nrm);
-            const double mn[3] = {-nrm[0], -nrm[1], -nrm[2]};
-            r[R_PHI] = phi;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-              r[R_N + k] = nrm[k];
-              r[R_RI + k] = p[k] - xI[k];
-              r[R_RJ + k] = p[k] - xJ[k];
-              r[R_P + k] = p[k];
-            }
-            r[R_SI] = 0.5 * fabs(dot3(nrm, tI));
-            r[R_SJ] = 0.5 * fabs(dot3(mn, tJ));
-            r[R_G] = phi >= 0.0 ? 0.0 : -phi * warm;
-          }
-          return true;
-  };
-  if (pass && e1 - e0 <= 64) {
-    // the contacts pass 0 found, in ascending bucket position
-    for (uint64_t m = acc[t]; m; m &= m - 1)
-      if (candidate(e0 + (int32_t)__builtin_ctzll(m))) ++c;
-  } else {
-    uint64_t bits = 0;
-    for (int32_t e = e0; e < e1; ++e)
-      if (candidate(e)) {
-        if (e - e0 < 64) bits |= 1ull << (e - e0);
-        ++c;
-      }
-    if (!pass) {
-      cnt_off[t] = c;
-      acc[t] = bits;
+  for (int k = 0; k < 3; ++k) {
+    r[R_N + k] = nrm[k];
+    r[R_RI + k] = p[k] - a.x[k];
+    r[R_RJ + k] = p[k] - b.x[k];
+    r[R_P + k] = p[k];
+  }
+  r[R_SI] = 0.5 * fabs(dot3(nrm, a.t));
+  r[R_SJ] = 0.5 * fabs(dot3(mn, b.t));
+  r[R_G] = phi >= 0.0 ? 0.0 : -phi * warm;
+}
+
+constexpr int kFilterThreads = 256;
+
+// Candidate pairs.  One thread per (body I, neighbour cell), threads in
+// bucket order (the bodies of one cell are neighbours in the grid and their
+// threads walk the same buckets; blocks are mapped XCD-aware, so each XCD
+// walks one band of the table), tests the J > I of that cell against the
+// reference's centre-distance filter (CollisionDetector.cpp:208-213).  A
+// bucket lists its bodies in ascending index, so the walk runs from its end
+// and stops at the first J <= I.  A conservative FP32 pre-filter on the
+// filter records comes first; the cell and the exact FP64 test only for the
+// pairs it keeps.  The thread's passing positions form one run of candidates
+// (slot u = I * S + cell, bucket position of J), in the workgroup's own
+// region of the candidate list: logical block l owns [l * cb, (l + 1) * cb),
+// its runs placed by a scan in the workgroup (no atomics: atomics on one
+// address serialise across the whole device).  cstart[u] = the run's first
+// candidate, bcount[l] = the region's fill.  A region that would overflow
+// writes nothing and raises *need to its total (the caller repeats the batch
+// with larger regions).
+template <int S>  // neighbour cells per body: 9 (one z-layer) or 27
+__global__ void __launch_bounds__(kFilterThreads)
+k_filter(int64_t N, int64_t gp, const int4* __restrict__ scell, const double2* __restrict__ sgeo,
+         const float4* __restrict__ frec, const int32_t* __restrict__ bstart,
+         const int32_t* __restrict__ perm, uint32_t mask, const InstDev* __restrict__ idev,
+         int32_t cb, int32_t* __restrict__ bcount, int32_t* __restrict__ need,
+         int32_t* __restrict__ cstart, int2* __restrict__ cand) {
+  __shared__ int32_t wsum[kFilterThreads / 64];
+  __shared__ int32_t btot;
+  const int64_t l = xcd_block(blockIdx.x, gp);
+  const int64_t t = l * kFilterThreads + threadIdx.x;
+  bool live = t < N * S;
+  int64_t I = 0, u = 0;
+  int32_t nx = 0, ny = 0, nz = 0, inst = 0, e0 = 0, e1 = 0;
+  double xI[3] = {0.0, 0.0, 0.0}, lenI = 0.0, cutoff = 0.0;
+  float fx = 0.f, fy = 0.f, fz = 0.f, flen = 0.f, fcut = 0.f, fh = 0.f;
+  if (live) {
+    const int64_t h = t / S;  // the home body's position in bucket order
+    const int sl = (int)(t - h * S);
+    I = perm[h];
+    u = I * S + sl;
+    const int4 cI = scell[h];
+    inst = cI.w;
+    live = (idev[inst].flags & kInstActive) != 0;
+    if (live) {
+      cutoff = idev[inst].cutoff;
+      const double2 g0 = sgeo[4 * h], g1 = sgeo[4 * h + 1], g3 = sgeo[4 * h + 3];
+      xI[0] = g0.x;
+      xI[1] = g0.y;
+      xI[2] = g1.x;
+      lenI = g3.x;
+      const int dx = sl % 3 - 1, dy = (sl / 3) % 3 - 1, dz = S == 27 ? sl / 9 - 1 : 0;
+      nx = cI.x + dx;
+      ny = cI.y + dy;
+      nz = cI.z + dz;
+      // home centre relative to the neighbour cell's corner
+      const float4 f = frec[h];
+      fh = (float)idev[inst].h;
+      fx = f.x - (float)dx * fh;
+      fy = f.y - (float)dy * fh;
+      fz = f.z - (float)dz * fh;
+      flen = f.w;
+      fcut = (float)cutoff;
+      const uint32_t bk = cell_slot(nx, ny, nz, inst, S == 9) & mask;
+      e0 = bstart[bk];
+      e1 = bstart[bk + 1];
     }
+  }
+  // Filter records hold centres relative to their cell's corner: the FP32
+  // centre difference is within about 4 * 2^-24 (h + |d|) of the FP64 one,
+  // and the pre-filter's margin, 1e-5 (reach + h), is far above that, so
+  // every pair that the FP64 test keeps passes it.
+  auto passes = [&](int32_t e) -> bool {
+    const float4 f = frec[e];
+    const float ex = fx - f.x, ey = fy - f.y, ez = fz - f.z;
+    const float fr = (flen + f.w) * 0.5f + fcut;
+    const float lim = fr + 1e-5f * (fr + fh);
+    if (ex * ex + ey * ey + ez * ez > lim * lim) return false;
+    const int4 cJ = scell[e];
+    if (cJ.x != nx || cJ.y != ny || cJ.z != nz || cJ.w != inst) return false;
+    const double2 g0 = sgeo[4 * (int64_t)e], g1 = sgeo[4 * (int64_t)e + 1];
+    const double lenJ = sgeo[4 * (int64_t)e + 3].x;
+    const double dd[3] = {xI[0] - g0.x, xI[1] - g0.y, xI[2] - g1.x};
+    const double rough = sqrt(dot3(dd, dd));
+    return !(rough > (lenI + lenJ) / 2.0 + cutoff);
+  };
+  int32_t n = 0;
+  uint64_t bits = 0;
+  for (int32_t e = e1 - 1; e >= e0; --e) {
+    const int32_t J = perm[e];
+    if (J <= I) break;  // the rest of the bucket has smaller indices
+    if (passes(e)) {
+      if (e - e0 < 64) bits |= 1ull << (e - e0);
+      ++n;
+    }
+  }
+  // the workgroup's runs: inclusive scan in each wavefront, then across them
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int32_t x = n;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[wv] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t tot = 0;
+#pragma unroll
+    for (int q = 0; q < kFilterThreads / 64; ++q) {
+      const int32_t v = wsum[q];
+      wsum[q] = tot;
+      tot += v;
+    }
+    btot = tot;
+    bcount[l] = tot <= cb ? tot : 0;
+    if (tot > cb) atomicMax(need, tot);  // rare: the batch is repeated
+  }
+  __syncthreads();
+  if (n == 0 || btot > cb) return;
+  int64_t k = l * cb + wsum[wv] + (x - n);
+  cstart[u] = (int32_t)k;
+  if (e1 - e0 <= 64) {
+    for (uint64_t m = bits; m; m &= m - 1, ++k)
+      cand[k] = make_int2((int32_t)u, e0 + (int32_t)__builtin_ctzll(m));
+  } else {
+    for (int32_t e = e1 - 1; e >= e0; --e) {
+      const int32_t J = perm[e];
+      if (J <= I) break;
+      if (passes(e)) cand[k++] = make_int2((int32_t)u, e);
+    }
+  }
+}
+
+// Every candidate through the reference's segment-segment test
+// (CollisionDetector.cpp:215-237): accept[k] = 1 for a contact, and
+// acnt[u] counts the contacts of slot u.  Candidates fill every lane, so
+// the FP64 DCP runs on dense wavefronts.  Workgroup l takes region l of the
+// list (the same XCD as the workgroup that filled it).
+__global__ void __launch_bounds__(256)
+k_pair_test(int S, int64_t gp, int32_t cb, const int32_t* __restrict__ bcount,
+            const int2* __restrict__ cand, const double2* __restrict__ geo,
+            const double2* __restrict__ sgeo, const int32_t* __restrict__ binst,
+            const InstDev* __restrict__ idev, uint8_t* __restrict__ accept,
+            int32_t* __restrict__ acnt) {
+  const int64_t l = xcd_block(blockIdx.x, gp);
+  const int32_t nb = bcount[l];
+  for (int32_t j = threadIdx.x; j < nb; j += blockDim.x) {
+    const int64_t k = l * cb + j;
+    const int2 c = cand[k];
+    const int64_t I = c.x / S;
+    Seg a, b;
+    seg_of(geo + 4 * I, a);
+    seg_of(sgeo + 4 * (int64_t)c.y, b);
+    DcpOut o;
+    const double phi = pair_phi(a, b, o);
+    const bool contact = !(phi > idev[binst[I]].cutoff);
+    accept[k] = contact ? 1 : 0;
+    if (contact) atomicAdd(&acnt[c.x], 1);
+  }
+}
+
+// The record of every accepted candidate at its place: slot u's contacts
+// start at aoff[u] and keep the run's (bucket) order.
+__global__ void __launch_bounds__(256)
+k_pair_write(int S, int64_t gp, int32_t cb, const int32_t* __restrict__ bcount,
+             const int2* __restrict__ cand, const uint8_t* __restrict__ accept,
+             const int32_t* __restrict__ cstart, const int32_t* __restrict__ aoff,
+             const double2* __restrict__ geo, const double2* __restrict__ sgeo,
+             const int32_t* __restrict__ perm, const int32_t* __restrict__ binst,
+             const InstDev* __restrict__ idev, int32_t* __restrict__ outI,
+             int32_t* __restrict__ outJ, double* __restrict__ rec) {
+  const int64_t l = xcd_block(blockIdx.x, gp);
+  const int32_t nb = bcount[l];
+  for (int32_t j = threadIdx.x; j < nb; j += blockDim.x) {
+    const int64_t k = l * cb + j;
+    if (!accept[k]) continue;
+    const int2 c = cand[k];
+    int32_t rank = 0;
+    for (int64_t q = cstart[c.x]; q < k; ++q) rank += accept[q];
+    const int64_t slot = (int64_t)aoff[c.x] + rank;
+    const int64_t I = c.x / S;
+    Seg a, b;
+    seg_of(geo + 4 * I, a);
+    seg_of(sgeo + 4 * (int64_t)c.y, b);
+    DcpOut o;
+    const double phi = pair_phi(a, b, o);
+    outI[slot] = (int32_t)I;
+    outJ[slot] = perm[c.y];
+    pair_record(a, b, o, phi, idev[binst[I]].warm, rec + R_LEN * slot);
   }
 }
 
@@ -438,6 +610,10 @@ size_t detect_scan_bytes(int64_t N, int S, int64_t T) {
   return std::max(a, b);
 }
 
+int64_t filter_grid(int64_t N, int S) {
+  return pad8((N * S + kFilterThreads - 1) / kFilterThreads);
+}
+
 size_t detect_sort_bytes(int64_t m, int64_t B) {
   size_t a = 0, b = 0;
   uint64_t* k = nullptr;
@@ -451,42 +627,56 @@ size_t detect_sort_bytes(int64_t m, int64_t B) {
 }
 
 // Stage 1 of a detection batch: the hashed cell list (a stable counting sort
-// of salted cell hashes), the counting pass over every (body, neighbour cell)
-// slot, and the offsets.  bounds_host[b] (B + 1 entries) receives the first
-// record of colony b in the batch; bounds_host[B] is the record count.
-void detect_count(DetectBufs& d, int32_t* bounds_host) {
+// of salted cell hashes), the candidate pairs, their contact test and the
+// offsets.  bounds_host[b] (B + 2 entries) receives the first record of
+// colony b in the batch; bounds_host[B] is the record count and
+// bounds_host[B + 1] is 0, or the candidate region size that some workgroup
+// needed beyond d.cand_block: then the function returns false and nothing
+// else is valid.
+bool detect_count(DetectBufs& d, int32_t* bounds_host) {
   const int64_t N = d.N, T = d.T, NS = N * d.S;
   const hipStream_t s = d.s;
   const uint32_t mask = (uint32_t)(T - 1);
   k_fill_i32<<<nblk(T + 1), 256, 0, s>>>(T + 1, 0, d.bcursor);
   if (N > 0)
-    k_cells<<<nblk(N), 256, 0, s>>>(N, d.body, d.binst, d.idev, mask, d.cell4, d.geo, d.bucket,
+    k_cells<<<nblk(N), 256, 0, s>>>(N, d.body, d.binst, d.idev, mask, d.S == 9, d.cell4, d.geo,
+                                    d.bucket,
                                     d.bcursor);
   scan_on(d, d.bcursor, d.bstart, T + 1);
   k_fill_i32<<<nblk(T + 1), 256, 0, s>>>(T + 1, 0, d.bcursor);
   if (N > 0) {
     k_scatter<<<nblk(N), 256, 0, s>>>(N, d.bucket, d.bstart, d.bcursor, d.perm);
     k_seg_sort<<<nblk(T), 256, 0, s>>>(T, d.bstart, d.perm);
+    k_stage<<<nblk(N), 256, 0, s>>>(N, d.perm, d.cell4, d.geo, d.idev, d.scell, d.sgeo, d.frec);
   }
+  // candidates, then their test; dslot counts each slot's contacts
+  int32_t* need = d.bbounds + d.B + 1;
+  k_fill_i32<<<1, 64, 0, s>>>(1, 0, need);
   k_fill_i32<<<nblk(NS + 1), 256, 0, s>>>(NS + 1, 0, d.dslot);
   if (N > 0) {
+    const int64_t gp = filter_grid(N, d.S);
     if (d.S == 9)
-      k_detect<9><<<nblk(NS), 256, 0, s>>>(0, N, d.cell4, d.geo, d.bstart, d.perm, mask, d.idev,
-                                           d.dslot, d.dacc, nullptr, nullptr, nullptr);
+      k_filter<9><<<(unsigned)gp, kFilterThreads, 0, s>>>(
+          N, gp, d.scell, d.sgeo, d.frec, d.bstart, d.perm, mask, d.idev, d.cand_block,
+          d.bcount, need, d.cstart, d.cand);
     else
-      k_detect<27><<<nblk(NS), 256, 0, s>>>(0, N, d.cell4, d.geo, d.bstart, d.perm, mask, d.idev,
-                                            d.dslot, d.dacc, nullptr, nullptr, nullptr);
+      k_filter<27><<<(unsigned)gp, kFilterThreads, 0, s>>>(
+          N, gp, d.scell, d.sgeo, d.frec, d.bstart, d.perm, mask, d.idev, d.cand_block,
+          d.bcount, need, d.cstart, d.cand);
+    k_pair_test<<<(unsigned)gp, 256, 0, s>>>(d.S, gp, d.cand_block, d.bcount, d.cand, d.geo,
+                                             d.sgeo, d.binst, d.idev, d.accept, d.dslot);
   }
   // per-slot offsets, then per-body offsets into dcount, then the colony
   // boundaries of the batch (bodies are stored colony after colony, and a
-  // stable sort by colony keeps them); bounds[B] is the batch's record count,
-  // so one read-back gives both
+  // stable sort by colony keeps them); bounds[B] is the batch's record count
+  // and bounds[B + 1] the candidate count, so one read-back gives all
   scan_on(d, d.dslot, d.dslot_off, NS + 1);
   k_body_offsets<<<nblk(N + 1), 256, 0, s>>>(N, d.S, d.dslot_off, d.dcount);
   k_batch_bounds<<<nblk(d.B + 1), 256, 0, s>>>(d.B, N, d.idev, d.dcount, d.bbounds);
-  PCC_HIP(hipMemcpyAsync(bounds_host, d.bbounds, sizeof(int32_t) * (d.B + 1),
+  PCC_HIP(hipMemcpyAsync(bounds_host, d.bbounds, sizeof(int32_t) * (d.B + 2),
                          hipMemcpyDeviceToHost, s));
   PCC_HIP(hipStreamSynchronize(s));
+  return bounds_host[d.B + 1] == 0;
 }
 
 // Stage 2: the writing pass of the m records found by detect_count, then
@@ -494,15 +684,12 @@ void detect_count(DetectBufs& d, int32_t* bounds_host) {
 // order (d.ridx and d.ridx2 may have been exchanged).
 void detect_write(DetectBufs& d, int64_t m) {
   if (m == 0) return;
-  const int64_t N = d.N, NS = N * d.S;
+  const int64_t N = d.N;
   const hipStream_t s = d.s;
-  const uint32_t mask = (uint32_t)(d.T - 1);
-  if (d.S == 9)
-    k_detect<9><<<nblk(NS), 256, 0, s>>>(1, N, d.cell4, d.geo, d.bstart, d.perm, mask, d.idev,
-                                         d.dslot_off, d.dacc, d.rI, d.rJ, d.rec);
-  else
-    k_detect<27><<<nblk(NS), 256, 0, s>>>(1, N, d.cell4, d.geo, d.bstart, d.perm, mask, d.idev,
-                                          d.dslot_off, d.dacc, d.rI, d.rJ, d.rec);
+  const int64_t gp = filter_grid(N, d.S);
+  k_pair_write<<<(unsigned)gp, 256, 0, s>>>(d.S, gp, d.cand_block, d.bcount, d.cand, d.accept,
+                                            d.cstart, d.dslot_off, d.geo, d.sgeo, d.perm,
+                                            d.binst, d.idev, d.rI, d.rJ, d.rec);
   k_segsort_by_J<<<nblk(N), 256, 0, s>>>(N, d.dcount, d.rJ, d.ridx);
   k_make_keys<<<nblk(m), 256, 0, s>>>(m, d.ridx, d.rec, d.rkey);
   sort_pairs_on<uint64_t>(d, d.rkey, d.rkey2, d.ridx, d.ridx2, m, 64);
@@ -527,15 +714,25 @@ static DetectBufs ctx_detect_bufs(pcc_ctx* ctx) {
   ctx->table_size = T;
   ctx->cell.ensure(4 * N + 4);
   ctx->dgeo.ensure(8 * N + 8);
+  ctx->scell.ensure(4 * N + 4);
+  ctx->sgeo.ensure(8 * N + 8);
+  ctx->sfrec.ensure(4 * N + 4);
   ctx->bucket.ensure(N + 1);
   ctx->perm.ensure(N + 1);
   ctx->bstart.ensure(T + 1);
   ctx->bcursor.ensure(T + 1);
   ctx->dslot.ensure(NS + 1);
   ctx->dslot_off.ensure(NS + 1);
-  ctx->dacc.ensure(NS + 1);
+  ctx->dcstart.ensure(NS + 1);
+  // candidate regions: a workgroup of 256 (body, cell) slots finds ~100
+  // candidates in dense 2D packings; regions grow when one overflows
+  const int64_t gp = filter_grid(N, S);
+  if (ctx->cand_block == 0) ctx->cand_block = kCandBlock;
+  ctx->dcand.ensure(2 * gp * ctx->cand_block + 2);
+  ctx->daccept.ensure(gp * ctx->cand_block + 1);
+  ctx->dbcount.ensure(gp + 1);
   ctx->dcount.ensure(N + 1);
-  ctx->bbounds.ensure(B + 1);
+  ctx->bbounds.ensure(B + 2);
   ctx->sort_tmp.ensure((int64_t)detect_scan_bytes(N, S, T) + 16);
   DetectBufs d;
   d.N = N;
@@ -547,13 +744,20 @@ static DetectBufs ctx_detect_bufs(pcc_ctx* ctx) {
   d.T = T;
   d.cell4 = reinterpret_cast<int4*>(ctx->cell.p);
   d.geo = reinterpret_cast<double2*>(ctx->dgeo.p);
+  d.scell = reinterpret_cast<int4*>(ctx->scell.p);
+  d.sgeo = reinterpret_cast<double2*>(ctx->sgeo.p);
+  d.frec = reinterpret_cast<float4*>(ctx->sfrec.p);
   d.bucket = ctx->bucket.p;
   d.perm = ctx->perm.p;
   d.bstart = ctx->bstart.p;
   d.bcursor = ctx->bcursor.p;
   d.dslot = ctx->dslot.p;
   d.dslot_off = ctx->dslot_off.p;
-  d.dacc = ctx->dacc.p;
+  d.cstart = ctx->dcstart.p;
+  d.cand = reinterpret_cast<int2*>(ctx->dcand.p);
+  d.accept = ctx->daccept.p;
+  d.bcount = ctx->dbcount.p;
+  d.cand_block = ctx->cand_block;
   d.dcount = ctx->dcount.p;
   d.bbounds = ctx->bbounds.p;
   d.tmp = ctx->sort_tmp.p;
@@ -569,8 +773,15 @@ void detect_batch(pcc_ctx* ctx) {
   const int64_t B = (int64_t)ctx->inst.size();
   KTimer tm(ctx, PCC_K_DETECT, 0.0);
   DetectBufs d = ctx_detect_bufs(ctx);
-  std::vector<int32_t> bounds(B + 1);
-  detect_count(d, bounds.data());
+  std::vector<int32_t> bounds(B + 2);
+  while (!detect_count(d, bounds.data())) {
+    // a workgroup found more candidates than its region holds: larger
+    // regions, and the batch again
+    int32_t cb = ctx->cand_block;
+    while (cb < bounds[B + 1]) cb *= 2;
+    ctx->cand_block = cb;
+    d = ctx_detect_bufs(ctx);
+  }
   const int64_t m = bounds[B];
   ctx->batch_count = m;
   ctx->rI_.ensure(m + 1);

@@ -265,7 +265,11 @@ struct pcc_ctx {
   // detection batch
   pcc::DArr<int32_t> dcount;   // N+1 (per-body record offsets)
   pcc::DArr<int32_t> dslot, dslot_off;  // per (body, neighbour cell) counts and offsets
-  pcc::DArr<uint64_t> dacc;  // per (body, neighbour cell): accepted bucket positions (pass 0)
+  pcc::DArr<int32_t> dcstart;  // per (body, neighbour cell): first candidate of its run
+  pcc::DArr<int32_t> dcand;    // candidate pairs (slot, bucket position of J)
+  pcc::DArr<uint8_t> daccept;  // per candidate: 1 if a contact
+  pcc::DArr<int32_t> dbcount;  // per filter workgroup: candidates in its region
+  int32_t cand_block = 0;      // candidate region per filter workgroup (grows on overflow)
   // division scratch (kept between calls)
   pcc::DArr<int32_t> dv_flag, dv_off, dv_bounds, dv_val, dv_perm, dv_perm2, dv_sinst;
   pcc::DArr<double> dv_stage, dv_body;
@@ -273,6 +277,9 @@ struct pcc_ctx {
   pcc::DArr<uint32_t> dv_ckey, dv_ckey2;
   pcc::DArr<uint8_t> dv_col;
   pcc::DArr<double> dgeo;    // per body: x, y, z, t (3), length, diameter (detection)
+  pcc::DArr<int32_t> scell;  // cell records in bucket order
+  pcc::DArr<double> sgeo;    // detection records in bucket order
+  pcc::DArr<float> sfrec;    // FP32 filter records in bucket order
   pcc::DArr<int32_t> rI_, rJ_; // raw pair ids
   pcc::DArr<double> rrec;      // raw geometry: 16 doubles per contact
   pcc::DArr<int32_t> ridx, ridx2;
@@ -326,7 +333,7 @@ struct pcc_ctx {
   pcc::DArr<int32_t> binst;                // N: instance of every body
   pcc::DArr<int32_t> bmap;                 // logical body block -> instance
   pcc::DArr<int32_t> cmap;                 // logical constraint block -> instance
-  pcc::DArr<int32_t> bbounds;              // B+1 detection-batch offsets
+  pcc::DArr<int32_t> bbounds;              // B+1 detection-batch offsets, then the candidate count
   pcc::DArr<unsigned long long> maxbits;   // B: max overlap bits
   pcc::ConstraintSet S2;                   // merge target (B > 1)
   pcc::DArr<int64_t> regroup_dst;
@@ -339,6 +346,10 @@ namespace pcc {
 // arrays, or a caller-independent static scratch).  Sizes: N bodies, S
 // neighbour cells per body (9 when every body has z = 0, else 27), hash table
 // of T (a power of two >= 2N) buckets, m records.
+// Candidate region of one filter workgroup (256 (body, neighbour cell)
+// slots), in candidate pairs: dense 2D packings fill ~100.
+constexpr int32_t kCandBlock = 512;
+
 struct DetectBufs {
   int64_t N = 0;
   BodyArrays body{};                 // body state (position, orientation, length, diameter read)
@@ -349,11 +360,18 @@ struct DetectBufs {
   int64_t T = 1024;
   int4* cell4 = nullptr;             // N
   double2* geo = nullptr;            // 4N
+  int4* scell = nullptr;             // N: cell4 in bucket order
+  double2* sgeo = nullptr;           // 4N: geo in bucket order
+  float4* frec = nullptr;            // N: FP32 filter records in bucket order
   uint32_t* bucket = nullptr;        // N
   int32_t *perm = nullptr, *bstart = nullptr, *bcursor = nullptr;  // N, T+1, T+1
   int32_t *dslot = nullptr, *dslot_off = nullptr;                  // N*S+1 each
-  uint64_t* dacc = nullptr;          // N*S+1
-  int32_t *dcount = nullptr, *bbounds = nullptr;                   // N+1, B+1
+  int32_t* cstart = nullptr;         // N*S+1: first candidate of each slot's run
+  int2* cand = nullptr;              // filter_grid * cand_block: (slot, bucket position of J)
+  uint8_t* accept = nullptr;         // filter_grid * cand_block
+  int32_t* bcount = nullptr;         // filter_grid: candidates in each workgroup's region
+  int32_t cand_block = kCandBlock;   // candidate region per workgroup
+  int32_t *dcount = nullptr, *bbounds = nullptr;                   // N+1, B+2
   int32_t *rI = nullptr, *rJ = nullptr;                            // m
   double* rec = nullptr;             // 16 m
   int32_t *ridx = nullptr, *ridx2 = nullptr;                       // m
@@ -365,7 +383,8 @@ struct DetectBufs {
 int64_t table_size_for(int64_t n);
 size_t detect_scan_bytes(int64_t N, int S, int64_t T);
 size_t detect_sort_bytes(int64_t m, int64_t B);
-void detect_count(DetectBufs& d, int32_t* bounds_host);
+bool detect_count(DetectBufs& d, int32_t* bounds_host);
+int64_t filter_grid(int64_t N, int S);  // workgroups of the candidate filter
 void detect_write(DetectBufs& d, int64_t m);
 void detect_records_out(const DetectBufs& d, int64_t m, const pcc_constraints& out_dev);
 

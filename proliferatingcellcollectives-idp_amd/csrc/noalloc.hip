@@ -6,8 +6,9 @@
 // already holds in device memory, written into caller-owned device arrays.
 // Every buffer the pipeline needs beyond the inputs and outputs lives in one
 // file-scope device array, `scratch`, of fixed size: a grid and records for
-// at most PCC_NOALLOC_MAX_BODIES bodies and PCC_NOALLOC_MAX_CONTACTS
-// contacts, and the rocPRIM temporary storage.  Its device address is looked
+// at most PCC_NOALLOC_MAX_BODIES bodies, PCC_NOALLOC_MAX_BLOCK_CANDIDATES
+// candidate pairs per filter workgroup and PCC_NOALLOC_MAX_CONTACTS contacts,
+// and the rocPRIM temporary storage.  Its device address is looked
 // up once per device with hipGetSymbolAddress and kept; host code never
 // passes the array's name itself to a kernel or a copy.
 //
@@ -37,6 +38,7 @@ constexpr int64_t kT = [] {  // hash table size of kN bodies (table_size_for)
   while (t < 2 * kN) t <<= 1;
   return t;
 }();
+constexpr int64_t kGp = (kN * kS + 255) / 256 / 8 * 8 + 8;  // filter workgroups, at most
 constexpr int64_t kTmp = 8ll << 20;  // rocPRIM temporary storage (checked at set-up)
 constexpr int kBodyFieldsRead = 9;   // x y z qw qx qy qz length diameter
 
@@ -44,8 +46,8 @@ constexpr int64_t pad(int64_t b) { return (b + 255) / 256 * 256; }
 
 // Byte layout of the scratch array.
 struct Layout {
-  int64_t body, binst, idev, flag, cell4, geo, bucket, perm, bstart, bcursor, dslot, dslot_off,
-      dacc, dcount, bbounds, rI, rJ, rec, ridx, ridx2, rkey, rkey2, tmp, total;
+  int64_t body, binst, idev, flag, cell4, geo, scell, sgeo, frec, bucket, perm, bstart, bcursor, dslot, dslot_off,
+      cstart, cand, accept, bcount, dcount, bbounds, rI, rJ, rec, ridx, ridx2, rkey, rkey2, tmp, total;
 };
 constexpr Layout make_layout() {
   Layout L{};
@@ -61,15 +63,21 @@ constexpr Layout make_layout() {
   L.flag = take(4);
   L.cell4 = take(16 * kN);
   L.geo = take(64 * kN);
+  L.scell = take(16 * kN);
+  L.sgeo = take(64 * kN);
+  L.frec = take(16 * kN);
   L.bucket = take(4 * (kN + 1));
   L.perm = take(4 * (kN + 1));
   L.bstart = take(4 * (kT + 1));
   L.bcursor = take(4 * (kT + 1));
   L.dslot = take(4 * (kN * kS + 1));
   L.dslot_off = take(4 * (kN * kS + 1));
-  L.dacc = take(8 * (kN * kS + 1));
+  L.cstart = take(4 * (kN * kS + 1));
+  L.cand = take(8 * kGp * kCandBlock);
+  L.accept = take(kGp * kCandBlock);
+  L.bcount = take(4 * kGp);
   L.dcount = take(4 * (kN + 1));
-  L.bbounds = take(4 * 2);
+  L.bbounds = take(4 * 3);
   L.rI = take(4 * (kM + 1));
   L.rJ = take(4 * (kM + 1));
   L.rec = take(8 * 16 * (kM + 1));
@@ -82,6 +90,7 @@ constexpr Layout make_layout() {
   return L;
 }
 constexpr Layout kL = make_layout();
+static_assert(kCandBlock == PCC_NOALLOC_MAX_BLOCK_CANDIDATES, "candidate region");
 static_assert(kL.total < (int64_t)96 << 20, "scratch of the no-allocation detection");
 
 }  // namespace
@@ -211,13 +220,20 @@ pcc_status detect_noalloc(int device, hipStream_t s, const pcc_params& p, double
   d.T = table_size_for(n);
   d.cell4 = static_cast<int4*>(at(kL.cell4));
   d.geo = static_cast<double2*>(at(kL.geo));
+  d.scell = static_cast<int4*>(at(kL.scell));
+  d.sgeo = static_cast<double2*>(at(kL.sgeo));
+  d.frec = static_cast<float4*>(at(kL.frec));
   d.bucket = static_cast<uint32_t*>(at(kL.bucket));
   d.perm = static_cast<int32_t*>(at(kL.perm));
   d.bstart = static_cast<int32_t*>(at(kL.bstart));
   d.bcursor = static_cast<int32_t*>(at(kL.bcursor));
   d.dslot = static_cast<int32_t*>(at(kL.dslot));
   d.dslot_off = static_cast<int32_t*>(at(kL.dslot_off));
-  d.dacc = static_cast<uint64_t*>(at(kL.dacc));
+  d.cstart = static_cast<int32_t*>(at(kL.cstart));
+  d.cand = static_cast<int2*>(at(kL.cand));
+  d.accept = static_cast<uint8_t*>(at(kL.accept));
+  d.bcount = static_cast<int32_t*>(at(kL.bcount));
+  d.cand_block = kCandBlock;
   d.dcount = static_cast<int32_t*>(at(kL.dcount));
   d.bbounds = static_cast<int32_t*>(at(kL.bbounds));
   d.rI = static_cast<int32_t*>(at(kL.rI));
@@ -231,8 +247,14 @@ pcc_status detect_noalloc(int device, hipStream_t s, const pcc_params& p, double
   d.tmp_bytes = (size_t)kTmp;
   d.s = s;
   if (d.T > kT) throw StatusError{PCC_E_DEVICE, "internal: hash table beyond the scratch"};
-  int32_t bounds[2] = {0, 0};
-  detect_count(d, bounds);
+  int32_t bounds[3] = {0, 0, 0};
+  if (filter_grid(n, d.S) > kGp) throw StatusError{PCC_E_DEVICE, "internal: filter grid beyond the scratch"};
+  if (!detect_count(d, bounds)) {
+    thread_error() = "pcc_detect_noalloc: " + std::to_string(bounds[2]) +
+                     " candidate pairs in one block of 256 (body, neighbour cell) slots exceed "
+                     "PCC_NOALLOC_MAX_BLOCK_CANDIDATES (" + std::to_string(kCandBlock) + ")";
+    return PCC_E_INVALID_ARGUMENT;
+  }
   const int64_t m = bounds[1];
   if (count) *count = m;
   if (m > kM) {

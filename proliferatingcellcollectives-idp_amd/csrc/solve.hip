@@ -61,7 +61,6 @@ void seg_sort_i32(pcc_ctx* ctx, int64_t nseg, const int32_t* start, int32_t* v);
 // the incidence records.
 static constexpr int kBbGridMax = 2048;
 static constexpr int64_t kTuneMinConstraints = 100000;
-static constexpr int kXcd = 8;
 
 namespace {
 
@@ -130,12 +129,6 @@ __device__ __forceinline__ double stored_g(double g) {
     return (double)(float)g;
 }
 
-// Logical block of hardware block h in a grid of gp blocks (gp % 8 == 0).
-__device__ __forceinline__ int64_t xcd_block(int64_t h, int64_t gp) {
-  return (h % kXcd) * (gp / kXcd) + h / kXcd;
-}
-
-inline int64_t pad8(int64_t g) { return (g + kXcd - 1) / kXcd * kXcd; }
 
 __global__ void k_inc_count(int64_t C, const int32_t* __restrict__ I, const int32_t* __restrict__ J,
                             int32_t* __restrict__ deg) {
