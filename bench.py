@@ -32,7 +32,8 @@ with the CPU restatement's soft step timed beside it.
 `--workload serving`: colony-growth requests with Poisson arrivals (20/s for
 10 s), 1k-100k cells each, hard or soft 50/50, growth rate U[0.5,1.5] ln2,
 200 steps each, served by the continuous-batching loop (pcc.serving) under an
-HBM budget; p50/p99 are over every request's per-step latencies.  The hard
+HBM budget and a per-model budget of resident cells; p50/p99 are over every
+request's per-step latencies, and per size bucket over the steady steps.  The hard
 colonies are relaxed by one untimed device step before the run, as growing
 colonies would be (--no-relax submits the raw packings).
 
@@ -231,6 +232,34 @@ def cpu_baseline(bodies, phases, sample_caps=(10, 110)):
         "est_step_s": est,
         "phases": detail,
     }
+
+
+def cpu_baseline_check(seed, n=40_000):
+    """The baseline's extrapolation checked against a measurement: on a
+    smaller colony of the same generator, one full FP64 step of the CPU
+    restatement (uncapped, to convergence) is timed and compared with the
+    estimate that the same fit (capped runs of each phase) gives at that
+    run's own per-phase iteration counts.  The Release-flag build rounds
+    differently from the GPU FP64 path, so its iteration counts are reported
+    beside the GPU's."""
+    from pcc.engine import ParticleManager
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+    b = G.dense_colony(n, seed=seed)
+    pm = ParticleManager(params_for(PREC_FP64))
+    pm.set_state(b)
+    pm.step()
+    gpu_phases = pm.phase_stats()
+    pm.close()
+    t = time.perf_counter()
+    st, res, _, _, ph, _ = oracle.step(params_for(PREC_FP64), b, fast=True)
+    measured = time.perf_counter() - t
+    cpu_phases = [(int(x), 0) for x in ph]
+    est = cpu_baseline(b, cpu_phases)
+    return {"n_cells": n, "est_step_s": est["est_step_s"], "measured_step_s": measured,
+            "est_over_measured": est["est_step_s"] / measured,
+            "cpu_phase_iterations": [x for x, _ in cpu_phases],
+            "gpu_phase_iterations": [int(x) for x, _ in gpu_phases], "cpu_status": int(st)}
 
 
 def ensemble_colonies(n_colonies, seed, lo=500, hi=50_000):
@@ -515,10 +544,25 @@ def main_serving(args):
     for mode, name in ((0, "hard"), (1, "soft")):
         x = per_mode[mode]
         done = [r for r in srv.results.values() if r.mode == mode]
+        # steady steps (after the warm-up batch) per size bucket: the batches
+        # whose step times a colony of that size sees
+        buckets = {}
+        for bk in range(len(srv.bucket_edges) + 1):
+            y = [v for r in srv.results.values() if r.mode == mode and srv._bucket(r.n_cells) == bk
+                 for v in r.step_ms[args.warm_steps:]]
+            y += [v for key, ms in srv.members.items() if key[0] == mode and key[1] == bk
+                  for m in ms for v in m.step_ms[args.warm_steps:]]
+            if y:
+                buckets[str(bk)] = {"steps": len(y), "p50_step_ms": float(np.percentile(y, 50)),
+                                    "p99_step_ms": float(np.percentile(y, 99))}
+        mw = [r.admitted - r.arrival for r in done]
         modes[name] = {"requests": sum(q.params.mode == mode for q in reqs),
                        "completed": len(done), "steps_recorded": len(x),
                        "p50_step_ms": float(np.percentile(x, 50)) if x else 0.0,
-                       "p99_step_ms": float(np.percentile(x, 99)) if x else 0.0}
+                       "p99_step_ms": float(np.percentile(x, 99)) if x else 0.0,
+                       "steady_by_bucket": buckets,
+                       "admission_wait_s": {"p50": float(np.percentile(mw, 50)) if mw else 0.0,
+                                            "max": float(max(mw)) if mw else 0.0}}
     line = {
         "metric": "cell-steps/s, continuous-batching colony server; p50/p99 per-step latency",
         "value": st["cell_steps_per_s"],
@@ -537,7 +581,8 @@ def main_serving(args):
                                 f"(log-uniform), "
                                 f"hard or soft 50/50, growth rate U[0.5,1.5] ln2, "
                                 f"{args.serve_steps} steps each (divide, then step), HBM "
-                                f"budget {args.hbm_budget_gb:g} GiB, mixed precision"),
+                                f"budget {args.hbm_budget_gb:g} GiB, resident-cell compute "
+                                f"budget {srv.max_resident}, mixed precision"),
                    "requests": len(reqs), "n_cells": ncell, "warm_steps": args.warm_steps,
                    "initial_states": ("hard colonies relaxed by one untimed device step "
                                       f"({relax_s:.1f} s before the run)" if args.relax
@@ -548,7 +593,9 @@ def main_serving(args):
         "p99_step_ms": st["p99_step_ms"],
         "serving": {k: st[k] for k in ("completed", "cell_steps", "wall_s", "batch_steps",
                                        "peak_reserved_bytes", "peak_device_bytes",
-                                       "budget_bytes")},
+                                       "budget_bytes", "peak_resident_cells",
+                                       "max_resident_cells")},
+        "bucket_edges_cells": list(srv.bucket_edges),
         "admission_wait_s": {"p50": float(np.percentile(waits, 50)) if waits else 0.0,
                              "max": float(max(waits)) if waits else 0.0},
         "completed_step_ms": {
@@ -707,6 +754,7 @@ def main():
     line["tuning"] = {k: v["tuning"] for k, v in out.items()}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(bodies, (out.get("fp64") or head)["phases"])
+        line["cpu_baseline"]["check"] = cpu_baseline_check(args.seed)
     if args.extras:
         # the other configured workloads, at their configured sizes and step
         # counts, as summaries beside the headline

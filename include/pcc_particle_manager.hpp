@@ -136,6 +136,7 @@ class ParticleManager {
       l0_[i] = p.l0;
       diam_[i] = p.diameter;
       ldot_[i] = p.ldot;
+      imp_[i] = p.impedance;  // input too: the Brownian turn scales with it
       age_[i] = p.age;
     }
     pcc_status s = pcc_set_state(ctx_, &b);

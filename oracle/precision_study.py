@@ -1,0 +1,141 @@
+"""Precision study behind the library's mixed-precision design (checker-side,
+CPU only): the phase-0 LCP of a dense random packing solved by BBPGD with
+four arithmetic variants of the operator applies.
+
+The input is a random disc packing (i.i.d. centres, lengths U[1,2],
+orientations U[0, pi), d = 0.5) at area fraction 0.7, whose LCP FP64 BBPGD
+solves to 1e-6 with impulses near 1e6 (at 0.9 crossing rods make it
+practically infeasible).  The LCP is the hard model's (HardModelGradient.cpp:
+89-158, BBPGD.cpp:32-127) without the growth coupling: gamma >= 0,
+g(gamma) = Phi + dt D M D^T gamma >= 0,
+gamma . g = 0, with D the contact Jacobian of the oracle's detection (rows
+[n, rI x n] at I and [-n, rJ x -n] at J), M the diagonal mobility
+(1/(xi l) translational, 12/(xi l^3) rotational, Physics.cpp:203-204) and
+Phi the signed distances.  BBPGD: projected steps
+gamma <- max(gamma - alpha g, 0), BB1 on even and BB2 on odd iterations,
+alpha_0 = 1/res_0, the projected-gradient residual
+max_i |gamma_i - max(gamma_i - g_i, 0)| checked every 10th iteration.
+
+Variants of the applies D^T gamma -> U = M F -> D U:
+  fp64          every operand and sum in FP64;
+  fp32_acc      the applies in FP32 arithmetic (operator, gamma, sums, U);
+  fp32_u        FP64 sums, the body velocities U rounded to FP32 once;
+  fp32_storage  the operator (D, M) stored in FP32, converted on load, FP64
+                sums (the library's mixed path).
+
+Run: make -C oracle && python oracle/precision_study.py [--n 3000] [--seed 7]
+"""
+import argparse
+import os
+import sys
+
+import numpy as np
+import scipy.sparse as sp
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [HERE, os.path.join(os.path.dirname(HERE), "proliferatingcellcollectives-idp_amd")]
+
+import oracle  # noqa: E402
+from pcc import generators as G  # noqa: E402
+from pcc.abi_types import reference_params  # noqa: E402
+
+
+def operator(bodies, cons, xi):
+    C = len(cons["gidI"])
+    n = bodies.n
+    rows, cols, vals = [], [], []
+    for key, rkey, sgn in (("gidI", "rPosI", 1.0), ("gidJ", "rPosJ", -1.0)):
+        b = cons[key].astype(np.int64)
+        nrm = sgn * cons["normal"]
+        tq = np.cross(cons[rkey], nrm)
+        for k in range(3):
+            rows.append(np.arange(C))
+            cols.append(6 * b + k)
+            vals.append(nrm[:, k])
+            rows.append(np.arange(C))
+            cols.append(6 * b + 3 + k)
+            vals.append(tq[:, k])
+    D = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(C, 6 * n))
+    ln = bodies.length
+    m = np.empty(6 * n)
+    for k in range(3):
+        m[k::6] = 1.0 / (ln * xi)
+        m[3 + k::6] = 12.0 / (ln ** 3 * xi)
+    return D, m
+
+
+def make_grad(D, m, phi, dt, variant):
+    if variant == "fp32_acc":
+        D32, DT32, m32 = D.astype(np.float32), D.T.tocsr().astype(np.float32), m.astype(np.float32)
+        phi32, dt32 = phi.astype(np.float32), np.float32(dt)
+        return lambda gam: (phi32 + dt32 * (D32 @ (m32 * (DT32 @ gam.astype(np.float32))))) \
+            .astype(np.float64)
+    if variant == "fp32_storage":
+        D = D.astype(np.float32).astype(np.float64)
+        m = m.astype(np.float32).astype(np.float64)
+    DT = D.T.tocsr()
+    if variant == "fp32_u":
+        return lambda gam: phi + dt * (D @ (m * (DT @ gam)).astype(np.float32).astype(np.float64))
+    return lambda gam: phi + dt * (D @ (m * (DT @ gam)))
+
+
+def bbpgd(grad, gamma0, tol, max_its):
+    res_of = lambda gm, g: float(np.max(np.abs(gm - np.maximum(gm - g, 0.0)))) if len(gm) else 0.0
+    gam = gamma0.copy()
+    g = grad(gam)
+    res = res_of(gam, g)
+    alpha = 1.0 / res if res > 0 else 1.0
+    k = 1
+    while True:
+        gn = np.maximum(gam - alpha * g, 0.0)
+        gr = grad(gn)
+        if k % 10 == 0:
+            res = res_of(gn, gr)
+            if res <= tol:
+                return gn, k, res
+        if k >= max_its:
+            return gn, k, res_of(gn, gr)
+        s, y = gn - gam, gr - g
+        if k % 2 == 0:
+            num, den = float(s @ s), float(s @ y)
+        else:
+            num, den = float(s @ y), float(y @ y)
+        if abs(den) < 10 * np.finfo(float).eps:
+            den += 10 * np.finfo(float).eps
+        alpha = num / den
+        gam, g = gn, gr
+        k += 1
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=3000)
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--max-its", type=int, default=20000)
+    ap.add_argument("--area-fraction", type=float, default=0.7)
+    ap.add_argument("--cutoff", type=float, default=0.05)
+    a = ap.parse_args()
+    p = reference_params(dt=1e-3, tolerance=a.tol)
+    b = G.random_disc(a.n, seed=a.seed, area_fraction=a.area_fraction)
+    cons = oracle.detect(b, a.cutoff)
+    D, m = operator(b, cons, p.xi)
+    phi = cons["signed_distance"]
+    gamma0 = np.where(phi < 0.0, -phi * 500.0, 0.0)
+    print(f"random packing n={a.n} seed={a.seed} area fraction {a.area_fraction}: {len(phi)} "
+          f"contacts at cutoff {a.cutoff}, "
+          f"max overlap {-phi.min():.3f}", flush=True)
+    ref = None
+    for v in ("fp64", "fp32_acc", "fp32_u", "fp32_storage"):
+        gam, its, res = bbpgd(make_grad(D, m, phi, p.dt, v), gamma0, a.tol, a.max_its)
+        if v == "fp64":
+            ref = gam
+        err = float(np.linalg.norm(gam - ref) / max(np.linalg.norm(ref), 1e-300))
+        state = "converged" if res <= a.tol else "not converged"
+        print(f"{v:13s} {state:13s} iterations {its:6d} residual {res:.2e} "
+              f"max gamma {gam.max():.3e} gamma error vs fp64 {err:.1e}", flush=True)
+
+
+if __name__ == "__main__":
+    main()

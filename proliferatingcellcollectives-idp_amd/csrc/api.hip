@@ -130,6 +130,40 @@ __global__ void k_reset(int64_t N, double* __restrict__ ldot, int32_t* __restric
   nc[i] = 0;
 }
 
+// Per-body outputs of the last step cleared after a division (F, U: 6 per
+// body; sigma, ldot_new; constraint counts), in one launch.
+__global__ void k_clear_outputs(int64_t N, double* __restrict__ F, double* __restrict__ U,
+                                double* __restrict__ sig, double* __restrict__ ldn,
+                                int32_t* __restrict__ nc) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > N) return;  // the +1 entries too, as the arrays' full length
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    F[6 * i + k] = 0.0;
+    U[6 * i + k] = 0.0;
+  }
+  sig[i] = 0.0;
+  ldn[i] = 0.0;
+  nc[i] = 0;
+}
+
+// Colony of every body from the colony records on the device (colonies are
+// stored one after another in record order).
+__global__ void k_binst(int64_t N, int64_t B, const InstDev* __restrict__ idev,
+                        int32_t* __restrict__ binst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int64_t lo = 0, hi = B - 1;  // last colony with boff <= i
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) / 2;
+    if (idev[mid].boff <= i)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  binst[i] = (int32_t)lo;
+}
+
 void reset_step(pcc_ctx* ctx) {
   const int64_t N = ctx->N;
   BodyArrays b = body_arrays(ctx->body.p, N);
@@ -143,8 +177,8 @@ void reset_step(pcc_ctx* ctx) {
   }
 }
 
-// Instance records to the device (synchronises the stream, so the host copy
-// may change right after).
+// Instance records to the device, through a pinned staging slot (no stream
+// synchronisation; the host copy may change right after).
 void upload_instances(pcc_ctx* ctx) {
   const int64_t B = (int64_t)ctx->inst.size();
   ctx->inst_host.resize(B);
@@ -186,9 +220,28 @@ void upload_instances(pcc_ctx* ctx) {
   }
   if (B == 0) return;
   ctx->idev.ensure(B);
-  PCC_HIP(hipMemcpyAsync(ctx->idev.p, ctx->inst_host.data(), sizeof(InstDev) * B,
-                         hipMemcpyHostToDevice, ctx->stream));
-  PCC_HIP(hipStreamSynchronize(ctx->stream));
+  constexpr int kSlots = 4;
+  if (ctx->inst_pin_cap < B) {
+    PCC_HIP(hipStreamSynchronize(ctx->stream));  // earlier copies from the old slots have run
+    DeviceLock lock(device_mutex());
+    if (ctx->inst_pin) PCC_HIP(hipHostFree(ctx->inst_pin));
+    ctx->inst_pin = nullptr;
+    ctx->inst_pin_cap = 0;
+    const int64_t cap = std::max<int64_t>(B, 16);
+    PCC_HIP(hipHostMalloc((void**)&ctx->inst_pin, sizeof(InstDev) * cap * kSlots,
+                          hipHostMallocDefault));
+    ctx->inst_pin_cap = cap;
+    for (auto& e : ctx->inst_ev)
+      if (!e) PCC_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  const int slot = ctx->inst_slot;
+  ctx->inst_slot = (slot + 1) % kSlots;
+  PCC_HIP(hipEventSynchronize(ctx->inst_ev[slot]));  // the slot's previous copy has run
+  InstDev* stage = ctx->inst_pin + (size_t)slot * ctx->inst_pin_cap;
+  memcpy(stage, ctx->inst_host.data(), sizeof(InstDev) * B);
+  PCC_HIP(hipMemcpyAsync(ctx->idev.p, stage, sizeof(InstDev) * B, hipMemcpyHostToDevice,
+                         ctx->stream));
+  PCC_HIP(hipEventRecord(ctx->inst_ev[slot], ctx->stream));
 }
 
 }  // namespace pcc
@@ -310,8 +363,9 @@ void hard_step(pcc_ctx* ctx) {
   begin_step(ctx);
   // planar kernels while every body lies in z = 0 with an in-plane
   // orientation and no Brownian input turns it out of the plane
-  ctx->planar = ctx->allow_planar && !brownian_offplane(ctx) && bodies_planar(ctx);
+  ctx->planar = ctx->allow_planar && !brownian_offplane(ctx) && state_planar(ctx);
   ctx->detect_flat = ctx->planar;  // z = 0 stays exact through a planar step
+  ctx->flat_known = ctx->planar;   // and after it
   std::vector<double> mo;
   for (int32_t k = 0;; ++k) {
     bool any = false;
@@ -378,22 +432,27 @@ void hard_step(pcc_ctx* ctx) {
 // solveSoftPotential (Soft.cpp:21-114) for every colony of the context.
 void soft_step(pcc_ctx* ctx) {
   begin_step(ctx);
-  ctx->detect_flat = ctx->allow_planar && bodies_planar(ctx);
+  const bool flat = state_planar(ctx);
+  ctx->detect_flat = ctx->allow_planar && flat;
+  // a planar state stays planar through the step unless a Brownian turn
+  // leaves the plane (Hertz forces between planar bodies have no z part)
+  ctx->flat_known = flat && !brownian_offplane(ctx);
   for (auto& I : ctx->inst) I.cutoff = 0.0;  // Soft.cpp:35
   upload_instances(ctx);
   detect_batch(ctx);
   append_batch(ctx, 0);
-  std::vector<double> mo;
-  max_overlap_inst(ctx, mo);  // Soft.cpp:37-41
-  for (size_t q = 0; q < ctx->inst.size(); ++q) {
-    ctx->inst[q].maxov = mo[q];
-    ctx->inst[q].res = mo[q];
-  }
+  max_overlap_launch(ctx);  // Soft.cpp:37-41, read at the end of the step
   build_incidence(ctx, false);
   brownian_inputs(ctx);  // U_ext of the step (Soft.cpp:94)
   soft_forces_and_integrate(ctx);
   ctx->brownian = false;
   PCC_HIP(hipStreamSynchronize(ctx->stream));
+  std::vector<double> mo;
+  max_overlap_read(ctx, mo);
+  for (size_t q = 0; q < ctx->inst.size(); ++q) {
+    ctx->inst[q].maxov = mo[q];
+    ctx->inst[q].res = mo[q];
+  }
 }
 
 pcc_status run_step(pcc_ctx* ctx) {
@@ -501,6 +560,7 @@ pcc_status set_colonies(pcc_ctx* ctx, int64_t B, const pcc_bodies* bodies,
     upload_binst(ctx);
     upload_instances(ctx);
     ctx->S.C = 0;
+    ctx->flat_known = false;
     ctx->has_state = true;
     ctx->has_snapshot = false;
     return PCC_OK;
@@ -704,6 +764,10 @@ void pcc_destroy(pcc_ctx* ctx) {
     }
     for (auto e : ctx->timing.free_events) (void)hipEventDestroy(e);
     if (ctx->state_host) (void)hipHostFree(ctx->state_host);
+    if (ctx->inst_pin) (void)hipHostFree(ctx->inst_pin);
+    if (ctx->ov_pin) (void)hipHostFree(ctx->ov_pin);
+    for (auto e : ctx->inst_ev)
+      if (e) (void)hipEventDestroy(e);
   }
   {
     StreamScope scope(s);  // pooled buffers go back to the pool in the stream's order
@@ -766,6 +830,7 @@ pcc_status pcc_restore(pcc_ctx* ctx) {
     PCC_HIP(hipMemcpyAsync(ctx->body.p, ctx->snap.p, sizeof(double) * kBodyFields * ctx->N,
                            hipMemcpyDeviceToDevice, ctx->stream));
     PCC_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->flat_known = false;
     for (size_t q = 0; q < ctx->inst.size() && q < ctx->snap_nstep.size(); ++q)
       ctx->inst[q].nstep = ctx->snap_nstep[q];
     return PCC_OK;
@@ -929,16 +994,15 @@ static pcc_status divide_impl(pcc_ctx* ctx, const uint64_t* seeds, int32_t step,
     PCC_HIP(hipSetDevice(ctx->device));
     divide_colonies(ctx, seeds, step, length_noise, angle_noise);
     {
-      const int64_t N = ctx->N;
+      const int64_t N = ctx->N, B = (int64_t)ctx->inst.size();
       ensure_body_arrays(ctx, N);
       hipStream_t s = ctx->stream;
-      PCC_HIP(hipMemsetAsync(ctx->F.p, 0, sizeof(double) * (6 * N + 6), s));
-      PCC_HIP(hipMemsetAsync(ctx->U.p, 0, sizeof(double) * (6 * N + 6), s));
-      PCC_HIP(hipMemsetAsync(ctx->sig.p, 0, sizeof(double) * (N + 1), s));
-      PCC_HIP(hipMemsetAsync(ctx->ldot_new.p, 0, sizeof(double) * (N + 1), s));
-      PCC_HIP(hipMemsetAsync(ctx->nc.p, 0, sizeof(int32_t) * (N + 1), s));
-      upload_binst(ctx);
+      k_clear_outputs<<<(unsigned)((N + 1 + 255) / 256), 256, 0, s>>>(N, ctx->F.p, ctx->U.p,
+                                                                      ctx->sig.p, ctx->ldot_new.p,
+                                                                      ctx->nc.p);
       upload_instances(ctx);
+      if (N > 0 && B > 0)
+        k_binst<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(N, B, ctx->idev.p, ctx->binst.p);
     }
     return PCC_OK;
   });
@@ -1064,6 +1128,7 @@ pcc_status pcc_batch_add(pcc_ctx* ctx, const pcc_bodies* bodies, const pcc_param
       PCC_HIP(hipMemcpyAsync(ctx->body.p + (size_t)k * newN + oldN, h.data() + (size_t)k * n,
                              sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
     PCC_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->flat_known = false;
     if (colony) *colony = (int64_t)ctx->inst.size() - 1;
     return PCC_OK;
   });

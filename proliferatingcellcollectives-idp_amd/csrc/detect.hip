@@ -75,26 +75,37 @@ __global__ void k_cells(int64_t N, BodyArrays b, const int32_t* __restrict__ bin
   atomicAdd(&count[bk], 1);
 }
 
+// Bodies into their buckets; cnt holds each bucket's count (from k_cells)
+// and ends at zero.  The order within a bucket is the order of arrival;
+// k_seg_sort then orders it by body index.
 __global__ void k_scatter(int64_t N, const uint32_t* __restrict__ bucket,
-                          const int32_t* __restrict__ start, int32_t* __restrict__ cursor,
+                          const int32_t* __restrict__ start, int32_t* __restrict__ cnt,
                           int32_t* __restrict__ perm) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   uint32_t bk = bucket[i];
-  int32_t pos = start[bk] + atomicAdd(&cursor[bk], 1);
+  int32_t pos = start[bk] + atomicSub(&cnt[bk], 1) - 1;
   perm[pos] = (int32_t)i;
 }
 
 // Every body's cell and detection record in bucket order (perm[pos] is the
 // body at position pos), and its filter record: FP32 centre relative to its
-// cell's corner (origin + cell * h per axis) and FP32 length.
-__global__ void k_stage(int64_t N, const int32_t* __restrict__ perm,
+// cell's corner (origin + cell * h per axis) and FP32 length.  Also clears
+// the body's S per-slot contact counts (and the closing entry and the
+// candidate-region flag) for the filter that follows.
+__global__ void k_stage(int64_t N, int S, const int32_t* __restrict__ perm,
                         const int4* __restrict__ cell4, const double2* __restrict__ geo,
                         const InstDev* __restrict__ idev, int4* __restrict__ scell,
-                        double2* __restrict__ sgeo, float4* __restrict__ frec) {
+                        double2* __restrict__ sgeo, float4* __restrict__ frec,
+                        int32_t* __restrict__ acnt, int32_t* __restrict__ need) {
   const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pos == 0) {
+    acnt[N * S] = 0;
+    *need = 0;
+  }
   if (pos >= N) return;
   const int64_t J = perm[pos];
+  for (int sl = 0; sl < S; ++sl) acnt[J * S + sl] = 0;
   const int4 c = cell4[J];
   scell[pos] = c;
   double2 g[4];
@@ -643,16 +654,17 @@ bool detect_count(DetectBufs& d, int32_t* bounds_host) {
                                     d.bucket,
                                     d.bcursor);
   scan_on(d, d.bcursor, d.bstart, T + 1);
-  k_fill_i32<<<nblk(T + 1), 256, 0, s>>>(T + 1, 0, d.bcursor);
+  // candidates, then their test; dslot counts each slot's contacts
+  int32_t* need = d.bbounds + d.B + 1;
   if (N > 0) {
     k_scatter<<<nblk(N), 256, 0, s>>>(N, d.bucket, d.bstart, d.bcursor, d.perm);
     k_seg_sort<<<nblk(T), 256, 0, s>>>(T, d.bstart, d.perm);
-    k_stage<<<nblk(N), 256, 0, s>>>(N, d.perm, d.cell4, d.geo, d.idev, d.scell, d.sgeo, d.frec);
+    k_stage<<<nblk(N), 256, 0, s>>>(N, d.S, d.perm, d.cell4, d.geo, d.idev, d.scell, d.sgeo,
+                                    d.frec, d.dslot, need);
+  } else {
+    k_fill_i32<<<1, 64, 0, s>>>(1, 0, need);
+    k_fill_i32<<<1, 64, 0, s>>>(1, 0, d.dslot);
   }
-  // candidates, then their test; dslot counts each slot's contacts
-  int32_t* need = d.bbounds + d.B + 1;
-  k_fill_i32<<<1, 64, 0, s>>>(1, 0, need);
-  k_fill_i32<<<nblk(NS + 1), 256, 0, s>>>(NS + 1, 0, d.dslot);
   if (N > 0) {
     const int64_t gp = filter_grid(N, d.S);
     if (d.S == 9)

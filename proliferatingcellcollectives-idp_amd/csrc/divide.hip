@@ -282,8 +282,7 @@ void divide_colonies(pcc_ctx* ctx, const uint64_t* seeds, int32_t step, double l
   DArr<double>& body = ctx->dv_body;
   body.ensure(kBodyFields * M + 1);
   k_div_gather<<<nblk(M), 256, 0, s>>>(M, stage.p, order, body.p);
-  PCC_HIP(hipStreamSynchronize(s));
-  ctx->body.swap(body);
+  ctx->body.swap(body);  // stream order keeps every later use after the gather
   // new colony ranges; per-body outputs of the last step no longer apply
   for (int64_t q = 0; q < B; ++q) {
     Instance& I = ctx->inst[q];

@@ -329,6 +329,19 @@ struct pcc_ctx {
   // colonies of this context (one for the single-colony entry points)
   std::vector<pcc::Instance> inst;
   std::vector<pcc::InstDev> inst_host;
+  // pinned staging of the colony records: kInstSlots slots of inst_pin_cap
+  // records, each reused once the copy recorded on its event has run
+  pcc::InstDev* inst_pin = nullptr;
+  int64_t inst_pin_cap = 0;
+  int inst_slot = 0;
+  hipEvent_t inst_ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // pinned copy of the per-colony max overlap (max_overlap_launch)
+  unsigned long long* ov_pin = nullptr;
+  int64_t ov_pin_cap = 0;
+  // planarity of the body state as of the end of the last step (cleared by
+  // every host write of body state; kept by device division)
+  bool flat_known = false;
+  bool flat_value = false;
   pcc::DArr<pcc::InstDev> idev;            // B
   pcc::DArr<int32_t> binst;                // N: instance of every body
   pcc::DArr<int32_t> bmap;                 // logical body block -> instance
@@ -393,6 +406,7 @@ void detect_batch(pcc_ctx* ctx);  // active colonies at their cutoffs; fills ctx
 void append_batch(pcc_ctx* ctx, int32_t iteration);               // appends sorted batch to S
 void build_incidence(pcc_ctx* ctx, bool records);
 bool bodies_planar(pcc_ctx* ctx);
+bool state_planar(pcc_ctx* ctx);  // bodies_planar, or the value known from the last step
 void bbpgd_solve(pcc_ctx* ctx);
 void integrate_hard(pcc_ctx* ctx);
 void grow(pcc_ctx* ctx);
@@ -406,6 +420,8 @@ void colony_median_speeds(pcc_ctx* ctx, std::vector<double>& median);
 double adapted_dt(double dt, double median, double cfl_factor, double tolerance, double alpha);
 void upload_instances(pcc_ctx* ctx);
 void max_overlap_inst(pcc_ctx* ctx, std::vector<double>& out);
+void max_overlap_launch(pcc_ctx* ctx);  // reduction and copy to pinned memory, no wait
+void max_overlap_read(pcc_ctx* ctx, std::vector<double>& out);  // after a stream sync
 void merge_batch(pcc_ctx* ctx);
 
 BodyArrays body_arrays(double* base, int64_t N);

@@ -138,14 +138,17 @@ __global__ void k_inc_count(int64_t C, const int32_t* __restrict__ I, const int3
   atomicAdd(&deg[J[c]], 1);
 }
 
+// Incidences into their bodies' ranges; cnt holds each body's count (from
+// k_inc_count) and ends at zero.  The order within a range is the order of
+// arrival; k_seg_sort then orders it.
 __global__ void k_inc_scatter(int64_t C, const int32_t* __restrict__ I,
                               const int32_t* __restrict__ J, const int32_t* __restrict__ rowptr,
-                              int32_t* __restrict__ cur, int32_t* __restrict__ inc) {
+                              int32_t* __restrict__ cnt, int32_t* __restrict__ inc) {
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= C) return;
   int32_t a = I[c], b = J[c];
-  inc[rowptr[a] + atomicAdd(&cur[a], 1)] = (int32_t)(2 * c);
-  inc[rowptr[b] + atomicAdd(&cur[b], 1)] = (int32_t)(2 * c + 1);
+  inc[rowptr[a] + atomicSub(&cnt[a], 1) - 1] = (int32_t)(2 * c);
+  inc[rowptr[b] + atomicSub(&cnt[b], 1) - 1] = (int32_t)(2 * c + 1);
 }
 
 // Solver order: positions of the constraints stably sorted by I.
@@ -1505,20 +1508,56 @@ __global__ void k_grow(int64_t N, const int32_t* __restrict__ binst,
 
 }  // namespace
 
-// Max overlap of every colony's current constraint set (Hard.cpp:125-130).
-void max_overlap_inst(pcc_ctx* ctx, std::vector<double>& out) {
+// Max overlap of every colony's current constraint set (Hard.cpp:125-130):
+// the reduction and its copy into pinned host memory are queued on the
+// stream; max_overlap_read takes the values after a stream synchronisation.
+void max_overlap_launch(pcc_ctx* ctx) {
   const int64_t B = (int64_t)ctx->inst.size(), C = ctx->S.C;
-  out.assign(B, 0.0);
-  if (C == 0) return;
+  if (ctx->ov_pin_cap < B) {
+    PCC_HIP(hipStreamSynchronize(ctx->stream));
+    DeviceLock lock(device_mutex());
+    if (ctx->ov_pin) PCC_HIP(hipHostFree(ctx->ov_pin));
+    ctx->ov_pin = nullptr;
+    ctx->ov_pin_cap = 0;
+    const int64_t cap = std::max<int64_t>(B, 16);
+    PCC_HIP(hipHostMalloc((void**)&ctx->ov_pin, sizeof(unsigned long long) * cap,
+                          hipHostMallocDefault));
+    ctx->ov_pin_cap = cap;
+  }
+  if (B == 0) return;
+  if (C == 0) {
+    for (int64_t b = 0; b < B; ++b) ctx->ov_pin[b] = 0ull;  // +0.0
+    return;
+  }
   hipStream_t s = ctx->stream;
   ctx->maxbits.ensure(B);
   PCC_HIP(hipMemsetAsync(ctx->maxbits.p, 0, sizeof(unsigned long long) * B, s));
   k_maxov<<<nblk(C), 256, 0, s>>>(C, ctx->S.inst.p, ctx->S.phi.p, ctx->maxbits.p);
-  std::vector<unsigned long long> h(B);
-  PCC_HIP(hipMemcpyAsync(h.data(), ctx->maxbits.p, sizeof(unsigned long long) * B,
+  PCC_HIP(hipMemcpyAsync(ctx->ov_pin, ctx->maxbits.p, sizeof(unsigned long long) * B,
                          hipMemcpyDeviceToHost, s));
-  PCC_HIP(hipStreamSynchronize(s));
-  for (int64_t b = 0; b < B; ++b) memcpy(&out[b], &h[b], sizeof(double));
+}
+
+void max_overlap_read(pcc_ctx* ctx, std::vector<double>& out) {
+  const int64_t B = (int64_t)ctx->inst.size();
+  out.assign(B, 0.0);
+  for (int64_t b = 0; b < B && b < ctx->ov_pin_cap; ++b) memcpy(&out[b], &ctx->ov_pin[b], sizeof(double));
+}
+
+void max_overlap_inst(pcc_ctx* ctx, std::vector<double>& out) {
+  max_overlap_launch(ctx);
+  PCC_HIP(hipStreamSynchronize(ctx->stream));
+  max_overlap_read(ctx, out);
+}
+
+// The planarity of the body state, from the last step when it kept the
+// state in the plane (planar kernels, no out-of-plane Brownian turn, device
+// division), otherwise checked on the device.
+bool state_planar(pcc_ctx* ctx) {
+  if (!ctx->flat_known) {
+    ctx->flat_value = bodies_planar(ctx);
+    ctx->flat_known = true;
+  }
+  return ctx->flat_value;
 }
 
 // 1 if every body lies in the z = 0 plane with an in-plane orientation.
@@ -1573,7 +1612,6 @@ void build_incidence(pcc_ctx* ctx, bool records) {
   PCC_HIP(hipMemsetAsync(ctx->inccur.p, 0, sizeof(int32_t) * (size_t)(N + 1), s));
   if (C > 0) k_inc_count<<<nblk(C), 256, 0, s>>>(C, ctx->S.I.p, ctx->S.J.p, ctx->inccur.p);
   exclusive_scan_i32(ctx, ctx->inccur.p, ctx->rowptr.p, N + 1);
-  PCC_HIP(hipMemsetAsync(ctx->inccur.p, 0, sizeof(int32_t) * (size_t)(N + 1), s));
   if (C == 0) {
     if (records) fold_gamma_old(ctx);  // every body's sum is 0
     return;

@@ -12,8 +12,12 @@ Between batch steps it
   * admits waiting requests, first come first served, while the estimated
     device footprint of the resident colonies plus the newcomer stays within
     an HBM budget (sized for the most cells the colony can reach by division
-    in its steps): an admitted colony joins its model's batch and its state
-    stays resident in HBM until its last step;
+    in its steps) and, per model, the resident cells stay within a compute
+    budget (the cells a batch step may carry: a step's time grows with its
+    cells, so this bounds the per-step latency of every resident colony; the
+    device is shared by the batches of both models): an admitted colony joins
+    its model's batch and its state stays resident in HBM until its last
+    step;
   * retires colonies that have run all their steps: the final state is
     downloaded and the colony leaves the batch, freeing its share of the
     budget for the next request.
@@ -51,6 +55,12 @@ from .engine import ColonyBatch
 # scratch; dense 2D packings hold ~10 constraints per cell over the ReLCP
 # phases of a step).  The run reports the measured peak beside it.
 DEFAULT_BYTES_PER_CELL = 12_000
+
+# Resident cells per model that admission allows (the compute budget).  A
+# hard step of a relaxed dense packing runs at ~1M cell-steps/s on one
+# MI355X (a few hundred BBPGD iterations per ReLCP phase), a soft step at
+# ~100M: these keep a batch step near a quarter of a second.
+DEFAULT_MAX_CELLS = {T.MODE_HARD: 250_000, T.MODE_SOFT: 4_000_000}
 
 
 @dataclass
@@ -120,8 +130,12 @@ class ColonyServer:
     def __init__(self, hbm_budget_bytes: int, precision: int = T.PREC_MIXED, device: int = 0,
                  bytes_per_cell: Optional[float] = None, overlap: bool = True,
                  bucket_edges: Sequence[int] = (4_000, 20_000), warm_steps: int = 2,
-                 divide: bool = True, length_noise: float = 0.0, angle_noise: float = 0.01):
+                 divide: bool = True, length_noise: float = 0.0, angle_noise: float = 0.01,
+                 max_cells_resident: Optional[Dict[int, int]] = None):
         self.budget = int(hbm_budget_bytes)
+        # compute budget: resident cells per model (None: HBM alone decides)
+        self.max_resident = dict(DEFAULT_MAX_CELLS if max_cells_resident is None
+                                 else max_cells_resident)
         self.precision = precision
         self.device = device
         self.overlap = overlap  # False: one worker steps every batch in turn
@@ -144,6 +158,8 @@ class ColonyServer:
         self.batch_steps = 0
         self.cell_steps = 0
         self._reserved = 0
+        self._cells = {T.MODE_HARD: 0, T.MODE_SOFT: 0}  # resident cells (admission bound)
+        self.peak_cells = {T.MODE_HARD: 0, T.MODE_SOFT: 0}
         self._lock = threading.Condition()
         self._stop = False
         self._errors: List[str] = []
@@ -173,16 +189,39 @@ class ColonyServer:
             req.params = req.params.replace(precision=self.precision)
         self.queue.append(req)
 
+    def cells_charge(self, req: Request) -> int:
+        """Cells a request counts against its model's compute budget: the
+        most it can reach in its steps (the HBM bound)."""
+        return max_cells(req, self.length_noise) if self.divide else req.bodies.n
+
     def _admit(self, now: float) -> int:
+        """First come, first served: a request that does not fit the HBM
+        budget stops admission; one that does not fit its model's compute
+        budget stops admission for that model only (later requests of the
+        other model may pass it, never those of its own)."""
         n = 0
         with self._lock:
-            while self.queue:
-                req = self.queue[0]
+            blocked = set()
+            i = 0
+            while i < len(self.queue):
+                req = self.queue[i]
+                mode = req.params.mode
+                if mode in blocked:
+                    i += 1
+                    continue
                 need = self.reservation(req)
                 if self._reserved + need > self.budget and self._reserved > 0:
-                    break  # first come, first served: the head waits for memory
-                self.queue.popleft()
+                    break  # the head waits for memory
+                cells = self.cells_charge(req)
+                cap = self.max_resident.get(mode)
+                if cap is not None and self._cells[mode] > 0 and self._cells[mode] + cells > cap:
+                    blocked.add(mode)  # waits for compute
+                    i += 1
+                    continue
+                del self.queue[i]
                 self._reserved += need
+                self._cells[mode] += cells
+                self.peak_cells[mode] = max(self.peak_cells[mode], self._cells[mode])
                 self.peak_reserved = max(self.peak_reserved, self._reserved)
                 key = (req.params.mode, self._bucket(req.bodies.n),
                        0 if self.warm_steps > 0 else 1)
@@ -259,6 +298,7 @@ class ColonyServer:
                     self.handoff[(key[0], key[1], 1)].append(m)
                     continue
                 self._reserved -= self.reservation(m.req)
+                self._cells[m.req.params.mode] -= self.cells_charge(m.req)
                 self.results[m.req.rid] = Result(
                     m.req.rid, key[0], m.req.bodies.n, m.req.steps, state, m.step_ms,
                     m.statuses, m.req.arrival, m.admitted, now)
@@ -344,6 +384,10 @@ class ColonyServer:
             "p99_step_ms": float(np.percentile(lat, 99)) if len(lat) else 0.0,
             "batch_steps": self.batch_steps,
             "peak_reserved_bytes": int(self.peak_reserved),
+            "peak_resident_cells": {"hard": int(self.peak_cells[T.MODE_HARD]),
+                                    "soft": int(self.peak_cells[T.MODE_SOFT])},
+            "max_resident_cells": {"hard": self.max_resident.get(T.MODE_HARD),
+                                   "soft": self.max_resident.get(T.MODE_SOFT)},
             "peak_device_bytes": int(self.peak_device_bytes),
             "budget_bytes": self.budget,
         }

@@ -122,6 +122,25 @@ def test_admission_respects_budget_and_order():
         assert np.array_equal(r[q.rid].state.length, q.bodies.length + q.steps)
 
 
+def test_compute_budget_bounds_resident_cells_per_model():
+    # HBM is plentiful; the hard model may hold 300 resident cells, the soft
+    # one is unbounded: hard requests take turns, the soft one runs at once
+    reqs = make_requests([(200, 0, 3, 0.0), (200, 0, 3, 0.0), (150, 0, 2, 0.0),
+                          (1000, 1, 2, 0.0)])
+    srv = serving.ColonyServer(10**12, bytes_per_cell=1000, divide=False,
+                               max_cells_resident={T.MODE_HARD: 300})
+    st = srv.run(reqs)
+    assert st["completed"] == 4
+    assert st["peak_resident_cells"]["hard"] <= 300
+    assert st["peak_resident_cells"]["soft"] == 1000
+    r = srv.results
+    assert r[1].admitted >= r[0].finished  # 200 + 200 > 300: waited for compute
+    assert r[2].admitted >= r[1].admitted  # first come, first served
+    assert r[3].admitted < r[0].finished   # the other model's budget is its own
+    for q in reqs:
+        assert np.array_equal(r[q.rid].state.length, q.bodies.length + q.steps)
+
+
 def test_late_arrival_joins_a_running_batch():
     # the first colony stays resident for at least 500 x 2 ms; the second
     # arrives well after its first step, also on a loaded machine
