@@ -244,6 +244,19 @@ void upload_instances(pcc_ctx* ctx) {
   PCC_HIP(hipEventRecord(ctx->inst_ev[slot], ctx->stream));
 }
 
+int32_t* readback_pin(pcc_ctx* ctx, int64_t n) {
+  if (ctx->rb_pin_cap < n) {
+    DeviceLock lock(device_mutex());
+    if (ctx->rb_pin) PCC_HIP(hipHostFree(ctx->rb_pin));
+    ctx->rb_pin = nullptr;
+    ctx->rb_pin_cap = 0;
+    const int64_t cap = std::max<int64_t>(n, 64);
+    PCC_HIP(hipHostMalloc((void**)&ctx->rb_pin, sizeof(int32_t) * cap, hipHostMallocDefault));
+    ctx->rb_pin_cap = cap;
+  }
+  return ctx->rb_pin;
+}
+
 }  // namespace pcc
 
 using namespace pcc;
@@ -766,6 +779,7 @@ void pcc_destroy(pcc_ctx* ctx) {
     if (ctx->state_host) (void)hipHostFree(ctx->state_host);
     if (ctx->inst_pin) (void)hipHostFree(ctx->inst_pin);
     if (ctx->ov_pin) (void)hipHostFree(ctx->ov_pin);
+    if (ctx->rb_pin) (void)hipHostFree(ctx->rb_pin);
     for (auto e : ctx->inst_ev)
       if (e) (void)hipEventDestroy(e);
   }
@@ -1241,12 +1255,12 @@ pcc_status pcc_device_bytes(pcc_ctx* ctx, int64_t* bytes) {
     add(S.coef32);
   };
   add(ctx->body); add(ctx->snap); add(ctx->F); add(ctx->U); add(ctx->sig); add(ctx->uext);
-  add(ctx->dv_flag); add(ctx->dv_off); add(ctx->dv_bounds); add(ctx->dv_val); add(ctx->dv_perm);
-  add(ctx->dv_perm2); add(ctx->dv_sinst); add(ctx->dv_stage); add(ctx->dv_body); add(ctx->dv_key);
-  add(ctx->dv_key2); add(ctx->dv_ckey); add(ctx->dv_ckey2); add(ctx->dv_col);
+  add(ctx->dv_flag); add(ctx->dv_off); add(ctx->dv_bounds); add(ctx->dv_perm);
+  add(ctx->dv_sinst); add(ctx->dv_stage); add(ctx->dv_body); add(ctx->dv_key); add(ctx->dv_col);
   add(ctx->ldot_new); add(ctx->nc); add(ctx->cell); add(ctx->bucket); add(ctx->perm);
   add(ctx->bstart); add(ctx->bcursor); add(ctx->dcount); add(ctx->dslot); add(ctx->dslot_off); add(ctx->dcstart); add(ctx->dcand); add(ctx->daccept); add(ctx->dbcount); add(ctx->dgeo); add(ctx->scell); add(ctx->sgeo); add(ctx->sfrec); add(ctx->rI_); add(ctx->rJ_);
   add(ctx->rrec); add(ctx->ridx); add(ctx->ridx2); add(ctx->rkey); add(ctx->rkey2);
+  add(ctx->xs_cnt); add(ctx->xs_off); add(ctx->xs_bounds);
   add(ctx->sort_tmp); add_set(ctx->S); add_set(ctx->S2); add(ctx->rowptr); add(ctx->inccur);
   add(ctx->inc); add(ctx->increc); add(ctx->increc32); add(ctx->incidx); add(ctx->gold_e); add(ctx->fold); add(ctx->plist);
   add(ctx->sord); add(ctx->spos); add(ctx->skey); add(ctx->skey2); add(ctx->sval); add(ctx->sI);

@@ -28,9 +28,8 @@
 //     scan         slot offsets, so each body's records keep the cell order
 //                  dz, dy, dx and the bucket order
 //     k_pair_write recomputes each accepted pair and writes its record
-//  5. k_segsort + radix sort: order = (contact x, I, J); with several colonies
-//                  in one context a second stable radix sort by colony makes
-//                  the order (colony, contact x, I, J)
+//                  with the keys of the order (colony, contact x, I, J)
+//  5. xsort_run:   that order by a bucket sort (xsort.hip)
 //
 // Colonies of one context share the hashed table: the colony index salts the
 // bucket hash and neighbour tests require the same colony, so each colony's
@@ -38,7 +37,6 @@
 #include <algorithm>
 #include <cstring>
 #include <vector>
-#include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
 #include "common.hpp"
@@ -368,7 +366,8 @@ k_pair_write(int S, int64_t gp, int32_t cb, const int32_t* __restrict__ bcount,
              const double2* __restrict__ geo, const double2* __restrict__ sgeo,
              const int32_t* __restrict__ perm, const int32_t* __restrict__ binst,
              const InstDev* __restrict__ idev, int32_t* __restrict__ outI,
-             int32_t* __restrict__ outJ, double* __restrict__ rec) {
+             int32_t* __restrict__ outJ, double* __restrict__ rec, uint64_t* __restrict__ xkey,
+             uint64_t* __restrict__ xtie, int32_t* __restrict__ xgrp) {
   const int64_t l = xcd_block(blockIdx.x, gp);
   const int32_t nb = bcount[l];
   for (int32_t j = threadIdx.x; j < nb; j += blockDim.x) {
@@ -384,9 +383,16 @@ k_pair_write(int S, int64_t gp, int32_t cb, const int32_t* __restrict__ bcount,
     seg_of(sgeo + 4 * (int64_t)c.y, b);
     DcpOut o;
     const double phi = pair_phi(a, b, o);
+    const int32_t J = perm[c.y];
+    const int32_t inst = binst[I];
     outI[slot] = (int32_t)I;
-    outJ[slot] = perm[c.y];
-    pair_record(a, b, o, phi, idev[binst[I]].warm, rec + R_LEN * slot);
+    outJ[slot] = J;
+    double* r = rec + R_LEN * slot;
+    pair_record(a, b, o, phi, idev[inst].warm, r);
+    // keys of the (colony, contact x, I, J) order
+    xkey[slot] = order_key(r[R_P]);
+    xtie[slot] = ((uint64_t)(uint32_t)I << 32) | (uint32_t)J;
+    xgrp[slot] = inst;
   }
 }
 
@@ -395,42 +401,6 @@ __global__ void k_body_offsets(int64_t N, int S, const int32_t* __restrict__ slo
                                int32_t* __restrict__ off) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i <= N) off[i] = slot_off[i * S];
-}
-
-// Per body segment of the batch: order its records by J (insertion sort of
-// record indices).
-__global__ void k_segsort_by_J(int64_t N, const int32_t* __restrict__ off,
-                               const int32_t* __restrict__ J, int32_t* __restrict__ idx) {
-  int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (I >= N) return;
-  int32_t a = off[I], e = off[I + 1];
-  for (int32_t i = a; i < e; ++i) idx[i] = i;
-  for (int32_t i = a + 1; i < e; ++i) {
-    int32_t x = idx[i];
-    int32_t jx = J[x];
-    int32_t j = i - 1;
-    while (j >= a && J[idx[j]] > jx) {
-      idx[j + 1] = idx[j];
-      --j;
-    }
-    idx[j + 1] = x;
-  }
-}
-
-__global__ void k_make_keys(int64_t m, const int32_t* __restrict__ idx,
-                           const double* __restrict__ rec, uint64_t* __restrict__ key) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  key[i] = order_key(rec[R_LEN * (int64_t)idx[i] + R_P]);
-}
-
-// Colony of each sorted batch record (key of the second, stable sort).
-__global__ void k_inst_keys(int64_t m, const int32_t* __restrict__ idx,
-                            const int32_t* __restrict__ rI_, const int32_t* __restrict__ binst,
-                            uint32_t* __restrict__ key) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  key[i] = (uint32_t)binst[rI_[idx[i]]];
 }
 
 // Batch offsets at the colony boundaries: colony b's records occupy
@@ -595,20 +565,6 @@ void scan_on(DetectBufs& d, const int32_t* in, int32_t* out, int64_t n) {
                                   rocprim::plus<int32_t>(), d.s));
 }
 
-template <class K>
-void sort_pairs_on(DetectBufs& d, K* k0, K* k1, int32_t* v0, int32_t* v1, int64_t m, int bits) {
-  size_t bytes = 0;
-  PCC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, k0, k1, v0, v1, (size_t)m, 0, bits, d.s));
-  if (bytes > d.tmp_bytes) throw StatusError{PCC_E_DEVICE, "internal: sort storage too small"};
-  PCC_HIP(rocprim::radix_sort_pairs(d.tmp, bytes, k0, k1, v0, v1, (size_t)m, 0, bits, d.s));
-}
-
-int colony_bits(int64_t B) {
-  int bits = 1;
-  while (((int64_t)1 << bits) < B) ++bits;
-  return bits;
-}
-
 }  // namespace
 
 size_t detect_scan_bytes(int64_t N, int S, int64_t T) {
@@ -626,15 +582,8 @@ int64_t filter_grid(int64_t N, int S) {
 }
 
 size_t detect_sort_bytes(int64_t m, int64_t B) {
-  size_t a = 0, b = 0;
-  uint64_t* k = nullptr;
-  uint32_t* k32 = nullptr;
-  int32_t* v = nullptr;
-  PCC_HIP(rocprim::radix_sort_pairs(nullptr, a, k, k, v, v, (size_t)m, 0, 64, (hipStream_t)0));
-  if (B > 1)
-    PCC_HIP(rocprim::radix_sort_pairs(nullptr, b, k32, k32, v, v, (size_t)m, 0, colony_bits(B),
-                                      (hipStream_t)0));
-  return std::max(a, b);
+  (void)B;
+  return xsort_tmp_bytes(m);
 }
 
 // Stage 1 of a detection batch: the hashed cell list (a stable counting sort
@@ -693,27 +642,34 @@ bool detect_count(DetectBufs& d, int32_t* bounds_host) {
 
 // Stage 2: the writing pass of the m records found by detect_count, then
 // their order (colony, contact x, I, J); on return d.ridx lists them in that
-// order (d.ridx and d.ridx2 may have been exchanged).
+// order.
 void detect_write(DetectBufs& d, int64_t m) {
   if (m == 0) return;
   const int64_t N = d.N;
   const hipStream_t s = d.s;
   const int64_t gp = filter_grid(N, d.S);
+  // ridx2: each record's colony (then its bucket); rkey / rkey2: its x key
+  // and (I, J) tie-break
   k_pair_write<<<(unsigned)gp, 256, 0, s>>>(d.S, gp, d.cand_block, d.bcount, d.cand, d.accept,
                                             d.cstart, d.dslot_off, d.geo, d.sgeo, d.perm,
-                                            d.binst, d.idev, d.rI, d.rJ, d.rec);
-  k_segsort_by_J<<<nblk(N), 256, 0, s>>>(N, d.dcount, d.rJ, d.ridx);
-  k_make_keys<<<nblk(m), 256, 0, s>>>(m, d.ridx, d.rec, d.rkey);
-  sort_pairs_on<uint64_t>(d, d.rkey, d.rkey2, d.ridx, d.ridx2, m, 64);
-  std::swap(d.ridx, d.ridx2);  // sorted order now in ridx
-  if (d.B > 1) {
-    // (colony, contact x, I, J): stable LSD sort by colony on top
-    uint32_t* k0 = reinterpret_cast<uint32_t*>(d.rkey);
-    uint32_t* k1 = reinterpret_cast<uint32_t*>(d.rkey2);
-    k_inst_keys<<<nblk(m), 256, 0, s>>>(m, d.ridx, d.rI, d.binst, k0);
-    sort_pairs_on<uint32_t>(d, k0, k1, d.ridx, d.ridx2, m, colony_bits(d.B));
-    std::swap(d.ridx, d.ridx2);
-  }
+                                            d.binst, d.idev, d.rI, d.rJ, d.rec, d.rkey, d.rkey2,
+                                            d.ridx2);
+  XSort x;
+  x.n = m;
+  x.G = d.B;
+  x.gstart = d.bbounds;  // records are colony-major (bodies are, and records are I-major)
+  x.key = d.rkey;
+  x.tie = d.rkey2;
+  x.grp = d.ridx2;
+  x.cnt = d.xs_cnt;
+  x.off = d.xs_off;
+  x.kmin = d.xs_kmin;
+  x.kmax = d.xs_kmax;
+  x.perm = d.ridx;
+  x.tmp = d.tmp;
+  x.tmp_bytes = d.tmp_bytes;
+  x.s = s;
+  xsort_run(x);
 }
 
 // Builds the hashed cell list of the current body state (stable counting sort).
@@ -785,8 +741,8 @@ void detect_batch(pcc_ctx* ctx) {
   const int64_t B = (int64_t)ctx->inst.size();
   KTimer tm(ctx, PCC_K_DETECT, 0.0);
   DetectBufs d = ctx_detect_bufs(ctx);
-  std::vector<int32_t> bounds(B + 2);
-  while (!detect_count(d, bounds.data())) {
+  int32_t* bounds = readback_pin(ctx, B + 2);
+  while (!detect_count(d, bounds)) {
     // a workgroup found more candidates than its region holds: larger
     // regions, and the batch again
     int32_t cb = ctx->cand_block;
@@ -803,6 +759,9 @@ void detect_batch(pcc_ctx* ctx) {
   ctx->ridx2.ensure(m + 1);
   ctx->rkey.ensure(m + 1);
   ctx->rkey2.ensure(m + 1);
+  ctx->xs_cnt.ensure(m + 1);
+  ctx->xs_off.ensure(m + 1);
+  ctx->xs_bounds.ensure(2 * B);
   ctx->sort_tmp.ensure((int64_t)detect_sort_bytes(m, B) + 16);
   for (int64_t q = 0; q < B; ++q) {
     ctx->inst[q].bstart = bounds[q];
@@ -816,10 +775,13 @@ void detect_batch(pcc_ctx* ctx) {
   d.ridx2 = ctx->ridx2.p;
   d.rkey = ctx->rkey.p;
   d.rkey2 = ctx->rkey2.p;
+  d.xs_cnt = ctx->xs_cnt.p;
+  d.xs_off = ctx->xs_off.p;
+  d.xs_kmin = ctx->xs_bounds.p;
+  d.xs_kmax = ctx->xs_bounds.p + B;
   d.tmp = ctx->sort_tmp.p;
   d.tmp_bytes = (size_t)ctx->sort_tmp.cap;
   detect_write(d, m);
-  if (d.ridx != ctx->ridx.p) ctx->ridx.swap(ctx->ridx2);
 }
 
 static void ensure_set_capacity(ConstraintSet& S, int64_t need, hipStream_t s) {

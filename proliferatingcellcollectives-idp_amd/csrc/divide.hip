@@ -13,7 +13,7 @@
 //                   x -/+ (l/2) t, each child turned in-plane by an angle drawn
 //                   from N(0, angle_noise) (Irwin-Hall, twelve uniforms), both
 //                   children one generation older than the parent
-//  4. radix sorts   stable by x, then stable by colony: gids by ascending x
+//  4. xsort_run     gids by ascending x per colony, ties in staging order
 //  5. k_div_gather  body fields into the new order
 // The draws are counter-based on (seed of the colony, step, index of the parent
 // within its colony), so a colony divides the same way alone and in a batch.
@@ -21,7 +21,6 @@
 #include <algorithm>
 #include <cstring>
 #include <vector>
-#include <rocprim/device/device_radix_sort.hpp>
 
 #include "common.hpp"
 #include "internal.hpp"
@@ -121,12 +120,13 @@ __global__ void k_div_bounds(int64_t B, int64_t N, const InstDev* __restrict__ i
 }
 
 // Staging: one thread per old body.  stage is field-major with stride M (new
-// total), key/val feed the x sort, sinst the colony of every staged body.
+// total), key (the order key of x) feeds the x sort, sinst the colony of every
+// staged body.
 __global__ void k_div_write(int64_t N, int64_t M, BodyArrays b, const int32_t* __restrict__ binst,
                             const DivColony* __restrict__ col, const int32_t* __restrict__ off,
                             int32_t step, double length_noise, double angle_noise,
                             double* __restrict__ stage, uint64_t* __restrict__ key,
-                            int32_t* __restrict__ val, int32_t* __restrict__ sinst) {
+                            int32_t* __restrict__ sinst) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const int32_t q = binst[i];
@@ -153,7 +153,6 @@ __global__ void k_div_write(int64_t N, int64_t M, BodyArrays b, const int32_t* _
     stage[11 * M + s] = ag;
     stage[12 * M + s] = im;
     key[s] = order_key(p[0]);
-    val[s] = (int32_t)s;
     sinst[s] = q;
   };
   if (!(L >= 2 * l0)) {
@@ -183,12 +182,6 @@ __global__ void k_div_write(int64_t N, int64_t M, BodyArrays b, const int32_t* _
   put(j, pl, ql, l1, age + 1.0, imp);
   const int64_t jr = c.noff + c.N + (off[i] - c.rank0);
   put(jr, pr, qr, l2, age + 1.0, 0.0);
-}
-
-__global__ void k_div_colony_keys(int64_t M, const int32_t* __restrict__ perm,
-                                  const int32_t* __restrict__ sinst, uint32_t* __restrict__ key) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < M) key[p] = (uint32_t)sinst[perm[p]];
 }
 
 __global__ void k_div_gather(int64_t M, const double* __restrict__ stage,
@@ -223,9 +216,10 @@ void divide_colonies(pcc_ctx* ctx, const uint64_t* seeds, int32_t step, double l
   k_div_flags<<<nblk(N), 256, 0, s>>>(N, b.len, b.l0, flag.p);
   exclusive_scan_i32(ctx, flag.p, off.p, N + 1);
   k_div_bounds<<<nblk(B + 1), 256, 0, s>>>(B, N, ctx->idev.p, off.p, bounds.p);
-  std::vector<int32_t> hb(B + 1);
-  PCC_HIP(hipMemcpyAsync(hb.data(), bounds.p, sizeof(int32_t) * (B + 1), hipMemcpyDeviceToHost, s));
+  int32_t* pin = readback_pin(ctx, B + 1);
+  PCC_HIP(hipMemcpyAsync(pin, bounds.p, sizeof(int32_t) * (B + 1), hipMemcpyDeviceToHost, s));
   PCC_HIP(hipStreamSynchronize(s));
+  const std::vector<int32_t> hb(pin, pin + B + 1);
   const int64_t D = hb[B] - hb[0];  // with no divider the cells are still renumbered by x
   const int64_t M = N + D;
   if (M >= (int64_t)1 << 30) throw StatusError{PCC_E_INVALID_ARGUMENT, "too many bodies"};
@@ -236,49 +230,52 @@ void divide_colonies(pcc_ctx* ctx, const uint64_t* seeds, int32_t step, double l
     hc[q] = {I.boff, I.N, noff, (int64_t)hb[q], seeds ? seeds[q] : 0};
     noff += I.N + (hb[q + 1] - hb[q]);
   }
-  ctx->dv_col.ensure((int64_t)sizeof(DivColony) * B);
+  // one upload: the colony records, then each colony's first new gid (B + 1
+  // entries, the groups of the x order)
+  const size_t col_bytes = (sizeof(DivColony) * B + 15) / 16 * 16;
+  std::vector<uint8_t> blob(col_bytes + sizeof(int32_t) * (B + 1));
+  memcpy(blob.data(), hc.data(), sizeof(DivColony) * B);
+  {
+    int32_t* gs = reinterpret_cast<int32_t*>(blob.data() + col_bytes);
+    for (int64_t q = 0; q < B; ++q) gs[q] = (int32_t)hc[q].noff;
+    gs[B] = (int32_t)M;
+  }
+  ctx->dv_col.ensure((int64_t)blob.size());
   DivColony* dcol = reinterpret_cast<DivColony*>(ctx->dv_col.p);
-  PCC_HIP(hipMemcpyAsync(dcol, hc.data(), sizeof(DivColony) * B, hipMemcpyHostToDevice, s));
+  const int32_t* gstart = reinterpret_cast<const int32_t*>(ctx->dv_col.p + col_bytes);
+  PCC_HIP(hipMemcpyAsync(ctx->dv_col.p, blob.data(), blob.size(), hipMemcpyHostToDevice, s));
   DArr<double>& stage = ctx->dv_stage;
   DArr<uint64_t>& key = ctx->dv_key;
-  DArr<uint64_t>& key2 = ctx->dv_key2;
-  DArr<int32_t>& val = ctx->dv_val;
   DArr<int32_t>& perm = ctx->dv_perm;
-  DArr<int32_t>& perm2 = ctx->dv_perm2;
   DArr<int32_t>& sinst = ctx->dv_sinst;
-  DArr<uint32_t>& ckey = ctx->dv_ckey;
-  DArr<uint32_t>& ckey2 = ctx->dv_ckey2;
   stage.ensure(kBodyFields * M + 1);
   key.ensure(M + 1);
-  key2.ensure(M + 1);
-  val.ensure(M + 1);
   perm.ensure(M + 1);
-  perm2.ensure(M + 1);
   sinst.ensure(M + 1);
-  ckey.ensure(M + 1);
-  ckey2.ensure(M + 1);
   k_div_write<<<nblk(N), 256, 0, s>>>(N, M, b, ctx->binst.p, dcol, off.p, step, length_noise,
-                                      angle_noise, stage.p, key.p, val.p, sinst.p);
-  // gids by ascending x within each colony: stable sort by x, then by colony
-  size_t bytes = 0;
-  PCC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, key.p, key2.p, val.p, perm.p, (size_t)M, 0, 64,
-                                    s));
-  ctx->sort_tmp.ensure((int64_t)bytes + 16);
-  PCC_HIP(rocprim::radix_sort_pairs(ctx->sort_tmp.p, bytes, key.p, key2.p, val.p, perm.p,
-                                    (size_t)M, 0, 64, s));
+                                      angle_noise, stage.p, key.p, sinst.p);
+  // gids by ascending x within each colony, ties in staging order
+  ctx->xs_cnt.ensure(M + 1);
+  ctx->xs_off.ensure(M + 1);
+  ctx->xs_bounds.ensure(2 * B);
+  ctx->sort_tmp.ensure((int64_t)xsort_tmp_bytes(M) + 16);
+  XSort xs;
+  xs.n = M;
+  xs.G = B;
+  xs.gstart = gstart;
+  xs.key = key.p;
+  xs.tie = nullptr;  // the staging index
+  xs.grp = sinst.p;
+  xs.cnt = ctx->xs_cnt.p;
+  xs.off = ctx->xs_off.p;
+  xs.kmin = ctx->xs_bounds.p;
+  xs.kmax = ctx->xs_bounds.p + B;
+  xs.perm = perm.p;
+  xs.tmp = ctx->sort_tmp.p;
+  xs.tmp_bytes = (size_t)ctx->sort_tmp.cap;
+  xs.s = s;
+  xsort_run(xs);
   const int32_t* order = perm.p;
-  if (B > 1) {
-    k_div_colony_keys<<<nblk(M), 256, 0, s>>>(M, perm.p, sinst.p, ckey.p);
-    int bits = 1;
-    while (((int64_t)1 << bits) < B) ++bits;
-    bytes = 0;
-    PCC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, ckey.p, ckey2.p, perm.p, perm2.p, (size_t)M,
-                                      0, bits, s));
-    ctx->sort_tmp.ensure((int64_t)bytes + 16);
-    PCC_HIP(rocprim::radix_sort_pairs(ctx->sort_tmp.p, bytes, ckey.p, ckey2.p, perm.p, perm2.p,
-                                      (size_t)M, 0, bits, s));
-    order = perm2.p;
-  }
   DArr<double>& body = ctx->dv_body;
   body.ensure(kBodyFields * M + 1);
   k_div_gather<<<nblk(M), 256, 0, s>>>(M, stage.p, order, body.p);

@@ -271,10 +271,9 @@ struct pcc_ctx {
   pcc::DArr<int32_t> dbcount;  // per filter workgroup: candidates in its region
   int32_t cand_block = 0;      // candidate region per filter workgroup (grows on overflow)
   // division scratch (kept between calls)
-  pcc::DArr<int32_t> dv_flag, dv_off, dv_bounds, dv_val, dv_perm, dv_perm2, dv_sinst;
+  pcc::DArr<int32_t> dv_flag, dv_off, dv_bounds, dv_perm, dv_sinst;
   pcc::DArr<double> dv_stage, dv_body;
-  pcc::DArr<uint64_t> dv_key, dv_key2;
-  pcc::DArr<uint32_t> dv_ckey, dv_ckey2;
+  pcc::DArr<uint64_t> dv_key;
   pcc::DArr<uint8_t> dv_col;
   pcc::DArr<double> dgeo;    // per body: x, y, z, t (3), length, diameter (detection)
   pcc::DArr<int32_t> scell;  // cell records in bucket order
@@ -284,6 +283,8 @@ struct pcc_ctx {
   pcc::DArr<double> rrec;      // raw geometry: 16 doubles per contact
   pcc::DArr<int32_t> ridx, ridx2;
   pcc::DArr<uint64_t> rkey, rkey2;
+  pcc::DArr<int32_t> xs_cnt, xs_off;          // bucket counts and offsets of the x orders
+  pcc::DArr<unsigned long long> xs_bounds;    // 2 per colony: smallest and largest x key
   pcc::DArr<uint8_t> sort_tmp;
   int64_t batch_count = 0;
 
@@ -338,6 +339,10 @@ struct pcc_ctx {
   // pinned copy of the per-colony max overlap (max_overlap_launch)
   unsigned long long* ov_pin = nullptr;
   int64_t ov_pin_cap = 0;
+  // pinned target of the small synchronous read-backs (detection and
+  // division counts): a copy into pageable memory takes a staging detour
+  int32_t* rb_pin = nullptr;
+  int64_t rb_pin_cap = 0;
   // planarity of the body state as of the end of the last step (cleared by
   // every host write of body state; kept by device division)
   bool flat_known = false;
@@ -389,6 +394,8 @@ struct DetectBufs {
   double* rec = nullptr;             // 16 m
   int32_t *ridx = nullptr, *ridx2 = nullptr;                       // m
   uint64_t *rkey = nullptr, *rkey2 = nullptr;                      // m
+  int32_t *xs_cnt = nullptr, *xs_off = nullptr;                    // m+1 each (x order)
+  unsigned long long *xs_kmin = nullptr, *xs_kmax = nullptr;       // B each
   void* tmp = nullptr;               // rocPRIM temporary storage
   size_t tmp_bytes = 0;
   hipStream_t s = nullptr;
@@ -400,6 +407,25 @@ bool detect_count(DetectBufs& d, int32_t* bounds_host);
 int64_t filter_grid(int64_t N, int S);  // workgroups of the candidate filter
 void detect_write(DetectBufs& d, int64_t m);
 void detect_records_out(const DetectBufs& d, int64_t m, const pcc_constraints& out_dev);
+
+// Order of n items by (group, x, tie) (xsort.hip).  The items of group g are
+// the contiguous range [gstart[g], gstart[g + 1]) of the item order (groups
+// ascend with the item index) and keep that range of output positions.
+struct XSort {
+  int64_t n = 0, G = 0;
+  const int32_t* gstart = nullptr;   // G + 1 entries, on the device
+  const uint64_t* key = nullptr;     // n: order_key of each item's x
+  const uint64_t* tie = nullptr;     // n: tie-break, or null for the item index
+  int32_t* grp = nullptr;            // n: each item's group (overwritten)
+  int32_t *cnt = nullptr, *off = nullptr;            // n + 1 each
+  unsigned long long *kmin = nullptr, *kmax = nullptr;  // G each
+  int32_t* perm = nullptr;           // n: the items in order (out)
+  void* tmp = nullptr;               // rocPRIM scan storage
+  size_t tmp_bytes = 0;
+  hipStream_t s = nullptr;
+};
+size_t xsort_tmp_bytes(int64_t n);
+void xsort_run(const XSort& a);
 
 // launchers (kernels in detect.hip / solve.hip / soft.hip)
 void detect_batch(pcc_ctx* ctx);  // active colonies at their cutoffs; fills ctx->ridx sorted
@@ -419,6 +445,7 @@ void divide_colonies(pcc_ctx* ctx, const uint64_t* seeds, int32_t step, double l
 void colony_median_speeds(pcc_ctx* ctx, std::vector<double>& median);
 double adapted_dt(double dt, double median, double cfl_factor, double tolerance, double alpha);
 void upload_instances(pcc_ctx* ctx);
+int32_t* readback_pin(pcc_ctx* ctx, int64_t n);  // pinned int32[n]; no copy may be pending
 void max_overlap_inst(pcc_ctx* ctx, std::vector<double>& out);
 void max_overlap_launch(pcc_ctx* ctx);  // reduction and copy to pinned memory, no wait
 void max_overlap_read(pcc_ctx* ctx, std::vector<double>& out);  // after a stream sync

@@ -47,7 +47,8 @@ constexpr int64_t pad(int64_t b) { return (b + 255) / 256 * 256; }
 // Byte layout of the scratch array.
 struct Layout {
   int64_t body, binst, idev, flag, cell4, geo, scell, sgeo, frec, bucket, perm, bstart, bcursor, dslot, dslot_off,
-      cstart, cand, accept, bcount, dcount, bbounds, rI, rJ, rec, ridx, ridx2, rkey, rkey2, tmp, total;
+      cstart, cand, accept, bcount, dcount, bbounds, rI, rJ, rec, ridx, ridx2, rkey, rkey2, xs_cnt,
+      xs_off, xs_bounds, tmp, total;
 };
 constexpr Layout make_layout() {
   Layout L{};
@@ -85,6 +86,9 @@ constexpr Layout make_layout() {
   L.ridx2 = take(4 * (kM + 1));
   L.rkey = take(8 * (kM + 1));
   L.rkey2 = take(8 * (kM + 1));
+  L.xs_cnt = take(4 * (kM + 1));
+  L.xs_off = take(4 * (kM + 1));
+  L.xs_bounds = take(8 * 2);
   L.tmp = take(kTmp);
   L.total = o;
   return L;
@@ -243,6 +247,10 @@ pcc_status detect_noalloc(int device, hipStream_t s, const pcc_params& p, double
   d.ridx2 = static_cast<int32_t*>(at(kL.ridx2));
   d.rkey = static_cast<uint64_t*>(at(kL.rkey));
   d.rkey2 = static_cast<uint64_t*>(at(kL.rkey2));
+  d.xs_cnt = static_cast<int32_t*>(at(kL.xs_cnt));
+  d.xs_off = static_cast<int32_t*>(at(kL.xs_off));
+  d.xs_kmin = static_cast<unsigned long long*>(at(kL.xs_bounds));
+  d.xs_kmax = d.xs_kmin + 1;
   d.tmp = at(kL.tmp);
   d.tmp_bytes = (size_t)kTmp;
   d.s = s;

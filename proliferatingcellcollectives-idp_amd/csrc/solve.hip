@@ -827,7 +827,7 @@ struct TileIn {
 // while tile j's body-velocity gathers are in flight.
 template <class CT, bool PLANAR>
 __global__ void __launch_bounds__(256) k_constr(ConstrArgs a) {
-  __shared__ double lds[4];
+  __shared__ double lds[20];
   const int64_t l = xcd_block(blockIdx.x, a.gp);
   if (l >= a.ncb) return;
   const int32_t inst = a.cmap[l];
@@ -914,19 +914,27 @@ __global__ void __launch_bounds__(256) k_constr(ConstrArgs a) {
     ok = okn;
     if (okn) cur = nxt;
   }
-  const double s0 = block_sum(v0, lds);
-  const double s1 = block_sum(v1, lds);
-  const double s2 = block_sum(v2, lds);
-  const double rm = block_max(vr, lds);
-  if (threadIdx.x == 0) {
-    a.partials[l] = s0;
-    a.partials[a.Gtot + l] = s1;
-    a.partials[2 * a.Gtot + l] = s2;
-    a.partials[3 * a.Gtot + l] = rm;
+  // block_sum x 3 (4 while tracing) and block_max with one LDS exchange: the
+  // same wave butterflies and the same ((w0 + w1) + w2) + w3 combination
+  const double w0 = wave_sum(v0), w1 = wave_sum(v1), w2 = wave_sum(v2), wm = wave_max(vr);
+  const double we = a.trace ? wave_sum(ve) : 0.0;
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & (PCC_WAVE - 1)) == 0) {
+    lds[wv] = w0;
+    lds[4 + wv] = w1;
+    lds[8 + wv] = w2;
+    lds[12 + wv] = wm;
+    lds[16 + wv] = we;
   }
-  if (a.trace) {
-    const double se = block_sum(ve, lds);
-    if (threadIdx.x == 0) a.partials[4 * a.Gtot + l] = se;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double rm = lds[12];
+    for (int i = 1; i < 4; ++i) rm = rm < lds[12 + i] ? lds[12 + i] : rm;
+    a.partials[l] = ((lds[0] + lds[1]) + lds[2]) + lds[3];
+    a.partials[a.Gtot + l] = ((lds[4] + lds[5]) + lds[6]) + lds[7];
+    a.partials[2 * a.Gtot + l] = ((lds[8] + lds[9]) + lds[10]) + lds[11];
+    a.partials[3 * a.Gtot + l] = rm;
+    if (a.trace) a.partials[4 * a.Gtot + l] = ((lds[16] + lds[17]) + lds[18]) + lds[19];
   }
 }
 

@@ -45,6 +45,29 @@ def test_divide_matches_oracle(length_noise, angle_noise):
     pm.close()
 
 
+def test_divide_ties_keep_staging_order():
+    """A column of vertical rods at one x, all at their division length: both
+    children of every cell keep the parent's x, so the new gids of the whole
+    column rest on the tie-break by staging order."""
+    from pcc.engine import ParticleManager
+    n = 200
+    pos = np.zeros((n, 3))
+    pos[:, 0] = 1.25
+    pos[:, 1] = 2.1 * np.arange(n)
+    h = math.sqrt(0.5)
+    q = np.tile([h, 0.0, 0.0, h], (n, 1))  # turned by pi/2: along y
+    b = G.Bodies(pos, q, np.full(n, 2.0))
+    b.age = (1 + np.arange(n) % 3).astype(np.int32)
+    pm = ParticleManager(reference_params())
+    pm.set_state(b)
+    got_n = pm.divide(seed=3, step=1, length_noise=0.0, angle_noise=0.0)
+    want = oracle.divide(b, 3, 1, 0.0, 0.0)
+    assert got_n == want.n == 2 * n
+    assert np.unique(want.pos[:, 0]).size < 4  # ties everywhere
+    same_bodies(pm.get_state(), want)
+    pm.close()
+
+
 def test_batch_divide_equals_alone():
     from pcc.engine import ColonyBatch
     cols = [with_dividers(G.dense_colony(n, seed=150 + i), every=2 + i)

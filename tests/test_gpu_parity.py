@@ -38,8 +38,24 @@ def random_3d(n, seed):
     return G.Bodies(pos[o], q[o], rng.uniform(1, 2, n)[o])
 
 
+def equal_x_lattice(cols=30, rows=40, seed=5):
+    """Horizontal rods of length 2 stacked in columns (spacing 0.48 < d): the
+    contacts of a column share one contact x, so the contact order rests on
+    its (I, J) tie-break; gids are shuffled so that I order is not x order."""
+    rng = np.random.default_rng(seed)
+    x, y = np.meshgrid(3.0 * np.arange(cols), 0.48 * np.arange(rows), indexing="ij")
+    n = cols * rows
+    pos = np.zeros((n, 3))
+    pos[:, 0] = x.ravel()
+    pos[:, 1] = y.ravel()
+    o = rng.permutation(n)
+    q = np.tile([1.0, 0.0, 0.0, 0.0], (n, 1))
+    return G.Bodies(pos[o], q, np.full(n, 2.0))
+
+
 DETECT_CASES = [
     ("disc2000_c0.05", lambda: G.random_disc(2000, seed=0), 0.05),
+    ("equal_x_lattice", lambda: equal_x_lattice(), 0.05),
     ("disc2000_c0.25", lambda: G.random_disc(2000, seed=1), 0.25),
     ("disc1500_c0", lambda: G.random_disc(1500, seed=2), 0.0),
     ("colony5000", lambda: G.packed_colony(5000, seed=3), 0.05),
