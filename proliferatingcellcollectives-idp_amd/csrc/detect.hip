@@ -397,19 +397,21 @@ k_pair_write(int S, int64_t gp, int32_t cb, const int32_t* __restrict__ bcount,
 }
 
 // Per-body record offsets from the per-(body, neighbour cell) offsets.
-__global__ void k_body_offsets(int64_t N, int S, const int32_t* __restrict__ slot_off,
-                               int32_t* __restrict__ off) {
+// Per-body record offsets from the per-(body, neighbour cell) offsets, and
+// the batch offsets at the colony boundaries: colony b's records occupy
+// [off[boff_b], off[boff_b + N_b]) of the batch.  Also clears the per-colony
+// x-key bounds of the x sort that follows the writing pass.
+__global__ void k_body_offsets(int64_t N, int S, int64_t B, const InstDev* __restrict__ idev,
+                               const int32_t* __restrict__ slot_off, int32_t* __restrict__ off,
+                               int32_t* __restrict__ bounds, unsigned long long* __restrict__ kmin,
+                               unsigned long long* __restrict__ kmax) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i <= N) off[i] = slot_off[i * S];
-}
-
-// Batch offsets at the colony boundaries: colony b's records occupy
-// [off[boff_b], off[boff_b + N_b]) of the sorted batch.
-__global__ void k_batch_bounds(int64_t B, int64_t N, const InstDev* __restrict__ idev,
-                               const int32_t* __restrict__ off, int32_t* __restrict__ out) {
-  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b > B) return;
-  out[b] = off[b < B ? idev[b].boff : N];
+  if (i <= B) bounds[i] = slot_off[(i < B ? idev[i].boff : N) * S];
+  if (i < B) {
+    kmin[i] = ~0ull;
+    kmax[i] = 0ull;
+  }
 }
 
 // Appends the sorted batch to the constraint set at offset C0 and builds the
@@ -597,7 +599,7 @@ bool detect_count(DetectBufs& d, int32_t* bounds_host) {
   const int64_t N = d.N, T = d.T, NS = N * d.S;
   const hipStream_t s = d.s;
   const uint32_t mask = (uint32_t)(T - 1);
-  k_fill_i32<<<nblk(T + 1), 256, 0, s>>>(T + 1, 0, d.bcursor);
+  if (!d.cursor_zero) k_fill_i32<<<nblk(T + 1), 256, 0, s>>>(T + 1, 0, d.bcursor);
   if (N > 0)
     k_cells<<<nblk(N), 256, 0, s>>>(N, d.body, d.binst, d.idev, mask, d.S == 9, d.cell4, d.geo,
                                     d.bucket,
@@ -632,8 +634,8 @@ bool detect_count(DetectBufs& d, int32_t* bounds_host) {
   // stable sort by colony keeps them); bounds[B] is the batch's record count
   // and bounds[B + 1] the candidate count, so one read-back gives all
   scan_on(d, d.dslot, d.dslot_off, NS + 1);
-  k_body_offsets<<<nblk(N + 1), 256, 0, s>>>(N, d.S, d.dslot_off, d.dcount);
-  k_batch_bounds<<<nblk(d.B + 1), 256, 0, s>>>(d.B, N, d.idev, d.dcount, d.bbounds);
+  k_body_offsets<<<nblk(std::max<int64_t>(N, d.B) + 1), 256, 0, s>>>(
+      N, d.S, d.B, d.idev, d.dslot_off, d.dcount, d.bbounds, d.xs_kmin, d.xs_kmax);
   PCC_HIP(hipMemcpyAsync(bounds_host, d.bbounds, sizeof(int32_t) * (d.B + 2),
                          hipMemcpyDeviceToHost, s));
   PCC_HIP(hipStreamSynchronize(s));
@@ -665,6 +667,7 @@ void detect_write(DetectBufs& d, int64_t m) {
   x.off = d.xs_off;
   x.kmin = d.xs_kmin;
   x.kmax = d.xs_kmax;
+  x.bounds_clear = true;  // by k_body_offsets
   x.perm = d.ridx;
   x.tmp = d.tmp;
   x.tmp_bytes = d.tmp_bytes;
@@ -701,6 +704,7 @@ static DetectBufs ctx_detect_bufs(pcc_ctx* ctx) {
   ctx->dbcount.ensure(gp + 1);
   ctx->dcount.ensure(N + 1);
   ctx->bbounds.ensure(B + 2);
+  ctx->xs_bounds.ensure(2 * B);
   ctx->sort_tmp.ensure((int64_t)detect_scan_bytes(N, S, T) + 16);
   DetectBufs d;
   d.N = N;
@@ -728,6 +732,9 @@ static DetectBufs ctx_detect_bufs(pcc_ctx* ctx) {
   d.cand_block = ctx->cand_block;
   d.dcount = ctx->dcount.p;
   d.bbounds = ctx->bbounds.p;
+  d.xs_kmin = ctx->xs_bounds.p;
+  d.xs_kmax = ctx->xs_bounds.p + B;
+  d.cursor_zero = ctx->bcursor_zero.known(ctx->bcursor.p, T + 1);
   d.tmp = ctx->sort_tmp.p;
   d.tmp_bytes = (size_t)ctx->sort_tmp.cap;
   d.s = ctx->stream;
@@ -742,6 +749,7 @@ void detect_batch(pcc_ctx* ctx) {
   KTimer tm(ctx, PCC_K_DETECT, 0.0);
   DetectBufs d = ctx_detect_bufs(ctx);
   int32_t* bounds = readback_pin(ctx, B + 2);
+  ctx->bcursor_zero.clear();
   while (!detect_count(d, bounds)) {
     // a workgroup found more candidates than its region holds: larger
     // regions, and the batch again
@@ -750,6 +758,8 @@ void detect_batch(pcc_ctx* ctx) {
     ctx->cand_block = cb;
     d = ctx_detect_bufs(ctx);
   }
+  // k_scatter consumed the bucket counts back to zero
+  ctx->bcursor_zero.mark(ctx->bcursor.p, d.T + 1);
   const int64_t m = bounds[B];
   ctx->batch_count = m;
   ctx->rI_.ensure(m + 1);
@@ -761,7 +771,6 @@ void detect_batch(pcc_ctx* ctx) {
   ctx->rkey2.ensure(m + 1);
   ctx->xs_cnt.ensure(m + 1);
   ctx->xs_off.ensure(m + 1);
-  ctx->xs_bounds.ensure(2 * B);
   ctx->sort_tmp.ensure((int64_t)detect_sort_bytes(m, B) + 16);
   for (int64_t q = 0; q < B; ++q) {
     ctx->inst[q].bstart = bounds[q];
@@ -777,8 +786,6 @@ void detect_batch(pcc_ctx* ctx) {
   d.rkey2 = ctx->rkey2.p;
   d.xs_cnt = ctx->xs_cnt.p;
   d.xs_off = ctx->xs_off.p;
-  d.xs_kmin = ctx->xs_bounds.p;
-  d.xs_kmax = ctx->xs_bounds.p + B;
   d.tmp = ctx->sort_tmp.p;
   d.tmp_bytes = (size_t)ctx->sort_tmp.cap;
   detect_write(d, m);

@@ -98,11 +98,12 @@ __device__ __forceinline__ void turn_z(double q[4], double d) {
   for (int k = 0; k < 4; ++k) q[k] = r[k] / norm;
 }
 
+// flag[N] (the scan's closing entry) is 0.
 __global__ void k_div_flags(int64_t N, const double* __restrict__ len,
                             const double* __restrict__ l0, int32_t* __restrict__ flag) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  flag[i] = len[i] >= 2 * l0[i] ? 1 : 0;
+  if (i > N) return;
+  flag[i] = i < N && len[i] >= 2 * l0[i] ? 1 : 0;
 }
 
 struct DivColony {
@@ -112,11 +113,19 @@ struct DivColony {
   uint64_t seed;
 };
 
+// Dividing cells before each colony; also clears the per-colony x-key
+// bounds of the x sort.
 __global__ void k_div_bounds(int64_t B, int64_t N, const InstDev* __restrict__ idev,
-                             const int32_t* __restrict__ off, int32_t* __restrict__ out) {
+                             const int32_t* __restrict__ off, int32_t* __restrict__ out,
+                             unsigned long long* __restrict__ kmin,
+                             unsigned long long* __restrict__ kmax) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b > B) return;
   out[b] = off[b < B ? idev[b].boff : N];
+  if (b < B) {
+    kmin[b] = ~0ull;
+    kmax[b] = 0ull;
+  }
 }
 
 // Staging: one thread per old body.  stage is field-major with stride M (new
@@ -212,10 +221,11 @@ void divide_colonies(pcc_ctx* ctx, const uint64_t* seeds, int32_t step, double l
   flag.ensure(N + 1);
   off.ensure(N + 1);
   bounds.ensure(B + 1);
-  PCC_HIP(hipMemsetAsync(flag.p, 0, sizeof(int32_t) * (N + 1), s));
-  k_div_flags<<<nblk(N), 256, 0, s>>>(N, b.len, b.l0, flag.p);
+  k_div_flags<<<nblk(N + 1), 256, 0, s>>>(N, b.len, b.l0, flag.p);
   exclusive_scan_i32(ctx, flag.p, off.p, N + 1);
-  k_div_bounds<<<nblk(B + 1), 256, 0, s>>>(B, N, ctx->idev.p, off.p, bounds.p);
+  ctx->xs_bounds.ensure(2 * B);
+  k_div_bounds<<<nblk(B + 1), 256, 0, s>>>(B, N, ctx->idev.p, off.p, bounds.p,
+                                           ctx->xs_bounds.p, ctx->xs_bounds.p + B);
   int32_t* pin = readback_pin(ctx, B + 1);
   PCC_HIP(hipMemcpyAsync(pin, bounds.p, sizeof(int32_t) * (B + 1), hipMemcpyDeviceToHost, s));
   PCC_HIP(hipStreamSynchronize(s));
@@ -257,7 +267,6 @@ void divide_colonies(pcc_ctx* ctx, const uint64_t* seeds, int32_t step, double l
   // gids by ascending x within each colony, ties in staging order
   ctx->xs_cnt.ensure(M + 1);
   ctx->xs_off.ensure(M + 1);
-  ctx->xs_bounds.ensure(2 * B);
   ctx->sort_tmp.ensure((int64_t)xsort_tmp_bytes(M) + 16);
   XSort xs;
   xs.n = M;
@@ -270,6 +279,7 @@ void divide_colonies(pcc_ctx* ctx, const uint64_t* seeds, int32_t step, double l
   xs.off = ctx->xs_off.p;
   xs.kmin = ctx->xs_bounds.p;
   xs.kmax = ctx->xs_bounds.p + B;
+  xs.bounds_clear = true;  // by k_div_bounds
   xs.perm = perm.p;
   xs.tmp = ctx->sort_tmp.p;
   xs.tmp_bytes = (size_t)ctx->sort_tmp.cap;

@@ -58,6 +58,25 @@ struct StreamScope {
 void* dev_alloc(size_t bytes, bool* pooled);
 void dev_free(void* p, bool pooled);
 
+// Leading entries of a count array known to be zero.  The scatters that
+// consume counts (atomicSub down to zero) leave their arrays clear, so a fill
+// is needed only for entries never cleared since the array was allocated.
+// Keyed by the array's address: a grown array is a new allocation made while
+// the old one is still held, so it never matches the address last marked.
+struct ZeroTrack {
+  const void* p = nullptr;
+  int64_t n = 0;
+  bool known(const void* q, int64_t k) const { return q == p && k <= n; }
+  void mark(const void* q, int64_t k) {
+    if (q != p) {
+      p = q;
+      n = 0;
+    }
+    n = k > n ? k : n;
+  }
+  void clear() { p = nullptr; n = 0; }
+};
+
 // Growable device array (never shrinks; contents are not preserved on growth
 // unless grow_keep is used).  Capacity doubles, so regrowth is rare.
 template <class T>
@@ -260,6 +279,7 @@ struct pcc_ctx {
   pcc::DArr<int32_t> perm;     // N
   pcc::DArr<int32_t> bstart;   // T+1
   pcc::DArr<int32_t> bcursor;  // T
+  pcc::ZeroTrack bcursor_zero;  // bucket counts left at zero by the last detection
   int64_t table_size = 0;
 
   // detection batch
@@ -291,6 +311,7 @@ struct pcc_ctx {
   // constraint set and phase operators
   pcc::ConstraintSet S;
   pcc::DArr<int32_t> rowptr, inccur, inc;  // N+1, N, 2C
+  pcc::ZeroTrack inccur_zero;              // incidence counts left at zero by the last build
   pcc::DArr<double> increc;                // per incidence: 8 (3D) or 4 (planar) doubles
   pcc::DArr<float> increc32;               // per incidence: 8 or 4 floats
   pcc::DArr<int32_t> incidx;               // planar: solver position per incidence
@@ -396,6 +417,7 @@ struct DetectBufs {
   uint64_t *rkey = nullptr, *rkey2 = nullptr;                      // m
   int32_t *xs_cnt = nullptr, *xs_off = nullptr;                    // m+1 each (x order)
   unsigned long long *xs_kmin = nullptr, *xs_kmax = nullptr;       // B each
+  bool cursor_zero = false;          // bcursor[0..T] known to be zero (no fill)
   void* tmp = nullptr;               // rocPRIM temporary storage
   size_t tmp_bytes = 0;
   hipStream_t s = nullptr;
@@ -419,6 +441,7 @@ struct XSort {
   int32_t* grp = nullptr;            // n: each item's group (overwritten)
   int32_t *cnt = nullptr, *off = nullptr;            // n + 1 each
   unsigned long long *kmin = nullptr, *kmax = nullptr;  // G each
+  bool bounds_clear = false;         // kmin / kmax already hold ~0 / 0 (else cleared here)
   int32_t* perm = nullptr;           // n: the items in order (out)
   void* tmp = nullptr;               // rocPRIM scan storage
   size_t tmp_bytes = 0;

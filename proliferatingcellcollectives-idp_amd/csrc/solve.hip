@@ -600,6 +600,34 @@ __global__ void __launch_bounds__(256) k_body_final(BodyArgs a, FinalOut fo) {
   fo.ld[i] = ld;
 }
 
+// Element idx of an array of T at `base`, addressed by a 32-bit byte offset.
+template <class T>
+__device__ __forceinline__ T* at(const void* base, uint32_t idx) {
+  return reinterpret_cast<T*>(const_cast<char*>(static_cast<const char*>(base)) +
+                              (size_t)(idx * (uint32_t)sizeof(T)));
+}
+
+// Iterate record as stored (FP64 pair, or the packed 12-byte record),
+// converted where it is used so that a prefetched tile holds fewer registers.
+template <class CT>
+struct ItRaw {
+  double2 v;
+  __device__ __forceinline__ void load(const double* __restrict__ it, int64_t c) {
+    v = reinterpret_cast<const double2*>(it)[c];
+  }
+  __device__ __forceinline__ double2 value() const { return v; }
+};
+template <>
+struct ItRaw<float> {
+  Rec12 v;
+  __device__ __forceinline__ void load(const double* __restrict__ it, int64_t c) {
+    v = reinterpret_cast<const Rec12*>(it)[c];
+  }
+  __device__ __forceinline__ double2 value() const {
+    return make_double2(__hiloint2double((int)v.hi, (int)v.lo), (double)v.g);
+  }
+};
+
 // Body apply of one BBPGD iteration, block-cooperative: a block owns 256
 // consecutive bodies and walks their incidence range in chunks.  Phase 1: every
 // lane takes incidences of the chunk in turn, gathers the constraint's
@@ -772,7 +800,7 @@ struct ConstrArgs {
 // registers: planar (nx, ny, tauI_z, tauJ_z, sI, sJ); 3D (n, rI x n,
 // rJ x -n, sI, sJ).
 template <class CT, bool PLANAR>
-struct Row {
+struct alignas(PLANAR && sizeof(CT) == 4 ? 8 : 16) Row {
   CT v[PLANAR ? 6 : 12];
 };
 
@@ -810,12 +838,12 @@ __device__ __forceinline__ Row<CT, PLANAR> load_row(const void* base, int64_t c)
   return r;
 }
 
-// Inputs of one constraint of a tile.
+// Inputs of one constraint of a tile, as stored.
 template <class CT, bool PLANAR>
 struct TileIn {
   int32_t I, J;
-  double2 yg;  // (gamma_p, g_p)
-  double phi;
+  ItRaw<CT> yg;  // (gamma_p, g_p)
+  CT phi;
   Row<CT, PLANAR> row;
 };
 
@@ -841,32 +869,31 @@ __global__ void __launch_bounds__(256) k_constr(ConstrArgs a) {
   const double malpha = -st->alpha;
   const bool save_prev = k > 0 && k % D.check == 0;
   constexpr int UW = PLANAR ? 4 : 8;  // doubles per body velocity record
-  auto fetch = [&](int64_t c, TileIn<CT, PLANAR>& t) {
-    t.I = a.I[c];
-    t.J = a.J[c];
-    t.yg = load_it<CT>(a.it, a.itg, c);
-    t.phi = load_phi<CT>(a.phi, c);
-    t.row = load_row<CT, PLANAR>(a.coef, c);
+  // Element addresses are the arrays' (uniform) bases plus 32-bit byte
+  // offsets (the host keeps every array of the set under 4 GiB), so the loads
+  // take the scalar-base form and the loop carries one index instead of a
+  // 64-bit pointer per array.
+  auto fetch = [&](uint32_t c, TileIn<CT, PLANAR>& t) {
+    t.I = *at<int32_t>(a.I, c);
+    t.J = *at<int32_t>(a.J, c);
+    t.yg.v = *at<decltype(t.yg.v)>(a.it, c);
+    t.phi = *at<CT>(a.phi, c);
+    t.row = *at<Row<CT, PLANAR>>(a.coef, c);
   };
   double v0 = 0.0, v1 = 0.0, v2 = 0.0, vr = 0.0, ve = 0.0;
-  int64_t c = D.coff + b * K * PCC_BLOCK + threadIdx.x;
-  bool ok = K > 0 && c < cend;
-  TileIn<CT, PLANAR> cur;
-  if (ok) fetch(c, cur);
-  for (int64_t j = 0; j < K && ok; ++j) {
-    const double2* UI = reinterpret_cast<const double2*>(a.ubuf + UW * (int64_t)cur.I);
-    const double2* UJ = reinterpret_cast<const double2*>(a.ubuf + UW * (int64_t)cur.J);
-    double2 ui[UW / 2], uj[UW / 2];
+  auto gatherU = [&](const TileIn<CT, PLANAR>& t, double2* ui, double2* uj) {
+    const double2* UI = at<double2>(a.ubuf, (uint32_t)t.I * (UW / 2));
+    const double2* UJ = at<double2>(a.ubuf, (uint32_t)t.J * (UW / 2));
 #pragma unroll
     for (int q = 0; q < UW / 2; ++q) {
       ui[q] = UI[q];
       uj[q] = UJ[q];
     }
-    const int64_t cn = c + PCC_BLOCK;
-    const bool okn = j + 1 < K && cn < cend;
-    TileIn<CT, PLANAR> nxt;
-    if (okn) fetch(cn, nxt);
-    const double2 yg = cur.yg;
+  };
+  auto apply = [&](const TileIn<CT, PLANAR>& cur, const double2* ui, const double2* uj,
+                   uint32_t c) {
+    const double2 yg = cur.yg.value();
+    const double phi = (double)cur.phi;
     const double gc = (k == 0) ? yg.x : proj_step(yg.x, malpha, yg.y);
     double pd = 0.0;  // D U_total, columns of body I (lower gid) first
     double pg = 0.0;  // L ldot_diff
@@ -896,9 +923,17 @@ __global__ void __launch_bounds__(256) k_constr(ConstrArgs a) {
       pg += (double)co[9] * ui[3].x;
       pg += (double)co[10] * uj[3].x;
     }
-    const double g = stored_g<CT>(cur.phi + (dt * pd + (-dt) * pg));
-    if (save_prev) a.gprev[c] = yg.x;
-    store_it<CT>(a.it, a.itg, c, gc, g);
+    const double g = stored_g<CT>(phi + (dt * pd + (-dt) * pg));
+    if constexpr (sizeof(CT) == 8) {
+      *at<double2>(a.it, c) = make_double2(gc, g);
+    } else {
+      Rec12 r;
+      r.lo = (uint32_t)__double2loint(gc);
+      r.hi = (uint32_t)__double2hiint(gc);
+      r.g = (float)g;
+      *at<Rec12>(a.it, c) = r;
+    }
+    if (save_prev) *at<double>(a.gprev, c) = yg.x;
     if (k > 0) {
       const double dgam = gc - yg.x;
       const double dphi = g - yg.y;
@@ -909,7 +944,25 @@ __global__ void __launch_bounds__(256) k_constr(ConstrArgs a) {
     const double pj = gc > 0 ? g : (g < 0.0 ? g : 0.0);
     const double ar = fabs(pj);
     vr = vr < ar ? ar : vr;
-    if (a.trace) ve += gc * cur.phi;  // energy: gamma . Phi
+    if (a.trace) ve += gc * phi;  // energy: gamma . Phi
+  };
+  const int64_t c0 = D.coff + b * K * PCC_BLOCK + threadIdx.x;
+  bool ok = K > 0 && c0 < cend;
+  uint32_t c = (uint32_t)c0;
+  const uint32_t cend32 = (uint32_t)cend;
+  const int32_t K32 = (int32_t)K;
+  TileIn<CT, PLANAR> cur;
+  if (ok) fetch(c, cur);
+  for (int32_t j = 0; j < K32 && ok; ++j) {
+    double2 ui[UW / 2], uj[UW / 2];
+    gatherU(cur, ui, uj);
+    const uint32_t cn = c + PCC_BLOCK;
+    const bool okn = j + 1 < K32 && cn < cend32;
+    // (measured: loading the next tile in every lane, clamped, instead of
+    // under okn, or no prefetch at all, was no faster)
+    TileIn<CT, PLANAR> nxt;
+    if (okn) fetch(cn, nxt);
+    apply(cur, ui, uj, c);
     c = cn;
     ok = okn;
     if (okn) cur = nxt;
@@ -1617,15 +1670,19 @@ void build_incidence(pcc_ctx* ctx, bool records) {
   ctx->rowptr.ensure(N + 1);
   ctx->inccur.ensure(N + 1);
   ctx->inc.ensure(2 * C + 1);
-  PCC_HIP(hipMemsetAsync(ctx->inccur.p, 0, sizeof(int32_t) * (size_t)(N + 1), s));
+  if (!ctx->inccur_zero.known(ctx->inccur.p, N + 1))
+    PCC_HIP(hipMemsetAsync(ctx->inccur.p, 0, sizeof(int32_t) * (size_t)(N + 1), s));
+  ctx->inccur_zero.clear();
   if (C > 0) k_inc_count<<<nblk(C), 256, 0, s>>>(C, ctx->S.I.p, ctx->S.J.p, ctx->inccur.p);
   exclusive_scan_i32(ctx, ctx->inccur.p, ctx->rowptr.p, N + 1);
   if (C == 0) {
+    ctx->inccur_zero.mark(ctx->inccur.p, N + 1);
     if (records) fold_gamma_old(ctx);  // every body's sum is 0
     return;
   }
   k_inc_scatter<<<nblk(C), 256, 0, s>>>(C, ctx->S.I.p, ctx->S.J.p, ctx->rowptr.p, ctx->inccur.p,
                                         ctx->inc.p);
+  ctx->inccur_zero.mark(ctx->inccur.p, N + 1);  // the scatter consumed the counts
   seg_sort_i32(ctx, N, ctx->rowptr.p, ctx->inc.p);
   if (!records) return;
   // solver order: stable radix sort of the constraints by I
@@ -1813,6 +1870,21 @@ double constr_bytes(int64_t N, int64_t C, bool mixed, bool planar) {
   return C * (8 + row + sc + 2 * yg) + N * ub;
 }
 
+static void launch_constr(bool planar, bool mixed, unsigned grid, hipStream_t s,
+                          const ConstrArgs& ca) {
+  if (planar) {
+    if (mixed)
+      k_constr<float, true><<<grid, 256, 0, s>>>(ca);
+    else
+      k_constr<double, true><<<grid, 256, 0, s>>>(ca);
+  } else {
+    if (mixed)
+      k_constr<float, false><<<grid, 256, 0, s>>>(ca);
+    else
+      k_constr<double, false><<<grid, 256, 0, s>>>(ca);
+  }
+}
+
 // Body-kernel variants the autotuner chooses from: staging size (incidences
 // per pass: planar 512 / 1024 / 2048, 3D 256 / 512 / 1024) times threads per
 // block (variants 0-2: 256, one body each; 3-5: 128, two bodies each), and
@@ -1856,7 +1928,7 @@ int body_threads_of(int variant) { return variant < 3 || variant >= 6 ? 256 : 12
 // Process-wide memory of the autotuner: the body-kernel variant chosen for a
 // (device architecture, planar, precision, N bucket, C/N bucket) class of
 // solves, so a new context (serving creates one per batch) reuses it instead
-// of timing the nine variants inside its first step.  N buckets by powers of
+// of timing every variant inside its first step.  N buckets by powers of
 // two of the bodies of the solve, C/N by halves of the mean incidences per
 // body.
 using TuneKey = std::tuple<std::string, int, int, int, int>;
@@ -1939,6 +2011,11 @@ void bbpgd_solve(pcc_ctx* ctx) {
   const int64_t B = (int64_t)ctx->inst.size();
   const bool mixed = ctx->params.precision == PCC_PREC_MIXED;
   const bool planar = ctx->planar;
+  // k_constr addresses the solver-order arrays by 32-bit byte offsets
+  const int64_t row_bytes = (planar ? 6 : 12) * (mixed ? 4 : 8);
+  if (C * std::max<int64_t>(row_bytes, 16) >= ((int64_t)1 << 32) || N * 64 >= ((int64_t)1 << 32))
+    throw StatusError{PCC_E_INVALID_ARGUMENT,
+                      "constraint set too large for one context (operator rows beyond 4 GiB)"};
   hipStream_t s = ctx->stream;
   solver_layout(ctx);
   ctx->ubuf.ensure(8 * N + 8);
@@ -2005,17 +2082,7 @@ void bbpgd_solve(pcc_ctx* ctx) {
     }
     {
       KTimer t(ctx, PCC_K_CONSTR_APPLY, 0.0);
-      if (planar) {
-        if (mixed)
-          k_constr<float, true><<<gc, 256, 0, s>>>(ca);
-        else
-          k_constr<double, true><<<gc, 256, 0, s>>>(ca);
-      } else {
-        if (mixed)
-          k_constr<float, false><<<gc, 256, 0, s>>>(ca);
-        else
-          k_constr<double, false><<<gc, 256, 0, s>>>(ca);
-      }
+      launch_constr(planar, mixed, gc, s, ca);
     }
     if (ctx->trace_on) {
       double* ep = ctx->epart.p;

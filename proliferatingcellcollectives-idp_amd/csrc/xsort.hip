@@ -179,7 +179,7 @@ size_t xsort_tmp_bytes(int64_t n) {
 void xsort_run(const XSort& a) {
   if (a.n <= 0) return;
   const hipStream_t s = a.s;
-  if (a.G > 0) k_xs_init<<<nblk(a.G), 256, 0, s>>>(a.G, a.kmin, a.kmax);
+  if (a.G > 0 && !a.bounds_clear) k_xs_init<<<nblk(a.G), 256, 0, s>>>(a.G, a.kmin, a.kmax);
   // at most ~512 reduction blocks (two atomics each per colony they cover)
   int ipt = 8;
   while ((a.n + 256 * (int64_t)ipt - 1) / (256 * (int64_t)ipt) > 512) ipt *= 2;
