@@ -21,7 +21,11 @@ Variants of the applies D^T gamma -> U = M F -> D U:
   fp32_acc      the applies in FP32 arithmetic (operator, gamma, sums, U);
   fp32_u        FP64 sums, the body velocities U rounded to FP32 once;
   fp32_storage  the operator (D, M) stored in FP32, converted on load, FP64
-                sums (the library's mixed path).
+                sums;
+  fp32_store_g  fp32_storage with Phi stored in FP32 and every evaluated
+                gradient g rounded to FP32 once (gamma stays FP64): the
+                library's mixed path, whose iterate record is gamma (FP64)
+                and g (FP32).
 
 Run: make -C oracle && python oracle/precision_study.py [--n 3000] [--seed 7]
 """
@@ -71,10 +75,13 @@ def make_grad(D, m, phi, dt, variant):
         phi32, dt32 = phi.astype(np.float32), np.float32(dt)
         return lambda gam: (phi32 + dt32 * (D32 @ (m32 * (DT32 @ gam.astype(np.float32))))) \
             .astype(np.float64)
-    if variant == "fp32_storage":
+    if variant in ("fp32_storage", "fp32_store_g"):
         D = D.astype(np.float32).astype(np.float64)
         m = m.astype(np.float32).astype(np.float64)
     DT = D.T.tocsr()
+    if variant == "fp32_store_g":
+        phi = phi.astype(np.float32).astype(np.float64)
+        return lambda gam: (phi + dt * (D @ (m * (DT @ gam)))).astype(np.float32).astype(np.float64)
     if variant == "fp32_u":
         return lambda gam: phi + dt * (D @ (m * (DT @ gam)).astype(np.float32).astype(np.float64))
     return lambda gam: phi + dt * (D @ (m * (DT @ gam)))
@@ -127,7 +134,7 @@ def main():
           f"contacts at cutoff {a.cutoff}, "
           f"max overlap {-phi.min():.3f}", flush=True)
     ref = None
-    for v in ("fp64", "fp32_acc", "fp32_u", "fp32_storage"):
+    for v in ("fp64", "fp32_acc", "fp32_u", "fp32_storage", "fp32_store_g"):
         gam, its, res = bbpgd(make_grad(D, m, phi, p.dt, v), gamma0, a.tol, a.max_its)
         if v == "fp64":
             ref = gam

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <mutex>
+#include <tuple>
 #include <utility>
 #include <string>
 #include <vector>
@@ -260,6 +261,7 @@ struct pcc_ctx {
   bool autotune = true;
   bool body_tuned = false;
   bool body_from_cache = false;  // the variant came from the process-wide tuning memory
+  std::tuple<std::string, int, int, int, int> tuned_key;  // class of the last tuned solve
   int body_variant = 1;
   float body_variant_ms[PCC_BODY_VARIANTS] = {};
 

@@ -14,8 +14,9 @@
 //             ascending constraint order (PETSc's MatMultTranspose row order),
 //             applies mobility, impedance exp(-lambda sigma) and the growth
 //             rate, and writes U and ldot - ldot_prev.  No float atomics.
-//             (k_body_final, one thread per body, is the same sum and is
-//             used for the final write-back of F, U, sigma.)
+//             The same block walk writes the converged iterate's F, U,
+//             sigma, impedance and ldot (MODE 1) and, once per phase in
+//             mixed precision, D^T gamma_old per body (MODE 2).
 //   k_constr  the same projection per constraint, D U and L dldot gathers,
 //             g = Phi + dt*phidot - dt*phidot_growth, BB dot-product and
 //             residual partials per block (fixed tile->block assignment and
@@ -505,100 +506,12 @@ __device__ __forceinline__ void unfold(const double* __restrict__ fold, int64_t 
   }
 }
 
-// D^T gamma_old per body with the FP32 incidence coefficients, in ascending
-// incidence order (once per phase, mixed precision only).
-template <bool PLANAR>
-__global__ void k_fold(int64_t N, const int32_t* __restrict__ rowptr, const void* __restrict__ rec,
-                       const double* __restrict__ gold, double* __restrict__ fold) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  double F[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  const int32_t e1 = rowptr[i + 1];
-  for (int32_t e = rowptr[i]; e < e1; ++e) {
-    const double g = gold[e];
-    if (PLANAR) {
-      double p[4];
-      load_prec<float>(rec, e, p);
-      F[0] += g * p[0];
-      F[1] += g * p[1];
-      F[5] += g * p[2];
-    } else {
-      double co[7];
-      int64_t c;
-      load_rec<float>(rec, e, co, c);
-#pragma unroll
-      for (int q = 0; q < 6; ++q) F[q] += g * co[q];
-    }
-  }
-  if (PLANAR) {
-    fold[3 * i] = F[0];
-    fold[3 * i + 1] = F[1];
-    fold[3 * i + 2] = F[5];
-  } else {
-#pragma unroll
-    for (int q = 0; q < 6; ++q) fold[6 * i + q] = F[q];
-  }
-}
-
-// Workspace of the converged iterate (HardModelGradient.cpp:89-121), one
-// thread per body of the colonies being integrated: F = D^T (gamma -
-// gamma_old), U = M F, sigma = L^T gamma, impedance, ldot; the same
-// ascending-constraint sums as the iteration kernel.
+// Outputs of the converged iterate's workspace (the final body walk): F and
+// U (6 per body), sigma, impedance, ldot; the fold walk writes D^T gamma_old
+// through F.
 struct FinalOut {
   double *F, *U, *sig, *imp, *ld;
 };
-
-template <class CT, bool PLANAR>
-__global__ void __launch_bounds__(256) k_body_final(BodyArgs a, FinalOut fo) {
-  constexpr bool FOLD = sizeof(CT) == 4;  // mixed: gamma_old enters as the per-body sum
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.N) return;
-  const InstDev& D = a.idev[a.binst[i]];
-  if (!(D.flags & kInstInteg)) return;
-  double F[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  double sig = 0.0;
-  const int32_t e1 = a.rowptr[i + 1];
-  for (int32_t e = a.rowptr[i]; e < e1; ++e) {
-    double co[7];
-    int64_t c;
-    if (PLANAR) {
-      double p[4];
-      load_prec<CT>(a.rec, e, p);
-      co[0] = p[0];
-      co[1] = p[1];
-      co[5] = p[2];
-      co[6] = p[3];
-      c = a.rc[e];
-    } else {
-      load_rec<CT>(a.rec, e, co, c);
-    }
-    const double gc = load_gamma<CT>(a.it, c);
-    const double dg = FOLD ? gc : gc - a.gold[e];
-    if (PLANAR) {
-      F[0] += dg * co[0];
-      F[1] += dg * co[1];
-      F[5] += dg * co[5];
-    } else {
-#pragma unroll
-      for (int q = 0; q < 6; ++q) F[q] += dg * co[q];
-    }
-    sig += gc * co[6];
-  }
-  if (FOLD) unfold(a.fold, i, F, PLANAR);
-  const double mt = a.mob[2 * i], mr = a.mob[2 * i + 1];
-  double u[6];
-  body_velocity(a.uext, i, mt, mr, F, u);
-  const double imp = det_exp(sig * D.mlam);
-  const double ld = (a.len[i] * imp) * D.itau;
-#pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    fo.F[6 * i + q] = F[q];
-    fo.U[6 * i + q] = u[q];
-  }
-  fo.sig[i] = sig;
-  fo.imp[i] = imp;
-  fo.ld[i] = ld;
-}
 
 // Element idx of an array of T at `base`, addressed by a 32-bit byte offset.
 template <class T>
@@ -642,24 +555,32 @@ struct ItRaw<float> {
 // lane issues the next chunk's incidence records as soon as the current
 // chunk's iterate gathers are in flight, so that their latency overlaps the
 // products, the LDS sums and the two barriers of the current chunk.
-template <class CT, bool PLANAR, int CHUNK, int TPB, bool PIPE>
-__global__ void __launch_bounds__(TPB) k_body_coop(BodyArgs a) {
-  constexpr int NC = PLANAR ? 4 : 7;  // products per incidence
+// MODE 0: the BBPGD body apply (velocity records of the iterate about to
+// run).  MODE 1: the workspace of the converged iterate for the colonies
+// being integrated (F = D^T (gamma - gamma_old), U = M F, sigma, impedance,
+// ldot into fo; HardModelGradient.cpp:89-121).  MODE 2 (mixed precision
+// only): D^T gamma_old per body with the FP32 coefficients, once per phase.
+// All three sum each body's incidences in ascending order.
+template <class CT, bool PLANAR, int CHUNK, int TPB, bool PIPE, int MODE>
+__device__ __forceinline__ void body_block(const BodyArgs& a, int64_t l,
+                                           double (*prod)[CHUNK], const FinalOut& fo) {
   constexpr bool FOLD = sizeof(CT) == 4;  // mixed: gamma_old enters as the per-body sum
+  constexpr bool GOLD = !FOLD || MODE == 2;  // gamma_old per incidence is read
+  static_assert(MODE != 2 || FOLD, "the fold is mixed precision");
   constexpr int R = PCC_BLOCK / TPB;      // bodies per thread
   static_assert(CHUNK % TPB == 0 && CHUNK >= TPB && PCC_BLOCK % TPB == 0, "staging shape");
-  __shared__ double prod[NC][CHUNK];
-  const int64_t l = xcd_block(blockIdx.x, a.gp);
   if (l >= a.nbb) return;
   const int32_t inst = a.bmap[l];
   const SolverState* st = a.st + inst;
-  if (st->skip | st->converged | st->failed) return;
   const InstDev& D = a.idev[inst];
+  if (MODE == 0 && (st->skip | st->converged | st->failed)) return;
+  if (MODE == 1 && !(D.flags & kInstInteg)) return;
+  if (MODE == 2 && st->skip) return;
   const int64_t bend = D.boff + D.N;
   const int64_t b0 = D.boff + (l - D.bb) * PCC_BLOCK;
   const int64_t b1 = b0 + PCC_BLOCK < bend ? b0 + PCC_BLOCK : bend;
-  const int64_t k = st->iter;
-  const double malpha = -st->alpha;
+  const int64_t k = MODE == 0 ? st->iter : 0;  // MODE 1: gamma as stored
+  const double malpha = MODE == 0 ? -st->alpha : 0.0;
   const int32_t E0 = a.rowptr[b0], E1 = a.rowptr[b1];
   int32_t e_my[R], e_end[R];
   double F[R][6], sig[R];
@@ -701,8 +622,8 @@ __global__ void __launch_bounds__(TPB) k_body_coop(BodyArgs a) {
     double gold[PER];
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-      yg[j] = load_it<CT>(a.it, a.itg, raw[j].index());
-      if (!FOLD) gold[j] = a.gold[ok[j] ? base + (int32_t)threadIdx.x + j * TPB : E1 - 1];
+      if (MODE != 2) yg[j] = load_it<CT>(a.it, a.itg, raw[j].index());
+      if (GOLD) gold[j] = a.gold[ok[j] ? base + (int32_t)threadIdx.x + j * TPB : E1 - 1];
     }
     IncRaw<CT, PLANAR> cur[PER];
 #pragma unroll
@@ -715,8 +636,8 @@ __global__ void __launch_bounds__(TPB) k_body_coop(BodyArgs a) {
     for (int j = 0; j < PER; ++j) {
       double co[7];
       cur[j].coef(co);
-      const double gc = (k == 0) ? yg[j].x : proj_step(yg[j].x, malpha, yg[j].y);
-      const double dg = FOLD ? gc : gc - gold[j];
+      const double gc = MODE == 2 ? gold[j] : (k == 0) ? yg[j].x : proj_step(yg[j].x, malpha, yg[j].y);
+      const double dg = (FOLD || MODE == 2) ? gc : gc - gold[j];
       const int q = (int)threadIdx.x + j * TPB;
       if (PLANAR) {
         prod[0][q] = dg * co[0];
@@ -752,12 +673,35 @@ __global__ void __launch_bounds__(TPB) k_body_coop(BodyArgs a) {
   for (int r = 0; r < R; ++r) {
     const int64_t i = b0 + threadIdx.x + r * TPB;
     if (i >= b1) continue;
+    if (MODE == 2) {
+      if (PLANAR) {
+        double* f = fo.F + 3 * i;
+        f[0] = F[r][0];
+        f[1] = F[r][1];
+        f[2] = F[r][5];
+      } else {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) fo.F[6 * i + q] = F[r][q];
+      }
+      continue;
+    }
     if (FOLD) unfold(a.fold, i, F[r], PLANAR);
     const double mt = a.mob[2 * i], mr = a.mob[2 * i + 1];
     double u[6];
     body_velocity(a.uext, i, mt, mr, F[r], u);
     const double imp = det_exp(sig[r] * D.mlam);
     const double ld = (a.len[i] * imp) * D.itau;
+    if (MODE == 1) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        fo.F[6 * i + q] = F[r][q];
+        fo.U[6 * i + q] = u[q];
+      }
+      fo.sig[i] = sig[r];
+      fo.imp[i] = imp;
+      fo.ld[i] = ld;
+      continue;
+    }
     if (PLANAR) {
       double2* o = reinterpret_cast<double2*>(a.ubuf + 4 * i);
       o[0] = make_double2(u[0], u[1]);
@@ -770,6 +714,13 @@ __global__ void __launch_bounds__(TPB) k_body_coop(BodyArgs a) {
       o[3] = make_double2(ld - a.ldot_prev[i], 0.0);
     }
   }
+}
+
+template <class CT, bool PLANAR, int CHUNK, int TPB, bool PIPE, int MODE = 0>
+__global__ void __launch_bounds__(TPB) k_body_coop(BodyArgs a, FinalOut fo) {
+  constexpr int NC = PLANAR ? 4 : 7;  // products per incidence
+  __shared__ double prod[NC][CHUNK];
+  body_block<CT, PLANAR, CHUNK, TPB, PIPE, MODE>(a, xcd_block(blockIdx.x, a.gp), prod, fo);
 }
 
 struct ConstrArgs {
@@ -854,9 +805,7 @@ struct TileIn {
 // software-pipelined: tile j+1's indices, row and iterate record are loaded
 // while tile j's body-velocity gathers are in flight.
 template <class CT, bool PLANAR>
-__global__ void __launch_bounds__(256) k_constr(ConstrArgs a) {
-  __shared__ double lds[20];
-  const int64_t l = xcd_block(blockIdx.x, a.gp);
+__device__ __forceinline__ void constr_block(const ConstrArgs& a, int64_t l, double* lds) {
   if (l >= a.ncb) return;
   const int32_t inst = a.cmap[l];
   const SolverState* st = a.st + inst;
@@ -989,6 +938,12 @@ __global__ void __launch_bounds__(256) k_constr(ConstrArgs a) {
     a.partials[3 * a.Gtot + l] = rm;
     if (a.trace) a.partials[4 * a.Gtot + l] = ((lds[16] + lds[17]) + lds[18]) + lds[19];
   }
+}
+
+template <class CT, bool PLANAR>
+__global__ void __launch_bounds__(256) k_constr(ConstrArgs a) {
+  __shared__ double lds[20];
+  constr_block<CT, PLANAR>(a, xcd_block(blockIdx.x, a.gp), lds);
 }
 
 // Energy of the iterate just evaluated (HardModelGradient.cpp:160-203), body
@@ -1638,24 +1593,6 @@ bool bodies_planar(pcc_ctx* ctx) {
 // no separate array.
 static float* iter_g(pcc_ctx*) { return nullptr; }
 
-// Body-incidence CSR over the whole constraint set, each body's incidences in
-// ascending constraint order, plus the per-incidence coefficient records (3D
-// or planar, FP64 and FP32) and, planar, the compact constraint rows.
-// Mixed precision: D^T gamma_old per body for the phase (rowptr, and for
-// C > 0 the FP32 records and gamma_old per incidence, must be current).
-static void fold_gamma_old(pcc_ctx* ctx) {
-  const int64_t N = ctx->N;
-  if (ctx->params.precision != PCC_PREC_MIXED || N == 0) return;
-  ctx->fold.ensure(6 * N + 6);
-  hipStream_t s = ctx->stream;
-  if (ctx->planar)
-    k_fold<true><<<nblk(N), 256, 0, s>>>(N, ctx->rowptr.p, ctx->increc32.p, ctx->gold_e.p,
-                                         ctx->fold.p);
-  else
-    k_fold<false><<<nblk(N), 256, 0, s>>>(N, ctx->rowptr.p, ctx->increc32.p, ctx->gold_e.p,
-                                          ctx->fold.p);
-}
-
 // The mixed body kernels read 3 or 6 doubles of D^T gamma_old per body.
 static void check_fold(pcc_ctx* ctx) {
   if (ctx->params.precision != PCC_PREC_MIXED) return;
@@ -1663,6 +1600,10 @@ static void check_fold(pcc_ctx* ctx) {
     throw StatusError{PCC_E_DEVICE, "internal: gamma_old sums missing for the mixed kernels"};
 }
 
+// Body-incidence CSR over the whole constraint set, each body's incidences in
+// ascending constraint order, plus the per-incidence coefficient records (3D
+// or planar, FP64 and FP32), gamma_old per incidence and, planar, the compact
+// constraint rows.
 void build_incidence(pcc_ctx* ctx, bool records) {
   const int64_t N = ctx->N, C = ctx->S.C;
   hipStream_t s = ctx->stream;
@@ -1677,7 +1618,6 @@ void build_incidence(pcc_ctx* ctx, bool records) {
   exclusive_scan_i32(ctx, ctx->inccur.p, ctx->rowptr.p, N + 1);
   if (C == 0) {
     ctx->inccur_zero.mark(ctx->inccur.p, N + 1);
-    if (records) fold_gamma_old(ctx);  // every body's sum is 0
     return;
   }
   k_inc_scatter<<<nblk(C), 256, 0, s>>>(C, ctx->S.I.p, ctx->S.J.p, ctx->rowptr.p, ctx->inccur.p,
@@ -1743,7 +1683,6 @@ void build_incidence(pcc_ctx* ctx, bool records) {
     k_inc_records<false><<<nblk(2 * C), 256, 0, s>>>(2 * C, ctx->inc.p, S.coef64.p, ctx->spos.p,
                                                      S.gamma.p, ctx->increc.p, ctx->increc32.p,
                                                      ctx->incidx.p, ctx->gold_e.p);
-  fold_gamma_old(ctx);
 }
 
 // Logical block layout of the BBPGD kernels over the active colonies:
@@ -1893,15 +1832,15 @@ static void launch_constr(bool planar, bool mixed, unsigned grid, hipStream_t s,
 template <class CT, bool PL, int C0>
 static void launch_body_t(int variant, unsigned grid, hipStream_t s, const BodyArgs& ba) {
   switch (variant) {
-    case 0: k_body_coop<CT, PL, C0, 256, false><<<grid, 256, 0, s>>>(ba); break;
-    case 1: k_body_coop<CT, PL, 2 * C0, 256, false><<<grid, 256, 0, s>>>(ba); break;
-    case 2: k_body_coop<CT, PL, 4 * C0, 256, false><<<grid, 256, 0, s>>>(ba); break;
-    case 3: k_body_coop<CT, PL, C0, 128, false><<<grid, 128, 0, s>>>(ba); break;
-    case 4: k_body_coop<CT, PL, 2 * C0, 128, false><<<grid, 128, 0, s>>>(ba); break;
-    case 5: k_body_coop<CT, PL, 4 * C0, 128, false><<<grid, 128, 0, s>>>(ba); break;
-    case 6: k_body_coop<CT, PL, 256, 256, true><<<grid, 256, 0, s>>>(ba); break;
-    case 7: k_body_coop<CT, PL, 512, 256, true><<<grid, 256, 0, s>>>(ba); break;
-    default: k_body_coop<CT, PL, 1024, 256, true><<<grid, 256, 0, s>>>(ba); break;
+    case 0: k_body_coop<CT, PL, C0, 256, false><<<grid, 256, 0, s>>>(ba, FinalOut{}); break;
+    case 1: k_body_coop<CT, PL, 2 * C0, 256, false><<<grid, 256, 0, s>>>(ba, FinalOut{}); break;
+    case 2: k_body_coop<CT, PL, 4 * C0, 256, false><<<grid, 256, 0, s>>>(ba, FinalOut{}); break;
+    case 3: k_body_coop<CT, PL, C0, 128, false><<<grid, 128, 0, s>>>(ba, FinalOut{}); break;
+    case 4: k_body_coop<CT, PL, 2 * C0, 128, false><<<grid, 128, 0, s>>>(ba, FinalOut{}); break;
+    case 5: k_body_coop<CT, PL, 4 * C0, 128, false><<<grid, 128, 0, s>>>(ba, FinalOut{}); break;
+    case 6: k_body_coop<CT, PL, 256, 256, true><<<grid, 256, 0, s>>>(ba, FinalOut{}); break;
+    case 7: k_body_coop<CT, PL, 512, 256, true><<<grid, 256, 0, s>>>(ba, FinalOut{}); break;
+    default: k_body_coop<CT, PL, 1024, 256, true><<<grid, 256, 0, s>>>(ba, FinalOut{}); break;
   }
 }
 
@@ -2036,10 +1975,20 @@ void bbpgd_solve(pcc_ctx* ctx) {
                            hipMemcpyHostToDevice, s));
     PCC_HIP(hipStreamSynchronize(s));
   }
-  check_fold(ctx);
+  if (mixed && N > 0) ctx->fold.ensure(6 * N + 6);
   BodyArgs ba = body_args(ctx);
   ConstrArgs ca = constr_args(ctx);
   const unsigned gb = (unsigned)ba.gp, gc = (unsigned)ca.gp;
+  if (mixed && N > 0) {
+    // D^T gamma_old of every body of the solve, once per phase
+    FinalOut fo{};
+    fo.F = ctx->fold.p;
+    if (planar)
+      k_body_coop<float, true, 512, 256, true, 2><<<gb, 256, 0, s>>>(ba, fo);
+    else
+      k_body_coop<float, false, 256, 256, true, 2><<<gb, 256, 0, s>>>(ba, fo);
+  }
+  check_fold(ctx);
   // Persistent path: when every active colony fits one workgroup, one launch
   // runs all their solves (a block per colony) to the end.
   {
@@ -2101,63 +2050,73 @@ void bbpgd_solve(pcc_ctx* ctx) {
     k_finish<<<(unsigned)B, 256, 0, s>>>(ca);
   };
   one_iteration();  // k = 0
-  if (ctx->autotune && !ctx->body_tuned && C >= kTuneMinConstraints) {
+  // Autotuner: the body-kernel variant for this solve's class (N bucket,
+  // C/N bucket), from the process-wide memory or timed here on the solve's
+  // own operators.  A context re-tunes when a solve falls in another class
+  // (the phases of a ReLCP step grow C/N, and larger staging pays off as it
+  // does).  The kernel only rewrites the velocity records of the iteration
+  // about to run, identically in every variant, so timing it changes no
+  // state.
+  if (ctx->autotune && C >= kTuneMinConstraints) {
     int64_t na = 0;
     for (const auto& I : ctx->inst)
       if (I.flags & kInstActive) na += I.N;
     const TuneKey key = tune_key(ctx, na, C);
-    std::lock_guard<std::mutex> g(tune_mutex());
-    auto it = tune_cache().find(key);
-    if (it != tune_cache().end()) {
-      ctx->body_variant = it->second.variant;
-      memcpy(ctx->body_variant_ms, it->second.ms, sizeof(ctx->body_variant_ms));
-      ctx->body_tuned = true;
-      ctx->body_from_cache = true;
-    }
-  }
-  if (ctx->autotune && !ctx->body_tuned && C >= kTuneMinConstraints && !ctx->timing.enabled) {
-    // Autotuner: time every staging size of the body kernel on this solve's
-    // operators.  The kernel only rewrites the velocity records of the
-    // iteration about to run, identically in every variant, so timing it
-    // changes no state.
-    float best = 0.f;
-    hipEvent_t e0, e1;
-    {
-      DeviceLock lock(device_mutex());
-      PCC_HIP(hipEventCreate(&e0));
-      PCC_HIP(hipEventCreate(&e1));
-    }
-    for (int v = 0; v < kBodyVariants; ++v) {
-      body(v);  // warm
-      PCC_HIP(hipEventRecord(e0, s));
-      for (int r = 0; r < 3; ++r) body(v);
-      PCC_HIP(hipEventRecord(e1, s));
-      PCC_HIP(hipEventSynchronize(e1));
-      float ms = 0.f;
-      PCC_HIP(hipEventElapsedTime(&ms, e0, e1));
-      ctx->body_variant_ms[v] = ms / 3;
-      if (v == 0 || ms < best) {
-        best = ms;
-        ctx->body_variant = v;
+    if (!ctx->body_tuned || key != ctx->tuned_key) {
+      bool found = false;
+      {
+        std::lock_guard<std::mutex> g(tune_mutex());
+        auto it = tune_cache().find(key);
+        if (it != tune_cache().end()) {
+          ctx->body_variant = it->second.variant;
+          memcpy(ctx->body_variant_ms, it->second.ms, sizeof(ctx->body_variant_ms));
+          found = true;
+        }
+      }
+      if (found) {
+        ctx->body_tuned = true;
+        ctx->body_from_cache = true;
+        ctx->tuned_key = key;
+      } else if (!ctx->timing.enabled) {
+        float best = 0.f;
+        hipEvent_t e0, e1;
+        {
+          DeviceLock lock(device_mutex());
+          PCC_HIP(hipEventCreate(&e0));
+          PCC_HIP(hipEventCreate(&e1));
+        }
+        constexpr int kReps = 8;
+        for (int v = 0; v < kBodyVariants; ++v) {
+          body(v);  // warm
+          PCC_HIP(hipEventRecord(e0, s));
+          for (int r = 0; r < kReps; ++r) body(v);
+          PCC_HIP(hipEventRecord(e1, s));
+          PCC_HIP(hipEventSynchronize(e1));
+          float ms = 0.f;
+          PCC_HIP(hipEventElapsedTime(&ms, e0, e1));
+          ctx->body_variant_ms[v] = ms / kReps;
+          if (v == 0 || ms < best) {
+            best = ms;
+            ctx->body_variant = v;
+          }
+        }
+        {
+          TuneEntry en;
+          en.variant = ctx->body_variant;
+          memcpy(en.ms, ctx->body_variant_ms, sizeof(en.ms));
+          std::lock_guard<std::mutex> g(tune_mutex());
+          tune_cache()[key] = en;
+        }
+        {
+          DeviceLock lock(device_mutex());
+          (void)hipEventDestroy(e0);
+          (void)hipEventDestroy(e1);
+        }
+        ctx->body_tuned = true;
+        ctx->body_from_cache = false;
+        ctx->tuned_key = key;
       }
     }
-    {
-      int64_t na = 0;
-      for (const auto& I : ctx->inst)
-        if (I.flags & kInstActive) na += I.N;
-      const TuneKey key = tune_key(ctx, na, C);
-      TuneEntry en;
-      en.variant = ctx->body_variant;
-      memcpy(en.ms, ctx->body_variant_ms, sizeof(en.ms));
-      std::lock_guard<std::mutex> g(tune_mutex());
-      tune_cache()[key] = en;
-    }
-    {
-      DeviceLock lock(device_mutex());
-      (void)hipEventDestroy(e0);
-      (void)hipEventDestroy(e1);
-    }
-    ctx->body_tuned = true;
   }
   const int chunk = 10;
   // The chunk of `chunk` iterations is launched as one captured HIP graph
@@ -2228,19 +2187,19 @@ void integrate_hard(pcc_ctx* ctx) {
   KTimer t(ctx, PCC_K_INTEGRATE, 0.0);
   if (N > 0) {
     check_fold(ctx);
-    BodyArgs ba = body_args(ctx);
+    BodyArgs ba = body_args(ctx);  // the solve's body blocks (solver_layout)
     const FinalOut fo = {ctx->F.p, ctx->U.p, ctx->sig.p, b.imp, ctx->ldot_new.p};
-    const unsigned g = nblk(N);
+    const unsigned gb = (unsigned)ba.gp, g = nblk(N);
     if (ctx->planar) {
       if (mixed)
-        k_body_final<float, true><<<g, 256, 0, s>>>(ba, fo);
+        k_body_coop<float, true, 512, 256, true, 1><<<gb, 256, 0, s>>>(ba, fo);
       else
-        k_body_final<double, true><<<g, 256, 0, s>>>(ba, fo);
+        k_body_coop<double, true, 512, 256, true, 1><<<gb, 256, 0, s>>>(ba, fo);
     } else {
       if (mixed)
-        k_body_final<float, false><<<g, 256, 0, s>>>(ba, fo);
+        k_body_coop<float, false, 256, 256, true, 1><<<gb, 256, 0, s>>>(ba, fo);
       else
-        k_body_final<double, false><<<g, 256, 0, s>>>(ba, fo);
+        k_body_coop<double, false, 256, 256, true, 1><<<gb, 256, 0, s>>>(ba, fo);
     }
     k_integrate<<<g, 256, 0, s>>>(N, b, ctx->binst.p, ctx->idev.p, ctx->U.p, ctx->ldot_new.p,
                                   b.ldot);

@@ -239,3 +239,4 @@ def test_every_body_kernel_variant_gives_the_same_bits(precision, monkeypatch):
             assert r == runs[0][0]
             assert_same_state(st, runs[0][1])
             assert_same_constraints(cons, runs[0][2])
+
