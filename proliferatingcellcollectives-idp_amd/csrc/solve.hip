@@ -804,6 +804,130 @@ struct TileIn {
 // k_finish combines the G block partials in a fixed order.  The tile loop is
 // software-pipelined: tile j+1's indices, row and iterate record are loaded
 // while tile j's body-velocity gathers are in flight.
+// The scalar update of one colony's BBPGD iteration (BBPGD.cpp:49-116):
+// its block partials reduced in a fixed order, then residual, convergence,
+// BB step size and iteration count on the device.
+__device__ __forceinline__ void finish_colony(const ConstrArgs& a, int32_t q, double* lds,
+                                              double (*l4)[4]) {
+  SolverState* st = a.st + q;
+  if (st->skip | st->converged | st->failed) return;
+  const InstDev& D = a.idev[q];
+  const int64_t k = st->iter;
+  const int64_t G = D.G, o = D.lb, T = a.Gtot;
+  // Thread t adds partials t, t + 256, ... (G <= kBbGridMax, so at most KP of
+  // them) in ascending order.  All of its loads are issued first, at an
+  // in-range index for the slots past G, and every sum is formed and then
+  // selected (a slot past G leaves it unchanged), so that no load waits
+  // behind a branch and they all overlap.
+  constexpr int KP = kBbGridMax / PCC_BLOCK;
+  double p0[KP], p1[KP], p2[KP], pm[KP];
+#pragma unroll
+  for (int j = 0; j < KP; ++j) {
+    const int64_t i = threadIdx.x + j * PCC_BLOCK;
+    const int64_t c = o + (i < G ? i : 0);
+    p0[j] = a.partials[c];
+    p1[j] = a.partials[T + c];
+    p2[j] = a.partials[2 * T + c];
+    pm[j] = a.partials[3 * T + c];
+  }
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, am = 0.0;
+#pragma unroll
+  for (int j = 0; j < KP; ++j) {
+    const bool in = threadIdx.x + j * PCC_BLOCK < G;
+    const double s0 = a0 + p0[j], s1 = a1 + p1[j], s2 = a2 + p2[j];
+    const double sm = am < pm[j] ? pm[j] : am;
+    a0 = in ? s0 : a0;
+    a1 = in ? s1 : a1;
+    a2 = in ? s2 : a2;
+    am = in ? sm : am;
+  }
+  // block_sum x 3 and block_max with one LDS exchange: the same wave
+  // butterflies and the same ((w0 + w1) + w2) + w3 / max chain over waves
+  {
+    const double w0 = wave_sum(a0), w1 = wave_sum(a1), w2 = wave_sum(a2), wm = wave_max(am);
+    if ((threadIdx.x & (PCC_WAVE - 1)) == 0) {
+      const int w = threadIdx.x >> 6;
+      l4[0][w] = w0;
+      l4[1][w] = w1;
+      l4[2][w] = w2;
+      l4[3][w] = wm;
+    }
+    __syncthreads();
+  }
+  double S0 = 0.0, S1 = 0.0, S2 = 0.0, R = 0.0;
+  if (threadIdx.x == 0) {
+    S0 = ((l4[0][0] + l4[0][1]) + l4[0][2]) + l4[0][3];
+    S1 = ((l4[1][0] + l4[1][1]) + l4[1][2]) + l4[1][3];
+    S2 = ((l4[2][0] + l4[2][1]) + l4[2][2]) + l4[2][3];
+    R = l4[3][0];
+    for (int i = 1; i < 4; ++i) R = R < l4[3][i] ? l4[3][i] : R;
+  }
+  // BBPGD trace record of this iterate (BBPGD.cpp:57-60,76-80): energy terms
+  // from the gamma.Phi block partials and the colony's body-block partials,
+  // each combined in the fixed strided order above
+  double E1 = 0.0, E2 = 0.0, E3 = 0.0;
+  if (a.trace) {
+    double a4 = 0.0, a5 = 0.0, a6 = 0.0;
+    for (int64_t i = threadIdx.x; i < G; i += PCC_BLOCK) a4 += a.partials[4 * T + o + i];
+    const int64_t nb = (D.N + PCC_BLOCK - 1) / PCC_BLOCK;
+    for (int64_t i = threadIdx.x; i < nb; i += PCC_BLOCK) {
+      a5 += a.epart[D.bb + i];
+      a6 += a.epart[a.nbb + D.bb + i];
+    }
+    E1 = block_sum(a4, lds);
+    E2 = block_sum(a5, lds);
+    E3 = block_sum(a6, lds);
+  }
+  if (threadIdx.x != 0) return;
+  auto record = [&](double res, double alpha) {
+    const int64_t n = a.trace_n[q];
+    if (n < a.trace_cap) {
+      pcc_bbpgd_record* r = a.trace_buf + (int64_t)q * a.trace_cap + n;
+      const double t2 = E2 * (D.dt / 2.0);
+      const double t3 = E3 * (D.dt / (D.lam * D.tau));
+      r->step = k;
+      r->recursive_iteration = D.phase;
+      r->total_constraints = D.C;
+      r->residual = res;
+      r->step_size = alpha;
+      r->linear = E1;
+      r->quadratic = t2;
+      r->growth = t3;
+      r->total = E1 + t2 + t3;
+    }
+    a.trace_n[q] = n + 1;
+  };
+  if (k == 0) {  // initial gradient: res = residual(g, Gamma), alpha = 1/res
+    st->res = R;
+    st->alpha = 1.0 / R;
+    st->iter = 1;
+    if (a.trace) record(R, st->alpha);
+    if (1 == D.maxit) st->failed = 1;
+    return;
+  }
+  if (k % D.check == 0) {
+    st->res = R;
+    if (a.trace) record(R, st->alpha);
+    if (R <= D.tol) {
+      st->converged = 1;
+      return;
+    }
+  }
+  double num, den;
+  if (k % 2 == 0) {  // BB1: (dg.dg)/(dg.dphi)
+    num = S0;
+    den = S1;
+  } else {  // BB2: (dphi.dg)/(dphi.dphi)
+    den = S2;
+    num = S1;
+  }
+  const double eps10 = 10 * 2.220446049250313080847e-16;
+  if (fabs(den) < eps10) den += eps10;
+  st->alpha = num / den;
+  st->iter = k + 1;
+  if (k + 1 == D.maxit) st->failed = 1;
+}
+
 template <class CT, bool PLANAR>
 __device__ __forceinline__ void constr_block(const ConstrArgs& a, int64_t l, double* lds) {
   if (l >= a.ncb) return;
@@ -1014,123 +1138,7 @@ __global__ void __launch_bounds__(256) k_energy_body(BodyArgs a, double* __restr
 __global__ void __launch_bounds__(256) k_finish(ConstrArgs a) {
   __shared__ double lds[4];
   __shared__ double l4[4][4];
-  SolverState* st = a.st + blockIdx.x;
-  if (st->skip | st->converged | st->failed) return;
-  const InstDev& D = a.idev[blockIdx.x];
-  const int64_t k = st->iter;
-  const int64_t G = D.G, o = D.lb, T = a.Gtot;
-  // Thread t adds partials t, t + 256, ... (G <= kBbGridMax, so at most KP of
-  // them) in ascending order.  All of its loads are issued first, at an
-  // in-range index for the slots past G, and every sum is formed and then
-  // selected (a slot past G leaves it unchanged), so that no load waits
-  // behind a branch and they all overlap.
-  constexpr int KP = kBbGridMax / PCC_BLOCK;
-  double p0[KP], p1[KP], p2[KP], pm[KP];
-#pragma unroll
-  for (int j = 0; j < KP; ++j) {
-    const int64_t i = threadIdx.x + j * PCC_BLOCK;
-    const int64_t c = o + (i < G ? i : 0);
-    p0[j] = a.partials[c];
-    p1[j] = a.partials[T + c];
-    p2[j] = a.partials[2 * T + c];
-    pm[j] = a.partials[3 * T + c];
-  }
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, am = 0.0;
-#pragma unroll
-  for (int j = 0; j < KP; ++j) {
-    const bool in = threadIdx.x + j * PCC_BLOCK < G;
-    const double s0 = a0 + p0[j], s1 = a1 + p1[j], s2 = a2 + p2[j];
-    const double sm = am < pm[j] ? pm[j] : am;
-    a0 = in ? s0 : a0;
-    a1 = in ? s1 : a1;
-    a2 = in ? s2 : a2;
-    am = in ? sm : am;
-  }
-  // block_sum x 3 and block_max with one LDS exchange: the same wave
-  // butterflies and the same ((w0 + w1) + w2) + w3 / max chain over waves
-  {
-    const double w0 = wave_sum(a0), w1 = wave_sum(a1), w2 = wave_sum(a2), wm = wave_max(am);
-    if ((threadIdx.x & (PCC_WAVE - 1)) == 0) {
-      const int w = threadIdx.x >> 6;
-      l4[0][w] = w0;
-      l4[1][w] = w1;
-      l4[2][w] = w2;
-      l4[3][w] = wm;
-    }
-    __syncthreads();
-  }
-  double S0 = 0.0, S1 = 0.0, S2 = 0.0, R = 0.0;
-  if (threadIdx.x == 0) {
-    S0 = ((l4[0][0] + l4[0][1]) + l4[0][2]) + l4[0][3];
-    S1 = ((l4[1][0] + l4[1][1]) + l4[1][2]) + l4[1][3];
-    S2 = ((l4[2][0] + l4[2][1]) + l4[2][2]) + l4[2][3];
-    R = l4[3][0];
-    for (int i = 1; i < 4; ++i) R = R < l4[3][i] ? l4[3][i] : R;
-  }
-  // BBPGD trace record of this iterate (BBPGD.cpp:57-60,76-80): energy terms
-  // from the gamma.Phi block partials and the colony's body-block partials,
-  // each combined in the fixed strided order above
-  double E1 = 0.0, E2 = 0.0, E3 = 0.0;
-  if (a.trace) {
-    double a4 = 0.0, a5 = 0.0, a6 = 0.0;
-    for (int64_t i = threadIdx.x; i < G; i += PCC_BLOCK) a4 += a.partials[4 * T + o + i];
-    const int64_t nb = (D.N + PCC_BLOCK - 1) / PCC_BLOCK;
-    for (int64_t i = threadIdx.x; i < nb; i += PCC_BLOCK) {
-      a5 += a.epart[D.bb + i];
-      a6 += a.epart[a.nbb + D.bb + i];
-    }
-    E1 = block_sum(a4, lds);
-    E2 = block_sum(a5, lds);
-    E3 = block_sum(a6, lds);
-  }
-  if (threadIdx.x != 0) return;
-  auto record = [&](double res, double alpha) {
-    const int64_t n = a.trace_n[blockIdx.x];
-    if (n < a.trace_cap) {
-      pcc_bbpgd_record* r = a.trace_buf + (int64_t)blockIdx.x * a.trace_cap + n;
-      const double t2 = E2 * (D.dt / 2.0);
-      const double t3 = E3 * (D.dt / (D.lam * D.tau));
-      r->step = k;
-      r->recursive_iteration = D.phase;
-      r->total_constraints = D.C;
-      r->residual = res;
-      r->step_size = alpha;
-      r->linear = E1;
-      r->quadratic = t2;
-      r->growth = t3;
-      r->total = E1 + t2 + t3;
-    }
-    a.trace_n[blockIdx.x] = n + 1;
-  };
-  if (k == 0) {  // initial gradient: res = residual(g, Gamma), alpha = 1/res
-    st->res = R;
-    st->alpha = 1.0 / R;
-    st->iter = 1;
-    if (a.trace) record(R, st->alpha);
-    if (1 == D.maxit) st->failed = 1;
-    return;
-  }
-  if (k % D.check == 0) {
-    st->res = R;
-    if (a.trace) record(R, st->alpha);
-    if (R <= D.tol) {
-      st->converged = 1;
-      return;
-    }
-  }
-  double num, den;
-  if (k % 2 == 0) {  // BB1: (dg.dg)/(dg.dphi)
-    num = S0;
-    den = S1;
-  } else {  // BB2: (dphi.dg)/(dphi.dphi)
-    den = S2;
-    num = S1;
-  }
-  const double eps10 = 10 * 2.220446049250313080847e-16;
-  if (fabs(den) < eps10) den += eps10;
-  st->alpha = num / den;
-  st->iter = k + 1;
-  if (k + 1 == D.maxit) st->failed = 1;
+  finish_colony(a, (int32_t)blockIdx.x, lds, l4);
 }
 
 // Persistent BBPGD for colonies small enough for one workgroup: a block of

@@ -22,9 +22,9 @@
 //   3. an exclusive scan of the counts gives each bucket's output range;
 //   4. k_xs_scatter places the items in their buckets (arrival order);
 //   5. k_xs_sort orders each bucket by (x, tie) with one thread per bucket
-//      (insertion sort; ~1.3 items per bucket in dense colonies; a bucket of
-//      k items costs O(k^2), so inputs with many equal x values are slower,
-//      never wrong).
+//      (~1.3 items per bucket in dense colonies: up to 8 sorted in
+//      registers, more by insertion sort, O(k^2) for k items, so inputs with
+//      many equal x values are slower, never wrong).
 #include <rocprim/device/device_scan.hpp>
 
 #include "common.hpp"
@@ -57,8 +57,10 @@ __global__ void k_xs_init(int64_t G, unsigned long long* __restrict__ kmin,
   }
 }
 
-// Block b covers items [b * 256 * ipt, (b + 1) * 256 * ipt), thread t the
-// items base + t + j * 256.  Groups ascend with the item index, so the block
+// Block b covers items [b * 256 * ipt, (b + 1) * 256 * ipt) in rounds of
+// 256 * 8, thread t taking base + t + j * 256 of a round; the eight loads of
+// a round are issued together (clamped to the last item; duplicates do not
+// change a min or max).  Groups ascend with the item index, so the block
 // visits the groups from its first item's to its last item's, one block
 // reduction each.  Also clears the bucket counts (n + 1 entries).
 __global__ void __launch_bounds__(256) k_xs_minmax(int64_t n, int ipt, const uint64_t* __restrict__ key,
@@ -66,6 +68,7 @@ __global__ void __launch_bounds__(256) k_xs_minmax(int64_t n, int ipt, const uin
                                                    unsigned long long* __restrict__ kmin,
                                                    unsigned long long* __restrict__ kmax,
                                                    int32_t* __restrict__ cnt) {
+  constexpr int U = 8;
   __shared__ uint64_t lmin[4], lmax[4];
   const int64_t base = (int64_t)blockIdx.x * 256 * ipt;
   if (base >= n) return;
@@ -79,12 +82,21 @@ __global__ void __launch_bounds__(256) k_xs_minmax(int64_t n, int ipt, const uin
   const int lane = threadIdx.x & (PCC_WAVE - 1), w = threadIdx.x >> 6;
   for (int32_t g = g0; g <= g1; ++g) {
     uint64_t mn = ~0ull, mx = 0ull;
-    for (int j = 0; j < ipt; ++j) {
-      const int64_t r = base + threadIdx.x + (int64_t)j * 256;
-      if (r <= last && grp[r] == g) {
-        const uint64_t k = key[r];
-        mn = k < mn ? k : mn;
-        mx = k > mx ? k : mx;
+    for (int j0 = 0; j0 < ipt; j0 += U) {
+      uint64_t k[U];
+      int32_t q[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        int64_t r = base + threadIdx.x + (int64_t)(j0 + u) * 256;
+        r = r <= last ? r : last;
+        k[u] = key[r];
+        q[u] = grp[r];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const bool in = q[u] == g;
+        mn = in && k[u] < mn ? k[u] : mn;
+        mx = in && k[u] > mx ? k[u] : mx;
       }
     }
 #pragma unroll
@@ -146,12 +158,49 @@ __device__ __forceinline__ bool xs_less(uint64_t ka, uint64_t ta, uint64_t kb, u
   return ka < kb || (ka == kb && ta < tb);
 }
 
+// One thread per bucket.  A bucket of up to kSmall items is loaded into
+// registers with independent loads and sorted there (a sorting network of
+// compare-exchanges over the valid prefix); larger buckets use an insertion
+// sort in place.
 __global__ void k_xs_sort(int64_t n, const int32_t* __restrict__ off,
                           const uint64_t* __restrict__ key, const uint64_t* __restrict__ tie,
                           int32_t* __restrict__ perm) {
+  constexpr int kSmall = 8;
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= n) return;
-  const int32_t lo = off[b], hi = off[b + 1];
+  const int32_t lo = off[b], hi = off[b + 1], m = hi - lo;
+  if (m <= 1) return;
+  if (m <= kSmall) {
+    int32_t x[kSmall];
+    uint64_t kx[kSmall], tx[kSmall];
+#pragma unroll
+    for (int i = 0; i < kSmall; ++i) x[i] = perm[lo + (i < m ? i : m - 1)];
+#pragma unroll
+    for (int i = 0; i < kSmall; ++i) {
+      kx[i] = key[x[i]];
+      tx[i] = tie ? tie[x[i]] : (uint64_t)x[i];
+    }
+    // odd-even transposition over the first m slots (m rounds)
+#pragma unroll
+    for (int r = 0; r < kSmall; ++r) {
+#pragma unroll
+      for (int i = r & 1; i + 1 < kSmall; i += 2) {
+        const bool sw = i + 1 < m && xs_less(kx[i + 1], tx[i + 1], kx[i], tx[i]);
+        const uint64_t k0 = kx[i], t0 = tx[i];
+        const int32_t x0 = x[i];
+        kx[i] = sw ? kx[i + 1] : k0;
+        tx[i] = sw ? tx[i + 1] : t0;
+        x[i] = sw ? x[i + 1] : x0;
+        kx[i + 1] = sw ? k0 : kx[i + 1];
+        tx[i + 1] = sw ? t0 : tx[i + 1];
+        x[i + 1] = sw ? x0 : x[i + 1];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kSmall; ++i)
+      if (i < m) perm[lo + i] = x[i];
+    return;
+  }
   for (int32_t i = lo + 1; i < hi; ++i) {
     const int32_t x = perm[i];
     const uint64_t kx = key[x], tx = tie ? tie[x] : (uint64_t)x;

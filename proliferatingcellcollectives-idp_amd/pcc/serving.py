@@ -12,12 +12,12 @@ Between batch steps it
   * admits waiting requests, first come first served, while the estimated
     device footprint of the resident colonies plus the newcomer stays within
     an HBM budget (sized for the most cells the colony can reach by division
-    in its steps) and, per model, the resident cells stay within a compute
-    budget (the cells a batch step may carry: a step's time grows with its
-    cells, so this bounds the per-step latency of every resident colony; the
-    device is shared by the batches of both models): an admitted colony joins
-    its model's batch and its state stays resident in HBM until its last
-    step;
+    in its steps) and, per model (hard: per model and size bucket), the
+    resident cells stay within a compute budget (the cells a batch step may
+    carry: a step's time grows with its cells, so this bounds the per-step
+    latency of every resident colony; the device is shared by the batches of
+    both models): an admitted colony joins its model's batch and its state
+    stays resident in HBM until its last step;
   * retires colonies that have run all their steps: the final state is
     downloaded and the colony leaves the batch, freeing its share of the
     budget for the next request.
@@ -56,11 +56,14 @@ from .engine import ColonyBatch
 # phases of a step).  The run reports the measured peak beside it.
 DEFAULT_BYTES_PER_CELL = 12_000
 
-# Resident cells per model that admission allows (the compute budget).  A
-# hard step of a relaxed dense packing runs at ~1M cell-steps/s on one
-# MI355X (a few hundred BBPGD iterations per ReLCP phase), a soft step at
-# ~100M: these keep a batch step near a quarter of a second.
-DEFAULT_MAX_CELLS = {T.MODE_HARD: 250_000, T.MODE_SOFT: 4_000_000}
+# Resident cells that admission allows (the compute budget), per model, or
+# per model and size bucket (a tuple, one cap per bucket).  A hard step of a
+# relaxed dense packing runs at ~1M cell-steps/s on one MI355X (a few hundred
+# BBPGD iterations per ReLCP phase), a soft step at ~100M.  The hard caps keep
+# a large-colony batch step near a quarter of a second and the small-colony
+# batches near 20-100 ms (charged at the most cells a colony can reach), each bucket with a budget of its own, so a large
+# colony waiting for room never holds back small ones.
+DEFAULT_MAX_CELLS = {T.MODE_HARD: (40_000, 100_000, 250_000), T.MODE_SOFT: 4_000_000}
 
 
 @dataclass
@@ -131,7 +134,7 @@ class ColonyServer:
                  bytes_per_cell: Optional[float] = None, overlap: bool = True,
                  bucket_edges: Sequence[int] = (4_000, 20_000), warm_steps: int = 2,
                  divide: bool = True, length_noise: float = 0.0, angle_noise: float = 0.01,
-                 max_cells_resident: Optional[Dict[int, int]] = None):
+                 max_cells_resident: Optional[Dict[int, object]] = None):
         self.budget = int(hbm_budget_bytes)
         # compute budget: resident cells per model (None: HBM alone decides)
         self.max_resident = dict(DEFAULT_MAX_CELLS if max_cells_resident is None
@@ -158,8 +161,9 @@ class ColonyServer:
         self.batch_steps = 0
         self.cell_steps = 0
         self._reserved = 0
-        self._cells = {T.MODE_HARD: 0, T.MODE_SOFT: 0}  # resident cells (admission bound)
+        self._cells = {T.MODE_HARD: 0, T.MODE_SOFT: 0}  # resident cells per model
         self.peak_cells = {T.MODE_HARD: 0, T.MODE_SOFT: 0}
+        self._bucket_cells: Dict[Tuple[int, int], int] = collections.defaultdict(int)
         self._lock = threading.Condition()
         self._stop = False
         self._errors: List[str] = []
@@ -194,11 +198,22 @@ class ColonyServer:
         most it can reach in its steps (the HBM bound)."""
         return max_cells(req, self.length_noise) if self.divide else req.bodies.n
 
+    def _budget_key(self, req: Request):
+        """The compute budget a request counts against: (model, bucket) when
+        the model's cap is per bucket, else (model, None)."""
+        mode = req.params.mode
+        cap = self.max_resident.get(mode)
+        if isinstance(cap, (tuple, list)):
+            b = self._bucket(req.bodies.n)
+            return (mode, b), cap[min(b, len(cap) - 1)]
+        return (mode, None), cap
+
     def _admit(self, now: float) -> int:
         """First come, first served: a request that does not fit the HBM
-        budget stops admission; one that does not fit its model's compute
-        budget stops admission for that model only (later requests of the
-        other model may pass it, never those of its own)."""
+        budget stops admission; one that does not fit its compute budget
+        (its model's, or its model's and size bucket's) stops admission for
+        that budget only: later requests of another budget may pass it, never
+        those of its own."""
         n = 0
         with self._lock:
             blocked = set()
@@ -206,20 +221,22 @@ class ColonyServer:
             while i < len(self.queue):
                 req = self.queue[i]
                 mode = req.params.mode
-                if mode in blocked:
+                bkey, cap = self._budget_key(req)
+                if bkey in blocked:
                     i += 1
                     continue
                 need = self.reservation(req)
                 if self._reserved + need > self.budget and self._reserved > 0:
                     break  # the head waits for memory
                 cells = self.cells_charge(req)
-                cap = self.max_resident.get(mode)
-                if cap is not None and self._cells[mode] > 0 and self._cells[mode] + cells > cap:
-                    blocked.add(mode)  # waits for compute
+                held = self._bucket_cells[bkey]
+                if cap is not None and held > 0 and held + cells > cap:
+                    blocked.add(bkey)  # waits for compute
                     i += 1
                     continue
                 del self.queue[i]
                 self._reserved += need
+                self._bucket_cells[bkey] += cells
                 self._cells[mode] += cells
                 self.peak_cells[mode] = max(self.peak_cells[mode], self._cells[mode])
                 self.peak_reserved = max(self.peak_reserved, self._reserved)
@@ -299,6 +316,7 @@ class ColonyServer:
                     continue
                 self._reserved -= self.reservation(m.req)
                 self._cells[m.req.params.mode] -= self.cells_charge(m.req)
+                self._bucket_cells[self._budget_key(m.req)[0]] -= self.cells_charge(m.req)
                 self.results[m.req.rid] = Result(
                     m.req.rid, key[0], m.req.bodies.n, m.req.steps, state, m.step_ms,
                     m.statuses, m.req.arrival, m.admitted, now)

@@ -141,6 +141,25 @@ def test_compute_budget_bounds_resident_cells_per_model():
         assert np.array_equal(r[q.rid].state.length, q.bodies.length + q.steps)
 
 
+def test_compute_budget_per_bucket_lets_small_colonies_pass():
+    # hard budgets per size bucket (edges 150 and 10^6 cells): a large colony
+    # waiting for room in its bucket does not hold back a small colony that
+    # arrived after it; within a bucket, first come first served still holds
+    reqs = make_requests([(400, 0, 3, 0.0), (400, 0, 3, 0.0), (100, 0, 2, 0.0),
+                          (120, 0, 2, 0.0)])
+    srv = serving.ColonyServer(10**12, bytes_per_cell=1000, divide=False,
+                               bucket_edges=(150, 10**6),
+                               max_cells_resident={T.MODE_HARD: (130, 500)})
+    st = srv.run(reqs)
+    assert st["completed"] == 4
+    r = srv.results
+    assert r[1].admitted >= r[0].finished  # 400 + 400 > 500: waits for its bucket
+    assert r[2].admitted < r[0].finished   # the small bucket's budget is its own
+    assert r[3].admitted >= r[2].finished  # 100 + 120 > 130 in the small bucket
+    for q in reqs:
+        assert np.array_equal(r[q.rid].state.length, q.bodies.length + q.steps)
+
+
 def test_late_arrival_joins_a_running_batch():
     # the first colony stays resident for at least 500 x 2 ms; the second
     # arrives well after its first step, also on a loaded machine
@@ -191,7 +210,9 @@ def slow_pair(warm_steps):
     b = make_requests([(777, 0, 5, 0.0)])[0]
     b.rid, b.arrival = 1, 0.25
     b.bodies.length[0] = 100.0  # fresh: slow first step
-    srv = serving.ColonyServer(10**12, bytes_per_cell=1000, warm_steps=warm_steps)
+    # compute budgets off: both colonies resident together
+    srv = serving.ColonyServer(10**12, bytes_per_cell=1000, warm_steps=warm_steps,
+                               max_cells_resident={})
     srv.run([a, b])
     return srv.results
 
