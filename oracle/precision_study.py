@@ -25,7 +25,9 @@ Variants of the applies D^T gamma -> U = M F -> D U:
   fp32_store_g  fp32_storage with Phi stored in FP32 and every evaluated
                 gradient g rounded to FP32 once (gamma stays FP64): the
                 library's mixed path, whose iterate record is gamma (FP64)
-                and g (FP32).
+                and g (FP32);
+  fp32_angle_g  fp32_store_g with each (planar) contact normal stored as its
+                FP32 angle and rebuilt as (cos, sin) in FP32 arithmetic.
 
 Run: make -C oracle && python oracle/precision_study.py [--n 3000] [--seed 7]
 """
@@ -44,13 +46,23 @@ from pcc import generators as G  # noqa: E402
 from pcc.abi_types import reference_params  # noqa: E402
 
 
-def operator(bodies, cons, xi):
+def angle_normals(normal):
+    """Planar normals through an FP32 angle: (cos t, sin t) in FP32."""
+    t = np.arctan2(normal[:, 1], normal[:, 0]).astype(np.float32)
+    out = np.zeros_like(normal)
+    out[:, 0] = np.cos(t).astype(np.float64)
+    out[:, 1] = np.sin(t).astype(np.float64)
+    return out
+
+
+def operator(bodies, cons, xi, normal=None):
     C = len(cons["gidI"])
     n = bodies.n
     rows, cols, vals = [], [], []
+    normal = cons["normal"] if normal is None else normal
     for key, rkey, sgn in (("gidI", "rPosI", 1.0), ("gidJ", "rPosJ", -1.0)):
         b = cons[key].astype(np.int64)
-        nrm = sgn * cons["normal"]
+        nrm = sgn * normal
         tq = np.cross(cons[rkey], nrm)
         for k in range(3):
             rows.append(np.arange(C))
@@ -75,11 +87,11 @@ def make_grad(D, m, phi, dt, variant):
         phi32, dt32 = phi.astype(np.float32), np.float32(dt)
         return lambda gam: (phi32 + dt32 * (D32 @ (m32 * (DT32 @ gam.astype(np.float32))))) \
             .astype(np.float64)
-    if variant in ("fp32_storage", "fp32_store_g"):
+    if variant in ("fp32_storage", "fp32_store_g", "fp32_angle_g"):
         D = D.astype(np.float32).astype(np.float64)
         m = m.astype(np.float32).astype(np.float64)
     DT = D.T.tocsr()
-    if variant == "fp32_store_g":
+    if variant in ("fp32_store_g", "fp32_angle_g"):
         phi = phi.astype(np.float32).astype(np.float64)
         return lambda gam: (phi + dt * (D @ (m * (DT @ gam)))).astype(np.float32).astype(np.float64)
     if variant == "fp32_u":
@@ -134,8 +146,10 @@ def main():
           f"contacts at cutoff {a.cutoff}, "
           f"max overlap {-phi.min():.3f}", flush=True)
     ref = None
-    for v in ("fp64", "fp32_acc", "fp32_u", "fp32_storage", "fp32_store_g"):
-        gam, its, res = bbpgd(make_grad(D, m, phi, p.dt, v), gamma0, a.tol, a.max_its)
+    Da, _ = operator(b, cons, p.xi, angle_normals(cons["normal"]))
+    for v in ("fp64", "fp32_acc", "fp32_u", "fp32_storage", "fp32_store_g", "fp32_angle_g"):
+        Dv = Da if v == "fp32_angle_g" else D
+        gam, its, res = bbpgd(make_grad(Dv, m, phi, p.dt, v), gamma0, a.tol, a.max_its)
         if v == "fp64":
             ref = gam
         err = float(np.linalg.norm(gam - ref) / max(np.linalg.norm(ref), 1e-300))
