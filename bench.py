@@ -11,9 +11,11 @@ arbitrary (1,0,0) of ArrayMath::normalize, and BBPGD then does not reach 1e-6
 (3M iterations at 5k and 20k cells).
 
 `--instances B` colonies (seeds seed..seed+B-1) are stepped together by the
-batched mixed-precision engine (one context, shared launches, captured HIP
-graphs); the plain FP64 single-colony path steps the same colonies one after
-another.  `value` is the batched mixed path's cell-steps/s.
+batched mixed-precision engine (one context, one step call, captured HIP
+graphs; a colony of this size fills the device alone, so the batch step runs
+the colonies' ReLCP loops one after another instead of in lockstep); the plain
+FP64 single-colony path steps the same colonies one after another in
+contexts of their own.  `value` is the batched mixed path's cell-steps/s.
 
 A "step" = restore the initial state from a device-resident snapshot (a
 device-to-device copy) + the step call.  The state is resident in HBM before
@@ -28,6 +30,10 @@ step.  The untimed warm-up step relaxes the packings' initial overlaps.
 `--workload soft`: the soft Hertz model on one 100,000-cell colony packing,
 E* = 1e4, r_eff = 0.125, dt = 1e-3/30, 100 time-loop steps, mixed and FP64,
 with the CPU restatement's soft step timed beside it.
+
+`--workload seeds`: one colony per seed (default 8 seeds) stepped alone in
+FP64 and in mixed precision; the mixed / FP64 step-time ratio split into the
+per-iteration ratio and the BBPGD iteration-count ratio (geometric means).
 
 `--workload serving`: colony-growth requests with Poisson arrivals (20/s for
 10 s), 1k-100k cells each, hard or soft 50/50, growth rate U[0.5,1.5] ln2,
@@ -68,9 +74,9 @@ def params_for(precision):
                             temperature=0.0)
 
 
-def run_single(colonies, precision, steps, warmup):
+def run_single(colonies, precision, steps, warmup, timing=True):
     """The single-colony path: one context per colony, colonies one after
-    another; per-kernel timing pass on the first colony."""
+    another; per-kernel timing pass on the first colony (if `timing`)."""
     from pcc.engine import ParticleManager
     total, lat, results, phases, tim, tuning = 0.0, [], [], None, None, None
     for q, bodies in enumerate(colonies):
@@ -91,6 +97,7 @@ def run_single(colonies, precision, steps, warmup):
         if q == 0:
             phases = pm.phase_stats()
             tuning = pm.tuning()
+        if q == 0 and timing:
             pm.set_timing(True)
             pm.restore()
             pm.step(0)
@@ -321,6 +328,64 @@ def main_ensemble(args):
         "bbpgd_iterations_per_step": e["bbpgd_iterations_per_step"],
         "roofline": e["roofline"],
         "tuning": e["tuning"],
+    }
+    print(json.dumps(line))
+
+
+def geomean(v):
+    return float(np.exp(np.mean(np.log(np.asarray(v, dtype=float)))))
+
+
+def main_seeds(args):
+    """mixed / FP64 over several colony seeds, split into the per-iteration
+    time ratio (what the storage precision buys) and the BBPGD iteration
+    count ratio (which differs between precisions from seed to seed): each
+    seed's colony is stepped alone by the single-colony path in both
+    precisions, one untimed step then one timed step."""
+    per = []
+    for s in range(args.seed, args.seed + args.seeds):
+        b = G.dense_colony(args.n, seed=s)
+        row = {"seed": s}
+        for name, prec in (("fp64", PREC_FP64), ("mixed", PREC_MIXED)):
+            r = run_single([b], prec, 1, 1, timing=False)
+            res = r["results"][0]
+            its = int(res["bbpgd_iterations"])
+            row[name] = {"ms": r["total_s"] * 1e3, "bbpgd_iterations": its,
+                         "us_per_iteration": r["total_s"] * 1e6 / max(its, 1),
+                         "phases": r["phases"]}
+        print(f"[bench] seed {s}: fp64 {row['fp64']['ms']:.1f} ms / {row['fp64']['bbpgd_iterations']}"
+              f" its, mixed {row['mixed']['ms']:.1f} ms / {row['mixed']['bbpgd_iterations']} its",
+              file=sys.stderr, flush=True)
+        per.append(row)
+    step = geomean([x["fp64"]["ms"] / x["mixed"]["ms"] for x in per])
+    per_it = geomean([x["fp64"]["us_per_iteration"] / x["mixed"]["us_per_iteration"] for x in per])
+    its = geomean([x["fp64"]["bbpgd_iterations"] / x["mixed"]["bbpgd_iterations"] for x in per])
+    fp64_cs = args.n * len(per) / (sum(x["fp64"]["ms"] for x in per) / 1e3)
+    mixed_cs = args.n * len(per) / (sum(x["mixed"]["ms"] for x in per) / 1e3)
+    line = {
+        "metric": "mixed / FP64 step-time ratio over colony seeds (single-colony path)",
+        "value": step,
+        "unit": "ratio",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": sum(x["mixed"]["ms"] for x in per) / len(per),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64 vs f64",
+        "data": "synthetic",
+        "config": {"workload": (f"hard-model single step (full ReLCP) of one {args.n}-cell "
+                                f"colony packing per seed (nematic domains, seeds {args.seed}.."
+                                f"{args.seed + args.seeds - 1}), stepped alone in FP64 and in "
+                                f"mixed precision; cutoff 0.05, BBPGD/ReLCP tol 1e-6, dt 1e-3"),
+                   "n_cells": args.n, "seeds": args.seeds},
+        "ratio_step_time_geomean": step,
+        "ratio_per_iteration_geomean": per_it,
+        "ratio_iterations_geomean": its,
+        "fp64_cell_steps_per_s": fp64_cs,
+        "mixed_cell_steps_per_s": mixed_cs,
+        "per_seed": per,
     }
     print(json.dumps(line))
 
@@ -623,7 +688,9 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--instances", type=int, default=2)
     ap.add_argument("--colonies", type=int, default=128)
-    ap.add_argument("--workload", choices=["colony300k", "ensemble", "serving", "soft"],
+    ap.add_argument("--seeds", type=int, default=8,
+                    help="--workload seeds: number of colony seeds")
+    ap.add_argument("--workload", choices=["colony300k", "ensemble", "serving", "soft", "seeds"],
                     default="colony300k")
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--duration", type=float, default=10.0)
@@ -649,6 +716,8 @@ def main():
         return main_serving(args)
     if args.workload == "soft":
         return main_soft(args)
+    if args.workload == "seeds":
+        return main_seeds(args)
     from pcc.engine import copy_bandwidth
     copy_gbs = copy_bandwidth(1 << 30, 10)
     cols = [G.dense_colony(args.n, seed=args.seed + q) for q in range(args.instances)]
@@ -709,7 +778,9 @@ def main():
             "workload": (f"hard-model single step (full ReLCP) of {len(cols)} independent "
                          f"{bodies.n}-cell colony packings (nematic domains, area fraction "
                          f"{G.area_fraction(bodies):.3f}, R={G.disc_radius(bodies):.0f}, seeds "
-                         f"{args.seed}..{args.seed + len(cols) - 1}) batched in one context; "
+                         f"{args.seed}..{args.seed + len(cols) - 1}) batched in one context "
+                         f"(colonies of >= 150000 cells run their ReLCP loops one after "
+                         f"another inside the batch step); "
                          f"lengths U[1,2], d=0.5, cutoff 0.05, BBPGD/ReLCP tol 1e-6, dt 1e-3"),
             "n_cells": bodies.n,
             "instances": len(cols),

@@ -371,14 +371,27 @@ void begin_step(pcc_ctx* ctx) {
   }
 }
 
-// solveHardModel (Hard.cpp:91-193) for every colony of the context.
-void hard_step(pcc_ctx* ctx) {
-  begin_step(ctx);
-  // planar kernels while every body lies in z = 0 with an in-plane
-  // orientation and no Brownian input turns it out of the plane
-  ctx->planar = ctx->allow_planar && !brownian_offplane(ctx) && state_planar(ctx);
-  ctx->detect_flat = ctx->planar;  // z = 0 stays exact through a planar step
-  ctx->flat_known = ctx->planar;   // and after it
+// Colonies whose ReLCP loops run together.  A colony of at least
+// ctx->solo_cells cells fills the device alone and runs its loop by itself;
+// the others share one loop in lockstep, which pays for small colonies (one
+// launch serves them all) but, in every phase, runs as many BBPGD
+// iterations as the group's slowest colony needs there.  Colonies never
+// interact, so the grouping changes no result.
+std::vector<std::vector<int64_t>> relcp_groups(const pcc_ctx* ctx) {
+  std::vector<std::vector<int64_t>> g(1);
+  for (int64_t q = 0; q < (int64_t)ctx->inst.size(); ++q) {
+    if (ctx->solo_cells > 0 && ctx->inst[q].N >= ctx->solo_cells)
+      g.push_back({q});
+    else
+      g[0].push_back(q);
+  }
+  if (g[0].empty()) g.erase(g.begin());
+  return g;
+}
+
+// The ReLCP loop of solveHardModel (Hard.cpp:113-183) for the colonies
+// marked active.
+void relcp_loop(pcc_ctx* ctx) {
   std::vector<double> mo;
   for (int32_t k = 0;; ++k) {
     bool any = false;
@@ -429,6 +442,21 @@ void hard_step(pcc_ctx* ctx) {
     ctx->brownian = false;  // U_ext = 0 from phase 1 on
     for (auto& I : ctx->inst)
       if (I.active) I.k = k + 1;
+  }
+}
+
+// solveHardModel (Hard.cpp:91-193) for every colony of the context.
+void hard_step(pcc_ctx* ctx) {
+  begin_step(ctx);
+  // planar kernels while every body lies in z = 0 with an in-plane
+  // orientation and no Brownian input turns it out of the plane
+  ctx->planar = ctx->allow_planar && !brownian_offplane(ctx) && state_planar(ctx);
+  ctx->detect_flat = ctx->planar;  // z = 0 stays exact through a planar step
+  ctx->flat_known = ctx->planar;   // and after it
+  for (const auto& grp : relcp_groups(ctx)) {
+    for (auto& I : ctx->inst) I.active = false;
+    for (int64_t q : grp) ctx->inst[q].active = true;
+    relcp_loop(ctx);
   }
   for (auto& I : ctx->inst) I.flags = I.status == PCC_OK ? kInstGrow : 0;
   upload_instances(ctx);
@@ -734,6 +762,7 @@ pcc_status pcc_create(const pcc_params* params, int device, pcc_ctx** out) {
   ctx->use_persist = !(persist_env && persist_env[0] == '0');
   if (const char* v = getenv("PCC_PERSIST_MAX_C")) ctx->persist_max_c = atoll(v);
   if (const char* v = getenv("PCC_PERSIST_MAX_N")) ctx->persist_max_n = atoll(v);
+  if (const char* v = getenv("PCC_SOLO_CELLS")) ctx->solo_cells = atoll(v);
   // PCC_BODY_VARIANT=v (0..PCC_BODY_VARIANTS-1) fixes the body-kernel variant
   if (const char* v = getenv("PCC_BODY_VARIANT")) {
     const int x = atoi(v);

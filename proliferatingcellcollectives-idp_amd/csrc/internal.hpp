@@ -257,6 +257,10 @@ struct pcc_ctx {
   bool use_persist = true;
   int64_t persist_max_c = 4096;
   int64_t persist_max_n = 1024;
+  // hard step of a batch: colonies of at least solo_cells cells run their
+  // ReLCP loop alone, one after another; the others share one lockstep loop
+  // (PCC_SOLO_CELLS=n sets the threshold, 0 keeps every colony in lockstep)
+  int64_t solo_cells = 150000;
   // autotuner (PCC_AUTOTUNE=0: off): body-kernel staging size
   bool autotune = true;
   bool body_tuned = false;

@@ -73,6 +73,21 @@ def test_batch_hard_equals_alone(precision):
     check_batch_equals_alone(params[0], colonies, params, steps=2)
 
 
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_solo_colonies_equal_alone(precision, monkeypatch):
+    """With a solo threshold of 1000 cells the colonies of 1500 and 2600
+    cells run their ReLCP loops alone, one after the other, after the
+    lockstep loop of the small ones (whose constraints stay in the set while
+    the solo loops append and regroup theirs); every colony still equals its
+    alone run bit for bit, a colony with rotational diffusion included."""
+    monkeypatch.setenv("PCC_SOLO_CELLS", "1000")
+    print(f"[batch-solo] precision={precision}", flush=True)
+    colonies = [G.packed_colony(n, seed=40 + i) for i, n in enumerate(SIZES)]
+    params = [hard(precision=precision, tau=1 / (LN2 * (0.5 + 0.2 * i)),
+                   temperature=0.05 if i == 4 else 0.0) for i in range(len(SIZES))]
+    check_batch_equals_alone(params[0], colonies, params, steps=2)
+
+
 def test_batch_solver_failures_stay_per_colony():
     """One colony exhausts BBPGD, one exhausts ReLCP phases, one is empty; the
     others are unaffected and every status is the one it has alone."""
