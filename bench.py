@@ -163,7 +163,15 @@ def run_batched(colonies, precision, steps, warmup, timing=True, restore=True, r
 PMC_SUMMARY = os.path.join(REPO, "profiles", "bbpgd_mixed_300k_colony_pmc.txt")
 
 
-def traffic_from_profile(kernel, instances):
+def schedule(n_cells):
+    """How a batch step runs colonies of n_cells cells: each ReLCP loop alone
+    ("solo", from the library's threshold of 150000 cells or PCC_SOLO_CELLS)
+    or all colonies in one lockstep loop."""
+    solo = int(os.environ.get("PCC_SOLO_CELLS", "150000"))
+    return "solo" if solo > 0 and n_cells >= solo else "lockstep"
+
+
+def traffic_from_profile(kernel, instances, sched):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary of this workload (FETCH_SIZE and WRITE_SIZE passes, each scaled
     by the factor that maps a copy of known size onto its true bytes), or
@@ -172,9 +180,9 @@ def traffic_from_profile(kernel, instances):
         return None
     for line in open(PMC_SUMMARY):
         f = line.split()
-        if len(f) == 4 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel \
-                and f[2] == f"instances={instances}":
-            return float(f[3])
+        if len(f) == 5 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel \
+                and f[2] == f"instances={instances}" and f[3] == f"schedule={sched}":
+            return float(f[4])
     return None
 
 
@@ -718,8 +726,9 @@ def main():
         return main_soft(args)
     if args.workload == "seeds":
         return main_seeds(args)
-    from pcc.engine import copy_bandwidth
+    from pcc.engine import copy_bandwidth, read_bandwidth
     copy_gbs = copy_bandwidth(1 << 30, 10)
+    read_gbs = read_bandwidth(1 << 31, 10)
     cols = [G.dense_colony(args.n, seed=args.seed + q) for q in range(args.instances)]
     bodies = cols[0]
     ncell = sum(c.n for c in cols)
@@ -796,11 +805,12 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": rf["gbs"] / HBM_PEAK_GBS,
-            "traffic": traffic_from_profile(rf["kernel"], len(cols)),
+            "traffic": traffic_from_profile(rf["kernel"], len(cols), schedule(bodies.n)),
             "avg_launch_us": rf["avg_us"],
             "algorithmic_bytes_per_launch": rf["bytes"],
             "achievable_copy_gbs": copy_gbs,
-            "frac_of_achievable": rf["gbs"] / copy_gbs,
+            "achievable_read_gbs": read_gbs,
+            "frac_of_achievable_read": rf["gbs"] / read_gbs,
         },
     }
     if "mixed" in out:

@@ -405,6 +405,13 @@ pcc_status pcc_device_bytes(pcc_ctx* ctx, int64_t* bytes);
  * counterpart. */
 pcc_status pcc_measure_copy_bandwidth(int device, int64_t bytes, int reps, double* gbs);
 
+/* Achievable device read bandwidth (bytes read per second / 1e9): a
+ * grid-stride sum over `bytes` bytes, four independent 32-byte loads in
+ * flight per thread, averaged over `reps` launches.  The BBPGD applies are
+ * read-dominated, so this is their practical ceiling.  No reference
+ * counterpart. */
+pcc_status pcc_measure_read_bandwidth(int device, int64_t bytes, int reps, double* gbs);
+
 /* Per-kernel GPU time accounting with HIP events on the context stream.
  * Kernel ids: see pcc_kernel_name(). */
 enum {

@@ -84,6 +84,7 @@ def lib():
         L.pcc_batch_divide.argtypes = [vp, C.POINTER(C.c_uint64), C.c_int32, C.c_double,
                                        C.c_double, C.POINTER(i64)]
         L.pcc_measure_copy_bandwidth.argtypes = [C.c_int, i64, C.c_int, C.POINTER(C.c_double)]
+        L.pcc_measure_read_bandwidth.argtypes = [C.c_int, i64, C.c_int, C.POINTER(C.c_double)]
         L.pcc_set_trace.argtypes = [vp, C.c_int, i64]
         L.pcc_get_trace.argtypes = [vp, i64, vp, i64, C.POINTER(i64)]
         for f in ("pcc_create", "pcc_set_params", "pcc_set_state", "pcc_get_state",
@@ -93,8 +94,8 @@ def lib():
                   "pcc_batch_set_params", "pcc_batch_size", "pcc_batch_step",
                   "pcc_batch_get_state", "pcc_batch_get_constraints",
                   "pcc_batch_get_phase_stats", "pcc_device_bytes",
-                  "pcc_measure_copy_bandwidth", "pcc_divide", "pcc_batch_divide",
-                  "pcc_get_tuning", "pcc_adapt_dt", "pcc_batch_adapt_dt", "pcc_set_trace",
+                  "pcc_measure_copy_bandwidth", "pcc_measure_read_bandwidth", "pcc_divide",
+                  "pcc_batch_divide", "pcc_get_tuning", "pcc_adapt_dt", "pcc_batch_adapt_dt", "pcc_set_trace",
                   "pcc_get_trace", "pcc_batch_get_step_count", "pcc_batch_set_step_count"):
             getattr(L, f).restype = C.c_int
         _lib = L
@@ -108,7 +109,8 @@ EXPORTED = ("pcc_default_params", "pcc_create", "pcc_destroy", "pcc_set_params",
             "pcc_batch_set_state", "pcc_batch_add", "pcc_batch_remove", "pcc_batch_set_params",
             "pcc_batch_size", "pcc_batch_step", "pcc_batch_get_state",
             "pcc_batch_get_constraints", "pcc_batch_get_phase_stats", "pcc_device_bytes",
-            "pcc_measure_copy_bandwidth", "pcc_divide", "pcc_batch_divide", "pcc_get_tuning",
+            "pcc_measure_copy_bandwidth", "pcc_measure_read_bandwidth", "pcc_divide",
+            "pcc_batch_divide", "pcc_get_tuning",
             "pcc_adapt_dt", "pcc_batch_adapt_dt", "pcc_set_trace", "pcc_get_trace",
             "pcc_batch_get_step_count", "pcc_batch_set_step_count")
 
@@ -119,6 +121,15 @@ def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 10, device: int = 0) -> fl
     st = lib().pcc_measure_copy_bandwidth(int(device), int(nbytes), int(reps), C.byref(g))
     if st != T.PCC_OK:
         raise PccError(st, "pcc_measure_copy_bandwidth failed")
+    return float(g.value)
+
+
+def read_bandwidth(nbytes: int = 1 << 30, reps: int = 10, device: int = 0) -> float:
+    """Achievable device read bandwidth in GB/s (bytes read)."""
+    g = C.c_double(0)
+    st = lib().pcc_measure_read_bandwidth(int(device), int(nbytes), int(reps), C.byref(g))
+    if st != T.PCC_OK:
+        raise PccError(st, "pcc_measure_read_bandwidth failed")
     return float(g.value)
 
 

@@ -17,13 +17,23 @@ def test_traffic_comes_from_the_committed_counter_summary():
     want = None
     for line in open(bench.PMC_SUMMARY):
         f = line.split()
-        if len(f) == 4 and f[:3] == ["traffic_bytes_per_launch", "body_apply", "instances=2"]:
-            want = float(f[3])
+        if len(f) == 5 and f[:4] == ["traffic_bytes_per_launch", "body_apply", "instances=2",
+                                     "schedule=solo"]:
+            want = float(f[4])
     assert want is not None and want > 0
-    assert bench.traffic_from_profile("body_apply", 2) == want
+    assert bench.traffic_from_profile("body_apply", 2, "solo") == want
     # another configuration has no committed summary: no figure rather than a wrong one
-    assert bench.traffic_from_profile("body_apply", 3) is None
-    assert bench.traffic_from_profile("no_such_kernel", 2) is None
+    assert bench.traffic_from_profile("body_apply", 3, "solo") is None
+    assert bench.traffic_from_profile("body_apply", 2, "lockstep") is None
+    assert bench.traffic_from_profile("no_such_kernel", 2, "solo") is None
+
+
+def test_schedule_follows_the_solo_threshold(monkeypatch):
+    monkeypatch.delenv("PCC_SOLO_CELLS", raising=False)
+    assert bench.schedule(300_000) == "solo"
+    assert bench.schedule(100_000) == "lockstep"
+    monkeypatch.setenv("PCC_SOLO_CELLS", "0")
+    assert bench.schedule(300_000) == "lockstep"
 
 
 def test_roofline_names_the_kernel_with_the_largest_share():
