@@ -600,6 +600,7 @@ def main_serving(args):
                        warm_steps=args.warm_steps)
     st = srv.run(reqs, time_limit=args.duration + args.serve_limit,
                  progress=lambda m: print(f"[serving] {m}", file=sys.stderr, flush=True))
+    batches = srv.step_summary()
     srv.close()
     waits = [r.admitted - r.arrival for r in srv.results.values()]
     # per-step latency of completed requests, split into warm-up and steady steps
@@ -679,6 +680,8 @@ def main_serving(args):
         "growth_factor": {"mean": float(np.mean(grown)) if grown else 0.0,
                           "max": float(max(grown)) if grown else 0.0},
         "per_model": modes,
+        # per batch "model-bucket-stage" (model 0 hard, 1 soft; stage 0 warm-up, 1 steady)
+        "batches": batches,
     }
     print(json.dumps(line))
 

@@ -160,6 +160,9 @@ class ColonyServer:
         self.peak_device_bytes = 0
         self.batch_steps = 0
         self.cell_steps = 0
+        # per batch step: (key, colonies, cells, divide ms, step ms, most BBPGD
+        # iterations of a colony, most ReLCP phases of a colony)
+        self.step_log: List[tuple] = []
         self._reserved = 0
         self._cells = {T.MODE_HARD: 0, T.MODE_SOFT: 0}  # resident cells per model
         self.peak_cells = {T.MODE_HARD: 0, T.MODE_SOFT: 0}
@@ -279,16 +282,21 @@ class ColonyServer:
         if self.divide:
             b.divide([division_seed(m.req.rid, m.done) for m in ms], 0, self.length_noise,
                      self.angle_noise)
+        td = time.perf_counter()
         res = b.step(self.batch_steps)
         t1 = time.perf_counter()
         dt_ms = (t1 - t) * 1e3
         cells = 0
+        sizes = b.sizes
         for q, m in enumerate(ms):
             m.done += 1
             m.step_ms.append(dt_ms)
             m.statuses.append(res[q]["status"])
-            cells += b.sizes[q]
+            cells += sizes[q]
         dev = b.device_bytes()
+        self.step_log.append((key, len(ms), cells, (td - t) * 1e3, (t1 - td) * 1e3,
+                              max(r["bbpgd_iterations"] for r in res),
+                              max(r["constraint_iterations"] for r in res)))
         gone = []  # (q, member, final state or None if moved), q descending
         for q in range(len(ms) - 1, -1, -1):
             m = ms[q]
@@ -409,6 +417,29 @@ class ColonyServer:
             "peak_device_bytes": int(self.peak_device_bytes),
             "budget_bytes": self.budget,
         }
+
+    def step_summary(self) -> dict:
+        """Per batch (model-bucket-stage): its batch steps with the median
+        colonies and cells, the divide and step times and the most BBPGD
+        iterations of a colony per step (p50 / p99)."""
+        out = {}
+        for key in self.keys:
+            rows = [r for r in self.step_log if r[0] == key]
+            if not rows:
+                continue
+            a = np.array([r[1:] for r in rows], dtype=float)
+            out["-".join(str(x) for x in key)] = {
+                "batch_steps": len(rows),
+                "colonies_p50": float(np.percentile(a[:, 0], 50)),
+                "cells_p50": float(np.percentile(a[:, 1], 50)),
+                "divide_ms_p50": float(np.percentile(a[:, 2], 50)),
+                "step_ms_p50": float(np.percentile(a[:, 3], 50)),
+                "step_ms_p99": float(np.percentile(a[:, 3], 99)),
+                "max_iterations_p50": float(np.percentile(a[:, 4], 50)),
+                "max_iterations_p99": float(np.percentile(a[:, 4], 99)),
+                "max_phases_p99": float(np.percentile(a[:, 5], 99)),
+            }
+        return out
 
     def close(self):
         for k, b in self.batches.items():
