@@ -27,7 +27,12 @@ Variants of the applies D^T gamma -> U = M F -> D U:
                 library's mixed path, whose iterate record is gamma (FP64)
                 and g (FP32);
   fp32_angle_g  fp32_store_g with each (planar) contact normal stored as its
-                FP32 angle and rebuilt as (cos, sin) in FP32 arithmetic.
+                FP32 angle and rebuilt as (cos, sin) in FP32 arithmetic;
+  fp32_minor_g  fp32_store_g with each (planar) contact normal stored as its
+                smaller-magnitude component in FP32 with the two low mantissa
+                bits cleared (they carry which component it is and the sign of
+                the other) and the other component rebuilt as
+                sqrt(1 - u^2) in FP32 (1 - u^2 >= 1/2, so well conditioned).
 
 Run: make -C oracle && python oracle/precision_study.py [--n 3000] [--seed 7]
 """
@@ -52,6 +57,22 @@ def angle_normals(normal):
     out = np.zeros_like(normal)
     out[:, 0] = np.cos(t).astype(np.float64)
     out[:, 1] = np.sin(t).astype(np.float64)
+    return out
+
+
+def minor_normals(normal):
+    """Planar normals through their smaller component u (FP32, 2 low mantissa
+    bits cleared) and the other rebuilt as sign * sqrt(1 - u u) in FP32."""
+    nx, ny = normal[:, 0], normal[:, 1]
+    ux = np.abs(nx) <= np.abs(ny)
+    u = np.where(ux, nx, ny).astype(np.float32)
+    u = (u.view(np.uint32) & np.uint32(0xFFFFFFFC)).view(np.float32)
+    ud = u.astype(np.float64)
+    w = (1.0 - ud * ud).astype(np.float32)  # one rounding of the exact 1 - u^2
+    v = np.sqrt(w).astype(np.float64) * np.where(np.where(ux, ny, nx) < 0, -1.0, 1.0)
+    out = np.zeros_like(normal)
+    out[:, 0] = np.where(ux, ud, v)
+    out[:, 1] = np.where(ux, v, ud)
     return out
 
 
@@ -87,11 +108,11 @@ def make_grad(D, m, phi, dt, variant):
         phi32, dt32 = phi.astype(np.float32), np.float32(dt)
         return lambda gam: (phi32 + dt32 * (D32 @ (m32 * (DT32 @ gam.astype(np.float32))))) \
             .astype(np.float64)
-    if variant in ("fp32_storage", "fp32_store_g", "fp32_angle_g"):
+    if variant in ("fp32_storage", "fp32_store_g", "fp32_angle_g", "fp32_minor_g"):
         D = D.astype(np.float32).astype(np.float64)
         m = m.astype(np.float32).astype(np.float64)
     DT = D.T.tocsr()
-    if variant in ("fp32_store_g", "fp32_angle_g"):
+    if variant in ("fp32_store_g", "fp32_angle_g", "fp32_minor_g"):
         phi = phi.astype(np.float32).astype(np.float64)
         return lambda gam: (phi + dt * (D @ (m * (DT @ gam)))).astype(np.float32).astype(np.float64)
     if variant == "fp32_u":
@@ -147,8 +168,10 @@ def main():
           f"max overlap {-phi.min():.3f}", flush=True)
     ref = None
     Da, _ = operator(b, cons, p.xi, angle_normals(cons["normal"]))
-    for v in ("fp64", "fp32_acc", "fp32_u", "fp32_storage", "fp32_store_g", "fp32_angle_g"):
-        Dv = Da if v == "fp32_angle_g" else D
+    Dm, _ = operator(b, cons, p.xi, minor_normals(cons["normal"]))
+    for v in ("fp64", "fp32_acc", "fp32_u", "fp32_storage", "fp32_store_g", "fp32_angle_g",
+              "fp32_minor_g"):
+        Dv = Da if v == "fp32_angle_g" else Dm if v == "fp32_minor_g" else D
         gam, its, res = bbpgd(make_grad(Dv, m, phi, p.dt, v), gamma0, a.tol, a.max_its)
         if v == "fp64":
             ref = gam
