@@ -334,7 +334,6 @@ struct pcc_ctx {
   pcc::DArr<double> iter;                  // 2 per constraint: (gamma, g), solver order
   pcc::DArr<double> gprev;                 // lagging iterate at convergence checks
   pcc::DArr<double> ubuf;                  // 8 per body: U[6], dldot, pad
-  pcc::DArr<double> mob;                   // 2 per body: 1/(xi l), 12/(xi l^3)
   pcc::DArr<double> partials;              // 4 * nblocks (5 while tracing)
   // BBPGD trace: per colony `trace_cap` records and a count per step
   bool trace_on = false;

@@ -264,15 +264,12 @@ __global__ void k_inc_records(int64_t E, const int32_t* __restrict__ inc,
   }
 }
 
-__global__ void k_mobility(int64_t N, const double* __restrict__ len,
-                           const int32_t* __restrict__ binst, const InstDev* __restrict__ idev,
-                           double* __restrict__ mob) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const double xi = idev[binst[i]].xi;
-  double l = len[i];
-  mob[2 * i] = 1.0 / (l * xi);               // Physics.cpp:203
-  mob[2 * i + 1] = 12.0 / (l * l * l * xi);  // Physics.cpp:204
+// Translational and rotational mobility of a body of length l (Physics.cpp:
+// 203-204), evaluated where a kernel needs it from the length it reads anyway
+// (the same two correctly rounded divisions in every kernel).
+__device__ __forceinline__ void mobility(double l, double xi, double& mt, double& mr) {
+  mt = 1.0 / (l * xi);
+  mr = 12.0 / (l * l * l * xi);
 }
 
 // Brownian angular velocities of the step (calculate_external_velocities,
@@ -324,7 +321,6 @@ struct BodyArgs {
   const double* fold;  // mixed kernels: D^T gamma_old per body (3D: 6, planar: x, y, z-torque)
   double* it;          // iterate records (gamma, g), solver order (mixed: gamma)
   float* itg;          // mixed: g, solver order
-  const double* mob;
   const double* len;
   const double* ldot_prev;
   const SolverState* st;  // one per colony
@@ -686,11 +682,13 @@ __device__ __forceinline__ void body_block(const BodyArgs& a, int64_t l,
       continue;
     }
     if (FOLD) unfold(a.fold, i, F[r], PLANAR);
-    const double mt = a.mob[2 * i], mr = a.mob[2 * i + 1];
+    const double li = a.len[i];
+    double mt, mr;
+    mobility(li, D.xi, mt, mr);
     double u[6];
     body_velocity(a.uext, i, mt, mr, F[r], u);
     const double imp = det_exp(sig[r] * D.mlam);
-    const double ld = (a.len[i] * imp) * D.itau;
+    const double ld = (li * imp) * D.itau;
     if (MODE == 1) {
 #pragma unroll
       for (int q = 0; q < 6; ++q) {
@@ -1114,7 +1112,9 @@ __global__ void __launch_bounds__(256) k_energy_body(BodyArgs a, double* __restr
       }
       sig += gc * co[6];
     }
-    const double mt = a.mob[2 * i], mr = a.mob[2 * i + 1];
+    const double li = a.len[i];
+    double mt, mr;
+    mobility(li, D.xi, mt, mr);
     double s = 0.0;
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
@@ -1122,7 +1122,7 @@ __global__ void __launch_bounds__(256) k_energy_body(BodyArgs a, double* __restr
       s += F[q] * u;
     }
     q2 = s;
-    q3 = a.len[i] * det_exp(sig * D.mlam);
+    q3 = li * det_exp(sig * D.mlam);
   }
   const double s2 = block_sum(q2, lds);
   const double s3 = block_sum(q3, lds);
@@ -1272,11 +1272,13 @@ __global__ void __launch_bounds__(kPersistThreads)
       }
       if (active) {
         if (FOLD) unfold(ba.fold, i, F, PLANAR);
-        const double mt = ba.mob[2 * i], mr = ba.mob[2 * i + 1];
+        const double li = ba.len[i];
+        double mt, mr;
+        mobility(li, D.xi, mt, mr);
         double u[6];
         body_velocity(ba.uext, i, mt, mr, F, u);
         const double imp = det_exp(sig * D.mlam);
-        const double ld = (ba.len[i] * imp) * D.itau;
+        const double ld = (li * imp) * D.itau;
         if (PLANAR) {
           double2* w = reinterpret_cast<double2*>(ba.ubuf + 4 * i);
           w[0] = make_double2(u[0], u[1]);
@@ -1751,7 +1753,6 @@ static BodyArgs body_args(pcc_ctx* ctx) {
   a.fold = ctx->fold.p;
   a.it = ctx->iter.p;
   a.itg = iter_g(ctx);
-  a.mob = ctx->mob.p;
   a.len = b.len;
   a.ldot_prev = b.ldot;
   a.st = ctx->state.p;
@@ -1797,14 +1798,15 @@ double body_bytes(int64_t N, int64_t C, bool mixed, bool planar) {
   // per incidence: record (8 or 4 coefficient slots [+ 4-byte index]) +
   // the iterate record's (gamma, g) (FP64 16 bytes; mixed 8 + 4) +
   // gamma_old (FP64 only)
-  // per body: rowptr, mobility (2), length, ldot_prev, velocity record out,
-  // and in mixed precision D^T gamma_old (6 or 3 doubles)
+  // per body: rowptr, length, ldot_prev, velocity record out, and in mixed
+  // precision D^T gamma_old (6 or 3 doubles); the mobility is evaluated from
+  // the length
   const double rec = planar ? 4 * sc + 4 : 8 * sc;
   const double ub = planar ? 32 : 64;
   const double yg = mixed ? 12 : 16;
   const double gold_e = mixed ? 0 : 8;
   const double fold = mixed ? (planar ? 24 : 48) : 0;
-  return 2.0 * C * (rec + yg + gold_e) + N * (4 + 16 + 16 + ub + fold);
+  return 2.0 * C * (rec + yg + gold_e) + N * (4 + 16 + ub + fold);
 }
 double constr_bytes(int64_t N, int64_t C, bool mixed, bool planar) {
   const double sc = mixed ? 4 : 8;
@@ -1966,14 +1968,10 @@ void bbpgd_solve(pcc_ctx* ctx) {
   hipStream_t s = ctx->stream;
   solver_layout(ctx);
   ctx->ubuf.ensure(8 * N + 8);
-  ctx->mob.ensure(2 * N + 2);
   ctx->partials.ensure((ctx->trace_on ? 5 : 4) * std::max<int64_t>(ctx->ncb_total, 1) + 4);
   if (ctx->trace_on) ctx->epart.ensure(2 * ctx->nbb_total + 2);
   ctx->state.ensure(B);
   ensure_state_host(ctx, 2 * B);
-  BodyArrays b = body_arrays(ctx->body.p, N);
-  if (N > 0)
-    k_mobility<<<nblk(N), 256, 0, s>>>(N, b.len, ctx->binst.p, ctx->idev.p, ctx->mob.p);
   {
     std::vector<SolverState> init(B);
     memset(init.data(), 0, sizeof(SolverState) * B);
