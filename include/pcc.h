@@ -365,6 +365,12 @@ pcc_status pcc_batch_divide(pcc_ctx* ctx, const uint64_t* seeds, int32_t step,
 pcc_status pcc_get_tuning(pcc_ctx* ctx, int32_t* variant, int32_t* body_chunk,
                           int32_t* body_threads, int32_t* tuned, double* variant_us);
 
+/* Counters of the context's multi-kernel BBPGD solves since its creation:
+ * *compactions = times a solve's grids were rebuilt over the colonies still
+ * iterating (once those hold at most half of the body blocks).  Any output
+ * pointer may be NULL.  No reference counterpart. */
+pcc_status pcc_get_solve_stats(pcc_ctx* ctx, int64_t* compactions);
+
 /* CFL time-step adaptation between steps (Domain::adaptDt, Domain.cpp:69-118),
  * on the device: the median over the colony's cells of |v| + ldot after the
  * last step (mean of the two middle values for an even count), then

@@ -763,6 +763,9 @@ pcc_status pcc_create(const pcc_params* params, int device, pcc_ctx** out) {
   if (const char* v = getenv("PCC_PERSIST_MAX_C")) ctx->persist_max_c = atoll(v);
   if (const char* v = getenv("PCC_PERSIST_MAX_N")) ctx->persist_max_n = atoll(v);
   if (const char* v = getenv("PCC_SOLO_CELLS")) ctx->solo_cells = atoll(v);
+  const char* compact_env = getenv("PCC_COMPACT");
+  ctx->compact = !(compact_env && compact_env[0] == '0');
+  if (const char* v = getenv("PCC_COMPACT_MIN_BLOCKS")) ctx->compact_min_blocks = atoll(v);
   // PCC_BODY_VARIANT=v (0..PCC_BODY_VARIANTS-1) fixes the body-kernel variant
   if (const char* v = getenv("PCC_BODY_VARIANT")) {
     const int x = atoi(v);
@@ -1238,6 +1241,12 @@ pcc_status pcc_get_tuning(pcc_ctx* ctx, int32_t* variant, int32_t* body_chunk,
   if (tuned) *tuned = ctx->body_tuned ? (ctx->body_from_cache ? 2 : 1) : 0;
   if (variant_us)
     for (int v = 0; v < kBodyVariants; ++v) variant_us[v] = ctx->body_variant_ms[v] * 1e3;
+  return PCC_OK;
+}
+
+pcc_status pcc_get_solve_stats(pcc_ctx* ctx, int64_t* compactions) {
+  if (!ctx) return PCC_E_INVALID_ARGUMENT;
+  if (compactions) *compactions = ctx->compactions;
   return PCC_OK;
 }
 

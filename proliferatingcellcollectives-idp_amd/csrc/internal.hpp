@@ -261,6 +261,12 @@ struct pcc_ctx {
   // ReLCP loop alone, one after another; the others share one lockstep loop
   // (PCC_SOLO_CELLS=n sets the threshold, 0 keeps every colony in lockstep)
   int64_t solo_cells = 150000;
+  // BBPGD grids rebuilt over the colonies still iterating once they hold at
+  // most half of the body blocks, for solves of at least compact_min_blocks
+  // body blocks (PCC_COMPACT=0: off; PCC_COMPACT_MIN_BLOCKS=n)
+  bool compact = true;
+  int64_t compact_min_blocks = 64;
+  int64_t compactions = 0;
   // autotuner (PCC_AUTOTUNE=0: off): body-kernel staging size
   bool autotune = true;
   bool body_tuned = false;

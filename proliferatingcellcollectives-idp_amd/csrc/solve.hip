@@ -63,6 +63,7 @@ void seg_sort_i32(pcc_ctx* ctx, int64_t nseg, const int32_t* start, int32_t* v);
 static constexpr int kBbGridMax = 2048;
 static constexpr int64_t kTuneMinConstraints = 100000;
 
+
 namespace {
 
 inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
@@ -1699,12 +1700,13 @@ void build_incidence(pcc_ctx* ctx, bool records) {
 // colony b gets ceil(N_b / 256) body blocks and G_b = min(Q_b, 2048) blocks
 // of K_b = ceil(Q_b / G_b) constraint tiles (Q_b = ceil(C_b / 256); a colony
 // with no constraints still gets one block, as a lone colony does).
-static void solver_layout(pcc_ctx* ctx) {
+// Logical blocks of the active colonies (with `keep`, only those it marks).
+static void solver_layout(pcc_ctx* ctx, const std::vector<char>* keep = nullptr) {
   std::vector<int32_t> bm, cm;
   int64_t lb = 0, bb = 0;
   for (size_t q = 0; q < ctx->inst.size(); ++q) {
     Instance& I = ctx->inst[q];
-    if (!(I.flags & kInstActive)) {
+    if (!(I.flags & kInstActive) || (keep && !(*keep)[q])) {
       I.G = 1;
       I.K = 0;
       I.lb = 0;
@@ -1984,7 +1986,7 @@ void bbpgd_solve(pcc_ctx* ctx) {
   if (mixed && N > 0) ctx->fold.ensure(6 * N + 6);
   BodyArgs ba = body_args(ctx);
   ConstrArgs ca = constr_args(ctx);
-  const unsigned gb = (unsigned)ba.gp, gc = (unsigned)ca.gp;
+  unsigned gb = (unsigned)ba.gp, gc = (unsigned)ca.gp;
   if (mixed && N > 0) {
     // D^T gamma_old of every body of the solve, once per phase
     FinalOut fo{};
@@ -2129,7 +2131,7 @@ void bbpgd_solve(pcc_ctx* ctx) {
   // (kernel nodes only; the state read-back below stays outside it), unless
   // per-kernel event timing is on.
   hipGraphExec_t exec = nullptr;
-  if (ctx->use_graphs && !ctx->timing.enabled) {
+  auto capture = [&]() {
     DeviceLock lock(device_mutex());
     hipGraph_t graph = nullptr;
     PCC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -2138,16 +2140,23 @@ void bbpgd_solve(pcc_ctx* ctx) {
     const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     PCC_HIP(e);
-  }
+  };
+  if (ctx->use_graphs && !ctx->timing.enabled) capture();
+  // graphs replaced after a compaction (possibly still in flight): destroyed
+  // after the final synchronisation
+  std::vector<hipGraphExec_t> retired;
   struct ExecGuard {
     hipGraphExec_t& e;
+    std::vector<hipGraphExec_t>& r;
     ~ExecGuard() {
-      if (e) {
+      if (e || !r.empty()) {
         DeviceLock lock(device_mutex());
-        (void)hipGraphExecDestroy(e);
+        if (e) (void)hipGraphExecDestroy(e);
+        for (auto x : r) (void)hipGraphExecDestroy(x);
       }
     }
-  } guard{exec};
+  } guard{exec, retired};
+  bool compacted = false;
   SolverState* hs = ctx->state_host;
   hipEvent_t ev[2];
   {
@@ -2169,12 +2178,41 @@ void bbpgd_solve(pcc_ctx* ctx) {
       PCC_HIP(hipEventSynchronize(ev[(j - 1) & 1]));
       const SolverState* h = hs + ((j - 1) & 1) * B;
       done = true;
-      for (int64_t q = 0; q < B && done; ++q)
-        if (!(h[q].skip | h[q].converged | h[q].failed)) done = false;
+      int64_t live_bb = 0;
+      for (int64_t q = 0; q < B; ++q)
+        if (!(h[q].skip | h[q].converged | h[q].failed)) {
+          done = false;
+          live_bb += ctx->inst[q].nbb;
+        }
+      // Active-set compaction: once the colonies still iterating hold at most
+      // half of the launches' body blocks, the grids are rebuilt over them
+      // alone (the finished colonies' blocks would only exit).  A colony's
+      // logical blocks and sums are the same in any layout, so no result
+      // changes; the maps are rewritten in stream order, after the chunk in
+      // flight, which still uses the old ones.
+      if (!done && ctx->compact && ctx->nbb_total >= ctx->compact_min_blocks &&
+          2 * live_bb <= ctx->nbb_total) {
+        std::vector<char> keep(B, 0);
+        for (int64_t q = 0; q < B; ++q)
+          keep[q] = !(h[q].skip | h[q].converged | h[q].failed);
+        solver_layout(ctx, &keep);
+        ba = body_args(ctx);
+        ca = constr_args(ctx);
+        gb = (unsigned)ba.gp;
+        gc = (unsigned)ca.gp;
+        if (exec) {
+          retired.push_back(exec);
+          exec = nullptr;
+          capture();
+        }
+        compacted = true;
+        ++ctx->compactions;
+      }
     }
     ++j;
   }
   PCC_HIP(hipStreamSynchronize(s));
+  if (compacted) solver_layout(ctx);  // every active colony's blocks, for the integration
   {
     DeviceLock lock(device_mutex());
     (void)hipEventDestroy(ev[0]);

@@ -85,6 +85,7 @@ def lib():
                                        C.c_double, C.POINTER(i64)]
         L.pcc_measure_copy_bandwidth.argtypes = [C.c_int, i64, C.c_int, C.POINTER(C.c_double)]
         L.pcc_measure_read_bandwidth.argtypes = [C.c_int, i64, C.c_int, C.POINTER(C.c_double)]
+        L.pcc_get_solve_stats.argtypes = [vp, C.POINTER(i64)]
         L.pcc_set_trace.argtypes = [vp, C.c_int, i64]
         L.pcc_get_trace.argtypes = [vp, i64, vp, i64, C.POINTER(i64)]
         for f in ("pcc_create", "pcc_set_params", "pcc_set_state", "pcc_get_state",
@@ -95,8 +96,9 @@ def lib():
                   "pcc_batch_get_state", "pcc_batch_get_constraints",
                   "pcc_batch_get_phase_stats", "pcc_device_bytes",
                   "pcc_measure_copy_bandwidth", "pcc_measure_read_bandwidth", "pcc_divide",
-                  "pcc_batch_divide", "pcc_get_tuning", "pcc_adapt_dt", "pcc_batch_adapt_dt", "pcc_set_trace",
-                  "pcc_get_trace", "pcc_batch_get_step_count", "pcc_batch_set_step_count"):
+                  "pcc_batch_divide", "pcc_get_tuning", "pcc_get_solve_stats", "pcc_adapt_dt",
+                  "pcc_batch_adapt_dt", "pcc_set_trace", "pcc_get_trace",
+                  "pcc_batch_get_step_count", "pcc_batch_set_step_count"):
             getattr(L, f).restype = C.c_int
         _lib = L
     return _lib
@@ -110,7 +112,7 @@ EXPORTED = ("pcc_default_params", "pcc_create", "pcc_destroy", "pcc_set_params",
             "pcc_batch_size", "pcc_batch_step", "pcc_batch_get_state",
             "pcc_batch_get_constraints", "pcc_batch_get_phase_stats", "pcc_device_bytes",
             "pcc_measure_copy_bandwidth", "pcc_measure_read_bandwidth", "pcc_divide",
-            "pcc_batch_divide", "pcc_get_tuning",
+            "pcc_batch_divide", "pcc_get_tuning", "pcc_get_solve_stats",
             "pcc_adapt_dt", "pcc_batch_adapt_dt", "pcc_set_trace", "pcc_get_trace",
             "pcc_batch_get_step_count", "pcc_batch_set_step_count")
 
@@ -213,6 +215,12 @@ class ParticleManager:
                 "body_threads": int(th.value), "prefetch": int(va.value) >= 6,
                 "tuned": bool(tu.value), "from_cache": int(tu.value) == 2,
                 "variant_us": [float(x) for x in us]}
+
+    def solve_stats(self) -> dict:
+        """Counters of this context's BBPGD solves: grid compactions."""
+        n = C.c_int64(0)
+        self._check(lib().pcc_get_solve_stats(self._ctx, C.byref(n)))
+        return {"compactions": int(n.value)}
 
     def set_trace(self, enable: bool = True, capacity: int = 100000):
         """BBPGD trace of the following steps (the reference's bbpgd_logger):

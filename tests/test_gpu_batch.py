@@ -88,6 +88,27 @@ def test_solo_colonies_equal_alone(precision, monkeypatch):
     check_batch_equals_alone(params[0], colonies, params, steps=2)
 
 
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_compacted_grids_equal_alone(precision, monkeypatch):
+    """Eight 3,000-cell colonies, half of them with a looser tolerance: once
+    the loose ones have converged, the solve rebuilds its grids over the
+    colonies still iterating (compaction from 8 body blocks on here).  Every
+    colony still equals its alone run bit for bit, and the batch did compact."""
+    monkeypatch.setenv("PCC_COMPACT_MIN_BLOCKS", "8")
+    print(f"[batch-compact] precision={precision}", flush=True)
+    from pcc.engine import ColonyBatch
+    colonies = [G.packed_colony(3000, seed=60 + i) for i in range(8)]
+    params = [hard(precision=precision, tolerance=1e-3 if i % 2 else 1e-6) for i in range(8)]
+    check_batch_equals_alone(params[0], colonies, params, steps=2)
+    cb = ColonyBatch(params[0])
+    cb.set_state(colonies, params)
+    cb.step(0)
+    n = cb.solve_stats()["compactions"]
+    print(f"  compactions in one step: {n}", flush=True)
+    assert n >= 1
+    cb.close()
+
+
 def test_batch_solver_failures_stay_per_colony():
     """One colony exhausts BBPGD, one exhausts ReLCP phases, one is empty; the
     others are unaffected and every status is the one it has alone."""
