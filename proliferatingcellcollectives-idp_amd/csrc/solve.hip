@@ -2185,13 +2185,14 @@ void bbpgd_solve(pcc_ctx* ctx) {
           live_bb += ctx->inst[q].nbb;
         }
       // Active-set compaction: once the colonies still iterating hold at most
-      // half of the launches' body blocks, the grids are rebuilt over them
-      // alone (the finished colonies' blocks would only exit).  A colony's
+      // compact_ratio (default one half) of the launches' body blocks, the
+      // grids are rebuilt over them alone (the finished colonies' blocks
+      // would only exit).  A colony's
       // logical blocks and sums are the same in any layout, so no result
       // changes; the maps are rewritten in stream order, after the chunk in
       // flight, which still uses the old ones.
       if (!done && ctx->compact && ctx->nbb_total >= ctx->compact_min_blocks &&
-          2 * live_bb <= ctx->nbb_total) {
+          (double)live_bb <= ctx->compact_ratio * (double)ctx->nbb_total) {
         std::vector<char> keep(B, 0);
         for (int64_t q = 0; q < B; ++q)
           keep[q] = !(h[q].skip | h[q].converged | h[q].failed);
