@@ -87,7 +87,15 @@ def test_trace_capacity_truncates_but_counts():
     pm.close()
 
 
-def test_batch_traces_equal_alone():
+@pytest.mark.parametrize("compact", [False, True], ids=["full_grids", "compacted"])
+def test_batch_traces_equal_alone(compact, monkeypatch):
+    """Per-colony traces of a batch equal the alone traces; with compaction
+    forced (from one body block on, as soon as one colony has finished), the
+    grids are rebuilt mid-solve, and the energy partials and block partials
+    the trace reads follow the layout."""
+    if compact:
+        monkeypatch.setenv("PCC_COMPACT_MIN_BLOCKS", "1")
+        monkeypatch.setenv("PCC_COMPACT_RATIO", "0.95")
     from pcc.engine import ColonyBatch, ParticleManager
     cols = [G.dense_colony(n, seed=230 + i) for i, n in enumerate([900, 2000, 150])]
     ps = [reference_params(dt=1e-2 * (1 + 0.3 * i), tolerance=1e-5, tau=1 / LN2)
@@ -96,6 +104,8 @@ def test_batch_traces_equal_alone():
     cb.set_state(cols, ps)
     cb.set_trace(True, 50000)
     cb.step(0)
+    if compact:
+        assert cb.solve_stats()["compactions"] >= 1
     for q in range(3):
         pm = ParticleManager(ps[q])
         pm.set_state(cols[q])
