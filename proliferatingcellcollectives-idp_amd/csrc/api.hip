@@ -766,6 +766,10 @@ pcc_status pcc_create(const pcc_params* params, int device, pcc_ctx** out) {
   const char* compact_env = getenv("PCC_COMPACT");
   ctx->compact = !(compact_env && compact_env[0] == '0');
   if (const char* v = getenv("PCC_COMPACT_MIN_BLOCKS")) ctx->compact_min_blocks = atoll(v);
+  if (const char* v = getenv("PCC_CHUNK_ITERS")) {
+    const int c = atoi(v);
+    if (c >= 1 && c <= 1000) ctx->chunk_iters = c;
+  }
   if (const char* v = getenv("PCC_COMPACT_RATIO")) {
     const double r = atof(v);
     if (r > 0.0 && r < 1.0) ctx->compact_ratio = r;

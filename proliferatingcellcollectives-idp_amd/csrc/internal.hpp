@@ -269,6 +269,8 @@ struct pcc_ctx {
   bool compact = true;
   int64_t compact_min_blocks = 64;
   double compact_ratio = 0.5;  // PCC_COMPACT_RATIO
+  // BBPGD iterations per speculative chunk (one captured graph; PCC_CHUNK_ITERS)
+  int chunk_iters = 10;
   int64_t compactions = 0;
   // autotuner (PCC_AUTOTUNE=0: off): body-kernel staging size
   bool autotune = true;

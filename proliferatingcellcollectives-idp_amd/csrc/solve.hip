@@ -62,6 +62,8 @@ void seg_sort_i32(pcc_ctx* ctx, int64_t nseg, const int32_t* start, int32_t* v);
 // the incidence records.
 static constexpr int kBbGridMax = 2048;
 static constexpr int64_t kTuneMinConstraints = 100000;
+// chunks after which a BBPGD solve's chunks double in length
+static constexpr int64_t kChunkGrowAfter = 10;
 
 
 namespace {
@@ -2126,10 +2128,13 @@ void bbpgd_solve(pcc_ctx* ctx) {
       }
     }
   }
-  const int chunk = 10;
   // The chunk of `chunk` iterations is launched as one captured HIP graph
   // (kernel nodes only; the state read-back below stays outside it), unless
-  // per-kernel event timing is on.
+  // per-kernel event timing is on.  A solve still running after
+  // kChunkGrowAfter chunks switches to chunks twice as long: fewer graph
+  // launches and flag read-backs for long solves (2 x 300k colonies: 738 ->
+  // 731 ms per step), the short chunks' early stop for short ones.
+  int chunk = ctx->chunk_iters;
   hipGraphExec_t exec = nullptr;
   auto capture = [&]() {
     DeviceLock lock(device_mutex());
@@ -2167,6 +2172,14 @@ void bbpgd_solve(pcc_ctx* ctx) {
   int64_t j = 0;
   bool done = false;
   while (!done) {
+    if (j == kChunkGrowAfter) {
+      chunk *= 2;
+      if (exec) {
+        retired.push_back(exec);
+        exec = nullptr;
+        capture();
+      }
+    }
     if (exec)
       PCC_HIP(hipGraphLaunch(exec, s));
     else
