@@ -844,7 +844,20 @@ void append_batch(pcc_ctx* ctx, int32_t iteration) {
         S.rI.p, S.rJ.p, S.p.p, S.sI.p, S.sJ.p, S.coef64.p, S.coef32.p, ctx->nc.p);
   }
   S.C = C0 + m;
-  if (B > 1 && m > 0) {
+  // The merge moves constraints only if some colony after the first one with
+  // new constraints already holds constraints: otherwise the old entries (all
+  // of colonies up to that one, in colony order) and the batch (in colony
+  // order) already form the colony-major set, and the merge is the identity
+  // (every phase 0, every soft step, a colony stepped alone in a batch).
+  bool moves = false;
+  {
+    bool appended = false;
+    for (const auto& q : ctx->inst) {
+      if (appended && q.C > 0) moves = true;
+      if (q.m > 0) appended = true;
+    }
+  }
+  if (B > 1 && m > 0 && moves) {
     const int64_t T = S.C;
     ConstraintSet& D = ctx->S2;
     ensure_set_capacity(D, T + 1, s);
