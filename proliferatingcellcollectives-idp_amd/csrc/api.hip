@@ -169,6 +169,8 @@ void reset_step(pcc_ctx* ctx) {
   BodyArrays b = body_arrays(ctx->body.p, N);
   if (N > 0) k_reset<<<(unsigned)((N + 255) / 256), 256, 0, ctx->stream>>>(N, b.ldot, ctx->nc.p);
   ctx->S.C = 0;
+  ctx->append_log.clear();
+  ctx->set_unmerged = false;
   for (auto& I : ctx->inst) {
     I.coff = 0;
     I.C = 0;
@@ -461,6 +463,7 @@ void hard_step(pcc_ctx* ctx) {
   for (auto& I : ctx->inst) I.flags = I.status == PCC_OK ? kInstGrow : 0;
   upload_instances(ctx);
   grow(ctx);  // Hard.cpp:185
+  merge_set(ctx);
   for (auto& I : ctx->inst) {
     if (I.status == PCC_OK && I.k == I.p.max_recursive_iterations) {  // Hard.cpp:187-190
       I.status = PCC_E_RELCP_MAX_ITERS;
@@ -487,6 +490,7 @@ void soft_step(pcc_ctx* ctx) {
   brownian_inputs(ctx);  // U_ext of the step (Soft.cpp:94)
   soft_forces_and_integrate(ctx);
   ctx->brownian = false;
+  merge_set(ctx);
   PCC_HIP(hipStreamSynchronize(ctx->stream));
   std::vector<double> mo;
   max_overlap_read(ctx, mo);
@@ -1311,7 +1315,7 @@ pcc_status pcc_device_bytes(pcc_ctx* ctx, int64_t* bytes) {
   add(ctx->inc); add(ctx->increc); add(ctx->increc32); add(ctx->incidx); add(ctx->gold_e); add(ctx->fold); add(ctx->plist);
   add(ctx->sord); add(ctx->spos); add(ctx->skey); add(ctx->skey2); add(ctx->sval); add(ctx->sI);
   add(ctx->sJ); add(ctx->sphi); add(ctx->prow); add(ctx->prow32); add(ctx->flag);
-  add(ctx->iter); add(ctx->gprev); add(ctx->ubuf); add(ctx->partials);
+  add(ctx->iter); add(ctx->gprev); add(ctx->ubuf); add(ctx->partials); add(ctx->merge_offs);
   add(ctx->state); add(ctx->scratch_d); add(ctx->hertz); add(ctx->hertz32); add(ctx->idev);
   add(ctx->binst); add(ctx->bmap); add(ctx->cmap); add(ctx->bbounds); add(ctx->maxbits);
   add(ctx->regroup_dst); add(ctx->trace); add(ctx->trace_n); add(ctx->epart);

@@ -470,14 +470,17 @@ __global__ void k_append(int64_t m, int64_t C0, int32_t iteration, const int32_t
   atomicAdd(&nc[J], 1);
 }
 
-// Instance-major merge: constraint g of the set (old entries first, then the
-// batch appended at C0) moves to its colony's new range, old constraints first.
-__global__ void k_regroup(int64_t total, int64_t C0, const InstDev* __restrict__ idev,
-                          const int32_t* __restrict__ inst_in, int64_t* __restrict__ dst) {
+// Colony-major merge at the end of a step: the set holds one block per append
+// (a ReLCP phase of the colonies that ran it), each in colony order, so a
+// constraint's destination is its position plus the offset of its (colony,
+// phase) block.
+__global__ void k_regroup_phases(int64_t total, const int32_t* __restrict__ inst,
+                                 const int32_t* __restrict__ iter,
+                                 const int64_t* __restrict__ offs, int32_t kmax,
+                                 int64_t* __restrict__ dst) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total) return;
-  const InstDev& D = idev[inst_in[g]];
-  dst[g] = g < C0 ? g - D.coff + D.coff_new : D.coff_new + D.C + (g - C0 - D.bstart);
+  dst[g] = g + offs[(int64_t)inst[g] * kmax + iter[g]];
 }
 
 template <class T, int W>
@@ -844,65 +847,98 @@ void append_batch(pcc_ctx* ctx, int32_t iteration) {
         S.rI.p, S.rJ.p, S.p.p, S.sI.p, S.sJ.p, S.coef64.p, S.coef32.p, ctx->nc.p);
   }
   S.C = C0 + m;
-  // The merge moves constraints only if some colony after the first one with
-  // new constraints already holds constraints: otherwise the old entries (all
-  // of colonies up to that one, in colony order) and the batch (in colony
-  // order) already form the colony-major set, and the merge is the identity
-  // (every phase 0, every soft step, a colony stepped alone in a batch).
-  bool moves = false;
+  // The set is left in append order: one block per append, each in colony
+  // order.  Within a colony that is the colony-major order (phase-major, then
+  // batch order), and nothing during the step depends on where the other
+  // colonies' constraints sit (the solver order is sorted by body index and
+  // a body's incidences by set index, and both keep a colony's relative
+  // order), so the merge into colony-major ranges is done once, at the end of
+  // the step (merge_set).  It moves constraints only if some colony after the
+  // first one with new constraints already holds constraints (not in phase
+  // 0, a soft step, or a colony stepped alone in a batch).
   {
     bool appended = false;
     for (const auto& q : ctx->inst) {
-      if (appended && q.C > 0) moves = true;
+      if (appended && q.C > 0) ctx->set_unmerged = true;
       if (q.m > 0) appended = true;
     }
   }
-  if (B > 1 && m > 0 && moves) {
-    const int64_t T = S.C;
-    ConstraintSet& D = ctx->S2;
-    ensure_set_capacity(D, T + 1, s);
-    ctx->regroup_dst.ensure(T + 1);
-    int64_t* dst = ctx->regroup_dst.p;
-    const unsigned g = nblk(T);
-    k_regroup<<<g, 256, 0, s>>>(T, C0, ctx->idev.p, S.inst.p, dst);
-    k_scatter_rows<int32_t, 1><<<g, 256, 0, s>>>(T, dst, S.inst.p, D.inst.p);
-    k_scatter_rows<int32_t, 1><<<g, 256, 0, s>>>(T, dst, S.I.p, D.I.p);
-    k_scatter_rows<int32_t, 1><<<g, 256, 0, s>>>(T, dst, S.J.p, D.J.p);
-    k_scatter_rows<int32_t, 1><<<g, 256, 0, s>>>(T, dst, S.gid.p, D.gid.p);
-    k_scatter_rows<int32_t, 1><<<g, 256, 0, s>>>(T, dst, S.iter.p, D.iter.p);
-    k_scatter_rows<double, 1><<<g, 256, 0, s>>>(T, dst, S.phi.p, D.phi.p);
-    k_scatter_rows<double, 1><<<g, 256, 0, s>>>(T, dst, S.gamma.p, D.gamma.p);
-    k_scatter_rows<double, 1><<<g, 256, 0, s>>>(T, dst, S.sI.p, D.sI.p);
-    k_scatter_rows<double, 1><<<g, 256, 0, s>>>(T, dst, S.sJ.p, D.sJ.p);
-    k_scatter_rows<double, 3><<<g, 256, 0, s>>>(T, dst, S.n.p, D.n.p);
-    k_scatter_rows<double, 3><<<g, 256, 0, s>>>(T, dst, S.rI.p, D.rI.p);
-    k_scatter_rows<double, 3><<<g, 256, 0, s>>>(T, dst, S.rJ.p, D.rJ.p);
-    k_scatter_rows<double, 3><<<g, 256, 0, s>>>(T, dst, S.p.p, D.p.p);
-    k_scatter_rows<double, 12><<<g, 256, 0, s>>>(T, dst, S.coef64.p, D.coef64.p);
-    k_scatter_rows<float, 12><<<g, 256, 0, s>>>(T, dst, S.coef32.p, D.coef32.p);
-    swap_arr(S.inst, D.inst);
-    swap_arr(S.I, D.I);
-    swap_arr(S.J, D.J);
-    swap_arr(S.gid, D.gid);
-    swap_arr(S.iter, D.iter);
-    swap_arr(S.phi, D.phi);
-    swap_arr(S.gamma, D.gamma);
-    swap_arr(S.sI, D.sI);
-    swap_arr(S.sJ, D.sJ);
-    swap_arr(S.n, D.n);
-    swap_arr(S.rI, D.rI);
-    swap_arr(S.rJ, D.rJ);
-    swap_arr(S.p, D.p);
-    swap_arr(S.coef64, D.coef64);
-    swap_arr(S.coef32, D.coef32);
-    std::swap(S.cap, D.cap);
-    D.C = 0;
+  for (size_t b = 0; b < ctx->inst.size(); ++b) {
+    const auto& q = ctx->inst[b];
+    if (q.m > 0) ctx->append_log.push_back({(int32_t)b, iteration, C0 + q.bstart, q.m});
   }
   for (auto& q : ctx->inst) {
     q.coff = q.coff_new_;
     q.C += q.m;
   }
   upload_instances(ctx);
+}
+
+// The colony-major merge of the step's constraint set (see append_batch):
+// colony b's phase-k block moves to coff_b + (its earlier phases' counts).
+void merge_set(pcc_ctx* ctx) {
+  const int64_t B = (int64_t)ctx->inst.size();
+  ConstraintSet& S = ctx->S;
+  if (!ctx->set_unmerged || B < 2 || S.C == 0) {
+    ctx->set_unmerged = false;
+    return;
+  }
+  hipStream_t s = ctx->stream;
+  int32_t kmax = 1;
+  for (const auto& e : ctx->append_log) kmax = std::max(kmax, e.phase + 1);
+  auto& offs = ctx->merge_offs_host;
+  offs.assign((size_t)B * kmax, 0);
+  std::vector<int64_t> next(B);
+  for (int64_t b = 0; b < B; ++b) next[b] = ctx->inst[b].coff;
+  // the log is in append order, and a colony's phases are appended in order
+  for (const auto& e : ctx->append_log) {
+    offs[(size_t)e.colony * kmax + e.phase] = next[e.colony] - e.start;
+    next[e.colony] += e.count;
+  }
+  const int64_t T = S.C;
+  ctx->merge_offs.ensure((int64_t)offs.size());
+  PCC_HIP(hipMemcpyAsync(ctx->merge_offs.p, offs.data(), sizeof(int64_t) * offs.size(),
+                         hipMemcpyHostToDevice, s));
+  ctx->regroup_dst.ensure(T + 1);
+  k_regroup_phases<<<nblk(T), 256, 0, s>>>(T, S.inst.p, S.iter.p, ctx->merge_offs.p, kmax,
+                                           ctx->regroup_dst.p);
+  ConstraintSet& D = ctx->S2;
+  ensure_set_capacity(D, T + 1, s);
+  const int64_t* dst = ctx->regroup_dst.p;
+  const unsigned g = nblk(T);
+  k_scatter_rows<int32_t, 1><<<g, 256, 0, s>>>(T, dst, S.inst.p, D.inst.p);
+  k_scatter_rows<int32_t, 1><<<g, 256, 0, s>>>(T, dst, S.I.p, D.I.p);
+  k_scatter_rows<int32_t, 1><<<g, 256, 0, s>>>(T, dst, S.J.p, D.J.p);
+  k_scatter_rows<int32_t, 1><<<g, 256, 0, s>>>(T, dst, S.gid.p, D.gid.p);
+  k_scatter_rows<int32_t, 1><<<g, 256, 0, s>>>(T, dst, S.iter.p, D.iter.p);
+  k_scatter_rows<double, 1><<<g, 256, 0, s>>>(T, dst, S.phi.p, D.phi.p);
+  k_scatter_rows<double, 1><<<g, 256, 0, s>>>(T, dst, S.gamma.p, D.gamma.p);
+  k_scatter_rows<double, 1><<<g, 256, 0, s>>>(T, dst, S.sI.p, D.sI.p);
+  k_scatter_rows<double, 1><<<g, 256, 0, s>>>(T, dst, S.sJ.p, D.sJ.p);
+  k_scatter_rows<double, 3><<<g, 256, 0, s>>>(T, dst, S.n.p, D.n.p);
+  k_scatter_rows<double, 3><<<g, 256, 0, s>>>(T, dst, S.rI.p, D.rI.p);
+  k_scatter_rows<double, 3><<<g, 256, 0, s>>>(T, dst, S.rJ.p, D.rJ.p);
+  k_scatter_rows<double, 3><<<g, 256, 0, s>>>(T, dst, S.p.p, D.p.p);
+  k_scatter_rows<double, 12><<<g, 256, 0, s>>>(T, dst, S.coef64.p, D.coef64.p);
+  k_scatter_rows<float, 12><<<g, 256, 0, s>>>(T, dst, S.coef32.p, D.coef32.p);
+  swap_arr(S.inst, D.inst);
+  swap_arr(S.I, D.I);
+  swap_arr(S.J, D.J);
+  swap_arr(S.gid, D.gid);
+  swap_arr(S.iter, D.iter);
+  swap_arr(S.phi, D.phi);
+  swap_arr(S.gamma, D.gamma);
+  swap_arr(S.sI, D.sI);
+  swap_arr(S.sJ, D.sJ);
+  swap_arr(S.n, D.n);
+  swap_arr(S.rI, D.rI);
+  swap_arr(S.rJ, D.rJ);
+  swap_arr(S.p, D.p);
+  swap_arr(S.coef64, D.coef64);
+  swap_arr(S.coef32, D.coef32);
+  std::swap(S.cap, D.cap);
+  D.C = 0;
+  ctx->set_unmerged = false;
 }
 
 // Copies the current (sorted) detection batch to host arrays.

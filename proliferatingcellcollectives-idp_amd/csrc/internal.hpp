@@ -391,6 +391,16 @@ struct pcc_ctx {
   pcc::DArr<int32_t> bbounds;              // B+1 detection-batch offsets, then the candidate count
   pcc::DArr<unsigned long long> maxbits;   // B: max overlap bits
   pcc::ConstraintSet S2;                   // merge target (B > 1)
+  // appends of the current step, in order (colony, its phase, first set
+  // index, count); set_unmerged: the set is not yet in colony-major order
+  struct AppendRec {
+    int32_t colony, phase;
+    int64_t start, count;
+  };
+  std::vector<AppendRec> append_log;
+  bool set_unmerged = false;
+  std::vector<int64_t> merge_offs_host;
+  pcc::DArr<int64_t> merge_offs;
   pcc::DArr<int64_t> regroup_dst;
   int64_t nbb_total = 0, ncb_total = 0;
 };
@@ -469,6 +479,7 @@ void xsort_run(const XSort& a);
 // launchers (kernels in detect.hip / solve.hip / soft.hip)
 void detect_batch(pcc_ctx* ctx);  // active colonies at their cutoffs; fills ctx->ridx sorted
 void append_batch(pcc_ctx* ctx, int32_t iteration);               // appends sorted batch to S
+void merge_set(pcc_ctx* ctx);  // the step's set into colony-major order (end of a step)
 void build_incidence(pcc_ctx* ctx, bool records);
 bool bodies_planar(pcc_ctx* ctx);
 bool state_planar(pcc_ctx* ctx);  // bodies_planar, or the value known from the last step

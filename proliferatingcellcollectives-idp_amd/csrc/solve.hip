@@ -1506,18 +1506,20 @@ __global__ void k_maxov(int64_t C, const int32_t* __restrict__ Sinst,
   __shared__ double lds[4];
   const int64_t c0 = (int64_t)blockIdx.x * blockDim.x;
   const int64_t c = c0 + threadIdx.x;
-  const int64_t cl = c0 + PCC_BLOCK - 1 < C ? c0 + PCC_BLOCK - 1 : C - 1;
   double v = 0.0;
   if (c < C) {
     const double p = phi[c];
     v = p < 0 ? -p : 0.0;
   }
+  // one block maximum when every constraint of the block belongs to one
+  // colony (the set need not be in colony-major order during a step)
   const int32_t first = Sinst[c0];
-  if (first == Sinst[cl]) {
+  const int32_t mine = c < C ? Sinst[c] : first;
+  if (__syncthreads_and(mine == first)) {
     const double r = block_max(v, lds);
     if (threadIdx.x == 0) atomicMax(&bits[first], (unsigned long long)__double_as_longlong(r));
   } else if (c < C) {
-    atomicMax(&bits[Sinst[c]], (unsigned long long)__double_as_longlong(v));
+    atomicMax(&bits[mine], (unsigned long long)__double_as_longlong(v));
   }
 }
 
