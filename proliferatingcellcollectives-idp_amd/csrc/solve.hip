@@ -190,18 +190,23 @@ __global__ void k_solver_rows(int64_t C, const int32_t* __restrict__ ord,
     static_cast<float*>(sphi)[p] = (float)phi[c];
   store_it<CT>(it, itg, p, gam[c], 0.0);
   const double* r = coef + 12 * c;
+  // only the row of the solve's precision is read
   if (PLANAR) {
     const double v[6] = {r[0], r[1], r[5], r[8], r[9], r[10]};
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
-      row[6 * p + k] = v[k];
-      row32[6 * p + k] = (float)v[k];
+      if constexpr (sizeof(CT) == 8)
+        row[6 * p + k] = v[k];
+      else
+        row32[6 * p + k] = (float)v[k];
     }
   } else {
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
-      row[12 * p + k] = r[k];
-      row32[12 * p + k] = (float)r[k];
+      if constexpr (sizeof(CT) == 8)
+        row[12 * p + k] = r[k];
+      else
+        row32[12 * p + k] = (float)r[k];
     }
   }
 }
@@ -209,8 +214,8 @@ __global__ void k_solver_rows(int64_t C, const int32_t* __restrict__ ord,
 // Incidence records, body-major: the coefficients of the D row on this body's
 // DOFs ([n, rI x n] for side I, [-n, rJ x -n] for side J; planar (c_x, c_y,
 // tau_z)), the stress coefficient, the constraint's solver position (3D: in
-// the last slot; planar: in rc) and gamma_old, in FP64 and FP32 copies.
-template <bool PLANAR>
+// the last slot; planar: in rc) and gamma_old, in the solve's precision.
+template <bool PLANAR, bool MIXED>
 __global__ void k_inc_records(int64_t E, const int32_t* __restrict__ inc,
                               const double* __restrict__ coef, const int32_t* __restrict__ spos,
                               const double* __restrict__ gam, double* __restrict__ rec,
@@ -237,10 +242,13 @@ __global__ void k_inc_records(int64_t E, const int32_t* __restrict__ inc,
       co[2] = row[8];
       co[3] = row[10];
     }
-    reinterpret_cast<double2*>(rec)[2 * e] = make_double2(co[0], co[1]);
-    reinterpret_cast<double2*>(rec)[2 * e + 1] = make_double2(co[2], co[3]);
-    reinterpret_cast<float4*>(rec32)[e] =
-        make_float4((float)co[0], (float)co[1], (float)co[2], (float)co[3]);
+    if (MIXED) {
+      reinterpret_cast<float4*>(rec32)[e] =
+          make_float4((float)co[0], (float)co[1], (float)co[2], (float)co[3]);
+    } else {
+      reinterpret_cast<double2*>(rec)[2 * e] = make_double2(co[0], co[1]);
+      reinterpret_cast<double2*>(rec)[2 * e + 1] = make_double2(co[2], co[3]);
+    }
     rc[e] = pos;
   } else {
     double co[8];
@@ -257,13 +265,16 @@ __global__ void k_inc_records(int64_t E, const int32_t* __restrict__ inc,
       co[6] = row[10];
     }
     co[7] = __longlong_as_double((long long)pos);
-    double* r = rec + 8 * e;
+    if (MIXED) {
+      float* r32 = rec32 + 8 * e;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) r[k] = co[k];
-    float* r32 = rec32 + 8 * e;
+      for (int k = 0; k < 7; ++k) r32[k] = (float)co[k];
+      r32[7] = __int_as_float(pos);
+    } else {
+      double* r = rec + 8 * e;
 #pragma unroll
-    for (int k = 0; k < 7; ++k) r32[k] = (float)co[k];
-    r32[7] = __int_as_float(pos);
+      for (int k = 0; k < 8; ++k) r[k] = co[k];
+    }
   }
 }
 
@@ -1663,10 +1674,13 @@ void build_incidence(pcc_ctx* ctx, bool records) {
   ctx->sphi.ensure(C + 1);
   ctx->iter.ensure(2 * (C + 1));  // (gamma, g): 16-byte FP64 pairs or 12-byte mixed records
   ctx->gprev.ensure(C + 1);
-  ctx->prow.ensure(W * (C + 1));
-  ctx->prow32.ensure(W * (C + 1));
   ConstraintSet& S = ctx->S;
   const bool mixed = ctx->params.precision == PCC_PREC_MIXED;
+  // rows and incidence records exist in the solve's precision only
+  if (mixed)
+    ctx->prow32.ensure(W * (C + 1));
+  else
+    ctx->prow.ensure(W * (C + 1));
   float* itg = iter_g(ctx);
 #define PCC_SOLVER_ROWS(CT, PL)                                                              \
   k_solver_rows<CT, PL><<<nblk(C), 256, 0, s>>>(C, ctx->sord.p, S.I.p, S.J.p, S.phi.p,      \
@@ -1686,25 +1700,35 @@ void build_incidence(pcc_ctx* ctx, bool records) {
   }
 #undef PCC_SOLVER_ROWS
   const int RW = ctx->planar ? 4 : 8;
-  ctx->increc.ensure(RW * (2 * C + 1));
-  ctx->increc32.ensure(RW * (2 * C + 1));
+  if (mixed)
+    ctx->increc32.ensure(RW * (2 * C + 1));
+  else
+    ctx->increc.ensure(RW * (2 * C + 1));
   ctx->incidx.ensure(2 * C + 1);
   ctx->gold_e.ensure(2 * C + 1);
-  if (ctx->planar)
-    k_inc_records<true><<<nblk(2 * C), 256, 0, s>>>(2 * C, ctx->inc.p, S.coef64.p, ctx->spos.p,
-                                                    S.gamma.p, ctx->increc.p, ctx->increc32.p,
-                                                    ctx->incidx.p, ctx->gold_e.p);
-  else
-    k_inc_records<false><<<nblk(2 * C), 256, 0, s>>>(2 * C, ctx->inc.p, S.coef64.p, ctx->spos.p,
-                                                     S.gamma.p, ctx->increc.p, ctx->increc32.p,
-                                                     ctx->incidx.p, ctx->gold_e.p);
+#define PCC_INC_RECORDS(PL, MX)                                                             \
+  k_inc_records<PL, MX><<<nblk(2 * C), 256, 0, s>>>(2 * C, ctx->inc.p, S.coef64.p, ctx->spos.p, \
+                                                    S.gamma.p, ctx->increc.p, ctx->increc32.p,  \
+                                                    ctx->incidx.p, ctx->gold_e.p)
+  if (ctx->planar) {
+    if (mixed)
+      PCC_INC_RECORDS(true, true);
+    else
+      PCC_INC_RECORDS(true, false);
+  } else {
+    if (mixed)
+      PCC_INC_RECORDS(false, true);
+    else
+      PCC_INC_RECORDS(false, false);
+  }
+#undef PCC_INC_RECORDS
 }
 
-// Logical block layout of the BBPGD kernels over the active colonies:
-// colony b gets ceil(N_b / 256) body blocks and G_b = min(Q_b, 2048) blocks
-// of K_b = ceil(Q_b / G_b) constraint tiles (Q_b = ceil(C_b / 256); a colony
-// with no constraints still gets one block, as a lone colony does).
-// Logical blocks of the active colonies (with `keep`, only those it marks).
+// Logical block layout of the BBPGD kernels over the active colonies (with
+// `keep`, only those it marks): colony b gets ceil(N_b / 256) body blocks
+// and G_b = min(Q_b, 2048) blocks of K_b = ceil(Q_b / G_b) constraint tiles
+// (Q_b = ceil(C_b / 256); a colony with no constraints still gets one
+// block, as a lone colony does).
 static void solver_layout(pcc_ctx* ctx, const std::vector<char>* keep = nullptr) {
   std::vector<int32_t> bm, cm;
   int64_t lb = 0, bb = 0;
