@@ -249,3 +249,20 @@ def test_division_before_each_step_seeded_per_colony():
         assert sorted(seen[r.rid]) == list(range(r.steps))
         assert serving.max_cells(r) > r.bodies.n
     assert st["peak_reserved_bytes"] >= 1000 * max(serving.max_cells(r) for r in reqs)
+
+
+def test_step_log_and_batch_summary():
+    """Every batch step is logged with its colonies, cells and the most
+    BBPGD iterations of a colony; the summary groups them per batch."""
+    reqs = make_requests([(100, 0, 3, 0.0), (120, 0, 3, 0.0), (90, 1, 2, 0.0)])
+    srv = serving.ColonyServer(10**12, bytes_per_cell=1000, warm_steps=0, divide=False)
+    srv.run(reqs)
+    hard = [r for r in srv.step_log if r[0][0] == 0]
+    soft = [r for r in srv.step_log if r[0][0] == 1]
+    assert sum(r[1] for r in hard) == 6 and sum(r[1] for r in soft) == 2
+    n0, n1 = reqs[0].bodies.n, reqs[1].bodies.n
+    assert all(r[2] in ((n0, n1) if r[1] == 1 else (n0 + n1,)) for r in hard)
+    summ = srv.step_summary()
+    assert set(summ) == {"0-0-1", "1-0-1"}
+    assert summ["0-0-1"]["batch_steps"] == len(hard)
+    assert summ["0-0-1"]["max_iterations_p99"] == 0.0  # the fake batch reports none
