@@ -197,7 +197,6 @@ void upload_instances(pcc_ctx* ctx) {
     d.lb = I.lb;
     d.bb = I.bb;
     d.bstart = I.bstart;
-    d.coff_new = I.coff_new_;
     d.maxit = p.max_bbpgd_iterations;
     d.dt = p.dt;
     d.xi = p.xi;

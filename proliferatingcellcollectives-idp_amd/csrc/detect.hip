@@ -822,10 +822,10 @@ static void swap_arr(DArr<T>& a, DArr<T>& b) {
   a.swap(b);
 }
 
-// Appends the sorted batch; with several colonies the set is then regrouped
-// instance-major (each colony's old constraints, then its new ones).  The
-// instance records must hold the colonies' current ranges, m and bstart; on
-// return they hold the new ranges.
+// Appends the sorted batch at the end of the set (merge_set puts the set in
+// colony-major order at the end of the step).  The instance records must
+// hold the colonies' current counts, m and bstart; on return their ranges
+// (coff, C) are the colony-major ones, which the solver order follows.
 void append_batch(pcc_ctx* ctx, int32_t iteration) {
   const int64_t m = ctx->batch_count;
   const int64_t B = (int64_t)ctx->inst.size();

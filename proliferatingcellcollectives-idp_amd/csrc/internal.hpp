@@ -161,10 +161,10 @@ enum { kInstActive = 1, kInstInteg = 2, kInstGrow = 4 };
 
 // One colony of a context as the kernels see it.  A context holds B >= 1
 // independent colonies; their bodies are concatenated (instance b owns bodies
-// [boff, boff + N)) and the constraint set is kept instance-major (instance b
-// owns constraints [coff, coff + C), in its own solver order), so every
-// per-instance sum and reduction has exactly the order of a context holding
-// that colony alone.
+// [boff, boff + N)) and the solver order is instance-major (instance b owns
+// solver positions [coff, coff + C)), as is the constraint set at the end of
+// a step, so every per-instance sum and reduction has exactly the order of a
+// context holding that colony alone.
 struct InstDev {
   int64_t boff, N;
   int64_t coff, C;
@@ -172,7 +172,6 @@ struct InstDev {
   int64_t lb;        // first logical constraint block (= first partial slot)
   int64_t bb;        // first logical body block
   int64_t bstart;    // first record of the instance in the sorted detection batch
-  int64_t coff_new;  // constraint offset once the current batch is merged
   int64_t maxit;
   double dt, xi, itau, mlam, kcc, tol, cutoff, warm, origin, h;
   double lam, tau;   // for the energy's growth term dt / (lambda tau)
@@ -218,7 +217,7 @@ struct Instance {
   int64_t boff = 0, N = 0;
   int64_t coff = 0, C = 0;   // constraint range of the current step
   int64_t m = 0, bstart = 0; // current detection batch
-  int64_t coff_new_ = 0;     // range offset once the batch is merged
+  int64_t coff_new_ = 0;     // range offset once the batch is appended
   double cutoff = 0.0;       // contact cutoff of the current detection
   int64_t G = 1, K = 0, lb = 0, bb = 0, nbb = 0;
   int32_t flags = 0;
