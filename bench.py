@@ -163,6 +163,29 @@ def run_batched(colonies, precision, steps, warmup, timing=True, restore=True, r
 PMC_SUMMARY = os.path.join(REPO, "profiles", "bbpgd_mixed_300k_colony_pmc.txt")
 
 
+TRACE_SUMMARY = os.path.join(REPO, "profiles", "bbpgd_mixed_300k_colony_kernel_trace.txt")
+
+
+def traced_from_profile(kernel, tuning):
+    """Average duration (us) of `kernel` ("body_apply" with the variant the
+    autotuner picked, or "constraint_apply") in the committed rocprofv3
+    kernel-trace summary of this workload, or None when the summary holds no
+    line for that kernel instantiation.  Events around each launch also time
+    the launch's dispatch and its end-of-kernel flush, so the traced average
+    is the shorter of the two."""
+    if not os.path.exists(TRACE_SUMMARY):
+        return None
+    if kernel == "body_apply":
+        name = (f"k_body_coop<float, true, {tuning['body_chunk']}, {tuning['body_threads']}, "
+                f"{'true' if tuning['prefetch'] else 'false'}, 0>")
+    else:
+        name = "k_constr<float, true>"
+    for line in open(TRACE_SUMMARY):
+        if line.startswith("pcc::" + name + " ") and "avg_us=" in line:
+            return float(line.rsplit("avg_us=", 1)[1])
+    return None
+
+
 def schedule(n_cells):
     """How a batch step runs colonies of n_cells cells: each ReLCP loop alone
     ("solo", from the library's threshold of 150000 cells or PCC_SOLO_CELLS)
@@ -827,6 +850,18 @@ def main():
                                          "algorithmic_bytes_per_launch": t["bytes_per_launch"],
                                          "gbs": t["bytes_per_launch"] / avg / 1e9,
                                          "frac": t["bytes_per_launch"] / avg / 1e9 / HBM_PEAK_GBS}
+                # the same kernel's average in the committed rocprofv3 trace
+                # of this workload, when that trace has its instantiation
+                tr = traced_from_profile(name, out["mixed"]["tuning"]) \
+                    if schedule(bodies.n) == "solo" and len(cols) == 2 else None
+                if tr:
+                    line["kernels"][name]["traced_avg_us"] = tr
+                    line["kernels"][name]["traced_frac"] = \
+                        t["bytes_per_launch"] / (tr * 1e-6) / 1e9 / HBM_PEAK_GBS
+        kr = line["kernels"].get(rf["kernel"], {})
+        if "traced_avg_us" in kr:
+            line["roofline"]["traced_avg_us"] = kr["traced_avg_us"]
+            line["roofline"]["traced_frac"] = kr["traced_frac"]
     if "fp64" in out and "mixed" in out:
         line["fp64"] = {k: out["fp64"][k] for k in ("cell_steps_per_s", "ms_per_step", "p50_ms",
                                                     "p99_ms", "bbpgd_iterations", "n_constraints",

@@ -28,6 +28,16 @@ def test_traffic_comes_from_the_committed_counter_summary():
     assert bench.traffic_from_profile("no_such_kernel", 2, "solo") is None
 
 
+def test_traced_average_comes_from_the_committed_kernel_trace():
+    body = bench.traced_from_profile("body_apply",
+                                     {"body_chunk": 512, "body_threads": 256, "prefetch": False})
+    constr = bench.traced_from_profile("constraint_apply", {})
+    assert body and constr and 0 < constr < body
+    # a variant the trace did not run has no figure
+    assert bench.traced_from_profile(
+        "body_apply", {"body_chunk": 2048, "body_threads": 128, "prefetch": False}) is None
+
+
 def test_schedule_follows_the_solo_threshold(monkeypatch):
     monkeypatch.delenv("PCC_SOLO_CELLS", raising=False)
     assert bench.schedule(300_000) == "solo"
