@@ -1312,8 +1312,8 @@ pcc_status pcc_device_bytes(pcc_ctx* ctx, int64_t* bytes) {
   add(ctx->xs_cnt); add(ctx->xs_off); add(ctx->xs_bounds);
   add(ctx->sort_tmp); add_set(ctx->S); add_set(ctx->S2); add(ctx->rowptr); add(ctx->inccur);
   add(ctx->inc); add(ctx->increc); add(ctx->increc32); add(ctx->incidx); add(ctx->gold_e); add(ctx->fold); add(ctx->plist);
-  add(ctx->sord); add(ctx->spos); add(ctx->skey); add(ctx->skey2); add(ctx->sval); add(ctx->sI);
-  add(ctx->sJ); add(ctx->sphi); add(ctx->prow); add(ctx->prow32); add(ctx->flag);
+  add(ctx->sord); add(ctx->spos); add(ctx->skey); add(ctx->skey2); add(ctx->sval); add(ctx->sIJ);
+  add(ctx->sphi); add(ctx->prow); add(ctx->prow32); add(ctx->flag);
   add(ctx->iter); add(ctx->gprev); add(ctx->ubuf); add(ctx->partials); add(ctx->merge_offs);
   add(ctx->state); add(ctx->scratch_d); add(ctx->hertz); add(ctx->hertz32); add(ctx->idev);
   add(ctx->binst); add(ctx->bmap); add(ctx->cmap); add(ctx->bbounds); add(ctx->maxbits);

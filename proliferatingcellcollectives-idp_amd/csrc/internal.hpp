@@ -336,7 +336,7 @@ struct pcc_ctx {
   pcc::DArr<int32_t> plist;                // colonies of a persistent BBPGD launch
   pcc::DArr<int32_t> sord, spos;           // solver order and its inverse
   pcc::DArr<int32_t> skey, skey2, sval;    // sort scratch
-  pcc::DArr<int32_t> sI, sJ;               // solver order
+  pcc::DArr<int2> sIJ;                     // solver order: (I, J) per constraint
   pcc::DArr<double> sphi;                  // solver order
   pcc::DArr<double> prow;                  // operator rows, solver order (6 planar, 12 3D)
   pcc::DArr<float> prow32;

@@ -175,15 +175,14 @@ template <class CT, bool PLANAR>
 __global__ void k_solver_rows(int64_t C, const int32_t* __restrict__ ord,
                               const int32_t* __restrict__ I, const int32_t* __restrict__ J,
                               const double* __restrict__ phi, const double* __restrict__ gam,
-                              const double* __restrict__ coef, int32_t* __restrict__ sI,
-                              int32_t* __restrict__ sJ, void* __restrict__ sphi,
+                              const double* __restrict__ coef, int2* __restrict__ sIJ,
+                              void* __restrict__ sphi,
                               double* __restrict__ it, float* __restrict__ itg,
                               double* __restrict__ row, float* __restrict__ row32) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= C) return;
   const int64_t c = ord[p];
-  sI[p] = I[c];
-  sJ[p] = J[c];
+  sIJ[p] = make_int2(I[c], J[c]);
   if constexpr (sizeof(CT) == 8)
     static_cast<double*>(sphi)[p] = phi[c];
   else
@@ -740,8 +739,7 @@ struct ConstrArgs {
   int64_t Gtot;         // stride of the partials arrays
   const int32_t* cmap;  // logical constraint block -> colony
   const InstDev* idev;
-  const int32_t* I;
-  const int32_t* J;
+  const int2* IJ;       // (I, J) per constraint, solver order
   const void* coef;     // 3D: 12 CT per constraint (n, rIxn, rJx-n, sI, sJ, pad); 2D: 6 CT
   const void* phi;      // Phi, solver order (mixed: FP32)
   double* it;
@@ -959,8 +957,9 @@ __device__ __forceinline__ void constr_block(const ConstrArgs& a, int64_t l, dou
   // take the scalar-base form and the loop carries one index instead of a
   // 64-bit pointer per array.
   auto fetch = [&](uint32_t c, TileIn<CT, PLANAR>& t) {
-    t.I = *at<int32_t>(a.I, c);
-    t.J = *at<int32_t>(a.J, c);
+    const int2 ij = *at<int2>(a.IJ, c);
+    t.I = ij.x;
+    t.J = ij.y;
     t.yg.v = *at<decltype(t.yg.v)>(a.it, c);
     t.phi = *at<CT>(a.phi, c);
     t.row = *at<Row<CT, PLANAR>>(a.coef, c);
@@ -1314,7 +1313,8 @@ __global__ void __launch_bounds__(kPersistThreads)
       for (int64_t j = 0; j < K; ++j) {
         const int64_t c = D.coff + (b * K + j) * PCC_BLOCK + lt;
         if (c >= cend) break;
-        const int32_t I = ca.I[c], J = ca.J[c];
+        const int2 ij = ca.IJ[c];
+        const int32_t I = ij.x, J = ij.y;
         const double2 yg = load_it<CT>(ca.it, ca.itg, c);
         const double phi = load_phi<CT>(ca.phi, c);
         const Row<CT, PLANAR> row = load_row<CT, PLANAR>(ca.coef, c);
@@ -1669,8 +1669,7 @@ void build_incidence(pcc_ctx* ctx, bool records) {
                                     ctx->sval.p, ctx->sord.p, (size_t)C, 0, bits, s));
   k_invert<<<nblk(C), 256, 0, s>>>(C, ctx->sord.p, ctx->spos.p);
   const int W = ctx->planar ? 6 : 12;
-  ctx->sI.ensure(C + 1);
-  ctx->sJ.ensure(C + 1);
+  ctx->sIJ.ensure(C + 1);
   ctx->sphi.ensure(C + 1);
   ctx->iter.ensure(2 * (C + 1));  // (gamma, g): 16-byte FP64 pairs or 12-byte mixed records
   ctx->gprev.ensure(C + 1);
@@ -1684,7 +1683,7 @@ void build_incidence(pcc_ctx* ctx, bool records) {
   float* itg = iter_g(ctx);
 #define PCC_SOLVER_ROWS(CT, PL)                                                              \
   k_solver_rows<CT, PL><<<nblk(C), 256, 0, s>>>(C, ctx->sord.p, S.I.p, S.J.p, S.phi.p,      \
-                                                S.gamma.p, S.coef64.p, ctx->sI.p, ctx->sJ.p, \
+                                                S.gamma.p, S.coef64.p, ctx->sIJ.p, \
                                                 ctx->sphi.p, ctx->iter.p, itg, ctx->prow.p,  \
                                                 ctx->prow32.p)
   if (ctx->planar) {
@@ -1801,8 +1800,7 @@ static ConstrArgs constr_args(pcc_ctx* ctx) {
   a.cmap = ctx->cmap.p;
   a.idev = ctx->idev.p;
   (void)S;
-  a.I = ctx->sI.p;
-  a.J = ctx->sJ.p;
+  a.IJ = ctx->sIJ.p;
   a.coef = mixed ? (const void*)ctx->prow32.p : (const void*)ctx->prow.p;
   a.phi = ctx->sphi.p;
   a.it = ctx->iter.p;
