@@ -5,10 +5,11 @@ lengths U[1,2], diameter 0.5, nematic domains with directions U[0,pi) packed
 at area fraction ~0.85 (R ~ 279) with ~1e-3..1e-2 overlaps, contact cutoff
 0.05 in every ReLCP phase, BBPGD and ReLCP tolerance 1e-6, dt = 1e-3,
 lambda = 1e-2, tau = 1/ln2, seeded and synthetic.  The full ReLCP loop runs
-(3 phases on this input).  I.i.d. uniform centres at area fraction 0.9 are
-not used: ~16% of their contacts are crossing rods whose normal is the
-arbitrary (1,0,0) of ArrayMath::normalize, and BBPGD then does not reach 1e-6
-(3M iterations at 5k and 20k cells).
+(3 phases on this input).  `--placement iid` steps i.i.d. uniform centres at
+area fraction 0.9 instead (the configured placement): ~15% of their contacts
+are crossing rods whose normal is the arbitrary (1,0,0) of
+ArrayMath::normalize, and their phase-0 LCP needs ~18,000 BBPGD iterations at
+5k and 20k cells (the nematic packing's needs 20).
 
 `--instances B` colonies (seeds seed..seed+B-1) are stepped together by the
 batched mixed-precision engine (one context, one step call, captured HIP
@@ -722,6 +723,9 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--instances", type=int, default=2)
     ap.add_argument("--colonies", type=int, default=128)
+    ap.add_argument("--placement", choices=["nematic", "iid"], default="nematic",
+                    help="headline colonies: nematic-domain packings, or i.i.d. centres at "
+                         "area fraction 0.9")
     ap.add_argument("--seeds", type=int, default=8,
                     help="--workload seeds: number of colony seeds")
     ap.add_argument("--workload", choices=["colony300k", "ensemble", "serving", "soft", "seeds"],
@@ -755,7 +759,8 @@ def main():
     from pcc.engine import copy_bandwidth, read_bandwidth
     copy_gbs = copy_bandwidth(1 << 30, 10)
     read_gbs = read_bandwidth(1 << 31, 10)
-    cols = [G.dense_colony(args.n, seed=args.seed + q) for q in range(args.instances)]
+    make = G.random_disc if args.placement == "iid" else G.dense_colony
+    cols = [make(args.n, seed=args.seed + q) for q in range(args.instances)]
     bodies = cols[0]
     ncell = sum(c.n for c in cols)
     print(f"[bench] {len(cols)} x n={bodies.n} R={G.disc_radius(bodies):.1f} "
@@ -811,7 +816,9 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": (f"hard-model single step (full ReLCP) of {len(cols)} independent "
-                         f"{bodies.n}-cell colony packings (nematic domains, area fraction "
+                         f"{bodies.n}-cell colony packings ("
+                         f"{'i.i.d. centres' if args.placement == 'iid' else 'nematic domains'}"
+                         f", area fraction "
                          f"{G.area_fraction(bodies):.3f}, R={G.disc_radius(bodies):.0f}, seeds "
                          f"{args.seed}..{args.seed + len(cols) - 1}) batched in one context "
                          f"(colonies of >= 150000 cells run their ReLCP loops one after "

@@ -10,7 +10,7 @@
     constraints, iteration count, residual); the mixed step's impulses are
     within 1e-5 of the oracle's normwise, its body velocities within 1e-5;
   - config 2's i.i.d. disc at area fraction 0.9, cutoff 0.05: its phase-0
-    LCP does not converge (crossing rods; residual ~0.7 after 3,000
+    LCP converges slowly (crossing rods; residual ~0.7 after 3,000
     iterations), so the test caps BBPGD at 100 iterations: both sides fail
     at the same iteration with the same residual, bit for bit, which pins
     the first 100 gradient evaluations and BB steps.
